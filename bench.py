@@ -1,0 +1,204 @@
+"""Benchmark: batched persistent range search on the 16M-point 3-D config.
+
+Workload (16,777,216 FP32 3-D points from a 64-component Gaussian mixture,
+sigma 0.01, clipped to the unit cube; MPHR tree, fanout autotuned over
+{32, 64, 128}; 100,000 box queries centred on mixture draws with volume
+selectivity log-uniform in [1e-6, 1e-2]; all queries in one persistent launch).
+
+One step = one persistent search of the whole query batch (per-query hit count
++ ID-set hash), with the tree and the queries already resident in HBM.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "query-co-processing-on-heterogeneous-architectures_amd")
+PEAK_HBM_GBS = 8000.0
+
+
+def load_qcp():
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("qcp", os.path.join(PKG, "qcp.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["qcp"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(st, q, D, E):
+    """Canonical-traversal bytes of one search launch: per query its box
+    (2D*4), every entry of every visited internal node (2D*4 each), every
+    entry of every visited leaf (E*4 each), plus the count (4) and hash (8)."""
+    return (q * (2 * D * 4 + 4 + 8) + st["internal_entries"] * 2 * D * 4 +
+            st["leaf_entries"] * E * 4)
+
+
+def cpu_baseline(qcp, ix, queries, got, budget_s, threads):
+    """The oracle's multi-threaded DFS (the reference CPU search's static query
+    split, src/tree/bvh.cpp:195-220) on the same tree, over a bounded prefix of
+    the queries; its results are also checked against the GPU's."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    ex = ix.export()
+    inf = ex["info"]
+    t = O.Tree.__new__(O.Tree)
+    t.sobjs = np.ascontiguousarray(ex["sobjs"])
+    t.n = inf["n"]
+    t.D, t.is_box, t.F, t.layout = inf["dims"], int(inf["obj_kind"] == 1), inf["fanout"], "packed"
+    t.n_nodes = inf["n_nodes"]
+    t.first, t.count, t.leaf = ex["first"], ex["count"], ex["leaf"]
+    t.boxes = np.ascontiguousarray(ex["boxes"])
+    # size the sample from a short probe
+    probe = min(len(queries), max(threads * 4, 64))
+    t0 = time.perf_counter()
+    c, h = t.search_dfs_mt(queries[:probe], threads)
+    dt = time.perf_counter() - t0
+    per_q = dt / probe
+    m = int(min(len(queries), max(probe, budget_s / max(per_q, 1e-9))))
+    t0 = time.perf_counter()
+    c, h = t.search_dfs_mt(queries[:m], threads)
+    dt = time.perf_counter() - t0
+    ok = bool(np.array_equal(c, got["counts"][:m]) and np.array_equal(h, got["hashes"][:m]))
+    return {"value": m / dt, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": "first %d of %d queries, oracle DFS over %d threads, %.1f s" %
+                      (m, len(queries), threads, dt),
+            "matches_gpu": ok}
+
+
+def read_traffic(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=float, default=1.0, help="shrink N and Q (debug only)")
+    ap.add_argument("--fanout", type=int, default=0, help="0 = autotune over {32,64,128}")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the box has one); --gpus %d is not "
+              "supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    qcp = load_qcp()
+    if qcp.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    log("generating workload c2 (scale %.3f)" % args.scale)
+    w = qcp.workload("c2", scale=args.scale)
+    pts, queries = w["objs"], w["queries"]
+    n, q, D, E = len(pts), len(queries), 3, 3
+    # autotune fanout on a query sample (untimed setup)
+    cands = [args.fanout] if args.fanout else [32, 64, 128]
+    best = None
+    tune = {}
+    sample = queries[: min(q, 20000)]
+    for F in cands:
+        ix = qcp.Index.build(pts, 3, fanout=F, layout="mphr", key_mode="ref")
+        with ix.batch(sample) as b:
+            b.run()
+            ms = min(b.run()["kernel_ms"] for _ in range(3))
+        tune[F] = ms
+        log("autotune F=%d: %.3f ms for %d queries; build %.2f ms" % (F, ms, len(sample),
+                                                                       ix.info()["build_ms"]))
+        if best is None or ms < tune[best[0]]:
+            if best is not None:
+                best[1].close()
+            best = (F, ix)
+        else:
+            ix.close()
+    F, ix = best
+    info = ix.info()
+    b = ix.batch(queries)
+    for _ in range(args.warmup):
+        b.run()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = b.run()
+        kms.append(st["kernel_ms"])
+    wall = time.perf_counter() - t0
+    got = b.results()
+    ms_step = wall * 1000.0 / args.steps
+    value = q * args.steps / wall
+    kavg = float(np.mean(kms))
+    abytes = algorithmic_bytes(st, q, D, E)
+    achieved = abytes / (kavg * 1e-3) / 1e9
+    log("persistent: %.3f ms/step wall, %.3f ms kernel, %.3e queries/s, hits %d, "
+        "int entries %d, leaf entries %d, %.1f GB/s algorithmic" %
+        (ms_step, kavg, value, st["total_hits"], st["internal_entries"], st["leaf_entries"],
+         achieved))
+    plain = None
+    if not args.no_plain:
+        pq = queries[: min(q, 20000)]
+        with ix.batch(pq) as pb:
+            pst = pb.run(mode="plain", plain_batch=128)
+            pres = pb.results()
+        plain = {"queries_per_s": len(pq) / (pst["total_ms"] * 1e-3), "sample_queries": len(pq),
+                 "batch": 128, "total_ms": pst["total_ms"], "launches": pst["launches"],
+                 "matches_persistent": bool(np.array_equal(pres["counts"], got["counts"][: len(pq)]))}
+        log("plain: %.3e queries/s over %d queries" % (plain["queries_per_s"], len(pq)))
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(qcp, ix, queries, got, args.cpu_budget, threads)
+        log("cpu baseline: %.3e queries/s (%s)" % (cpu["value"], cpu["sample"]))
+    traffic = read_traffic("traffic_c2_search.json")
+    kname = "k_search"
+    out = {
+        "metric": "range queries/sec (batched persistent, 16M-point 3-D config)",
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox4x32-10 seed 1234)",
+        "config": {"workload": "16,777,216 3-D FP32 points, 64-Gaussian mixture (sigma 0.01); "
+                               "100,000 box queries, selectivity log-uniform [1e-6,1e-2]; "
+                               "MPHR tree, one persistent launch",
+                   "n": n, "queries": q, "fanout": F, "autotune_ms": tune,
+                   "tree_levels": info["level_count"], "build_ms": info["build_ms"],
+                   "total_hits": int(st["total_hits"]), "outputs": "counts+hash"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "kernel": kname, "kernel_ms": kavg, "algorithmic_bytes": abytes},
+        "plain": plain,
+        "speedup_vs_plain": (value / plain["queries_per_s"]) if plain else None,
+        "cpu_baseline": cpu,
+    }
+    b.close()
+    ix.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

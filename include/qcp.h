@@ -1,0 +1,200 @@
+/*
+ * qcp.h -- C ABI of the MI355X-native Hilbert bulk-load + range-search library
+ * (libqcp.so).  Plain C types only: pointers, sizes and status codes.
+ *
+ * Every entry point names the reference interface it replaces (paths are
+ * relative to the reference repository root).  Semantics carried over:
+ *   - objects are rank-sorted by a Hilbert key of their lower corner (or centre)
+ *     and addressed by their 1-based Hilbert rank ("rank ID"), exactly as the
+ *     reference overwrites each branch index with its rank
+ *     (src/sort/thrust_sorter.cpp:39-47);
+ *   - a query box overlaps an entry unless q.lo[d] > e.hi[d] or q.hi[d] < e.lo[d]
+ *     for some d, in FP32 (src/node/node_soa.cpp:94-113);
+ *   - per-query hit counts (the reference's per-block h_hit[],
+ *     src/tree/mphr.cpp:244-252) plus, new here, sorted rank-ID lists and an
+ *     order-independent ID-set hash.
+ * Conventions that differ from the reference on purpose: every error is a
+ * status code (nothing calls exit()), every handle owns its device memory and
+ * HIP stream (no device globals, no singletons), and inputs are caller-owned.
+ * A handle is used by one host thread at a time; distinct handles are
+ * independent.
+ */
+#ifndef QCP_H
+#define QCP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  QCP_OK = 0,
+  QCP_EINVAL = -1,   /* bad argument or configuration                         */
+  QCP_ENOMEM = -2,   /* device or host allocation failed                      */
+  QCP_EHIP = -3,     /* a HIP runtime call failed                             */
+  QCP_EIO = -4,      /* file open/read/write failed                           */
+  QCP_EFORMAT = -5,  /* index file malformed, truncated or checksum mismatch  */
+  QCP_EBUSY = -6,    /* result buffer too small / admission refused           */
+  QCP_EDOMAIN = -7,  /* an object coordinate lies outside the key quantiser's domain */
+  QCP_ENODEV = -8    /* no usable GPU                                         */
+} qcp_status;
+
+enum { QCP_LAYOUT_MPHR = 0, QCP_LAYOUT_RTREE = 1, QCP_LAYOUT_BVH = 2 };
+enum { QCP_KEY_REF = 0, QCP_KEY_B16 = 1 };      /* (ll)(1e6f*p) @ 31|20 bits, or 16 bits */
+enum { QCP_SRC_LO = 0, QCP_SRC_CENTER = 1 };    /* key from lower corner or box centre  */
+enum { QCP_OBJ_POINTS = 0, QCP_OBJ_BOXES = 1 };
+
+enum { QCP_SEARCH_PERSISTENT = 0, QCP_SEARCH_PLAIN = 1 };
+enum { QCP_OUT_COUNTS = 1, QCP_OUT_HASH = 2, QCP_OUT_IDS = 4, QCP_OUT_VISITS = 8 };
+
+typedef struct {
+  uint32_t dims;      /* 2 or 3                                                     */
+  uint32_t fanout;    /* entries per node: 16..256 (MPHR/RTREE), 16..256 (BVH: reference 128) */
+  uint32_t layout;    /* QCP_LAYOUT_*                                               */
+  uint32_t key_mode;  /* QCP_KEY_*                                                  */
+  uint32_t key_src;   /* QCP_SRC_*                                                  */
+  uint32_t obj_kind;  /* QCP_OBJ_*: points are N x D floats, boxes N x 2D [lo, hi] */
+  uint32_t device;    /* HIP device ordinal                                         */
+  uint32_t flags;     /* reserved, 0                                                */
+} qcp_build_cfg;
+
+typedef struct {
+  uint64_t n;                 /* objects                                  */
+  uint64_t n_nodes;           /* tree nodes (root included)               */
+  uint32_t height;            /* node levels (root .. leaves)             */
+  uint32_t dims, fanout, layout, obj_kind, key_mode, key_src, key_bits;
+  uint64_t level_count[32];   /* nodes per level, root first              */
+  uint64_t tree_bytes;        /* device bytes of node boxes + objects     */
+  double build_ms;            /* device time of the last build (HIP events) */
+  double build_phase_ms[6];   /* keys, sort, gather, pack, topology, total */
+} qcp_index_info;
+
+typedef struct {
+  uint32_t mode;           /* QCP_SEARCH_*                                          */
+  uint32_t outputs;        /* QCP_OUT_* mask                                        */
+  uint32_t plain_batch;    /* plain form: queries per launch (reference default 128) */
+  uint32_t block_waves;    /* persistent form: waves per workgroup (0 = tuned)      */
+  uint32_t grid_blocks;    /* persistent form: workgroups (0 = fill the device)     */
+  uint32_t leaf_chunks;    /* persistent form: 64-entry leaf chunks in flight per wave (0 = tuned) */
+  uint32_t donate;         /* persistent form: subtree donation (work stealing) on/off */
+  uint32_t reserved;
+} qcp_search_cfg;
+
+typedef struct {
+  double kernel_ms;          /* device time of the search kernels (HIP events)  */
+  double total_ms;           /* device time of the whole search, setup included */
+  uint64_t total_hits;
+  uint64_t internal_entries; /* entries of visited internal nodes (canonical traversal) */
+  uint64_t leaf_entries;     /* entries of visited leaf nodes (canonical traversal)     */
+  uint64_t donated_items;    /* subtrees handed to other waves                          */
+  uint64_t launches;         /* kernel launches issued by the search                    */
+} qcp_search_stats;
+
+typedef struct qcp_index qcp_index;
+typedef struct qcp_batch qcp_batch;
+
+/* Status text.  Replaces the reference's cudaErrCheck/exit() reporting
+ * (src/common/macro.h:100-108). */
+const char *qcp_strerror(qcp_status s);
+
+/* Number of visible HIP devices (0 when none); never initialises a context. */
+int qcp_device_count(void);
+
+/* Build an index on the GPU: Hilbert keys, stable radix sort by (key, input
+ * index), rank-ordered SoA objects, bottom-up (MPHR/RTREE) or rank-prefix
+ * (BVH) node boxes, all resident in HBM.  Replaces tree::Tree::Build
+ * (src/tree/tree.h:24) as implemented by MPHR::Build (src/tree/mphr.cpp:35-136),
+ * BVH::Build (src/tree/bvh.cpp:25-63) and RTree::Build (src/tree/rtree.cpp:25-63),
+ * together with the upload of manager::ChunkManager::Init/CopyNode
+ * (src/manager/chunk_manager.cpp:16-40).  objs is host memory, row-major. */
+qcp_status qcp_build(const qcp_build_cfg *cfg, const float *objs, uint64_t n,
+                     qcp_index **out);
+
+/* Same, with objs already in device memory of cfg->device (no PCIe copy). */
+qcp_status qcp_build_device(const qcp_build_cfg *cfg, const float *d_objs, uint64_t n,
+                            qcp_index **out);
+
+/* Rebuild an existing index in place from new device-resident objects of the
+ * same count and configuration (timed builds). */
+qcp_status qcp_rebuild_device(qcp_index *ix, const float *d_objs);
+
+void qcp_index_destroy(qcp_index *ix);
+
+/* Shape and timing of a built index.  Replaces the reference's build-time
+ * logging of level counts and phase times (src/tree/tree.cpp:426-428,492). */
+qcp_status qcp_index_info_get(const qcp_index *ix, qcp_index_info *out);
+
+/* Copy the built tree back to the host for inspection and parity checks:
+ *   sorted_keys[n]   Hilbert keys in rank order          (may be NULL)
+ *   perm[n]          input index of each rank            (may be NULL)
+ *   sobjs[n*E]       objects in rank order, row-major     (may be NULL)
+ *   node_boxes[n_nodes*2D] node MBRs, BFS order, [lo(D),hi(D)] (may be NULL)
+ *   node_first/node_count[n_nodes], node_leaf[n_nodes]   (may be NULL)
+ * Replaces the reference's tree dump (MPHR::DumpToFile, src/tree/mphr.cpp:169-194)
+ * as an in-memory export. */
+qcp_status qcp_export(const qcp_index *ix, uint64_t *sorted_keys, uint32_t *perm,
+                      float *sobjs, float *node_boxes, uint32_t *node_first,
+                      uint32_t *node_count, uint8_t *node_leaf);
+
+/* Save / load the device-resident index (versioned header + checksum).
+ * Replaces MPHR::DumpToFile / DumpFromFile (src/tree/mphr.cpp:138-194) and
+ * BVH::DumpToFile / DumpFromFile (src/tree/bvh.cpp:65-163). */
+qcp_status qcp_save(const qcp_index *ix, const char *path);
+qcp_status qcp_load(const char *path, uint32_t device, qcp_index **out);
+
+/* Upload a query set to HBM.  Replaces io::DataSet::GetDeviceQuery
+ * (src/io/dataset.cpp:82-88); queries are Q x 2D floats [lo(D), hi(D)]. */
+qcp_status qcp_batch_create(qcp_index *ix, const float *boxes, uint64_t q,
+                            qcp_batch **out);
+/* Same, from device memory on the index's device. */
+qcp_status qcp_batch_create_device(qcp_index *ix, const float *d_boxes, uint64_t q,
+                                   qcp_batch **out);
+void qcp_batch_destroy(qcp_batch *b);
+
+/* Run the range search of a resident query batch.  Replaces tree::Tree::Search
+ * (src/tree/tree.h:47-48) as implemented by MPHR::Search (src/tree/mphr.cpp:196-272,
+ * cfg->mode = QCP_SEARCH_PLAIN: one workgroup per query, plain_batch queries per
+ * launch, a synchronous count copy after each launch) and, new here, the
+ * persistent work-stealing form (QCP_SEARCH_PERSISTENT: all queries in one
+ * launch).  With QCP_OUT_IDS the sorted rank-ID lists are produced by a count
+ * pass, a device prefix sum and an ordered write pass.  cfg may be NULL
+ * (persistent, counts+hash, tuned). */
+qcp_status qcp_batch_run(qcp_batch *b, const qcp_search_cfg *cfg, qcp_search_stats *stats);
+
+/* Results of the last run (host copies).  counts[q], hashes[q],
+ * offsets[q+1] (exclusive prefix of counts), ids[offsets[Q]], visits[2q]
+ * (root visits, node visits: plain form only).  Each pointer may be NULL. */
+qcp_status qcp_batch_results(const qcp_batch *b, uint32_t *counts, uint64_t *hashes,
+                             uint64_t *offsets, uint32_t *ids, uint64_t ids_cap,
+                             uint32_t *visits);
+uint64_t qcp_batch_total_hits(const qcp_batch *b);
+
+/* One-shot search of host queries (upload, run, copy back).  Same semantics
+ * as qcp_batch_run; counts/hashes/offsets/ids as in qcp_batch_results. */
+qcp_status qcp_search(qcp_index *ix, const float *boxes, uint64_t q,
+                      const qcp_search_cfg *cfg, uint32_t *counts, uint64_t *hashes,
+                      uint64_t *offsets, uint32_t *ids, uint64_t ids_cap,
+                      qcp_search_stats *stats);
+
+/* ---- seeded generator (replaces generator/cuRAND.cpp:17-75, without cuRAND) ----
+ * Philox4x32-10 (Salmon et al., SC'11).  Word i of (seed, stream) is output
+ * word i%4 of philox(counter = {i/4 low, i/4 high, seed high, 0},
+ * key = {seed low, stream}); it is order independent and parallel. */
+void qcp_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void qcp_gen_words(uint32_t *out, uint64_t n, uint64_t seed, uint32_t stream,
+                   uint64_t offset);
+/* u = (w >> 8) * 2^-24 in [0, 1). */
+void qcp_gen_uniform(float *out, uint64_t n, uint64_t seed, uint32_t stream,
+                     uint64_t offset);
+/* Standard normals, Box-Muller in double from word pairs (2i, 2i+1):
+ * u1 = ((w0 >> 8) + 1) * 2^-24, u2 = (w1 >> 8) * 2^-24,
+ * z[2i] = sqrt(-2 ln u1) cos(2 pi u2), z[2i+1] = sqrt(-2 ln u1) sin(2 pi u2). */
+void qcp_gen_normal(double *out, uint64_t n, uint64_t seed, uint32_t stream,
+                    uint64_t offset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QCP_H */

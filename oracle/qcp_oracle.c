@@ -1,0 +1,750 @@
+/*
+ * qcp_oracle.c -- CPU oracle (test infrastructure only; see qcp_oracle.h).
+ *
+ * Plain C11 restatement of the reference's CPU semantics for the Hilbert
+ * bulk-load + range-search path.  Citations are relative to the reference
+ * repository root.  Built with gcc -O2 (no fast-math, SSE float arithmetic),
+ * so every FP32 operation here is one IEEE single-precision operation, as in
+ * the reference's host code.
+ */
+#include "qcp_oracle.h"
+
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef uint64_t u64;
+typedef int64_t i64;
+
+/* ones(T,k) of src/mapper/hilbert_macro.h:14 (valid for 1 <= k <= 64). */
+#define ONES(k) ((((u64)2) << ((k) - 1)) - 1)
+
+/* rotateRight / rotateLeft of src/mapper/hilbert_macro.h:18-22. */
+static u64 rot_right(u64 arg, unsigned nrots, unsigned nd) {
+  return ((arg >> nrots) | (arg << (nd - nrots))) & ONES(nd);
+}
+static u64 rot_left(u64 arg, unsigned nrots, unsigned nd) {
+  return ((arg << nrots) | (arg >> (nd - nrots))) & ONES(nd);
+}
+
+/* adjust_rotation of src/mapper/hilbert_macro.h:4-12:
+ * rotation = (rotation + 1 + ffs(bits & -bits & nd1Ones)) % nDims. */
+static unsigned adjust_rotation(unsigned rotation, unsigned nd, u64 bits, u64 nd1ones) {
+  bits &= (0 - bits) & nd1ones;
+  while (bits) {
+    bits >>= 1;
+    ++rotation;
+  }
+  if (++rotation >= nd) rotation -= nd;
+  return rotation;
+}
+
+/* HilbertMapper::bitTranspose, src/mapper/hilbert_mapper.cpp:135-183.
+ * The reference works on signed long long; every value here stays below
+ * 2^63, so unsigned arithmetic gives identical bits. */
+uint64_t orc_bit_transpose(unsigned nd, unsigned nb, uint64_t in_coords) {
+  unsigned const nd1 = nd - 1;
+  unsigned in_b = nb;
+  unsigned ut_b;
+  u64 in_field_ends = 1;
+  u64 in_mask = ONES(in_b);
+  u64 coords = 0;
+
+  while ((ut_b = in_b / 2)) {
+    unsigned const shift_amt = nd1 * ut_b;
+    u64 const ut_field_ends = in_field_ends | (in_field_ends << (shift_amt + ut_b));
+    u64 const ut_mask = (ut_field_ends << ut_b) - ut_field_ends;
+    u64 ut_coords = 0;
+    unsigned d;
+    if (in_b & 1) {
+      u64 const in_field_starts = in_field_ends << (in_b - 1);
+      unsigned odd_shift = 2 * shift_amt;
+      for (d = 0; d < nd; ++d) {
+        u64 in = in_coords & in_mask;
+        in_coords >>= in_b;
+        coords |= (in & in_field_starts) << odd_shift++;
+        in &= ~in_field_starts;
+        in = (in | (in << shift_amt)) & ut_mask;
+        ut_coords |= in << (d * ut_b);
+      }
+    } else {
+      for (d = 0; d < nd; ++d) {
+        u64 in = in_coords & in_mask;
+        in_coords >>= in_b;
+        in = (in | (in << shift_amt)) & ut_mask;
+        ut_coords |= in << (d * ut_b);
+      }
+    }
+    in_coords = ut_coords;
+    in_b = ut_b;
+    in_field_ends = ut_field_ends;
+    in_mask = ut_mask;
+  }
+  coords |= in_coords;
+  return coords;
+}
+
+/* HilbertMapper::MappingIntoSingle, src/mapper/hilbert_mapper.cpp:15-68,
+ * after its quantisation step (lines 21-23, see orc_quant_ref). */
+uint64_t orc_hilbert_c2i(unsigned nd, unsigned nb, const uint64_t *coord) {
+  if (nd > 1) {
+    unsigned const ndb = nd * nb;
+    u64 index;
+    unsigned d;
+    u64 coords = 0;
+    for (d = nd; d--;) {
+      coords <<= nb;
+      coords |= coord[d];
+    }
+    if (nb > 1) {
+      u64 const nd_ones = ONES(nd);
+      u64 const nd1_ones = nd_ones >> 1;
+      unsigned b = ndb;
+      unsigned rotation = 0;
+      u64 flip_bit = 0;
+      u64 const nthbits = ONES(ndb) / nd_ones;
+      coords = orc_bit_transpose(nd, nb, coords);
+      coords ^= coords >> nd;
+      index = 0;
+      do {
+        u64 bits = (coords >> (b -= nd)) & nd_ones;
+        bits = rot_right(flip_bit ^ bits, rotation, nd);
+        index <<= nd;
+        index |= bits;
+        flip_bit = (u64)1 << rotation;
+        rotation = adjust_rotation(rotation, nd, bits, nd1_ones);
+      } while (b);
+      index ^= nthbits >> 1;
+    } else {
+      index = coords;
+    }
+    for (d = 1; d < ndb; d *= 2) index ^= index >> d;
+    return index;
+  }
+  return coord[0];
+}
+
+/* HilbertMapper::MappingIntoMulti, src/mapper/hilbert_mapper.cpp:79-124,
+ * stopping before the division by 1e6 (line 129): integer coordinates out. */
+void orc_hilbert_i2c(unsigned nd, unsigned nb, uint64_t index, uint64_t *coord) {
+  if (nd > 1) {
+    u64 coords;
+    u64 const nb_ones = ONES(nb);
+    unsigned d;
+    if (nb > 1) {
+      unsigned const ndb = nd * nb;
+      u64 const nd_ones = ONES(nd);
+      u64 const nd1_ones = nd_ones >> 1;
+      unsigned b = ndb;
+      unsigned rotation = 0;
+      u64 flip_bit = 0;
+      u64 const nthbits = ONES(ndb) / nd_ones;
+      index ^= (index ^ nthbits) >> 1;
+      coords = 0;
+      do {
+        u64 bits = (index >> (b -= nd)) & nd_ones;
+        coords <<= nd;
+        coords |= rot_left(bits, rotation, nd) ^ flip_bit;
+        flip_bit = (u64)1 << rotation;
+        rotation = adjust_rotation(rotation, nd, bits, nd1_ones);
+      } while (b);
+      for (b = nd; b < ndb; b *= 2) coords ^= coords >> b;
+      coords = orc_bit_transpose(nb, nd, coords);
+    } else {
+      coords = index ^ (index >> 1);
+    }
+    for (d = 0; d < nd; ++d) {
+      coord[d] = coords & nb_ones;
+      coords >>= nb;
+    }
+  } else {
+    coord[0] = index;
+  }
+}
+
+/* getIntBits, src/mapper/hilbert_mapper.cpp:185-199 (little-endian byte order,
+ * whichByte of src/mapper/hilbert_macro.h:33). */
+static u64 get_int_bits(unsigned nd, unsigned nbytes, const char *c, unsigned y) {
+  unsigned const bit = y % 8;
+  unsigned const offs = y / 8;
+  unsigned d;
+  u64 bits = 0;
+  c += offs;
+  for (d = 0; d < nd; ++d) {
+    bits |= (u64)((((unsigned char)*c) >> bit) & 1) << d;
+    c += nbytes;
+  }
+  return bits;
+}
+
+/* hilbert_cmp / hilbert_cmp_work, src/mapper/hilbert_mapper.cpp:218-262.
+ * Never called by the reference; used here as an independent ordering check. */
+int orc_hilbert_cmp(unsigned nd, unsigned nbytes, unsigned nb,
+                    const void *c1v, const void *c2v) {
+  const char *c1 = (const char *)c1v;
+  const char *c2 = (const char *)c2v;
+  u64 const one = 1;
+  u64 const nd1_ones = ONES(nd) >> 1;
+  u64 bits = one << (nd - 1);
+  u64 index = bits;
+  unsigned rotation = 0;
+  unsigned max = 0, y = nb;
+  while (y-- > max) {
+    u64 reflection = get_int_bits(nd, nbytes, c1, y);
+    u64 diff = reflection ^ get_int_bits(nd, nbytes, c2, y);
+    bits ^= reflection;
+    bits = rot_right(bits, rotation, nd);
+    if (diff) {
+      unsigned d;
+      diff = rot_right(diff, rotation, nd);
+      for (d = 1; d < nd; d *= 2) {
+        index ^= index >> d;
+        bits ^= bits >> d;
+        diff ^= diff >> d;
+      }
+      return (((index ^ y ^ nb) & 1) == (bits < (bits ^ diff))) ? -1 : 1;
+    }
+    index ^= bits;
+    reflection ^= one << rotation;
+    rotation = adjust_rotation(rotation, nd, bits, nd1_ones);
+    bits = reflection;
+  }
+  return 0;
+}
+
+/* Quantiser of src/mapper/hilbert_mapper.cpp:22: (ll)(1000000*p), an FP32
+ * multiply followed by truncation toward zero. */
+int64_t orc_quant_ref(float p) {
+  float const x = 1000000.0f * p;
+  return (int64_t)x;
+}
+
+/* 16-bit quantiser (defined by this build for the b16 key mode):
+ * q = clamp(trunc(p * 65536), 0, 65535); NaN maps to 0. */
+uint32_t orc_quant_b16(float p) {
+  float const x = p * 65536.0f;
+  if (x >= 65535.0f) return 65535u;
+  if (x > 0.0f) return (uint32_t)x;
+  return 0u;
+}
+
+/* number_of_bits of src/tree/tree.cpp:597. */
+unsigned orc_ref_bits(unsigned D) { return D > 2 ? 20u : 31u; }
+
+/* Tree::Thread_Mapping, src/tree/tree.cpp:596-605: the key source is the
+ * first D entries of the branch (the lower corner); the center source
+ * ((lo+hi)*0.5f) is the alternative key source of this build. */
+int orc_keys(const float *objs, uint64_t n, unsigned D, int is_box,
+             int key_mode, int key_src, uint64_t *keys) {
+  unsigned const E = is_box ? 2 * D : D;
+  unsigned const nb = key_mode == ORC_KEY_REF ? orc_ref_bits(D) : 16u;
+  int bad = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const float *o = objs + i * E;
+    uint64_t c[4];
+    for (unsigned d = 0; d < D; ++d) {
+      float p = o[d];
+      if (is_box && key_src == ORC_SRC_CENTER) p = (o[d] + o[D + d]) * 0.5f;
+      if (key_mode == ORC_KEY_REF) {
+        if (!(p >= 0.0f)) { bad = 1; c[d] = 0; continue; }
+        int64_t q = orc_quant_ref(p);
+        if (q >= ((int64_t)1 << nb)) { bad = 1; q = 0; }
+        c[d] = (uint64_t)q;
+      } else {
+        c[d] = orc_quant_b16(p);
+      }
+    }
+    keys[i] = orc_hilbert_c2i(D, nb, c);
+  }
+  return bad;
+}
+
+/* ---- sort ------------------------------------------------------------------ */
+/* The reference sorts Branch structs by key with an unstable sort
+ * (src/sort/thrust_sorter.cpp:20, src/sort/parallel_sorter.cpp:27) and then
+ * overwrites each key with its 1-based rank (src/sort/thrust_sorter.cpp:39-47).
+ * The tie order is unspecified there; this build defines it as input index
+ * ascending (a stable sort).  Bottom-up merge sort on (key, index). */
+void orc_sort_stable(const uint64_t *keys, uint64_t n, uint32_t *perm) {
+  uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  for (uint64_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+  uint32_t *src = perm, *dst = tmp;
+  for (uint64_t w = 1; w < n; w *= 2) {
+    for (uint64_t lo = 0; lo < n; lo += 2 * w) {
+      uint64_t mid = lo + w < n ? lo + w : n;
+      uint64_t hi = lo + 2 * w < n ? lo + 2 * w : n;
+      uint64_t a = lo, b = mid, k = lo;
+      while (a < mid && b < hi) {
+        if (keys[src[b]] < keys[src[a]]) dst[k++] = src[b++];
+        else dst[k++] = src[a++];
+      }
+      while (a < mid) dst[k++] = src[a++];
+      while (b < hi) dst[k++] = src[b++];
+    }
+    uint32_t *t = src; src = dst; dst = t;
+  }
+  if (src != perm) memcpy(perm, src, sizeof(uint32_t) * n);
+  free(tmp);
+}
+
+/* ---- packed levels ----------------------------------------------------------- */
+/* Tree::GetLevelNodeCount, src/tree/tree.cpp:654-667: n <- ceil(n/F) until 1,
+ * root first.  N == 1 (an empty list in the reference, which then asserts at
+ * src/tree/tree.cpp:675) is defined here as a single leaf root; N == 0 gives
+ * zero levels. */
+int orc_level_counts(uint64_t n, unsigned F, uint64_t *counts, int max_levels) {
+  uint64_t tmp[64];
+  int h = 0;
+  if (n == 0) return 0;
+  if (n == 1) { if (max_levels >= 1) counts[0] = 1; return 1; }
+  uint64_t cur = n;
+  while (cur > 1) {
+    cur = cur / F + ((cur % F) ? 1 : 0);
+    tmp[h++] = cur;
+  }
+  if (h > max_levels) return -1;
+  for (int i = 0; i < h; ++i) counts[i] = tmp[h - 1 - i];
+  return h;
+}
+
+/* Reduction with the reference's padding rule: slots beyond the entry count
+ * (thread >= branch_count) are padded with lower 1.0f and upper 0.0f before the
+ * min/max tree (src/tree/tree.cpp:1070-1075 and :893-899, src/node/node.cpp:27-35).
+ * Min/max of non-NaN values is order independent, so a linear scan gives the
+ * tree reduction's value (see the +/-0.0 note in the design notes). */
+static void box_of_entries(const float *ent, unsigned D, unsigned stride,
+                           unsigned lo_off, unsigned hi_off, uint64_t cnt,
+                           unsigned F, float *out) {
+  for (unsigned d = 0; d < D; ++d) {
+    float lo, hi;
+    if (cnt < F) { lo = 1.0f; hi = 0.0f; }
+    else { lo = ent[lo_off + d]; hi = ent[hi_off + d]; }
+    for (uint64_t i = 0; i < cnt; ++i) {
+      float l = ent[i * stride + lo_off + d];
+      float h = ent[i * stride + hi_off + d];
+      if (lo > l) lo = l;
+      if (hi < h) hi = h;
+    }
+    out[d] = lo;
+    out[D + d] = hi;
+  }
+}
+
+/* Tree::Bottom_Up + global_BottomUpBuild_ILP (src/tree/tree.cpp:410-495,
+ * 1025-1133) and CopyBranchToLeafNode (702-745): leaf j holds ranks
+ * jF+1..min(jF+F,N); each parent entry is the padded min/max of its child. */
+uint64_t orc_pack(const float *sobjs, uint64_t n, unsigned D, int is_box,
+                  unsigned F, float *node_boxes) {
+  uint64_t counts[64];
+  int h = orc_level_counts(n, F, counts, 64);
+  if (h <= 0) return 0;
+  uint64_t off[65];
+  off[0] = 0;
+  for (int l = 0; l < h; ++l) off[l + 1] = off[l] + counts[l];
+  unsigned const B = 2 * D;
+  unsigned const E = is_box ? 2 * D : D;
+  /* leaves */
+  for (uint64_t j = 0; j < counts[h - 1]; ++j) {
+    uint64_t first = j * F;
+    uint64_t cnt = n - first < F ? n - first : F;
+    box_of_entries(sobjs + first * E, D, E, 0, is_box ? D : 0, cnt, F,
+                   node_boxes + (off[h - 1] + j) * B);
+  }
+  for (int l = h - 2; l >= 0; --l) {
+    for (uint64_t j = 0; j < counts[l]; ++j) {
+      uint64_t first = j * F;
+      uint64_t cnt = counts[l + 1] - first < F ? counts[l + 1] - first : F;
+      box_of_entries(node_boxes + (off[l + 1] + first) * B, D, B, 0, D, cnt, F,
+                     node_boxes + (off[l] + j) * B);
+    }
+  }
+  return off[h];
+}
+
+int64_t orc_packed_topology(uint64_t n, unsigned F, int64_t cap,
+                            uint32_t *node_first, uint32_t *node_count,
+                            uint8_t *node_leaf, uint8_t *node_level) {
+  uint64_t counts[64];
+  int h = orc_level_counts(n, F, counts, 64);
+  if (h <= 0) return 0;
+  uint64_t off[65];
+  off[0] = 0;
+  for (int l = 0; l < h; ++l) off[l + 1] = off[l] + counts[l];
+  if ((int64_t)off[h] > cap) return -1;
+  for (int l = 0; l < h; ++l) {
+    uint64_t below = (l == h - 1) ? n : counts[l + 1];
+    for (uint64_t j = 0; j < counts[l]; ++j) {
+      uint64_t id = off[l] + j;
+      uint64_t first = j * F;
+      uint64_t cnt = below - first < F ? below - first : F;
+      node_first[id] = (uint32_t)((l == h - 1) ? first : off[l + 1] + first);
+      node_count[id] = (uint32_t)cnt;
+      node_leaf[id] = (uint8_t)(l == h - 1);
+      node_level[id] = (uint8_t)l;
+    }
+  }
+  return (int64_t)off[h];
+}
+
+/* ---- BVH ----------------------------------------------------------------------- */
+/* Tree::GetSplitOffset, src/tree/tree.cpp:267-302.  After the sort the branch
+ * index is its 1-based rank (src/sort/thrust_sorter.cpp:39-47), so the "codes"
+ * are offset+1. */
+static uint32_t bvh_split_offset(uint32_t start, uint32_t end) {
+  u64 const first_code = (u64)start + 1;
+  u64 const last_code = (u64)end + 1;
+  if (first_code >= last_code) return 0;
+  unsigned const common_prefix = (unsigned)__builtin_clzll(first_code ^ last_code);
+  uint32_t split = start;
+  uint32_t step = end - start;
+  do {
+    step = (step + 1) >> 1;
+    uint32_t ns = split + step;
+    if (ns < end) {
+      u64 const split_code = (u64)ns + 1;
+      unsigned const split_prefix = (unsigned)__builtin_clzll(first_code ^ split_code);
+      if (split_prefix > common_prefix) split = ns;
+    }
+  } while (step > 1);
+  return split;
+}
+
+static int cmp_u32(const void *a, const void *b) {
+  uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* Tree::GetSplitPosition, src/tree/tree.cpp:304-350: BFS of binary splits
+ * until F split points; returns the sorted positions (count in *np). */
+static void bvh_split_positions(uint32_t start, uint32_t end, unsigned F,
+                                uint32_t *pos, unsigned *np) {
+  uint32_t qs[512], qe[512];
+  unsigned qh = 0, qt = 0, n = 0;
+  qs[qt] = start; qe[qt] = end; qt++;
+  pos[n++] = start;
+  pos[n++] = end;
+  while (qh != qt && n < F) {
+    uint32_t s = qs[qh], e = qe[qh];
+    qh++;
+    uint32_t so = bvh_split_offset(s, e);
+    if (!so) continue;
+    pos[n++] = so;
+    if (so - s >= F) { qs[qt] = s; qe[qt] = so; qt++; }
+    if ((e - so + 1) >= F) { qs[qt] = so + 1; qe[qt] = e; qt++; }
+  }
+  qsort(pos, n, sizeof(uint32_t), cmp_u32);
+  *np = n;
+}
+
+/* Tree::CreateNode, src/tree/tree.cpp:352-408, emitted in BFS order (the order
+ * BVH::DumpToFile writes, src/tree/bvh.cpp:94-163).  A node holding <= F entries
+ * is a leaf; otherwise children are the ranges between consecutive split
+ * positions, each later range starting one past the previous end (line 381). */
+int64_t orc_bvh_topology(uint64_t n, unsigned F, int64_t cap,
+                         uint32_t *node_first, uint32_t *node_count,
+                         uint8_t *node_leaf, uint8_t *node_level) {
+  if (n == 0) return 0;
+  /* BFS queue of (start, end, level); node ids are assigned in dequeue order,
+   * and children of a node are enqueued consecutively. */
+  uint32_t *qs = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+  uint32_t *qe = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+  uint8_t *ql = (uint8_t *)malloc(cap);
+  int64_t head = 0, tail = 0;
+  uint32_t *pos = (uint32_t *)malloc(sizeof(uint32_t) * (F + 2));
+  qs[tail] = 0; qe[tail] = (uint32_t)(n - 1); ql[tail] = 0; tail++;
+  while (head < tail) {
+    int64_t id = head++;
+    uint32_t s = qs[id], e = qe[id];
+    uint64_t nd = (uint64_t)(e - s) + 1;
+    node_level[id] = ql[id];
+    if (nd <= F) {
+      node_first[id] = s;
+      node_count[id] = (uint32_t)nd;
+      node_leaf[id] = 1;
+      continue;
+    }
+    unsigned np;
+    bvh_split_positions(s, e, F, pos, &np);
+    if (tail + (int64_t)(np - 1) > cap) { tail = -1; break; }
+    node_first[id] = (uint32_t)tail;
+    node_count[id] = np - 1;
+    node_leaf[id] = 0;
+    for (unsigned c = 0; c + 1 < np; ++c) {
+      qs[tail] = pos[c];
+      qe[tail] = pos[c + 1];
+      ql[tail] = (uint8_t)(ql[id] + 1);
+      tail++;
+      pos[c + 1] += 1;
+    }
+  }
+  free(qs); free(qe); free(ql); free(pos);
+  return tail;
+}
+
+/* Node::GetMBB (src/node/node.cpp:19-70) applied bottom-up over an explicit
+ * tree in BFS order (children always have larger ids than their parent). */
+void orc_tree_boxes(const float *sobjs, unsigned D, int is_box, unsigned F,
+                    int64_t n_nodes, const uint32_t *node_first,
+                    const uint32_t *node_count, const uint8_t *node_leaf,
+                    float *node_boxes) {
+  unsigned const B = 2 * D;
+  unsigned const E = is_box ? 2 * D : D;
+  for (int64_t i = n_nodes - 1; i >= 0; --i) {
+    if (node_leaf[i])
+      box_of_entries(sobjs + (uint64_t)node_first[i] * E, D, E, 0, is_box ? D : 0,
+                     node_count[i], F, node_boxes + i * B);
+    else
+      box_of_entries(node_boxes + (uint64_t)node_first[i] * B, D, B, 0, D,
+                     node_count[i], F, node_boxes + i * B);
+  }
+}
+
+/* ---- search ------------------------------------------------------------------------- */
+/* Order-independent set hash of rank IDs: the sum over IDs of the splitmix64
+ * finalizer (defined by this build for count+hash result mode). */
+uint64_t orc_id_hash(uint32_t id) {
+  u64 z = (u64)id + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+/* Node_SOA::IsOverlap / Node::IsOverlap (src/node/node_soa.cpp:94-113,
+ * src/node/node.cpp:156-168): rejected iff q.lo[d] > e.hi[d] or q.hi[d] < e.lo[d]. */
+static int overlap(const float *q, const float *lo, const float *hi, unsigned D) {
+  for (unsigned d = 0; d < D; ++d)
+    if (q[d] > hi[d] || q[D + d] < lo[d]) return 0;
+  return 1;
+}
+
+typedef struct {
+  uint32_t cnt;
+  uint64_t hash;
+  uint32_t *ids;
+  uint64_t ie, le;
+} dfs_acc;
+
+/* BVH::TraverseInternalNodes, src/tree/bvh.cpp:256-278 (iterative form with an
+ * explicit stack; children visited left to right, so IDs come out ascending). */
+static void dfs_one(const float *sobjs, unsigned D, int is_box,
+                    const uint32_t *node_first, const uint32_t *node_count,
+                    const uint8_t *node_leaf, const float *node_boxes,
+                    const float *q, dfs_acc *acc) {
+  unsigned const B = 2 * D;
+  unsigned const E = is_box ? 2 * D : D;
+  unsigned const hi_off = is_box ? D : 0;
+  /* stack of (node, next child) */
+  int64_t st_node[128];
+  uint32_t st_next[128];
+  int sp = 0;
+  st_node[0] = 0; st_next[0] = 0;
+  if (node_leaf[0]) acc->le += node_count[0]; else acc->ie += node_count[0];
+  while (sp >= 0) {
+    int64_t nd = st_node[sp];
+    if (node_leaf[nd]) {
+      uint32_t f = node_first[nd];
+      for (uint32_t i = 0; i < node_count[nd]; ++i) {
+        const float *o = sobjs + (uint64_t)(f + i) * E;
+        if (overlap(q, o, o + hi_off, D)) {
+          uint32_t id = f + i + 1;
+          if (acc->ids) acc->ids[acc->cnt] = id;
+          acc->cnt++;
+          acc->hash += orc_id_hash(id);
+        }
+      }
+      sp--;
+      continue;
+    }
+    if (st_next[sp] >= node_count[nd]) { sp--; continue; }
+    uint32_t c = node_first[nd] + st_next[sp]++;
+    const float *b = node_boxes + (uint64_t)c * B;
+    if (overlap(q, b, b + D, D)) {
+      if (node_leaf[c]) acc->le += node_count[c]; else acc->ie += node_count[c];
+      sp++;
+      st_node[sp] = c;
+      st_next[sp] = 0;
+    }
+  }
+}
+
+void orc_search_dfs(const float *sobjs, unsigned D, int is_box,
+                    int64_t n_nodes, const uint32_t *node_first,
+                    const uint32_t *node_count, const uint8_t *node_leaf,
+                    const float *node_boxes,
+                    const float *queries, uint64_t q0, uint64_t q1,
+                    uint32_t *counts, uint64_t *hashes,
+                    const uint64_t *offsets, uint32_t *ids,
+                    uint64_t *int_entries, uint64_t *leaf_entries) {
+  for (uint64_t q = q0; q < q1; ++q) {
+    dfs_acc acc = {0, 0, ids ? ids + offsets[q] : NULL, 0, 0};
+    if (n_nodes > 0)
+      dfs_one(sobjs, D, is_box, node_first, node_count, node_leaf, node_boxes,
+              queries + q * 2 * D, &acc);
+    if (counts) counts[q] = acc.cnt;
+    if (hashes) hashes[q] = acc.hash;
+    if (int_entries) int_entries[q] = acc.ie;
+    if (leaf_entries) leaf_entries[q] = acc.le;
+  }
+}
+
+typedef struct {
+  const float *sobjs; unsigned D; int is_box; int64_t n_nodes;
+  const uint32_t *nf, *nc; const uint8_t *nl; const float *nb;
+  const float *queries; uint64_t q0, q1; uint32_t *counts; uint64_t *hashes;
+} mt_arg;
+
+static void *mt_worker(void *p) {
+  mt_arg *a = (mt_arg *)p;
+  orc_search_dfs(a->sobjs, a->D, a->is_box, a->n_nodes, a->nf, a->nc, a->nl, a->nb,
+                 a->queries, a->q0, a->q1, a->counts, a->hashes, NULL, NULL, NULL, NULL);
+  return NULL;
+}
+
+/* BVH::Search thread split (src/tree/bvh.cpp:195-220): chunk = Q/T, the first
+ * thread also takes the remainder. */
+void orc_search_dfs_mt(const float *sobjs, unsigned D, int is_box,
+                       int64_t n_nodes, const uint32_t *node_first,
+                       const uint32_t *node_count, const uint8_t *node_leaf,
+                       const float *node_boxes,
+                       const float *queries, uint64_t nq, unsigned nthreads,
+                       uint32_t *counts, uint64_t *hashes) {
+  if (nthreads < 1) nthreads = 1;
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+  mt_arg *args = (mt_arg *)malloc(sizeof(mt_arg) * nthreads);
+  uint64_t chunk = nq / nthreads;
+  uint64_t s = 0, e = chunk + nq % nthreads;
+  for (unsigned t = 0; t < nthreads; ++t) {
+    mt_arg a = {sobjs, D, is_box, n_nodes, node_first, node_count, node_leaf,
+                node_boxes, queries, s, e, counts, hashes};
+    args[t] = a;
+    pthread_create(&th[t], NULL, mt_worker, &args[t]);
+    s = e;
+    e += chunk;
+  }
+  for (unsigned t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+  free(th);
+  free(args);
+}
+
+/* global_RestartScanning_and_ParentCheck, src/tree/mphr.cpp:303-438, serial,
+ * over the packed levels (root first; leaves are the last level; node (l,j)
+ * covers ranks j*F^(h-l)+1 .. min(N,(j+1)*F^(h-l)), so its "last index" is that
+ * upper bound; a leaf's parent is (l-1, j/F); the next leaf is j+1).
+ * The reference's partially zeroed t_hit[] (lines 327-330) is not reproduced:
+ * the count is the number of overlapping entries. */
+void orc_search_mprs(const float *sobjs, uint64_t n, unsigned D, int is_box,
+                     unsigned F, const float *node_boxes,
+                     const float *queries, uint64_t nq,
+                     uint32_t *counts, uint64_t *hashes,
+                     const uint64_t *offsets, uint32_t *ids,
+                     uint32_t *visits) {
+  uint64_t lc[64];
+  int h = orc_level_counts(n, F, lc, 64);
+  uint64_t off[65], span[65];
+  unsigned const B = 2 * D;
+  unsigned const E = is_box ? 2 * D : D;
+  unsigned const hi_off = is_box ? D : 0;
+  off[0] = 0;
+  for (int l = 0; l < h; ++l) off[l + 1] = off[l] + lc[l];
+  /* span[l] = ranks covered by one full node at level l */
+  if (h > 0) {
+    span[h - 1] = F;
+    for (int l = h - 2; l >= 0; --l) span[l] = span[l + 1] * F;
+  }
+  for (uint64_t qi = 0; qi < nq; ++qi) {
+    const float *q = queries + qi * 2 * D;
+    uint32_t cnt = 0, root_visit = 0, node_visit = 0;
+    uint64_t hash = 0;
+    uint32_t *out = ids ? ids + offsets[qi] : NULL;
+    if (h == 0) {
+      if (counts) counts[qi] = 0;
+      if (hashes) hashes[qi] = 0;
+      if (visits) { visits[2 * qi] = 0; visits[2 * qi + 1] = 0; }
+      continue;
+    }
+#define LAST_INDEX(l, j) (((j) + 1) * span[l] < n ? ((j) + 1) * span[l] : n)
+    int lvl = 0;
+    uint64_t j = 0;
+    uint64_t visited = 0;
+    uint64_t const last = n;
+    root_visit++;
+    while (visited < last) {
+      while (lvl < h - 1) {
+        uint64_t below = lc[lvl + 1];
+        uint64_t first = j * F;
+        uint64_t c = below - first < F ? below - first : F;
+        uint64_t pick = UINT64_MAX;
+        for (uint64_t k = 0; k < c; ++k) {
+          uint64_t ch = first + k;
+          const float *b = node_boxes + (off[lvl + 1] + ch) * B;
+          if (LAST_INDEX(lvl + 1, ch) > visited && overlap(q, b, b + D, D)) {
+            pick = ch;
+            break;
+          }
+        }
+        if (pick == UINT64_MAX) {
+          visited = LAST_INDEX(lvl, j);
+          lvl = 0; j = 0;
+          root_visit++;
+          break;
+        }
+        lvl++;
+        j = pick;
+        node_visit++;
+      }
+      while (lvl == h - 1) {
+        uint64_t first = j * F;
+        uint64_t c = n - first < F ? n - first : F;
+        int is_hit = 0;
+        for (uint64_t k = 0; k < c; ++k) {
+          const float *o = sobjs + (first + k) * E;
+          if (overlap(q, o, o + hi_off, D)) {
+            uint32_t id = (uint32_t)(first + k + 1);
+            if (out) out[cnt] = id;
+            cnt++;
+            hash += orc_id_hash(id);
+            is_hit = 1;
+          }
+        }
+        visited = LAST_INDEX(lvl, j);
+        if (visited == last) break;
+        if (is_hit) {
+          j++;
+          node_visit++;
+        } else {
+          lvl--;
+          j /= F;
+          if (lvl == 0) root_visit++; else node_visit++;
+        }
+      }
+    }
+#undef LAST_INDEX
+    if (counts) counts[qi] = cnt;
+    if (hashes) hashes[qi] = hash;
+    if (visits) { visits[2 * qi] = root_visit; visits[2 * qi + 1] = node_visit; }
+  }
+}
+
+void orc_search_brute(const float *sobjs, uint64_t n, unsigned D, int is_box,
+                      const float *queries, uint64_t nq,
+                      uint32_t *counts, uint64_t *hashes,
+                      const uint64_t *offsets, uint32_t *ids) {
+  unsigned const E = is_box ? 2 * D : D;
+  unsigned const hi_off = is_box ? D : 0;
+  for (uint64_t qi = 0; qi < nq; ++qi) {
+    const float *q = queries + qi * 2 * D;
+    uint32_t cnt = 0;
+    uint64_t hash = 0;
+    uint32_t *out = ids ? ids + offsets[qi] : NULL;
+    for (uint64_t i = 0; i < n; ++i) {
+      const float *o = sobjs + i * E;
+      if (overlap(q, o, o + hi_off, D)) {
+        if (out) out[cnt] = (uint32_t)(i + 1);
+        cnt++;
+        hash += orc_id_hash((uint32_t)(i + 1));
+      }
+    }
+    if (counts) counts[qi] = cnt;
+    if (hashes) hashes[qi] = hash;
+  }
+}

@@ -1,0 +1,521 @@
+// build_kernels.hip -- GPU bulk-load pipeline for gfx950:
+//   Hilbert keys -> stable LSD radix sort (8-bit digits, LDS histograms and
+//   wave-level multisplit ranking) -> gather into rank-ordered SoA objects ->
+//   node MBRs (packed levels bottom-up, or explicit BVH levels deepest first).
+//
+// Reference steps replaced: Tree::AssignHilbertIndexToBranches
+// (src/tree/tree.cpp:596-639), Sorter::Sort + ReassignHilbertIndexes
+// (src/sort/sorter.cpp:11-29, src/sort/thrust_sorter.cpp:12-47),
+// CopyBranchToLeafNode + Transformer::Transform (src/tree/tree.cpp:683-745,
+// src/transformer/transformer.cpp:13-88), global_BottomUpBuild_ILP
+// (src/tree/tree.cpp:1025-1133) and Node::GetMBB (src/node/node.cpp:19-70).
+#include <hip/hip_runtime.h>
+
+#include "hilbert_key.h"
+#include "qcp_device.h"
+#include "qcp_kernels.h"
+
+namespace qcp {
+
+// ---------------------------------------------------------------------------
+// Hilbert keys: one thread per object.  Key source is the lower corner (the
+// reference reads only the first D entries of the branch, src/tree/tree.cpp:600)
+// or the box centre (lo+hi)*0.5f.
+template <int D, bool BOX, bool REF, bool CENTER>
+__global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, uint64_t n,
+                                              unsigned nb, uint64_t* __restrict__ keys,
+                                              uint32_t* __restrict__ idx,
+                                              unsigned long long* key_or, uint32_t* bad) {
+  constexpr int E = BOX ? 2 * D : D;
+  uint64_t acc = 0;
+  bool any_bad = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const float* o = objs + i * E;
+    float v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = o[e];
+    uint32_t c[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      float p = v[d];
+      if (CENTER && BOX) p = (v[d] + v[D + d]) * 0.5f;
+      if (REF) {
+        any_bad |= !quant_ref(p, nb, &c[d]);
+      } else {
+        c[d] = quant_b16(p);
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) any_bad |= (v[e] != v[e]);  // NaN objects are rejected
+    uint64_t k = hilbert_key<D>(c, nb);
+    keys[i] = k;
+    idx[i] = (uint32_t)i;
+    acc |= k;
+  }
+  // wave OR, one atomic per wave
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc |= __shfl_xor(acc, off, kWave);
+  if (lane_id() == 0 && acc) atomicOr(key_or, (unsigned long long)acc);
+  if (any_bad) atomicOr(bad, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Radix sort.  Tile = 256 threads x 16 keys; key i of a tile sits at
+// tile_start + it*256 + tid, so each 256-key sub-tile is in index order and
+// LSD passes stay stable.
+constexpr int kSortThreads = 256;
+constexpr int kSortKPT = 16;
+constexpr int kSortTile = kSortThreads * kSortKPT;
+
+__global__ __launch_bounds__(kSortThreads) void k_hist(const uint64_t* __restrict__ keys,
+                                                       uint64_t n, unsigned shift,
+                                                       uint32_t* __restrict__ hist,
+                                                       uint32_t ntiles) {
+  __shared__ uint32_t cnt[256];
+  const int tid = threadIdx.x;
+  cnt[tid] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+#pragma unroll
+  for (int it = 0; it < kSortKPT; ++it) {
+    uint64_t i = base + (uint64_t)it * kSortThreads + tid;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 0xFF], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)tid * ntiles + blockIdx.x] = cnt[tid];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_scatter(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned shift,
+    const uint32_t* __restrict__ offs, uint32_t ntiles) {
+  constexpr int W = kSortThreads / kWave;
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[W][256];
+  const int tid = threadIdx.x;
+  const int w = tid / kWave;
+  const uint32_t lane = lane_id();
+  run[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
+#pragma unroll
+  for (int ww = 0; ww < W; ++ww) wcnt[ww][tid] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+  for (int it = 0; it < kSortKPT; ++it) {
+    uint64_t i = base + (uint64_t)it * kSortThreads + tid;
+    bool valid = i < n;
+    uint64_t k = valid ? kin[i] : 0;
+    uint32_t v = valid ? vin[i] : 0;
+    uint32_t dig = (uint32_t)(k >> shift) & 0xFF;
+    // wave-level multisplit: lanes holding the same digit
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bb = ballot((dig >> b) & 1u);
+      peers &= ((dig >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t r = rank_below(peers);
+    if (valid && r == 0) wcnt[w][dig] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[dig] + r;
+      for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][dig];
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+#pragma unroll
+    for (int ww = 0; ww < W; ++ww) {
+      tot += wcnt[ww][tid];
+      wcnt[ww][tid] = 0;
+    }
+    run[tid] += tot;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan (three phases).  Each block owns `per_block`
+// consecutive elements, processed 2048 at a time.
+constexpr int kScanThreads = 256;
+constexpr int kScanIPT = 8;
+constexpr int kScanChunk = kScanThreads * kScanIPT;
+
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* sh, T* total) {
+  // sh has kScanThreads entries
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    T t = tid >= off ? sh[tid - off] : (T)0;
+    __syncthreads();
+    sh[tid] += t;
+    __syncthreads();
+  }
+  T incl = sh[tid];
+  *total = sh[kScanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const TI* __restrict__ in,
+                                                              uint64_t n, uint64_t per_block,
+                                                              TO* __restrict__ sums) {
+  __shared__ TO sh[kScanThreads];
+  const uint64_t b0 = (uint64_t)blockIdx.x * per_block;
+  const uint64_t b1 = b0 + per_block < n ? b0 + per_block : n;
+  TO acc = 0;
+  for (uint64_t i = b0 + threadIdx.x; i < b1; i += kScanThreads) acc += (TO)in[i];
+  TO tot;
+  block_excl_scan<TO>(acc, sh, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of nb <= kScanThreads*64 block sums, in place;
+// total written to sums[nb].
+template <typename TO>
+__global__ __launch_bounds__(kScanThreads) void k_scan_top(TO* sums, uint32_t nb) {
+  __shared__ TO sh[kScanThreads];
+  const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
+  const uint32_t s0 = threadIdx.x * per;
+  TO acc = 0;
+  for (uint32_t i = 0; i < per; ++i)
+    if (s0 + i < nb) acc += sums[s0 + i];
+  TO tot;
+  TO ex = block_excl_scan<TO>(acc, sh, &tot);
+  for (uint32_t i = 0; i < per; ++i) {
+    if (s0 + i < nb) {
+      TO t = sums[s0 + i];
+      sums[s0 + i] = ex;
+      ex += t;
+    }
+  }
+  if (threadIdx.x == 0) sums[nb] = tot;
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const TI* __restrict__ in,
+                                                             uint64_t n, uint64_t per_block,
+                                                             const TO* __restrict__ sums,
+                                                             TO* __restrict__ out) {
+  __shared__ TO sh[kScanThreads];
+  const uint64_t b0 = (uint64_t)blockIdx.x * per_block;
+  const uint64_t b1 = b0 + per_block < n ? b0 + per_block : n;
+  TO carry = sums[blockIdx.x];
+  for (uint64_t c0 = b0; c0 < b1; c0 += kScanChunk) {
+    // each thread owns kScanIPT consecutive elements of this chunk
+    const uint64_t t0 = c0 + (uint64_t)threadIdx.x * kScanIPT;
+    TO v[kScanIPT];
+    TO loc = 0;
+#pragma unroll
+    for (int j = 0; j < kScanIPT; ++j) {
+      v[j] = (t0 + j < b1) ? (TO)in[t0 + j] : (TO)0;
+      loc += v[j];
+    }
+    TO tot;
+    TO ex = block_excl_scan<TO>(loc, sh, &tot) + carry;
+#pragma unroll
+    for (int j = 0; j < kScanIPT; ++j) {
+      if (t0 + j < b1) out[t0 + j] = ex;
+      ex += v[j];
+    }
+    carry += tot;
+  }
+  // the last block also writes the grand total after the end (out[n]) when
+  // the caller sized out for n+1 elements
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0 && sums != nullptr) {
+    // sums[gridDim.x] holds the grand total (k_scan_top)
+    out[n] = sums[gridDim.x];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gather the rank-ordered objects into SoA: obj[e][r] = objs[perm[r]][e].
+template <int E>
+__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ objs,
+                                                const uint32_t* __restrict__ perm, uint64_t n,
+                                                float* __restrict__ obj, uint64_t stride) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    const float* o = objs + (uint64_t)perm[r] * E;
+    float v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = o[e];
+#pragma unroll
+    for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = v[e];
+  }
+}
+
+// Box of `cnt` entries with the reference's padding rule: when cnt < F the
+// reduction also sees lower 1.0f / upper 0.0f (src/tree/tree.cpp:1070-1075).
+// One wave per node; lane handles entries lane + 64*w.
+template <int D, int MW>
+__device__ __forceinline__ void wave_box(const float* __restrict__ src, uint64_t stride,
+                                         int hi_row, uint64_t first, uint32_t cnt, uint32_t F,
+                                         float* __restrict__ dst, uint64_t dstride,
+                                         uint64_t dst_idx) {
+  const uint32_t lane = lane_id();
+  float lo[D], hi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    lo[d] = __builtin_huge_valf();
+    hi[d] = -__builtin_huge_valf();
+  }
+#pragma unroll
+  for (int w = 0; w < MW; ++w) {
+    uint32_t k = w * kWave + lane;
+    if (k < cnt) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        float l = src[(uint64_t)d * stride + first + k];
+        float h = src[(uint64_t)(hi_row + d) * stride + first + k];
+        lo[d] = (lo[d] > l) ? l : lo[d];
+        hi[d] = (hi[d] < h) ? h : hi[d];
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    float l = wave_min(lo[d]);
+    float h = wave_max(hi[d]);
+    if (cnt < F) {
+      l = (l > 1.0f) ? 1.0f : l;
+      h = (h < 0.0f) ? 0.0f : h;
+    }
+    lo[d] = l;
+    hi[d] = h;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      dst[(uint64_t)d * dstride + dst_idx] = lo[d];
+      dst[(uint64_t)(D + d) * dstride + dst_idx] = hi[d];
+    }
+  }
+}
+
+// Packed leaves: leaf j holds ranks [jF, min(n, jF+F)).
+template <int D, bool BOX, int MW>
+__global__ __launch_bounds__(256) void k_leaf_boxes(const float* __restrict__ obj,
+                                                    uint64_t ostride, uint64_t n, uint32_t F,
+                                                    uint64_t nleaves, float* __restrict__ nbox,
+                                                    uint64_t nstride, uint64_t leaf_off) {
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (wave >= nleaves) return;
+  const uint64_t first = wave * F;
+  const uint32_t cnt = (uint32_t)((n - first) < F ? (n - first) : F);
+  wave_box<D, MW>(obj, ostride, BOX ? D : 0, first, cnt, F, nbox, nstride, leaf_off + wave);
+}
+
+// Packed internal level: parent j boxes children [jF, min(child_cnt, jF+F)).
+template <int D, int MW>
+__global__ __launch_bounds__(256) void k_level_boxes(float* __restrict__ nbox,
+                                                     uint64_t nstride, uint64_t child_off,
+                                                     uint64_t child_cnt, uint32_t F,
+                                                     uint64_t parent_off,
+                                                     uint64_t parent_cnt) {
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (wave >= parent_cnt) return;
+  const uint64_t first = wave * F;
+  const uint32_t cnt = (uint32_t)((child_cnt - first) < F ? (child_cnt - first) : F);
+  wave_box<D, MW>(nbox, nstride, D, child_off + first, cnt, F, nbox, nstride,
+                  parent_off + wave);
+}
+
+// Explicit tree (BVH): nodes [lo, hi) of one depth; leaves box their objects,
+// internal nodes their children (which live at the next depth, already done).
+template <int D, bool BOX, int MW>
+__global__ __launch_bounds__(256) void k_tree_boxes(const float* __restrict__ obj,
+                                                    uint64_t ostride,
+                                                    const uint32_t* __restrict__ nfirst,
+                                                    const uint32_t* __restrict__ ncount,
+                                                    uint32_t F, uint64_t lo, uint64_t hi,
+                                                    float* __restrict__ nbox,
+                                                    uint64_t nstride) {
+  const uint64_t node = lo + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (node >= hi) return;
+  const uint32_t first = nfirst[node];
+  const uint32_t cw = ncount[node];
+  const uint32_t cnt = cw & ~kLeafBit;
+  if (cw & kLeafBit)
+    wave_box<D, MW>(obj, ostride, BOX ? D : 0, first, cnt, F, nbox, nstride, node);
+  else
+    wave_box<D, MW>(nbox, nstride, D, first, cnt, F, nbox, nstride, node);
+}
+
+// Fill helpers (kernels, so that build and search stay graph-capturable).
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_f32(float* p, uint64_t n, float v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+static inline unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 65535u * 4) {
+  uint64_t g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+hipError_t launch_keys(const float* objs, uint64_t n, unsigned D, bool box, bool ref,
+                       bool center, unsigned nb, uint64_t* keys, uint32_t* idx,
+                       unsigned long long* key_or, uint32_t* bad, hipStream_t s) {
+  unsigned g = grid_for(n, 256, 8192);
+#define QCP_KEYS(DD, BB, RR, CC) \
+  k_keys<DD, BB, RR, CC><<<g, 256, 0, s>>>(objs, n, nb, keys, idx, key_or, bad)
+  if (D == 2) {
+    if (box) {
+      if (ref) { if (center) QCP_KEYS(2, true, true, true); else QCP_KEYS(2, true, true, false); }
+      else { if (center) QCP_KEYS(2, true, false, true); else QCP_KEYS(2, true, false, false); }
+    } else {
+      if (ref) QCP_KEYS(2, false, true, false); else QCP_KEYS(2, false, false, false);
+    }
+  } else if (D == 3) {
+    if (box) {
+      if (ref) { if (center) QCP_KEYS(3, true, true, true); else QCP_KEYS(3, true, true, false); }
+      else { if (center) QCP_KEYS(3, true, false, true); else QCP_KEYS(3, true, false, false); }
+    } else {
+      if (ref) QCP_KEYS(3, false, true, false); else QCP_KEYS(3, false, false, false);
+    }
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef QCP_KEYS
+  return hipGetLastError();
+}
+
+uint32_t sort_tiles(uint64_t n) { return (uint32_t)((n + kSortTile - 1) / kSortTile); }
+
+template <typename TI, typename TO>
+static hipError_t scan_impl(const TI* in, uint64_t n, TO* out, TO* block_sums,
+                            uint32_t max_blocks, hipStream_t s) {
+  // out must hold n+1 elements; block_sums max_blocks+1
+  if (n == 0) {
+    k_fill_u32<<<1, 64, 0, s>>>((uint32_t*)out, sizeof(TO) / 4, 0u);
+    return hipGetLastError();
+  }
+  uint64_t per_block = kScanChunk;
+  while ((n + per_block - 1) / per_block > max_blocks) per_block += kScanChunk;
+  uint32_t nb = (uint32_t)((n + per_block - 1) / per_block);
+  k_scan_reduce<TI, TO><<<nb, kScanThreads, 0, s>>>(in, n, per_block, block_sums);
+  k_scan_top<TO><<<1, kScanThreads, 0, s>>>(block_sums, nb);
+  k_scan_apply<TI, TO><<<nb, kScanThreads, 0, s>>>(in, n, per_block, block_sums, out);
+  return hipGetLastError();
+}
+
+hipError_t scan_u32_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* block_sums,
+                        uint32_t max_blocks, hipStream_t s) {
+  return scan_impl<uint32_t, uint32_t>(in, n, out, block_sums, max_blocks, s);
+}
+hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
+                        unsigned long long* block_sums, uint32_t max_blocks, hipStream_t s) {
+  return scan_impl<uint32_t, unsigned long long>(in, n, out, block_sums, max_blocks, s);
+}
+
+hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
+                            uint32_t* vout, uint64_t n, unsigned shift, uint32_t* hist,
+                            uint32_t* hist_scan, uint32_t* block_sums, uint32_t max_blocks,
+                            hipStream_t s) {
+  uint32_t nt = sort_tiles(n);
+  k_hist<<<nt, kSortThreads, 0, s>>>(kin, n, shift, hist, nt);
+  hipError_t e = scan_u32_u32(hist, (uint64_t)nt * 256, hist_scan, block_sums, max_blocks, s);
+  if (e != hipSuccess) return e;
+  k_scatter<<<nt, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, hist_scan, nt);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, unsigned E,
+                         float* obj, uint64_t stride, hipStream_t s) {
+  unsigned g = grid_for(n, 256, 16384);
+  switch (E) {
+    case 2: k_gather<2><<<g, 256, 0, s>>>(objs, perm, n, obj, stride); break;
+    case 3: k_gather<3><<<g, 256, 0, s>>>(objs, perm, n, obj, stride); break;
+    case 4: k_gather<4><<<g, 256, 0, s>>>(objs, perm, n, obj, stride); break;
+    case 6: k_gather<6><<<g, 256, 0, s>>>(objs, perm, n, obj, stride); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+#define QCP_MW_SWITCH(MWV, ...)                        \
+  switch (MWV) {                                       \
+    case 1: { constexpr int MW = 1; __VA_ARGS__; } break; \
+    case 2: { constexpr int MW = 2; __VA_ARGS__; } break; \
+    case 3: { constexpr int MW = 3; __VA_ARGS__; } break; \
+    case 4: { constexpr int MW = 4; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue;              \
+  }
+
+hipError_t launch_leaf_boxes(const float* obj, uint64_t ostride, uint64_t n, unsigned D,
+                             bool box, uint32_t F, uint64_t nleaves, float* nbox,
+                             uint64_t nstride, uint64_t leaf_off, hipStream_t s) {
+  unsigned g = (unsigned)((nleaves * kWave + 255) / 256);
+  int mwv = (int)((F + 63) / 64);
+  if (D == 2) {
+    if (box) { QCP_MW_SWITCH(mwv, k_leaf_boxes<2, true, MW><<<g, 256, 0, s>>>(obj, ostride, n, F, nleaves, nbox, nstride, leaf_off)) }
+    else { QCP_MW_SWITCH(mwv, k_leaf_boxes<2, false, MW><<<g, 256, 0, s>>>(obj, ostride, n, F, nleaves, nbox, nstride, leaf_off)) }
+  } else if (D == 3) {
+    if (box) { QCP_MW_SWITCH(mwv, k_leaf_boxes<3, true, MW><<<g, 256, 0, s>>>(obj, ostride, n, F, nleaves, nbox, nstride, leaf_off)) }
+    else { QCP_MW_SWITCH(mwv, k_leaf_boxes<3, false, MW><<<g, 256, 0, s>>>(obj, ostride, n, F, nleaves, nbox, nstride, leaf_off)) }
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_level_boxes(float* nbox, uint64_t nstride, unsigned D, uint64_t child_off,
+                              uint64_t child_cnt, uint32_t F, uint64_t parent_off,
+                              uint64_t parent_cnt, hipStream_t s) {
+  unsigned g = (unsigned)((parent_cnt * kWave + 255) / 256);
+  int mwv = (int)((F + 63) / 64);
+  if (D == 2) {
+    QCP_MW_SWITCH(mwv, k_level_boxes<2, MW><<<g, 256, 0, s>>>(nbox, nstride, child_off, child_cnt, F, parent_off, parent_cnt))
+  } else if (D == 3) {
+    QCP_MW_SWITCH(mwv, k_level_boxes<3, MW><<<g, 256, 0, s>>>(nbox, nstride, child_off, child_cnt, F, parent_off, parent_cnt))
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_tree_boxes(const float* obj, uint64_t ostride, const uint32_t* nfirst,
+                             const uint32_t* ncount, unsigned D, bool box, uint32_t F,
+                             uint64_t lo, uint64_t hi, float* nbox, uint64_t nstride,
+                             hipStream_t s) {
+  if (hi <= lo) return hipSuccess;
+  unsigned g = (unsigned)(((hi - lo) * kWave + 255) / 256);
+  int mwv = (int)((F + 63) / 64);
+  if (D == 2) {
+    if (box) { QCP_MW_SWITCH(mwv, k_tree_boxes<2, true, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, F, lo, hi, nbox, nstride)) }
+    else { QCP_MW_SWITCH(mwv, k_tree_boxes<2, false, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, F, lo, hi, nbox, nstride)) }
+  } else if (D == 3) {
+    if (box) { QCP_MW_SWITCH(mwv, k_tree_boxes<3, true, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, F, lo, hi, nbox, nstride)) }
+    else { QCP_MW_SWITCH(mwv, k_tree_boxes<3, false, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, F, lo, hi, nbox, nstride)) }
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_fill_u32<<<grid_for(n, 256, 4096), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_fill_f32<<<grid_for(n, 256, 4096), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace qcp

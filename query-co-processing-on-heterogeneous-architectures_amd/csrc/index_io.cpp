@@ -1,0 +1,152 @@
+// index_io.cpp -- save / load of a device-resident index.
+//
+// Replaces MPHR::DumpToFile / DumpFromFile (src/tree/mphr.cpp:138-194) and the
+// BVH dump (src/tree/bvh.cpp:65-163), whose files carry no header, version or
+// checksum.  Format (little-endian):
+//   magic "QCPIDX01" | u32 version | u32 header bytes | qcp_build_cfg |
+//   u64 n | u64 n_nodes | i32 h | u32 pad | u64 level_off[h+1] |
+//   u64 payload bytes | u64 FNV-1a 64 of the payload | payload
+// payload = sorted keys (8n) | perm (4n) | objects SoA (4En) | node boxes SoA
+//           (8D n_nodes) | [explicit layouts] node first (4 n_nodes) |
+//           node count (4 n_nodes)
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/qcp.h"
+
+namespace {
+
+constexpr char kMagic[8] = {'Q', 'C', 'P', 'I', 'D', 'X', '0', '1'};
+constexpr uint32_t kVersion = 1;
+
+uint64_t fnv1a(const uint8_t* p, size_t n, uint64_t h) {
+  for (size_t i = 0; i < n; ++i) {
+    h ^= p[i];
+    h *= 0x100000001B3ull;
+  }
+  return h;
+}
+
+struct FileCloser {
+  FILE* f;
+  ~FileCloser() {
+    if (f) fclose(f);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+qcp_status qcp_save(const qcp_index* ix, const char* path) {
+  if (!ix || !path) return QCP_EINVAL;
+  qcp_index_info in;
+  qcp_status s = qcp_index_info_get(ix, &in);
+  if (s != QCP_OK) return s;
+  const uint64_t n = in.n, m = in.n_nodes;
+  const uint32_t D = in.dims;
+  const uint32_t E = in.obj_kind == QCP_OBJ_BOXES ? 2 * D : D;
+  const bool expl = in.layout == QCP_LAYOUT_BVH;
+  std::vector<uint64_t> keys(n);
+  std::vector<uint32_t> perm(n);
+  std::vector<float> objs((size_t)E * n), boxes((size_t)2 * D * m);
+  std::vector<uint32_t> first(m), count(m);
+  std::vector<uint8_t> leaf(m);
+  s = qcp_export(ix, keys.data(), perm.data(), objs.data(), boxes.data(), first.data(),
+                 count.data(), leaf.data());
+  if (s != QCP_OK) return s;
+  std::vector<uint8_t> payload;
+  auto put = [&](const void* p, size_t bytes) {
+    const uint8_t* b = (const uint8_t*)p;
+    payload.insert(payload.end(), b, b + bytes);
+  };
+  put(keys.data(), 8 * n);
+  put(perm.data(), 4 * n);
+  put(objs.data(), 4 * objs.size());
+  put(boxes.data(), 4 * boxes.size());
+  if (expl) {
+    for (uint64_t i = 0; i < m; ++i) count[i] |= leaf[i] ? 0x80000000u : 0u;
+    put(first.data(), 4 * m);
+    put(count.data(), 4 * m);
+  }
+  qcp_build_cfg cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.dims = D;
+  cfg.fanout = in.fanout;
+  cfg.layout = in.layout;
+  cfg.key_mode = in.key_mode;
+  cfg.key_src = in.key_src;
+  cfg.obj_kind = in.obj_kind;
+  int32_t h = (int32_t)in.height;
+  std::vector<uint64_t> level_off(h + 1, 0);
+  for (int l = 0; l < h; ++l) level_off[l + 1] = level_off[l] + in.level_count[l];
+  FileCloser fc{fopen(path, "wb")};
+  if (!fc.f) return QCP_EIO;
+  const uint32_t hdr = (uint32_t)(8 + 4 + 4 + sizeof(cfg) + 8 + 8 + 4 + 4 + 8 * (h + 1) + 8 + 8);
+  const uint64_t pbytes = payload.size();
+  const uint64_t sum = fnv1a(payload.data(), payload.size(), 0xCBF29CE484222325ull);
+  const uint32_t pad = 0;
+  bool ok = fwrite(kMagic, 8, 1, fc.f) == 1 && fwrite(&kVersion, 4, 1, fc.f) == 1 &&
+            fwrite(&hdr, 4, 1, fc.f) == 1 && fwrite(&cfg, sizeof(cfg), 1, fc.f) == 1 &&
+            fwrite(&n, 8, 1, fc.f) == 1 && fwrite(&m, 8, 1, fc.f) == 1 &&
+            fwrite(&h, 4, 1, fc.f) == 1 && fwrite(&pad, 4, 1, fc.f) == 1 &&
+            fwrite(level_off.data(), 8, h + 1, fc.f) == (size_t)(h + 1) &&
+            fwrite(&pbytes, 8, 1, fc.f) == 1 && fwrite(&sum, 8, 1, fc.f) == 1 &&
+            (pbytes == 0 || fwrite(payload.data(), 1, pbytes, fc.f) == pbytes);
+  return ok ? QCP_OK : QCP_EIO;
+}
+
+// Implemented in qcp_api.cpp: rebuild an index's device state from exported arrays.
+__attribute__((visibility("hidden"))) qcp_status qcp_internal_restore(const qcp_build_cfg* cfg, uint64_t n, const uint64_t* keys,
+                                const uint32_t* perm, const float* objs, const float* boxes,
+                                const uint32_t* first, const uint32_t* count, uint64_t n_nodes,
+                                qcp_index** out);
+
+qcp_status qcp_load(const char* path, uint32_t device, qcp_index** out) {
+  if (!path || !out) return QCP_EINVAL;
+  *out = nullptr;
+  FileCloser fc{fopen(path, "rb")};
+  if (!fc.f) return QCP_EIO;
+  char magic[8];
+  uint32_t ver = 0, hdr = 0, pad = 0;
+  qcp_build_cfg cfg;
+  uint64_t n = 0, m = 0, pbytes = 0, sum = 0;
+  int32_t h = 0;
+  if (fread(magic, 8, 1, fc.f) != 1 || memcmp(magic, kMagic, 8) != 0) return QCP_EFORMAT;
+  if (fread(&ver, 4, 1, fc.f) != 1 || ver != kVersion) return QCP_EFORMAT;
+  if (fread(&hdr, 4, 1, fc.f) != 1) return QCP_EFORMAT;
+  if (fread(&cfg, sizeof(cfg), 1, fc.f) != 1) return QCP_EFORMAT;
+  if (fread(&n, 8, 1, fc.f) != 1 || fread(&m, 8, 1, fc.f) != 1) return QCP_EFORMAT;
+  if (fread(&h, 4, 1, fc.f) != 1 || fread(&pad, 4, 1, fc.f) != 1) return QCP_EFORMAT;
+  if (h < 0 || h > 64 || cfg.dims < 2 || cfg.dims > 3 || n >= 0xFFFFFFF0ull || m > 2 * n + 2)
+    return QCP_EFORMAT;
+  std::vector<uint64_t> level_off(h + 1);
+  if (fread(level_off.data(), 8, h + 1, fc.f) != (size_t)(h + 1)) return QCP_EFORMAT;
+  if (fread(&pbytes, 8, 1, fc.f) != 1 || fread(&sum, 8, 1, fc.f) != 1) return QCP_EFORMAT;
+  const uint32_t D = cfg.dims;
+  const uint32_t E = cfg.obj_kind == QCP_OBJ_BOXES ? 2 * D : D;
+  const bool expl = cfg.layout == QCP_LAYOUT_BVH;
+  const uint64_t expect = 8 * n + 4 * n + 4ull * E * n + 8ull * D * m + (expl ? 8 * m : 0);
+  if (pbytes != expect) return QCP_EFORMAT;
+  std::vector<uint8_t> payload(pbytes);
+  if (pbytes && fread(payload.data(), 1, pbytes, fc.f) != pbytes) return QCP_EFORMAT;
+  if (fnv1a(payload.data(), payload.size(), 0xCBF29CE484222325ull) != sum) return QCP_EFORMAT;
+  const uint8_t* p = payload.data();
+  const uint64_t* keys = (const uint64_t*)p;
+  p += 8 * n;
+  const uint32_t* perm = (const uint32_t*)p;
+  p += 4 * n;
+  const float* objs = (const float*)p;
+  p += 4ull * E * n;
+  const float* boxes = (const float*)p;
+  p += 8ull * D * m;
+  const uint32_t* first = expl ? (const uint32_t*)p : nullptr;
+  const uint32_t* count = expl ? (const uint32_t*)(p + 4 * m) : nullptr;
+  cfg.device = device;
+  return qcp_internal_restore(&cfg, n, keys, perm, objs, boxes, first, count, m, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,828 @@
+// qcp_api.cpp -- host runtime and C ABI of libqcp.so.
+//
+// An index handle owns its device (HBM) tree and a HIP stream; there are no
+// device globals and no singletons (the reference keeps the one tree of the
+// process in a ChunkManager singleton plus a __device__ pointer,
+// src/manager/chunk_manager.cpp:11-50).  Every launch goes to the handle's
+// stream; host synchronisation happens only where a size must be known on the
+// host (sort pass count, ID buffer size) and at the end of a call.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "../../include/qcp.h"
+#include "bvh_topology.h"
+#include "qcp_kernels.h"
+
+using namespace qcp;
+
+#define HIP_TRY(expr)                       \
+  do {                                      \
+    hipError_t _e = (expr);                 \
+    if (_e != hipSuccess) {                 \
+      last_hip_error(_e, #expr, __LINE__);  \
+      return QCP_EHIP;                      \
+    }                                       \
+  } while (0)
+
+#define QCP_TRY(expr)                 \
+  do {                                \
+    qcp_status _s = (expr);           \
+    if (_s != QCP_OK) return _s;      \
+  } while (0)
+
+static void last_hip_error(hipError_t e, const char* what, int line) {
+  if (getenv("QCP_DEBUG"))
+    fprintf(stderr, "qcp: HIP error %s (%d) at line %d: %s\n", hipGetErrorString(e), (int)e,
+            line, what);
+}
+
+namespace {
+
+constexpr uint64_t kObjSlack = 64 * 16;   // entries read past the end by masked chunks
+constexpr uint64_t kNodeSlack = 256 + 64;
+constexpr uint32_t kMaxScanBlocks = 4096;
+
+template <typename T>
+qcp_status dalloc(T** p, uint64_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  if (hipMalloc((void**)p, sizeof(T) * count) != hipSuccess) {
+    *p = nullptr;
+    return QCP_ENOMEM;
+  }
+  return QCP_OK;
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) hipFree((void*)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct qcp_index {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  qcp_build_cfg cfg{};
+  uint32_t D = 0, F = 0, E = 0, nb = 0;
+  bool box = false, expl = false;
+  uint64_t n = 0;
+  int h = 0;
+  std::vector<uint64_t> level_count;  // root first
+  std::vector<uint64_t> level_off;    // h+1 entries
+  uint64_t n_nodes = 0;
+  uint64_t obj_stride = 0, node_stride = 0;
+  // device
+  float* d_obj = nullptr;        // [E][obj_stride]
+  float* d_nbox = nullptr;       // [2D][node_stride]
+  uint32_t* d_nfirst = nullptr;  // explicit layouts
+  uint32_t* d_ncount = nullptr;
+  uint64_t* d_keys = nullptr;    // sorted keys [n]
+  uint32_t* d_perm = nullptr;    // [n]
+  // build scratch
+  uint64_t* d_keys2 = nullptr;
+  uint32_t* d_perm2 = nullptr;
+  uint32_t* d_hist = nullptr;
+  uint32_t* d_hist_scan = nullptr;
+  uint32_t* d_block_sums = nullptr;
+  unsigned long long* d_key_or = nullptr;
+  uint32_t* d_bad = nullptr;
+  float* d_in = nullptr;         // device copy of host input
+  hipEvent_t ev[8] = {};
+  int cus = 0;
+  qcp_index_info info{};
+};
+
+struct qcp_batch {
+  qcp_index* ix = nullptr;
+  uint64_t q = 0;
+  float* d_queries = nullptr;
+  uint32_t* d_counts = nullptr;
+  unsigned long long* d_hashes = nullptr;
+  unsigned long long* d_offsets = nullptr;
+  unsigned long long* d_scan_sums = nullptr;
+  uint32_t* d_ids = nullptr;
+  uint64_t ids_cap = 0;
+  uint32_t* d_visits = nullptr;
+  QueueCtl* d_ctl = nullptr;
+  unsigned long long* d_slots = nullptr;
+  uint32_t slot_cap = 0;
+  uint32_t* h_counts = nullptr;   // pinned, plain-form per-batch copies
+  uint32_t* h_visits = nullptr;
+  hipEvent_t ev[4] = {};
+  uint32_t last_outputs = 0;
+  uint32_t last_mode = 0;
+  uint64_t total_hits = 0;
+  bool has_offsets = false;
+};
+
+static qcp_status set_device(int dev) {
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) return QCP_ENODEV;
+  if (dev < 0 || dev >= cnt) return QCP_EINVAL;
+  HIP_TRY(hipSetDevice(dev));
+  return QCP_OK;
+}
+
+static void index_free_scratch(qcp_index* ix) {
+  dfree(ix->d_keys2);
+  dfree(ix->d_perm2);
+  dfree(ix->d_hist);
+  dfree(ix->d_hist_scan);
+  dfree(ix->d_block_sums);
+}
+
+static void index_free(qcp_index* ix) {
+  if (!ix) return;
+  hipSetDevice(ix->device);
+  if (ix->stream) hipStreamSynchronize(ix->stream);
+  index_free_scratch(ix);
+  dfree(ix->d_obj);
+  dfree(ix->d_nbox);
+  dfree(ix->d_nfirst);
+  dfree(ix->d_ncount);
+  dfree(ix->d_keys);
+  dfree(ix->d_perm);
+  dfree(ix->d_key_or);
+  dfree(ix->d_bad);
+  dfree(ix->d_in);
+  for (auto& e : ix->ev)
+    if (e) hipEventDestroy(e);
+  if (ix->stream) hipStreamDestroy(ix->stream);
+  delete ix;
+}
+
+static qcp_status validate_cfg(const qcp_build_cfg* cfg) {
+  if (!cfg) return QCP_EINVAL;
+  if (cfg->dims != 2 && cfg->dims != 3) return QCP_EINVAL;
+  if (cfg->fanout < 2 || cfg->fanout > 256) return QCP_EINVAL;
+  if (cfg->layout > QCP_LAYOUT_BVH) return QCP_EINVAL;
+  if (cfg->key_mode > QCP_KEY_B16 || cfg->key_src > QCP_SRC_CENTER) return QCP_EINVAL;
+  if (cfg->obj_kind > QCP_OBJ_BOXES) return QCP_EINVAL;
+  if (cfg->key_src == QCP_SRC_CENTER && cfg->obj_kind != QCP_OBJ_BOXES) return QCP_EINVAL;
+  return QCP_OK;
+}
+
+// Level sizes of a packed tree, root first (Tree::GetLevelNodeCount,
+// src/tree/tree.cpp:654-667); N == 1 is one leaf root, N == 0 no levels.
+static std::vector<uint64_t> packed_levels(uint64_t n, uint32_t F) {
+  std::vector<uint64_t> lv;
+  if (n == 0) return lv;
+  if (n == 1) return {1};
+  uint64_t cur = n;
+  while (cur > 1) {
+    cur = cur / F + ((cur % F) ? 1 : 0);
+    lv.insert(lv.begin(), cur);
+  }
+  return lv;
+}
+
+// Allocate the index and its topology for n objects.
+static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** out) {
+  QCP_TRY(validate_cfg(cfg));
+  if (n >= 0xFFFFFFF0ull) return QCP_EINVAL;  // ranks and node ids are 32-bit
+  QCP_TRY(set_device((int)cfg->device));
+  qcp_index* ix = new (std::nothrow) qcp_index();
+  if (!ix) return QCP_ENOMEM;
+  ix->device = (int)cfg->device;
+  ix->cfg = *cfg;
+  ix->D = cfg->dims;
+  ix->F = cfg->fanout;
+  ix->box = cfg->obj_kind == QCP_OBJ_BOXES;
+  ix->E = ix->box ? 2 * ix->D : ix->D;
+  ix->expl = cfg->layout == QCP_LAYOUT_BVH;
+  ix->nb = cfg->key_mode == QCP_KEY_REF ? (ix->D > 2 ? 20u : 31u) : 16u;
+  ix->n = n;
+  auto fail = [&](qcp_status s) {
+    index_free(ix);
+    return s;
+  };
+  if (hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(QCP_EHIP);
+  for (auto& e : ix->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(QCP_EHIP);
+  hipDeviceGetAttribute(&ix->cus, hipDeviceAttributeMultiprocessorCount, ix->device);
+  if (ix->cus <= 0) ix->cus = 256;
+
+  std::vector<uint32_t> h_first, h_count;
+  if (!ix->expl) {
+    ix->level_count = packed_levels(n, ix->F);
+    ix->h = (int)ix->level_count.size();
+    if (ix->h > kMaxLevels) return fail(QCP_EINVAL);
+    ix->level_off.assign(ix->h + 1, 0);
+    for (int l = 0; l < ix->h; ++l) ix->level_off[l + 1] = ix->level_off[l] + ix->level_count[l];
+    ix->n_nodes = ix->level_off[ix->h];
+  } else {
+    BvhTopology t;
+    if (!bvh_topology(n, ix->F, kMaxDepth, &t)) return fail(QCP_EINVAL);
+    ix->n_nodes = t.first.size();
+    ix->h = (int)t.level_off.size() - 1;
+    if (ix->h < 0) ix->h = 0;
+    ix->level_off.assign(t.level_off.begin(), t.level_off.end());
+    ix->level_count.clear();
+    for (int l = 0; l < ix->h; ++l) ix->level_count.push_back(ix->level_off[l + 1] - ix->level_off[l]);
+    h_first.swap(t.first);
+    h_count.swap(t.count);
+  }
+  ix->obj_stride = ((n + kObjSlack + 63) / 64) * 64;
+  ix->node_stride = ((ix->n_nodes + kNodeSlack + 63) / 64) * 64;
+  qcp_status s;
+  if ((s = dalloc(&ix->d_obj, (uint64_t)ix->E * ix->obj_stride)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_nbox, (uint64_t)2 * ix->D * ix->node_stride)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_keys, n)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_perm, n)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_key_or, 1)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_bad, 1)) != QCP_OK) return fail(s);
+  if (launch_fill_f32(ix->d_obj, (uint64_t)ix->E * ix->obj_stride, 0.0f, ix->stream) != hipSuccess ||
+      launch_fill_f32(ix->d_nbox, (uint64_t)2 * ix->D * ix->node_stride, 0.0f, ix->stream) != hipSuccess)
+    return fail(QCP_EHIP);
+  if (ix->expl) {
+    if ((s = dalloc(&ix->d_nfirst, ix->node_stride)) != QCP_OK) return fail(s);
+    if ((s = dalloc(&ix->d_ncount, ix->node_stride)) != QCP_OK) return fail(s);
+    if (launch_fill_u32(ix->d_nfirst, ix->node_stride, 0u, ix->stream) != hipSuccess ||
+        launch_fill_u32(ix->d_ncount, ix->node_stride, 0u, ix->stream) != hipSuccess)
+      return fail(QCP_EHIP);
+    if (ix->n_nodes) {
+      if (hipMemcpyAsync(ix->d_nfirst, h_first.data(), 4 * ix->n_nodes, hipMemcpyHostToDevice,
+                         ix->stream) != hipSuccess ||
+          hipMemcpyAsync(ix->d_ncount, h_count.data(), 4 * ix->n_nodes, hipMemcpyHostToDevice,
+                         ix->stream) != hipSuccess ||
+          hipStreamSynchronize(ix->stream) != hipSuccess)
+        return fail(QCP_EHIP);
+    }
+  }
+  // sort scratch
+  const uint32_t nt = sort_tiles(n > 0 ? n : 1);
+  if ((s = dalloc(&ix->d_keys2, n)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_perm2, n)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_hist, (uint64_t)nt * 256)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_hist_scan, (uint64_t)nt * 256 + 1)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_block_sums, kMaxScanBlocks + 1)) != QCP_OK) return fail(s);
+  *out = ix;
+  return QCP_OK;
+}
+
+// The build pipeline on the index's stream.
+static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
+  hipStream_t s = ix->stream;
+  const uint64_t n = ix->n;
+  HIP_TRY(hipEventRecord(ix->ev[0], s));
+  uint64_t key_or = 0;
+  uint32_t bad = 0;
+  if (n > 0) {
+    HIP_TRY(launch_fill_u32((uint32_t*)ix->d_key_or, 2, 0u, s));
+    HIP_TRY(launch_fill_u32(ix->d_bad, 1, 0u, s));
+    HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
+                        ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, ix->d_perm,
+                        ix->d_key_or, ix->d_bad, s));
+    HIP_TRY(hipMemcpyAsync(&key_or, ix->d_key_or, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&bad, ix->d_bad, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipEventRecord(ix->ev[1], s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (bad) return QCP_EDOMAIN;
+    // LSD radix sort over the significant bytes of the keys
+    const unsigned bits = key_or ? 64 - __builtin_clzll(key_or) : 0;
+    const unsigned passes = (bits + 7) / 8;
+    uint64_t* kin = ix->d_keys;
+    uint32_t* vin = ix->d_perm;
+    uint64_t* kout = ix->d_keys2;
+    uint32_t* vout = ix->d_perm2;
+    for (unsigned p = 0; p < passes; ++p) {
+      HIP_TRY(launch_sort_pass(kin, vin, kout, vout, n, 8 * p, ix->d_hist, ix->d_hist_scan,
+                               ix->d_block_sums, kMaxScanBlocks, s));
+      std::swap(kin, kout);
+      std::swap(vin, vout);
+    }
+    if (kin != ix->d_keys) {
+      std::swap(ix->d_keys, ix->d_keys2);
+      std::swap(ix->d_perm, ix->d_perm2);
+    }
+    HIP_TRY(hipEventRecord(ix->ev[2], s));
+    HIP_TRY(launch_gather(d_objs, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s));
+    HIP_TRY(hipEventRecord(ix->ev[3], s));
+    if (!ix->expl) {
+      const int h = ix->h;
+      HIP_TRY(launch_leaf_boxes(ix->d_obj, ix->obj_stride, n, ix->D, ix->box, ix->F,
+                                ix->level_count[h - 1], ix->d_nbox, ix->node_stride,
+                                ix->level_off[h - 1], s));
+      for (int l = h - 2; l >= 0; --l)
+        HIP_TRY(launch_level_boxes(ix->d_nbox, ix->node_stride, ix->D, ix->level_off[l + 1],
+                                   ix->level_count[l + 1], ix->F, ix->level_off[l],
+                                   ix->level_count[l], s));
+    } else {
+      for (int l = ix->h - 1; l >= 0; --l)
+        HIP_TRY(launch_tree_boxes(ix->d_obj, ix->obj_stride, ix->d_nfirst, ix->d_ncount, ix->D,
+                                  ix->box, ix->F, ix->level_off[l], ix->level_off[l + 1],
+                                  ix->d_nbox, ix->node_stride, s));
+    }
+  } else {
+    HIP_TRY(hipEventRecord(ix->ev[1], s));
+    HIP_TRY(hipEventRecord(ix->ev[2], s));
+    HIP_TRY(hipEventRecord(ix->ev[3], s));
+  }
+  HIP_TRY(hipEventRecord(ix->ev[4], s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms[4] = {0, 0, 0, 0}, tot = 0;
+  hipEventElapsedTime(&ms[0], ix->ev[0], ix->ev[1]);
+  hipEventElapsedTime(&ms[1], ix->ev[1], ix->ev[2]);
+  hipEventElapsedTime(&ms[2], ix->ev[2], ix->ev[3]);
+  hipEventElapsedTime(&ms[3], ix->ev[3], ix->ev[4]);
+  hipEventElapsedTime(&tot, ix->ev[0], ix->ev[4]);
+  qcp_index_info& in = ix->info;
+  memset(&in, 0, sizeof(in));
+  in.n = n;
+  in.n_nodes = ix->n_nodes;
+  in.height = (uint32_t)ix->h;
+  in.dims = ix->D;
+  in.fanout = ix->F;
+  in.layout = ix->cfg.layout;
+  in.obj_kind = ix->cfg.obj_kind;
+  in.key_mode = ix->cfg.key_mode;
+  in.key_src = ix->cfg.key_src;
+  in.key_bits = ix->nb * ix->D;
+  for (int l = 0; l < ix->h && l < 32; ++l) in.level_count[l] = ix->level_count[l];
+  in.tree_bytes = (uint64_t)ix->E * 4 * n + (uint64_t)2 * ix->D * 4 * ix->n_nodes;
+  in.build_ms = tot;
+  for (int i = 0; i < 4; ++i) in.build_phase_ms[i] = ms[i];
+  in.build_phase_ms[5] = tot;
+  return QCP_OK;
+}
+
+extern "C" {
+
+const char* qcp_strerror(qcp_status s) {
+  switch (s) {
+    case QCP_OK: return "ok";
+    case QCP_EINVAL: return "invalid argument";
+    case QCP_ENOMEM: return "out of memory";
+    case QCP_EHIP: return "HIP runtime error";
+    case QCP_EIO: return "I/O error";
+    case QCP_EFORMAT: return "malformed index file";
+    case QCP_EBUSY: return "buffer too small or admission refused";
+    case QCP_EDOMAIN: return "object coordinate outside the key quantiser's domain";
+    case QCP_ENODEV: return "no GPU";
+  }
+  return "unknown status";
+}
+
+int qcp_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+qcp_status qcp_build_device(const qcp_build_cfg* cfg, const float* d_objs, uint64_t n,
+                            qcp_index** out) {
+  if (!out || (n && !d_objs)) return QCP_EINVAL;
+  *out = nullptr;
+  qcp_index* ix = nullptr;
+  QCP_TRY(index_alloc(cfg, n, &ix));
+  qcp_status s = index_run_build(ix, d_objs);
+  if (s != QCP_OK) {
+    index_free(ix);
+    return s;
+  }
+  *out = ix;
+  return QCP_OK;
+}
+
+qcp_status qcp_build(const qcp_build_cfg* cfg, const float* objs, uint64_t n, qcp_index** out) {
+  if (!out || (n && !objs)) return QCP_EINVAL;
+  *out = nullptr;
+  qcp_index* ix = nullptr;
+  QCP_TRY(index_alloc(cfg, n, &ix));
+  const uint64_t bytes = (uint64_t)ix->E * 4 * n;
+  qcp_status s = dalloc(&ix->d_in, n ? (uint64_t)ix->E * n : 1);
+  if (s == QCP_OK && n &&
+      (hipMemcpyAsync(ix->d_in, objs, bytes, hipMemcpyHostToDevice, ix->stream) != hipSuccess))
+    s = QCP_EHIP;
+  if (s == QCP_OK) s = index_run_build(ix, ix->d_in);
+  dfree(ix->d_in);
+  if (s != QCP_OK) {
+    index_free(ix);
+    return s;
+  }
+  *out = ix;
+  return QCP_OK;
+}
+
+qcp_status qcp_rebuild_device(qcp_index* ix, const float* d_objs) {
+  if (!ix || (ix->n && !d_objs)) return QCP_EINVAL;
+  QCP_TRY(set_device(ix->device));
+  if (!ix->d_keys2) return QCP_EINVAL;
+  return index_run_build(ix, d_objs);
+}
+
+void qcp_index_destroy(qcp_index* ix) { index_free(ix); }
+
+qcp_status qcp_index_info_get(const qcp_index* ix, qcp_index_info* out) {
+  if (!ix || !out) return QCP_EINVAL;
+  *out = ix->info;
+  return QCP_OK;
+}
+
+qcp_status qcp_export(const qcp_index* ix, uint64_t* sorted_keys, uint32_t* perm, float* sobjs,
+                      float* node_boxes, uint32_t* node_first, uint32_t* node_count,
+                      uint8_t* node_leaf) {
+  if (!ix) return QCP_EINVAL;
+  QCP_TRY(set_device(ix->device));
+  const uint64_t n = ix->n;
+  hipStream_t s = ix->stream;
+  if (sorted_keys && n) HIP_TRY(hipMemcpyAsync(sorted_keys, ix->d_keys, 8 * n, hipMemcpyDeviceToHost, s));
+  if (perm && n) HIP_TRY(hipMemcpyAsync(perm, ix->d_perm, 4 * n, hipMemcpyDeviceToHost, s));
+  std::vector<float> soa;
+  if (sobjs && n) {
+    soa.resize((size_t)ix->E * n);
+    for (uint32_t e = 0; e < ix->E; ++e)
+      HIP_TRY(hipMemcpyAsync(soa.data() + (size_t)e * n, ix->d_obj + (uint64_t)e * ix->obj_stride,
+                             4 * n, hipMemcpyDeviceToHost, s));
+  }
+  std::vector<float> nb;
+  const uint64_t m = ix->n_nodes;
+  if (node_boxes && m) {
+    nb.resize((size_t)2 * ix->D * m);
+    for (uint32_t e = 0; e < 2 * ix->D; ++e)
+      HIP_TRY(hipMemcpyAsync(nb.data() + (size_t)e * m, ix->d_nbox + (uint64_t)e * ix->node_stride,
+                             4 * m, hipMemcpyDeviceToHost, s));
+  }
+  std::vector<uint32_t> cnt;
+  if (ix->expl && m) {
+    if (node_first) HIP_TRY(hipMemcpyAsync(node_first, ix->d_nfirst, 4 * m, hipMemcpyDeviceToHost, s));
+    cnt.resize(m);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), ix->d_ncount, 4 * m, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  if (sobjs)
+    for (uint64_t r = 0; r < n; ++r)
+      for (uint32_t e = 0; e < ix->E; ++e) sobjs[r * ix->E + e] = soa[(size_t)e * n + r];
+  if (node_boxes)
+    for (uint64_t i = 0; i < m; ++i)
+      for (uint32_t e = 0; e < 2 * ix->D; ++e) node_boxes[i * 2 * ix->D + e] = nb[(size_t)e * m + i];
+  if (ix->expl) {
+    for (uint64_t i = 0; i < m; ++i) {
+      if (node_count) node_count[i] = cnt[i] & 0x7FFFFFFFu;
+      if (node_leaf) node_leaf[i] = (cnt[i] & 0x80000000u) ? 1 : 0;
+    }
+  } else {
+    for (int l = 0; l < ix->h; ++l) {
+      const bool leaf = l == ix->h - 1;
+      const uint64_t below = leaf ? n : ix->level_count[l + 1];
+      for (uint64_t j = 0; j < ix->level_count[l]; ++j) {
+        const uint64_t id = ix->level_off[l] + j;
+        const uint64_t f = j * ix->F;
+        const uint64_t c = std::min<uint64_t>(ix->F, below - f);
+        if (node_first) node_first[id] = (uint32_t)(leaf ? f : ix->level_off[l + 1] + f);
+        if (node_count) node_count[id] = (uint32_t)c;
+        if (node_leaf) node_leaf[id] = leaf ? 1 : 0;
+      }
+    }
+  }
+  return QCP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+
+void qcp_batch_destroy(qcp_batch* b) {
+  if (!b) return;
+  hipSetDevice(b->ix->device);
+  hipStreamSynchronize(b->ix->stream);
+  dfree(b->d_queries);
+  dfree(b->d_counts);
+  dfree(b->d_hashes);
+  dfree(b->d_offsets);
+  dfree(b->d_scan_sums);
+  dfree(b->d_ids);
+  dfree(b->d_visits);
+  dfree(b->d_ctl);
+  dfree(b->d_slots);
+  if (b->h_counts) hipHostFree(b->h_counts);
+  if (b->h_visits) hipHostFree(b->h_visits);
+  for (auto& e : b->ev)
+    if (e) hipEventDestroy(e);
+  delete b;
+}
+
+static qcp_status batch_alloc(qcp_index* ix, uint64_t q, qcp_batch** out) {
+  if (q >= 0xFFFFFFF0ull) return QCP_EINVAL;
+  qcp_batch* b = new (std::nothrow) qcp_batch();
+  if (!b) return QCP_ENOMEM;
+  b->ix = ix;
+  b->q = q;
+  auto fail = [&](qcp_status s) {
+    qcp_batch_destroy(b);
+    return s;
+  };
+  qcp_status s;
+  if ((s = dalloc(&b->d_queries, (uint64_t)2 * ix->D * q)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&b->d_counts, q)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&b->d_hashes, q)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&b->d_offsets, q + 1)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&b->d_scan_sums, kMaxScanBlocks + 1)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&b->d_visits, 2 * q)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&b->d_ctl, 1)) != QCP_OK) return fail(s);
+  for (auto& e : b->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(QCP_EHIP);
+  *out = b;
+  return QCP_OK;
+}
+
+qcp_status qcp_batch_create(qcp_index* ix, const float* boxes, uint64_t q, qcp_batch** out) {
+  if (!ix || !out || (q && !boxes)) return QCP_EINVAL;
+  *out = nullptr;
+  QCP_TRY(set_device(ix->device));
+  qcp_batch* b = nullptr;
+  QCP_TRY(batch_alloc(ix, q, &b));
+  if (q) {
+    if (hipMemcpyAsync(b->d_queries, boxes, (uint64_t)2 * ix->D * 4 * q, hipMemcpyHostToDevice,
+                       ix->stream) != hipSuccess ||
+        hipStreamSynchronize(ix->stream) != hipSuccess) {
+      qcp_batch_destroy(b);
+      return QCP_EHIP;
+    }
+  }
+  *out = b;
+  return QCP_OK;
+}
+
+qcp_status qcp_batch_create_device(qcp_index* ix, const float* d_boxes, uint64_t q,
+                                   qcp_batch** out) {
+  if (!ix || !out || (q && !d_boxes)) return QCP_EINVAL;
+  *out = nullptr;
+  QCP_TRY(set_device(ix->device));
+  qcp_batch* b = nullptr;
+  QCP_TRY(batch_alloc(ix, q, &b));
+  if (q) {
+    if (hipMemcpyAsync(b->d_queries, d_boxes, (uint64_t)2 * ix->D * 4 * q,
+                       hipMemcpyDeviceToDevice, ix->stream) != hipSuccess ||
+        hipStreamSynchronize(ix->stream) != hipSuccess) {
+      qcp_batch_destroy(b);
+      return QCP_EHIP;
+    }
+  }
+  *out = b;
+  return QCP_OK;
+}
+
+static SearchArgs make_args(const qcp_batch* b) {
+  const qcp_index* ix = b->ix;
+  SearchArgs a;
+  memset(&a, 0, sizeof(a));
+  a.obj = ix->d_obj;
+  a.nbox = ix->d_nbox;
+  a.nfirst = ix->d_nfirst;
+  a.ncount = ix->d_ncount;
+  a.queries = b->d_queries;
+  a.obj_stride = ix->obj_stride;
+  a.node_stride = ix->node_stride;
+  a.n = ix->n;
+  a.nq = (uint32_t)b->q;
+  a.F = ix->F;
+  a.h = ix->h;
+  a.n_nodes = (uint32_t)ix->n_nodes;
+  if (!ix->expl) {
+    for (int l = 0; l <= ix->h && l <= kMaxLevels; ++l) a.lvl_off[l] = (uint32_t)ix->level_off[l];
+    for (int l = 0; l < ix->h && l < kMaxLevels; ++l) a.lvl_cnt[l] = (uint32_t)ix->level_count[l];
+    uint64_t sp = ix->F;
+    for (int l = ix->h - 1; l >= 0; --l) {
+      a.span[l] = sp;
+      sp = (sp > (1ull << 40)) ? sp : sp * ix->F;
+    }
+  }
+  a.counts = b->d_counts;
+  a.hashes = b->d_hashes;
+  a.offsets = b->d_offsets;
+  a.ids = b->d_ids;
+  a.visits = b->d_visits;
+  a.ctl = b->d_ctl;
+  a.slots = b->d_slots;
+  a.slot_cap = b->slot_cap;
+  return a;
+}
+
+static unsigned persistent_grid(const qcp_index* ix, bool ids, unsigned block) {
+  const void* k = search_kernel_ptr(ix->D, ix->box, ix->expl, ids, (int)((ix->F + 63) / 64));
+  int per_cu = 0;
+  if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)block, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 2;
+  return (unsigned)(ix->cus * per_cu);
+}
+
+qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_stats* stats) {
+  if (!b) return QCP_EINVAL;
+  qcp_search_cfg cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.mode = QCP_SEARCH_PERSISTENT;
+  cfg.outputs = QCP_OUT_COUNTS | QCP_OUT_HASH;
+  if (cfg_in) cfg = *cfg_in;
+  if (cfg.mode > QCP_SEARCH_PLAIN) return QCP_EINVAL;
+  qcp_index* ix = b->ix;
+  QCP_TRY(set_device(ix->device));
+  hipStream_t s = ix->stream;
+  const uint64_t q = b->q;
+  qcp_search_stats st;
+  memset(&st, 0, sizeof(st));
+  b->has_offsets = false;
+  b->last_mode = cfg.mode;
+  b->last_outputs = cfg.outputs;
+  if (cfg.mode == QCP_SEARCH_PLAIN) {
+    if (ix->expl) return QCP_EINVAL;  // MPRS needs the packed level layout
+    if (cfg.outputs & QCP_OUT_IDS) return QCP_EINVAL;
+    const uint32_t batch = cfg.plain_batch ? cfg.plain_batch : 128;
+    if (!b->h_counts && q) {
+      HIP_TRY(hipHostMalloc((void**)&b->h_counts, 4 * q, 0));
+      HIP_TRY(hipHostMalloc((void**)&b->h_visits, 8 * q, 0));
+    }
+    SearchArgs a = make_args(b);
+    HIP_TRY(hipEventRecord(b->ev[0], s));
+    uint64_t total = 0;
+    for (uint64_t q0 = 0; q0 < q; q0 += batch) {
+      const uint32_t nbk = (uint32_t)std::min<uint64_t>(batch, q - q0);
+      a.q_begin = (uint32_t)q0;
+      HIP_TRY(launch_mprs(a, ix->D, ix->box, nbk, s));
+      // the reference copies the per-block results back synchronously after
+      // every launch (src/tree/mphr.cpp:244-246)
+      HIP_TRY(hipMemcpyAsync(b->h_counts + q0, b->d_counts + q0, 4ull * nbk, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(b->h_visits + 2 * q0, b->d_visits + 2 * q0, 8ull * nbk,
+                             hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      for (uint32_t i = 0; i < nbk; ++i) total += b->h_counts[q0 + i];
+      st.launches++;
+    }
+    HIP_TRY(hipEventRecord(b->ev[1], s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0;
+    hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+    st.kernel_ms = ms;
+    st.total_ms = ms;
+    st.total_hits = total;
+    b->total_hits = total;
+    if (stats) *stats = st;
+    return QCP_OK;
+  }
+  // persistent form
+  const unsigned block = 256;
+  unsigned grid = cfg.grid_blocks ? cfg.grid_blocks : persistent_grid(ix, false, block);
+  SearchArgs a = make_args(b);
+  HIP_TRY(hipEventRecord(b->ev[0], s));
+  HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, false, grid, block, s));
+  HIP_TRY(hipEventRecord(b->ev[1], s));
+  st.launches = 1;
+  QueueCtl ctl;
+  if (cfg.outputs & QCP_OUT_IDS) {
+    HIP_TRY(scan_u32_u64(b->d_counts, q, b->d_offsets, b->d_scan_sums, kMaxScanBlocks, s));
+    unsigned long long total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, b->d_offsets + q, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (total > b->ids_cap || !b->d_ids) {
+      dfree(b->d_ids);
+      b->ids_cap = 0;
+      QCP_TRY(dalloc(&b->d_ids, total + 64));
+      b->ids_cap = total + 64;
+    }
+    a.ids = b->d_ids;
+    unsigned grid2 = cfg.grid_blocks ? cfg.grid_blocks : persistent_grid(ix, true, block);
+    HIP_TRY(hipEventRecord(b->ev[2], s));
+    HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, true, grid2, block, s));
+    HIP_TRY(hipEventRecord(b->ev[3], s));
+    st.launches = 2;
+    b->has_offsets = true;
+  }
+  HIP_TRY(hipMemcpyAsync(&ctl, b->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float k1 = 0, k2 = 0, tot = 0;
+  hipEventElapsedTime(&k1, b->ev[0], b->ev[1]);
+  if (cfg.outputs & QCP_OUT_IDS) {
+    hipEventElapsedTime(&k2, b->ev[2], b->ev[3]);
+    hipEventElapsedTime(&tot, b->ev[0], b->ev[3]);
+  } else {
+    tot = k1;
+  }
+  st.kernel_ms = k1 + k2;
+  st.total_ms = tot;
+  st.internal_entries = ctl.internal_entries;
+  st.leaf_entries = ctl.leaf_entries;
+  st.total_hits = ctl.total_hits;
+  st.donated_items = ctl.donated;
+  b->total_hits = ctl.total_hits;
+  if (stats) *stats = st;
+  return QCP_OK;
+}
+
+uint64_t qcp_batch_total_hits(const qcp_batch* b) { return b ? b->total_hits : 0; }
+
+qcp_status qcp_batch_results(const qcp_batch* b, uint32_t* counts, uint64_t* hashes,
+                             uint64_t* offsets, uint32_t* ids, uint64_t ids_cap,
+                             uint32_t* visits) {
+  if (!b) return QCP_EINVAL;
+  const qcp_index* ix = b->ix;
+  QCP_TRY(set_device(ix->device));
+  hipStream_t s = ix->stream;
+  const uint64_t q = b->q;
+  if (counts && q) HIP_TRY(hipMemcpyAsync(counts, b->d_counts, 4 * q, hipMemcpyDeviceToHost, s));
+  if (hashes && q) HIP_TRY(hipMemcpyAsync(hashes, b->d_hashes, 8 * q, hipMemcpyDeviceToHost, s));
+  if (visits && q) {
+    if (b->last_mode != QCP_SEARCH_PLAIN) return QCP_EINVAL;
+    HIP_TRY(hipMemcpyAsync(visits, b->d_visits, 8 * q, hipMemcpyDeviceToHost, s));
+  }
+  if (offsets || ids) {
+    if (!b->has_offsets) return QCP_EINVAL;
+    unsigned long long total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, b->d_offsets + q, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (offsets) HIP_TRY(hipMemcpyAsync(offsets, b->d_offsets, 8 * (q + 1), hipMemcpyDeviceToHost, s));
+    if (ids) {
+      if (ids_cap < total) return QCP_EBUSY;
+      if (total) HIP_TRY(hipMemcpyAsync(ids, b->d_ids, 4 * total, hipMemcpyDeviceToHost, s));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  return QCP_OK;
+}
+
+qcp_status qcp_search(qcp_index* ix, const float* boxes, uint64_t q, const qcp_search_cfg* cfg,
+                      uint32_t* counts, uint64_t* hashes, uint64_t* offsets, uint32_t* ids,
+                      uint64_t ids_cap, qcp_search_stats* stats) {
+  qcp_batch* b = nullptr;
+  QCP_TRY(qcp_batch_create(ix, boxes, q, &b));
+  qcp_status s = qcp_batch_run(b, cfg, stats);
+  if (s == QCP_OK) {
+    const bool want_ids = cfg && (cfg->outputs & QCP_OUT_IDS);
+    s = qcp_batch_results(b, counts, hashes, want_ids ? offsets : nullptr,
+                          want_ids ? ids : nullptr, ids_cap, nullptr);
+  }
+  qcp_batch_destroy(b);
+  return s;
+}
+
+// Device state of an index from host arrays (the payload of a saved index).
+// The topology is recomputed from (n, cfg) and must equal the stored one.
+__attribute__((visibility("hidden"))) qcp_status qcp_internal_restore(
+    const qcp_build_cfg* cfg, uint64_t n, const uint64_t* keys, const uint32_t* perm,
+    const float* objs, const float* boxes, const uint32_t* first, const uint32_t* count,
+    uint64_t n_nodes, qcp_index** out) {
+  qcp_index* ix = nullptr;
+  QCP_TRY(index_alloc(cfg, n, &ix));
+  auto fail = [&](qcp_status s) {
+    index_free(ix);
+    return s;
+  };
+  if (ix->n_nodes != n_nodes) return fail(QCP_EFORMAT);
+  hipStream_t s = ix->stream;
+  if (ix->expl && n_nodes) {
+    std::vector<uint32_t> f(n_nodes), c(n_nodes);
+    if (hipMemcpy(f.data(), ix->d_nfirst, 4 * n_nodes, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(c.data(), ix->d_ncount, 4 * n_nodes, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(QCP_EHIP);
+    if (memcmp(f.data(), first, 4 * n_nodes) || memcmp(c.data(), count, 4 * n_nodes))
+      return fail(QCP_EFORMAT);
+  }
+  std::vector<float> soa;
+  if (n) {
+    if (hipMemcpyAsync(ix->d_keys, keys, 8 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(ix->d_perm, perm, 4 * n, hipMemcpyHostToDevice, s) != hipSuccess)
+      return fail(QCP_EHIP);
+    soa.resize((size_t)ix->E * n);
+    for (uint64_t r = 0; r < n; ++r)
+      for (uint32_t e = 0; e < ix->E; ++e) soa[(size_t)e * n + r] = objs[r * ix->E + e];
+    for (uint32_t e = 0; e < ix->E; ++e)
+      if (hipMemcpyAsync(ix->d_obj + (uint64_t)e * ix->obj_stride, soa.data() + (size_t)e * n,
+                         4 * n, hipMemcpyHostToDevice, s) != hipSuccess)
+        return fail(QCP_EHIP);
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(QCP_EHIP);
+  }
+  std::vector<float> nb((size_t)2 * ix->D * n_nodes);
+  for (uint64_t i = 0; i < n_nodes; ++i)
+    for (uint32_t e = 0; e < 2 * ix->D; ++e) nb[(size_t)e * n_nodes + i] = boxes[i * 2 * ix->D + e];
+  for (uint32_t e = 0; e < 2 * ix->D && n_nodes; ++e)
+    if (hipMemcpyAsync(ix->d_nbox + (uint64_t)e * ix->node_stride, nb.data() + (size_t)e * n_nodes,
+                       4 * n_nodes, hipMemcpyHostToDevice, s) != hipSuccess)
+      return fail(QCP_EHIP);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(QCP_EHIP);
+  qcp_index_info& in = ix->info;
+  memset(&in, 0, sizeof(in));
+  in.n = n;
+  in.n_nodes = ix->n_nodes;
+  in.height = (uint32_t)ix->h;
+  in.dims = ix->D;
+  in.fanout = ix->F;
+  in.layout = ix->cfg.layout;
+  in.obj_kind = ix->cfg.obj_kind;
+  in.key_mode = ix->cfg.key_mode;
+  in.key_src = ix->cfg.key_src;
+  in.key_bits = ix->nb * ix->D;
+  for (int l = 0; l < ix->h && l < 32; ++l) in.level_count[l] = ix->level_count[l];
+  in.tree_bytes = (uint64_t)ix->E * 4 * n + (uint64_t)2 * ix->D * 4 * ix->n_nodes;
+  *out = ix;
+  return QCP_OK;
+}
+
+}  // extern "C"

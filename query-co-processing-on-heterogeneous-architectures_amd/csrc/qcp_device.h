@@ -1,0 +1,119 @@
+// qcp_device.h -- device-side helpers and kernel argument blocks shared by the
+// build and search kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qcp {
+
+constexpr int kWave = 64;
+constexpr int kMaxLevels = 16;    // packed levels: F >= 16 and N < 2^32 give <= 8
+constexpr int kMaxDepth = 48;     // explicit (BVH) trees: DFS frames per wave
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+// Device-wide work queue state of one persistent search launch.
+struct QueueCtl {
+  uint32_t head;      // next ticket (tickets < nq are root items of query t)
+  uint32_t tail;      // donated slots claimed
+  uint32_t pending;   // items not finished (root items + donations)
+  uint32_t done;      // set once pending reaches zero
+  uint32_t spare[4];
+  unsigned long long internal_entries;
+  unsigned long long leaf_entries;
+  unsigned long long donated;
+  unsigned long long total_hits;
+};
+
+// Tree + query batch as seen by the search kernels.
+struct SearchArgs {
+  const float* obj;          // [E][obj_stride] rank-ordered objects, SoA
+  const float* nbox;         // [2D][node_stride] node MBRs, BFS order, SoA
+  const uint32_t* nfirst;    // explicit layouts: first child node / first rank
+  const uint32_t* ncount;    // explicit layouts: count | kLeafBit for leaves
+  const float* queries;      // [nq][2D] row-major [lo(D), hi(D)]
+  uint64_t obj_stride;
+  uint64_t node_stride;
+  uint64_t n;                // objects
+  uint32_t nq;
+  uint32_t F;
+  int32_t h;                 // node levels (packed); 0 = empty tree
+  uint32_t n_nodes;
+  uint32_t lvl_off[kMaxLevels + 1];
+  uint32_t lvl_cnt[kMaxLevels];
+  uint64_t span[kMaxLevels];   // ranks under one full node of each level
+  uint32_t* counts;          // [nq]
+  unsigned long long* hashes;// [nq]
+  const unsigned long long* offsets;  // [nq+1] (ID write pass)
+  uint32_t* ids;             // ID write pass output
+  uint32_t* visits;          // plain form: [2*nq]
+  QueueCtl* ctl;
+  unsigned long long* slots; // donated items (q << 32 | node)
+  uint32_t slot_cap;
+  uint32_t donate;
+  uint32_t q_begin;          // plain form: first query of the launch
+};
+
+__device__ __forceinline__ uint64_t id_hash(uint32_t id) {
+  // splitmix64 finalizer of the ID: the per-ID term of the order-independent
+  // set hash (sum over IDs, mod 2^64).
+  uint64_t z = (uint64_t)id + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// popcount of the ballot bits below this lane
+__device__ __forceinline__ uint32_t rank_below(uint64_t b) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    float o = __shfl_xor(v, off, kWave);
+    v = (v > o) ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    float o = __shfl_xor(v, off, kWave);
+    v = (v < o) ? o : v;
+  }
+  return v;
+}
+
+// The reference's closed-interval test: rejected iff q.lo > e.hi or q.hi < e.lo
+// in some dimension (src/node/node_soa.cpp:94-113).
+template <int D>
+__device__ __forceinline__ bool overlaps(const float* ql, const float* qh,
+                                         const float* lo, const float* hi) {
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) ok = ok && !(ql[d] > hi[d] || qh[d] < lo[d]);
+  return ok;
+}
+
+}  // namespace qcp

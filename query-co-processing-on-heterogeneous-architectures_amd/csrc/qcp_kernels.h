@@ -1,0 +1,46 @@
+// qcp_kernels.h -- host-callable launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qcp_device.h"
+
+namespace qcp {
+
+// build
+hipError_t launch_keys(const float* objs, uint64_t n, unsigned D, bool box, bool ref,
+                       bool center, unsigned nb, uint64_t* keys, uint32_t* idx,
+                       unsigned long long* key_or, uint32_t* bad, hipStream_t s);
+uint32_t sort_tiles(uint64_t n);
+hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
+                            uint32_t* vout, uint64_t n, unsigned shift, uint32_t* hist,
+                            uint32_t* hist_scan, uint32_t* block_sums, uint32_t max_blocks,
+                            hipStream_t s);
+hipError_t scan_u32_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* block_sums,
+                        uint32_t max_blocks, hipStream_t s);
+hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
+                        unsigned long long* block_sums, uint32_t max_blocks, hipStream_t s);
+hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, unsigned E,
+                         float* obj, uint64_t stride, hipStream_t s);
+hipError_t launch_leaf_boxes(const float* obj, uint64_t ostride, uint64_t n, unsigned D,
+                             bool box, uint32_t F, uint64_t nleaves, float* nbox,
+                             uint64_t nstride, uint64_t leaf_off, hipStream_t s);
+hipError_t launch_level_boxes(float* nbox, uint64_t nstride, unsigned D, uint64_t child_off,
+                              uint64_t child_cnt, uint32_t F, uint64_t parent_off,
+                              uint64_t parent_cnt, hipStream_t s);
+hipError_t launch_tree_boxes(const float* obj, uint64_t ostride, const uint32_t* nfirst,
+                             const uint32_t* ncount, unsigned D, bool box, uint32_t F,
+                             uint64_t lo, uint64_t hi, float* nbox, uint64_t nstride,
+                             hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s);
+
+// search
+hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, bool ids,
+                         unsigned grid, unsigned block, hipStream_t s);
+hipError_t launch_mprs(const SearchArgs& a, unsigned D, bool box, unsigned nblocks,
+                       hipStream_t s);
+const void* search_kernel_ptr(unsigned D, bool box, bool expl, bool ids, int mw);
+
+}  // namespace qcp

@@ -1,0 +1,535 @@
+// search_kernels.hip -- range search over a resident tree on gfx950.
+//
+// Two forms:
+//  * k_mprs: the reference's plain form, MPRS "restart scanning + parent
+//    check" (src/tree/mphr.cpp:303-438) with one workgroup per query, launched
+//    once per batch of queries by the host (src/tree/mphr.cpp:231-258).  A
+//    workgroup is a single wave here: the reference's LDS min-reduction over
+//    192 threads (FindMinOnGPU, src/common/macro.h:24-42) becomes a ballot and
+//    a find-first-set, and the partially zeroed t_hit[] sum
+//    (src/tree/mphr.cpp:327-330,429-437) becomes a popcount of the hit ballot.
+//  * k_search: the persistent form.  All queries in one launch; each wave
+//    takes work items (a query and a subtree root) from a device-wide ticket
+//    queue and walks the canonical top-down traversal with a DFS stack of
+//    (node, remaining-child bitmask) frames in LDS.  Children that are leaves
+//    are streamed in 64-entry chunks, several chunks in flight per wave, from
+//    the rank-ordered SoA object arrays; hits are compacted with ballot +
+//    mbcnt.  With subtree donation on, a wave that sees idle waves hands the
+//    unvisited siblings of its shallowest frame to the queue (work stealing).
+//    Visiting children in ascending order makes ID lists come out sorted.
+#include <hip/hip_runtime.h>
+
+#include "qcp_device.h"
+#include "qcp_kernels.h"
+
+namespace qcp {
+
+template <int MW>
+struct Mask {
+  uint64_t w[MW];
+  __device__ __forceinline__ bool empty() const {
+    uint64_t o = 0;
+#pragma unroll
+    for (int i = 0; i < MW; ++i) o |= w[i];
+    return o == 0;
+  }
+  // lowest set bit, or -1
+  __device__ __forceinline__ int lowest() const {
+#pragma unroll
+    for (int i = 0; i < MW; ++i)
+      if (w[i]) return i * 64 + __builtin_ctzll(w[i]);
+    return -1;
+  }
+  __device__ __forceinline__ bool test(int k) const {
+    int i = k >> 6;
+    uint64_t v = 0;
+#pragma unroll
+    for (int t = 0; t < MW; ++t)
+      if (t == i) v = w[t];
+    return (v >> (k & 63)) & 1ull;
+  }
+  __device__ __forceinline__ void clear(int k) {
+#pragma unroll
+    for (int t = 0; t < MW; ++t)
+      if (t == (k >> 6)) w[t] &= ~(1ull << (k & 63));
+  }
+  // end (exclusive) of the run of consecutive set bits of `this` starting at k
+  __device__ __forceinline__ int run_end(int k) const {
+    int e = k;
+#pragma unroll
+    for (int t = 0; t < MW; ++t) {
+      if (e >= t * 64 && e < (t + 1) * 64) {
+        uint64_t x = ~(w[t] >> (e & 63));
+        int r = x ? __builtin_ctzll(x) : 64 - (e & 63);
+        r = r > 64 - (e & 63) ? 64 - (e & 63) : r;
+        e += r;
+      }
+    }
+    return e;
+  }
+  __device__ __forceinline__ void clear_range(int k, int e) {
+#pragma unroll
+    for (int t = 0; t < MW; ++t) {
+      int lo = k > t * 64 ? k - t * 64 : 0;
+      int hi = e < (t + 1) * 64 ? e - t * 64 : 64;
+      if (hi > lo) {
+        uint64_t m = (hi - lo == 64) ? ~0ull : (((1ull << (hi - lo)) - 1ull) << lo);
+        w[t] &= ~m;
+      }
+    }
+  }
+  __device__ __forceinline__ Mask operator&(const Mask& o) const {
+    Mask r;
+#pragma unroll
+    for (int i = 0; i < MW; ++i) r.w[i] = w[i] & o.w[i];
+    return r;
+  }
+};
+
+template <int MW>
+struct Frame {
+  uint32_t node;
+  uint32_t level;
+  Mask<MW> m;
+  Mask<MW> lm;
+};
+
+// Per-wave traversal context (all members wave-uniform except h_lane).
+template <int D>
+struct Ctx {
+  float ql[D], qh[D];
+  uint32_t q;
+  uint32_t cnt;           // hits of this item
+  uint64_t out_base;      // ID write pass: next output slot
+  uint64_t h_lane;        // per-lane partial hash
+  uint64_t ie, le;        // canonical entries visited
+};
+
+// Evaluate the children [cbase, cbase+cc) (node ids) of an internal node
+// against the query; for explicit layouts also their leaf flags.
+template <int D, int MW, bool EXPL>
+__device__ __forceinline__ void eval_children(const SearchArgs& a, const Ctx<D>& c,
+                                              uint32_t cbase, uint32_t cc, Mask<MW>& m,
+                                              Mask<MW>& lm) {
+  const uint32_t lane = lane_id();
+  float lo[MW][D], hi[MW][D];
+  uint32_t cw[MW];
+#pragma unroll
+  for (int w = 0; w < MW; ++w) {
+    const uint64_t g = (uint64_t)cbase + w * kWave + lane;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      lo[w][d] = a.nbox[(uint64_t)d * a.node_stride + g];
+      hi[w][d] = a.nbox[(uint64_t)(D + d) * a.node_stride + g];
+    }
+    if (EXPL) cw[w] = a.ncount[g];
+  }
+#pragma unroll
+  for (int w = 0; w < MW; ++w) {
+    const uint32_t k = w * kWave + lane;
+    bool hit = (k < cc) && overlaps<D>(c.ql, c.qh, lo[w], hi[w]);
+    m.w[w] = ballot(hit);
+    lm.w[w] = EXPL ? ballot((cw[w] & kLeafBit) != 0) : ~0ull;
+  }
+}
+
+// Stream the leaf runs at the bottom of mask m (bits that are also set in lm),
+// U 64-entry chunks in flight, stopping at the first non-leaf set bit.
+// run_range(k, k2, &s, &e) maps a run of children [k, k2) to its rank range.
+template <int D, bool BOX, int MW, bool IDS, int U, typename RunRange>
+__device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<MW>& m,
+                                          const Mask<MW>& lm, RunRange run_range) {
+  constexpr int E = BOX ? 2 * D : D;
+  const uint32_t lane = lane_id();
+  uint64_t cs = 0, ce = 0;
+  while (true) {
+    uint64_t st[U];
+    uint32_t len[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (cs >= ce) {
+        int k = m.lowest();
+        if (k >= 0 && lm.test(k)) {
+          int k2 = (m & lm).run_end(k);
+          m.clear_range(k, k2);
+          run_range(k, k2, cs, ce);
+          c.le += ce - cs;
+        }
+      }
+      uint64_t rem = ce > cs ? ce - cs : 0;
+      len[u] = (uint32_t)(rem < kWave ? rem : kWave);
+      st[u] = cs;
+      cs += len[u];
+    }
+    if (len[0] == 0) break;
+    float v[U][E];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (len[u]) {
+        const uint64_t i = st[u] + lane;
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[u][e] = a.obj[(uint64_t)e * a.obj_stride + i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (len[u]) {
+        const float* lo = v[u];
+        const float* hi = BOX ? v[u] + D : v[u];
+        bool hit = (lane < len[u]) && overlaps<D>(c.ql, c.qh, lo, hi);
+        uint64_t b = ballot(hit);
+        uint32_t nb = (uint32_t)__popcll(b);
+        if (IDS) {
+          if (hit) a.ids[c.out_base + rank_below(b)] = (uint32_t)(st[u] + lane + 1);
+          c.out_base += nb;
+        } else {
+          if (hit) c.h_lane += id_hash((uint32_t)(st[u] + lane + 1));
+        }
+        c.cnt += nb;
+      }
+    }
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void load_query(const SearchArgs& a, uint32_t q, Ctx<D>& c) {
+  const float* qp = a.queries + (uint64_t)q * 2 * D;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    c.ql[d] = qp[d];
+    c.qh[d] = qp[D + d];
+  }
+}
+
+__device__ __forceinline__ int level_of(const SearchArgs& a, uint32_t node) {
+  int l = 0;
+  while (l + 1 < a.h && node >= a.lvl_off[l + 1]) ++l;
+  return l;
+}
+
+// Canonical top-down traversal of the subtree rooted at `node`.
+template <int D, bool BOX, int MW, bool EXPL, bool IDS, int U>
+__device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* stk) {
+  const uint32_t F = a.F;
+  int level = EXPL ? 0 : level_of(a, node);
+  // --- node metadata
+  auto packed_range = [&](int lv, uint32_t j, int k, int k2, uint64_t& s, uint64_t& e) {
+    // children of (lv, j) are leaves: child index j*F+k .. covers ranks
+    uint64_t c0 = (uint64_t)j * F + (uint64_t)k;
+    uint64_t c1 = (uint64_t)j * F + (uint64_t)k2;
+    s = c0 * F;
+    e = c1 * F < a.n ? c1 * F : a.n;
+  };
+  bool is_leaf;
+  uint32_t first, cnt;
+  if (EXPL) {
+    uint32_t cw = a.ncount[node];
+    first = a.nfirst[node];
+    cnt = cw & ~kLeafBit;
+    is_leaf = (cw & kLeafBit) != 0;
+  } else {
+    is_leaf = (level == a.h - 1);
+    uint32_t j = node - a.lvl_off[level];
+    first = j * F;
+    uint64_t below = is_leaf ? a.n : a.lvl_cnt[level + 1];
+    cnt = (uint32_t)((below - first) < F ? (below - first) : F);
+  }
+  if (is_leaf) {
+    Mask<MW> m, lm;
+#pragma unroll
+    for (int i = 0; i < MW; ++i) { m.w[i] = 0; lm.w[i] = ~0ull; }
+    m.w[0] = 1;  // a single pseudo-run covering this leaf's entries
+    uint64_t s0 = first, e0 = (uint64_t)first + cnt;
+    scan_runs<D, BOX, MW, IDS, U>(a, c, m, lm,
+                                  [&](int, int, uint64_t& s, uint64_t& e) { s = s0; e = e0; });
+    return;
+  }
+  Mask<MW> m, lm;
+  uint32_t cbase = EXPL ? first : a.lvl_off[level + 1] + first;
+  eval_children<D, MW, EXPL>(a, c, cbase, cnt, m, lm);
+  c.ie += cnt;
+  uint32_t cur = node;
+  int sp = 0;
+  while (true) {
+    // leaf children at the bottom of the mask are streamed right away
+    if (EXPL) {
+      const uint32_t cf = a.nfirst[cur];
+      scan_runs<D, BOX, MW, IDS, U>(a, c, m, lm, [&](int k, int k2, uint64_t& s, uint64_t& e) {
+        s = a.nfirst[cf + k];
+        e = (uint64_t)a.nfirst[cf + k2 - 1] + (a.ncount[cf + k2 - 1] & ~kLeafBit);
+      });
+    } else if (level + 1 == a.h - 1) {
+      const uint32_t j = cur - a.lvl_off[level];
+      scan_runs<D, BOX, MW, IDS, U>(a, c, m, lm, [&](int k, int k2, uint64_t& s, uint64_t& e) {
+        packed_range(level, j, k, k2, s, e);
+      });
+    }
+    if (m.empty()) {
+      if (sp == 0) break;
+      --sp;
+      cur = stk[sp].node;
+      level = (int)stk[sp].level;
+      m = stk[sp].m;
+      lm = stk[sp].lm;
+      continue;
+    }
+    const int k = m.lowest();
+    m.clear(k);
+    uint32_t child, cfirst, ccnt;
+    if (EXPL) {
+      child = a.nfirst[cur] + k;
+      cfirst = a.nfirst[child];
+      ccnt = a.ncount[child] & ~kLeafBit;
+    } else {
+      const uint32_t j = cur - a.lvl_off[level];
+      const uint32_t cj = j * F + k;
+      child = a.lvl_off[level + 1] + cj;
+      cfirst = cj * F;
+      uint64_t below = a.lvl_cnt[level + 2];
+      ccnt = (uint32_t)((below - cfirst) < F ? (below - cfirst) : F);
+      cfirst += a.lvl_off[level + 2];
+    }
+    Mask<MW> cm, clm;
+    eval_children<D, MW, EXPL>(a, c, cfirst, ccnt, cm, clm);
+    c.ie += ccnt;
+    if (cm.empty()) continue;
+    if (!m.empty()) {
+      stk[sp].node = cur;
+      stk[sp].level = (uint32_t)level;
+      stk[sp].m = m;
+      stk[sp].lm = lm;
+      ++sp;
+    }
+    cur = child;
+    level += 1;
+    m = cm;
+    lm = clm;
+  }
+}
+
+template <int D, bool BOX, int MW, bool EXPL, bool IDS, int U>
+__global__ __launch_bounds__(256) void k_search(SearchArgs a) {
+  constexpr int kDepth = EXPL ? kMaxDepth : kMaxLevels;
+  __shared__ Frame<MW> stack[256 / kWave][kDepth];
+  Frame<MW>* stk = stack[threadIdx.x / kWave];
+  const uint32_t lane = lane_id();
+  uint64_t ie = 0, le = 0, hits = 0;
+  while (true) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&a.ctl->head, 1u);
+    t = uni(t);
+    if (t >= a.nq) break;
+    const uint32_t q = t;
+    Ctx<D> c;
+    c.q = q;
+    c.cnt = 0;
+    c.h_lane = 0;
+    c.ie = 0;
+    c.le = 0;
+    c.out_base = IDS ? a.offsets[q] : 0;
+    load_query<D>(a, q, c);
+    if (a.h > 0) walk<D, BOX, MW, EXPL, IDS, U>(a, c, 0u, stk);
+    if (!IDS) {
+      uint64_t hsum = wave_sum_u64(c.h_lane);
+      if (lane == 0) {
+        a.counts[q] = c.cnt;
+        a.hashes[q] = hsum;
+      }
+    }
+    ie += c.ie;
+    le += c.le;
+    hits += c.cnt;
+  }
+  if (lane == 0) {
+    atomicAdd(&a.ctl->internal_entries, (unsigned long long)ie);
+    atomicAdd(&a.ctl->leaf_entries, (unsigned long long)le);
+    atomicAdd(&a.ctl->total_hits, (unsigned long long)hits);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plain MPRS, one wave per query (packed layouts only).
+template <int D, bool BOX, int MW>
+__global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
+  constexpr int E = BOX ? 2 * D : D;
+  const uint32_t lane = lane_id();
+  const uint32_t q = a.q_begin + blockIdx.x;
+  if (q >= a.nq) return;
+  Ctx<D> c;
+  load_query<D>(a, q, c);
+  const uint32_t F = a.F;
+  const int h = a.h;
+  const uint64_t n = a.n;
+  uint32_t cnt = 0, rootv = 0, nodev = 0;
+  uint64_t hl = 0;
+  if (h > 0) {
+    auto last_index = [&](int l, uint64_t j) -> uint64_t {
+      uint64_t e = (j + 1) * a.span[l];
+      return e < n ? e : n;
+    };
+    uint64_t visited = 0;
+    const uint64_t last = n;
+    int lvl = 0;
+    uint64_t j = 0;
+    rootv = 1;
+    while (visited < last) {
+      while (lvl < h - 1) {
+        const uint64_t first = j * F;
+        const uint64_t below = a.lvl_cnt[lvl + 1];
+        const uint32_t cc = (uint32_t)((below - first) < F ? (below - first) : F);
+        const uint64_t gbase = a.lvl_off[lvl + 1] + first;
+        int pick = -1;
+#pragma unroll
+        for (int w = 0; w < MW; ++w) {
+          const uint32_t k = w * kWave + lane;
+          float lo[D], hi[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            lo[d] = a.nbox[(uint64_t)d * a.node_stride + gbase + k];
+            hi[d] = a.nbox[(uint64_t)(D + d) * a.node_stride + gbase + k];
+          }
+          bool ok = (k < cc) && last_index(lvl + 1, first + k) > visited &&
+                    overlaps<D>(c.ql, c.qh, lo, hi);
+          uint64_t b = ballot(ok);
+          if (pick < 0 && b) pick = w * kWave + __builtin_ctzll(b);
+        }
+        if (pick < 0) {
+          visited = last_index(lvl, j);
+          lvl = 0;
+          j = 0;
+          ++rootv;
+          break;
+        }
+        ++lvl;
+        j = first + (uint64_t)pick;
+        ++nodev;
+      }
+      while (lvl == h - 1) {
+        const uint64_t first = j * F;
+        const uint32_t cc = (uint32_t)((n - first) < F ? (n - first) : F);
+        bool any = false;
+#pragma unroll
+        for (int w = 0; w < MW; ++w) {
+          const uint32_t k = w * kWave + lane;
+          float v[E];
+#pragma unroll
+          for (int e = 0; e < E; ++e) v[e] = a.obj[(uint64_t)e * a.obj_stride + first + k];
+          bool hit = (k < cc) && overlaps<D>(c.ql, c.qh, v, BOX ? v + D : v);
+          uint64_t b = ballot(hit);
+          cnt += (uint32_t)__popcll(b);
+          if (hit) hl += id_hash((uint32_t)(first + k + 1));
+          any = any || (b != 0);
+        }
+        visited = last_index(lvl, j);
+        if (visited == last) break;
+        if (any) {
+          ++j;
+          ++nodev;
+        } else {
+          --lvl;
+          j /= F;
+          if (lvl == 0) ++rootv; else ++nodev;
+        }
+      }
+    }
+  }
+  uint64_t hs = wave_sum_u64(hl);
+  if (lane == 0) {
+    a.counts[q] = cnt;
+    a.hashes[q] = hs;
+    if (a.visits) {
+      a.visits[2 * (uint64_t)q] = rootv;
+      a.visits[2 * (uint64_t)q + 1] = nodev;
+    }
+  }
+}
+
+__global__ void k_reset_ctl(QueueCtl* ctl, uint32_t pending) {
+  if (threadIdx.x == 0) {
+    ctl->head = 0;
+    ctl->tail = 0;
+    ctl->pending = pending;
+    ctl->done = 0;
+    ctl->internal_entries = 0;
+    ctl->leaf_entries = 0;
+    ctl->donated = 0;
+    ctl->total_hits = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+#define QCP_MW_SWITCH(MWV, ...)                        \
+  switch (MWV) {                                       \
+    case 1: { constexpr int MW = 1; __VA_ARGS__; } break; \
+    case 2: { constexpr int MW = 2; __VA_ARGS__; } break; \
+    case 3: { constexpr int MW = 3; __VA_ARGS__; } break; \
+    case 4: { constexpr int MW = 4; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue;              \
+  }
+
+template <int D, bool BOX, bool EXPL, bool IDS>
+static hipError_t launch_search_t(const SearchArgs& a, int mw, unsigned grid, unsigned block,
+                                  hipStream_t s) {
+  constexpr int U = 4;
+  QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, IDS, U><<<grid, block, 0, s>>>(a));
+  return hipGetLastError();
+}
+
+template <int D, bool BOX>
+static hipError_t launch_search_d(const SearchArgs& a, bool expl, bool ids, unsigned grid,
+                                  unsigned block, hipStream_t s) {
+  int mw = (int)((a.F + 63) / 64);
+  if (expl) {
+    return ids ? launch_search_t<D, BOX, true, true>(a, mw, grid, block, s)
+               : launch_search_t<D, BOX, true, false>(a, mw, grid, block, s);
+  }
+  return ids ? launch_search_t<D, BOX, false, true>(a, mw, grid, block, s)
+             : launch_search_t<D, BOX, false, false>(a, mw, grid, block, s);
+}
+
+hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, bool ids,
+                         unsigned grid, unsigned block, hipStream_t s) {
+  k_reset_ctl<<<1, 64, 0, s>>>(a.ctl, a.nq);
+  if (D == 2) return box ? launch_search_d<2, true>(a, expl, ids, grid, block, s)
+                         : launch_search_d<2, false>(a, expl, ids, grid, block, s);
+  if (D == 3) return box ? launch_search_d<3, true>(a, expl, ids, grid, block, s)
+                         : launch_search_d<3, false>(a, expl, ids, grid, block, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_mprs(const SearchArgs& a, unsigned D, bool box,
+                       unsigned nblocks, hipStream_t s) {
+  int mw = (int)((a.F + 63) / 64);
+  if (D == 2) {
+    if (box) { QCP_MW_SWITCH(mw, k_mprs<2, true, MW><<<nblocks, 64, 0, s>>>(a)) }
+    else { QCP_MW_SWITCH(mw, k_mprs<2, false, MW><<<nblocks, 64, 0, s>>>(a)) }
+  } else if (D == 3) {
+    if (box) { QCP_MW_SWITCH(mw, k_mprs<3, true, MW><<<nblocks, 64, 0, s>>>(a)) }
+    else { QCP_MW_SWITCH(mw, k_mprs<3, false, MW><<<nblocks, 64, 0, s>>>(a)) }
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+const void* search_kernel_ptr(unsigned D, bool box, bool expl, bool ids, int mw) {
+  // used for occupancy queries
+#define QCP_PTR(DD, BB, EE, II)                                              \
+  switch (mw) {                                                              \
+    case 1: return (const void*)&k_search<DD, BB, 1, EE, II, 4>;             \
+    case 2: return (const void*)&k_search<DD, BB, 2, EE, II, 4>;             \
+    case 3: return (const void*)&k_search<DD, BB, 3, EE, II, 4>;             \
+    case 4: return (const void*)&k_search<DD, BB, 4, EE, II, 4>;             \
+    default: return nullptr;                                                 \
+  }
+#define QCP_PTR_E(DD, BB)                                                        \
+  if (expl) { if (ids) { QCP_PTR(DD, BB, true, true) } else { QCP_PTR(DD, BB, true, false) } } \
+  else { if (ids) { QCP_PTR(DD, BB, false, true) } else { QCP_PTR(DD, BB, false, false) } }
+  if (D == 2) { if (box) { QCP_PTR_E(2, true) } else { QCP_PTR_E(2, false) } }
+  if (D == 3) { if (box) { QCP_PTR_E(3, true) } else { QCP_PTR_E(3, false) } }
+  return nullptr;
+#undef QCP_PTR
+#undef QCP_PTR_E
+}
+
+}  // namespace qcp

@@ -1,0 +1,388 @@
+"""ctypes front end of libqcp.so (the MI355X Hilbert bulk-load + range-search
+library).  Mirrors the reference's tree API (src/tree/tree.h:15-155):
+
+    Index.build(...)      ~ tree::Tree::Build (src/tree/mphr.cpp:35-136, bvh.cpp:25-63)
+    Index.search(...)     ~ tree::Tree::Search (src/tree/mphr.cpp:196-272)
+    Index.save/.load      ~ DumpToFile / DumpFromFile (src/tree/mphr.cpp:138-194)
+
+Nothing here computes: every call goes through the C ABI of the gfx950 library,
+and loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "_build", "libqcp.so")
+
+LAYOUT = {"mphr": 0, "rtree": 1, "bvh": 2}
+KEY = {"ref": 0, "b16": 1}
+SRC = {"lo": 0, "center": 1}
+OBJ = {"points": 0, "boxes": 1}
+MODE = {"persistent": 0, "plain": 1}
+OUT_COUNTS, OUT_HASH, OUT_IDS, OUT_VISITS = 1, 2, 4, 8
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP runtime error",
+          -4: "I/O error", -5: "malformed index file", -6: "buffer too small",
+          -7: "coordinate outside the key quantiser's domain", -8: "no GPU"}
+
+
+class QcpError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__("%s: %s (%d)" % (what, STATUS.get(status, "?"), status))
+
+
+class BuildCfg(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in
+                ("dims", "fanout", "layout", "key_mode", "key_src", "obj_kind", "device", "flags")]
+
+
+class IndexInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("n_nodes", ctypes.c_uint64), ("height", ctypes.c_uint32),
+                ("dims", ctypes.c_uint32), ("fanout", ctypes.c_uint32), ("layout", ctypes.c_uint32),
+                ("obj_kind", ctypes.c_uint32), ("key_mode", ctypes.c_uint32),
+                ("key_src", ctypes.c_uint32), ("key_bits", ctypes.c_uint32),
+                ("level_count", ctypes.c_uint64 * 32), ("tree_bytes", ctypes.c_uint64),
+                ("build_ms", ctypes.c_double), ("build_phase_ms", ctypes.c_double * 6)]
+
+
+class SearchCfg(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in
+                ("mode", "outputs", "plain_batch", "block_waves", "grid_blocks", "leaf_chunks",
+                 "donate", "reserved")]
+
+
+class SearchStats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("total_hits", ctypes.c_uint64), ("internal_entries", ctypes.c_uint64),
+                ("leaf_entries", ctypes.c_uint64), ("donated_items", ctypes.c_uint64),
+                ("launches", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+P = ctypes.c_void_p
+
+
+def lib():
+    """Load libqcp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libqcp.so not built at %s (run __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.qcp_strerror.restype = ctypes.c_char_p
+        L.qcp_strerror.argtypes = [ctypes.c_int]
+        L.qcp_device_count.restype = ctypes.c_int
+        L.qcp_build.argtypes = [ctypes.POINTER(BuildCfg), P, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.qcp_build_device.argtypes = [ctypes.POINTER(BuildCfg), P, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.qcp_rebuild_device.argtypes = [P, P]
+        L.qcp_index_destroy.argtypes = [P]
+        L.qcp_index_destroy.restype = None
+        L.qcp_index_info_get.argtypes = [P, ctypes.POINTER(IndexInfo)]
+        L.qcp_export.argtypes = [P, P, P, P, P, P, P, P]
+        L.qcp_save.argtypes = [P, ctypes.c_char_p]
+        L.qcp_load.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(P)]
+        L.qcp_batch_create.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.qcp_batch_create_device.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.qcp_batch_destroy.argtypes = [P]
+        L.qcp_batch_destroy.restype = None
+        L.qcp_batch_run.argtypes = [P, ctypes.POINTER(SearchCfg), ctypes.POINTER(SearchStats)]
+        L.qcp_batch_results.argtypes = [P, P, P, P, P, ctypes.c_uint64, P]
+        L.qcp_batch_total_hits.argtypes = [P]
+        L.qcp_batch_total_hits.restype = ctypes.c_uint64
+        L.qcp_search.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(SearchCfg), P, P, P, P,
+                                 ctypes.c_uint64, ctypes.POINTER(SearchStats)]
+        L.qcp_philox4x32_10.argtypes = [P, P, P]
+        L.qcp_philox4x32_10.restype = None
+        for fn in ("qcp_gen_words", "qcp_gen_uniform", "qcp_gen_normal"):
+            getattr(L, fn).argtypes = [P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                       ctypes.c_uint64]
+            getattr(L, fn).restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def check(st, what):
+    if st != 0:
+        raise QcpError(st, what)
+
+
+def device_count():
+    return lib().qcp_device_count()
+
+
+# ---------------------------------------------------------------------------
+class Index:
+    """A device-resident tree built by libqcp.so."""
+
+    def __init__(self, handle, D, obj_kind):
+        self._h = handle
+        self.D = D
+        self.obj_kind = obj_kind
+
+    @classmethod
+    def build(cls, objs, dims, fanout=64, layout="mphr", key_mode="ref", key_src="lo",
+              obj_kind="points", device=0):
+        objs = np.ascontiguousarray(objs, dtype=np.float32)
+        E = dims * (2 if obj_kind == "boxes" else 1)
+        n = objs.size // E if objs.size else 0
+        cfg = BuildCfg(dims, fanout, LAYOUT[layout], KEY[key_mode], SRC[key_src],
+                       OBJ[obj_kind], device, 0)
+        h = P()
+        check(lib().qcp_build(ctypes.byref(cfg), _p(objs) if n else None, n, ctypes.byref(h)),
+              "qcp_build")
+        return cls(h, dims, obj_kind)
+
+    @classmethod
+    def load(cls, path, device=0):
+        h = P()
+        check(lib().qcp_load(path.encode(), device, ctypes.byref(h)), "qcp_load")
+        ix = cls(h, 0, "points")
+        inf = ix.info()
+        ix.D = inf["dims"]
+        ix.obj_kind = "boxes" if inf["obj_kind"] == 1 else "points"
+        return ix
+
+    def close(self):
+        if self._h:
+            lib().qcp_index_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        inf = IndexInfo()
+        check(lib().qcp_index_info_get(self._h, ctypes.byref(inf)), "qcp_index_info_get")
+        d = {k: getattr(inf, k) for k, _ in IndexInfo._fields_}
+        d["level_count"] = [int(x) for x in inf.level_count[: inf.height]]
+        d["build_phase_ms"] = list(inf.build_phase_ms)
+        return d
+
+    def export(self):
+        inf = self.info()
+        n, m, D = inf["n"], inf["n_nodes"], inf["dims"]
+        E = D * (2 if inf["obj_kind"] == 1 else 1)
+        keys = np.zeros(max(n, 1), np.uint64)
+        perm = np.zeros(max(n, 1), np.uint32)
+        sobjs = np.zeros((max(n, 1), E), np.float32)
+        boxes = np.zeros((max(m, 1), 2 * D), np.float32)
+        first = np.zeros(max(m, 1), np.uint32)
+        count = np.zeros(max(m, 1), np.uint32)
+        leaf = np.zeros(max(m, 1), np.uint8)
+        check(lib().qcp_export(self._h, _p(keys), _p(perm), _p(sobjs), _p(boxes), _p(first),
+                               _p(count), _p(leaf)), "qcp_export")
+        return {"keys": keys[:n], "perm": perm[:n], "sobjs": sobjs[:n], "boxes": boxes[:m],
+                "first": first[:m], "count": count[:m], "leaf": leaf[:m], "info": inf}
+
+    def save(self, path):
+        check(lib().qcp_save(self._h, path.encode()), "qcp_save")
+
+    def batch(self, queries):
+        return Batch(self, queries)
+
+    def search(self, queries, mode="persistent", ids=False, visits=False, plain_batch=128,
+               grid_blocks=0, donate=0):
+        with self.batch(queries) as b:
+            st = b.run(mode=mode, ids=ids, plain_batch=plain_batch, grid_blocks=grid_blocks,
+                       donate=donate)
+            res = b.results(ids=ids, visits=visits)
+            res["stats"] = st
+            return res
+
+
+class Batch:
+    """A query set resident in HBM, searchable repeatedly (bench timing)."""
+
+    def __init__(self, index, queries):
+        self.index = index
+        q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, 2 * index.D)
+        self.q = q.shape[0]
+        self._h = P()
+        check(lib().qcp_batch_create(index._h, _p(q) if self.q else None, self.q,
+                                     ctypes.byref(self._h)), "qcp_batch_create")
+
+    def close(self):
+        if self._h:
+            lib().qcp_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=0,
+            hash=True):
+        outputs = OUT_COUNTS | (OUT_HASH if hash else 0) | (OUT_IDS if ids else 0)
+        cfg = SearchCfg(MODE[mode], outputs, plain_batch, 0, grid_blocks, 0, donate, 0)
+        st = SearchStats()
+        check(lib().qcp_batch_run(self._h, ctypes.byref(cfg), ctypes.byref(st)), "qcp_batch_run")
+        self._ids = ids
+        self._mode = mode
+        return st.as_dict()
+
+    def results(self, ids=False, visits=False):
+        q = self.q
+        counts = np.zeros(max(q, 1), np.uint32)
+        hashes = np.zeros(max(q, 1), np.uint64)
+        out = {}
+        offsets = idsb = vis = None
+        cap = 0
+        if ids:
+            total = lib().qcp_batch_total_hits(self._h)
+            offsets = np.zeros(q + 1, np.uint64)
+            idsb = np.zeros(max(int(total), 1), np.uint32)
+            cap = int(total)
+        if visits:
+            vis = np.zeros(max(2 * q, 2), np.uint32)
+        check(lib().qcp_batch_results(self._h, _p(counts), _p(hashes), _p(offsets), _p(idsb), cap,
+                                      _p(vis)), "qcp_batch_results")
+        out["counts"] = counts[:q]
+        out["hashes"] = hashes[:q]
+        if ids:
+            out["offsets"] = offsets
+            out["ids"] = idsb[: int(offsets[-1])]
+        if visits:
+            out["visits"] = vis[: 2 * q].reshape(q, 2)
+        return out
+
+
+# ---------------------------------------------------------------------------
+# seeded generator (Philox4x32-10) and the benchmark workloads
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib().qcp_philox4x32_10(_p(c), _p(k), _p(o))
+    return o
+
+
+def gen_words(n, seed, stream, offset=0):
+    out = np.empty(max(n, 1), np.uint32)
+    lib().qcp_gen_words(_p(out), n, seed, stream, offset)
+    return out[:n]
+
+
+def gen_uniform(n, seed, stream, offset=0):
+    out = np.empty(max(n, 1), np.float32)
+    lib().qcp_gen_uniform(_p(out), n, seed, stream, offset)
+    return out[:n]
+
+
+def gen_normal(n, seed, stream, offset=0):
+    out = np.empty(max(n, 1), np.float64)
+    lib().qcp_gen_normal(_p(out), n, seed, stream, offset)
+    return out[:n]
+
+
+SEED = 1234        # echoes the reference generator's seed (generator/cuRAND.cpp:49)
+CLIP_HI = np.float32(1.0) - np.float32(2.0 ** -24)
+# streams: 0 data, 1 query centres, 2 query sizes, 3 arrivals, 4 mixture centres,
+# 5 mixture components, 6 mixture noise, 7 query components, 8 query noise
+
+
+def boxes_from(centers, sides):
+    """Query boxes lo = c - 0.5f*side, hi = c + 0.5f*side in FP32, not clipped."""
+    c = np.asarray(centers, np.float32)
+    h = np.float32(0.5) * np.asarray(sides, np.float32)
+    return np.ascontiguousarray(np.concatenate([c - h, c + h], axis=1), np.float32)
+
+
+def uniform_points(n, D, seed=SEED, stream=0):
+    return gen_uniform(n * D, seed, stream).reshape(n, D)
+
+
+def mixture_points(n, D, k, sigma, seed=SEED, comp_stream=5, noise_stream=6, centers=None):
+    """n points from a k-component equal-weight Gaussian mixture with the given
+    centres (uniform in [0,1)^D from stream 4 when None), clipped to
+    [0, 1-2^-24]."""
+    if centers is None:
+        centers = gen_uniform(k * D, seed, 4).reshape(k, D)
+    comp = (gen_words(n, seed, comp_stream).astype(np.uint64) * np.uint64(k)) >> np.uint64(32)
+    z = gen_normal(n * D, seed, noise_stream).reshape(n, D)
+    p = (centers[comp].astype(np.float64) + sigma * z).astype(np.float32)
+    np.clip(p, np.float32(0.0), CLIP_HI, out=p)
+    return p, centers
+
+
+def workload(name, scale=1.0, seed=SEED):
+    """Synthetic inputs of one benchmark configuration.  `scale` shrinks N and Q
+    proportionally (parity runs); 1.0 is the full size."""
+    if name == "c1":
+        n, q = int(1048576 * scale), max(1, int(1000 * min(1.0, scale * 4)))
+        pts = uniform_points(n, 2, seed)
+        cen = gen_uniform(q * 2, seed, 1).reshape(q, 2)
+        side = np.full((q, 2), np.float32(np.sqrt(np.float32(1e-4))), np.float32)
+        return {"objs": pts, "queries": boxes_from(cen, side), "dims": 2, "obj_kind": "points",
+                "fanout": 64, "key_mode": "b16", "layout": "mphr"}
+    if name == "c2":
+        n, q = int(16777216 * scale), max(1, int(100000 * scale))
+        pts, centers = mixture_points(n, 3, 64, 0.01, seed)
+        qc, _ = mixture_points(q, 3, 64, 0.01, seed, comp_stream=7, noise_stream=8,
+                               centers=centers)
+        u = gen_uniform(q, seed, 2).astype(np.float64)
+        sel = 10.0 ** (-6.0 + 4.0 * u)
+        side = np.repeat((sel ** (1.0 / 3.0)).astype(np.float32)[:, None], 3, axis=1)
+        return {"objs": pts, "queries": boxes_from(qc, side), "dims": 3, "obj_kind": "points",
+                "fanout": 64, "key_mode": "ref", "layout": "mphr"}
+    if name == "c3":
+        n, q = int(67108864 * scale), max(1, int(1000000 * scale))
+        lo = gen_uniform(n * 2, seed, 0).reshape(n, 2)
+        u = gen_uniform(n * 2, seed, 9).astype(np.float64)
+        side = (-1e-4 * np.log1p(-u)).astype(np.float32).reshape(n, 2)
+        objs = np.concatenate([lo, lo + side], axis=1).astype(np.float32)
+        cen = gen_uniform(q * 2, seed, 1).reshape(q, 2)
+        heavy = gen_uniform(q, seed, 2) < np.float32(0.1)
+        s = np.where(heavy, np.float32(np.sqrt(1e-3)), np.float32(1e-3)).astype(np.float32)
+        return {"objs": objs, "queries": boxes_from(cen, np.repeat(s[:, None], 2, 1)), "dims": 2,
+                "obj_kind": "boxes", "fanout": 128, "key_mode": "ref", "key_src": "center",
+                "layout": "rtree"}
+    if name == "c4":
+        n, q = int(33554432 * scale), max(1, int(100000 * scale))
+        pts = uniform_points(n, 3, seed)
+        cen = gen_uniform(q * 3, seed, 1).reshape(q, 3)
+        u = gen_uniform(q, seed, 2).astype(np.float64)
+        sel = 10.0 ** (-7.0 + 4.0 * u)
+        side = np.repeat((sel ** (1.0 / 3.0)).astype(np.float32)[:, None], 3, axis=1)
+        return {"objs": pts, "queries": boxes_from(cen, side), "dims": 3, "obj_kind": "points",
+                "fanout": 64, "key_mode": "ref", "layout": "mphr"}
+    if name == "c5":
+        n, q = int(134217728 * scale), max(1, int(500000 * scale))
+        nh = int(n * 0.8)
+        hot_c = gen_uniform(16 * 2, seed, 4).reshape(16, 2)
+        w = np.arange(1, 17, dtype=np.float64) ** -1.2
+        cdf = np.cumsum(w / w.sum())
+        u = gen_uniform(nh, seed, 5).astype(np.float64)
+        comp = np.minimum(np.searchsorted(cdf, u, side="right"), 15)
+        z = gen_normal(nh * 2, seed, 6).reshape(nh, 2)
+        hot = (hot_c[comp].astype(np.float64) + 0.002 * z).astype(np.float32)
+        cold = gen_uniform((n - nh) * 2, seed, 0).reshape(n - nh, 2)
+        pts = np.concatenate([hot, cold]).astype(np.float32)
+        np.clip(pts, np.float32(0.0), CLIP_HI, out=pts)
+        pick = (gen_words(q, seed, 7).astype(np.uint64) * np.uint64(n)) >> np.uint64(32)
+        cen = pts[pick.astype(np.int64)]
+        sides = (np.float32(1e-4) + gen_uniform(q * 2, seed, 2).reshape(q, 2) *
+                 np.float32(1e-2 - 1e-4)).astype(np.float32)
+        return {"objs": pts, "queries": boxes_from(cen, sides), "dims": 2, "obj_kind": "points",
+                "fanout": 64, "key_mode": "ref", "layout": "mphr"}
+    raise ValueError(name)

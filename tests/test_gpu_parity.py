@@ -1,0 +1,161 @@
+"""GPU parity: the HIP build and search paths against the CPU oracle, bit-exact.
+
+Each case builds on the GPU through the C ABI (libqcp.so), exports the tree,
+and compares keys, rank permutation, rank-ordered objects, node MBRs and
+topology with the oracle's, then the per-query counts, ID-set hashes and sorted
+rank-ID lists of the persistent form (and the plain MPRS form for packed
+layouts, including its root/node visit counters).
+"""
+import numpy as np
+import pytest
+
+from helpers import (assert_bits_equal, assert_results_equal, rand_boxes, rand_points,
+                     rand_queries)
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_tree(O, objs, D, is_box, F, key_mode, key_src, layout):
+    return O.build_packed(objs, D, is_box, F, O.KEY_REF if key_mode == "ref" else O.KEY_B16,
+                          O.SRC_CENTER if key_src == "center" else O.SRC_LO,
+                          layout="bvh" if layout == "bvh" else "packed")
+
+
+def check_case(gpu, O, objs, D, is_box, F, key_mode="ref", key_src="lo", layout="mphr",
+               nq=200, seed=0, plain=True):
+    tag = "D=%d box=%d F=%d key=%s src=%s layout=%s n=%d" % (D, is_box, F, key_mode, key_src,
+                                                             layout, len(objs))
+    print("case", tag, flush=True)
+    ix = gpu.Index.build(objs, D, fanout=F, layout=layout, key_mode=key_mode, key_src=key_src,
+                         obj_kind="boxes" if is_box else "points")
+    t = _oracle_tree(O, objs, D, is_box, F, key_mode, key_src, layout)
+    ex = ix.export()
+    n = len(objs)
+    assert ex["info"]["n"] == n
+    assert_bits_equal(ex["keys"], t.keys, tag + " sorted keys")
+    assert_bits_equal(ex["perm"], t.perm, tag + " permutation")
+    assert_bits_equal(ex["sobjs"], t.sobjs, tag + " rank-ordered objects")
+    assert ex["info"]["n_nodes"] == t.n_nodes, tag
+    assert_bits_equal(ex["first"], t.first, tag + " node first")
+    assert_bits_equal(ex["count"], t.count, tag + " node count")
+    assert_bits_equal(ex["leaf"], t.leaf, tag + " node leaf flag")
+    assert_bits_equal(ex["boxes"], t.boxes[: t.n_nodes], tag + " node MBRs")
+    queries = rand_queries(nq, D, seed + 17, objs if n else None)
+    exp = t.search_dfs(queries, want_ids=True)
+    got = ix.search(queries, mode="persistent", ids=True)
+    assert_results_equal(got, exp, tag + " persistent")
+    st = got["stats"]
+    assert st["internal_entries"] == int(exp["int_entries"].sum()), tag + " internal entries"
+    assert st["leaf_entries"] == int(exp["leaf_entries"].sum()), tag + " leaf entries"
+    assert st["total_hits"] == int(exp["counts"].sum())
+    if plain and layout != "bvh":
+        mp = t.search_mprs(queries)
+        assert_bits_equal(mp["counts"], exp["counts"], tag + " oracle MPRS vs DFS")
+        pl = ix.search(queries, mode="plain", visits=True, plain_batch=64)
+        assert_bits_equal(pl["counts"], exp["counts"], tag + " plain counts")
+        assert_bits_equal(pl["hashes"], exp["hashes"], tag + " plain hashes")
+        assert_bits_equal(pl["visits"], mp["visits"], tag + " plain visit counters")
+    ix.close()
+    return exp
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("F", [16, 32, 64, 128, 192, 256])
+def test_points_ref_mphr(gpu, O, D, F):
+    pts = rand_points(20000, D, seed=D * 100 + F)
+    check_case(gpu, O, pts, D, False, F)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_points_b16(gpu, O, D):
+    pts = rand_points(30000, D, seed=7 + D, dup_frac=0.1)
+    check_case(gpu, O, pts, D, False, 64, key_mode="b16")
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("src", ["lo", "center"])
+@pytest.mark.parametrize("layout", ["rtree", "bvh"])
+def test_boxes(gpu, O, D, src, layout):
+    objs = rand_boxes(25000, D, seed=11 * D + len(src), mean_side=0.01)
+    check_case(gpu, O, objs, D, True, 128, key_src=src, layout=layout)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("F", [32, 128])
+def test_points_bvh(gpu, O, D, F):
+    pts = rand_points(40000, D, seed=5 + F)
+    check_case(gpu, O, pts, D, False, F, layout="bvh")
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 4095, 4096, 4097])
+def test_edge_sizes(gpu, O, n):
+    pts = rand_points(n, 2, seed=n)
+    check_case(gpu, O, pts, 2, False, 64, nq=64)
+    check_case(gpu, O, pts, 2, False, 64, layout="bvh", nq=64)
+
+
+def test_all_duplicates(gpu, O):
+    pts = np.tile(np.array([[0.25, 0.75]], np.float32), (5000, 1))
+    check_case(gpu, O, pts, 2, False, 64, nq=32)
+
+
+def test_empty_index(gpu):
+    ix = gpu.Index.build(np.zeros((0, 3), np.float32), 3)
+    q = rand_queries(10, 3, 1)
+    r = ix.search(q, ids=True)
+    assert (r["counts"] == 0).all() and len(r["ids"]) == 0
+
+
+def test_empty_query_set(gpu):
+    ix = gpu.Index.build(rand_points(1000, 2, 1), 2)
+    r = ix.search(np.zeros((0, 4), np.float32), ids=True)
+    assert len(r["counts"]) == 0
+
+
+def test_domain_error(gpu):
+    pts = rand_points(100, 2, 3)
+    pts[5, 1] = -0.5
+    with pytest.raises(gpu.QcpError) as e:
+        gpu.Index.build(pts, 2, key_mode="ref")
+    assert e.value.status == -7
+    pts[5, 1] = np.nan
+    with pytest.raises(gpu.QcpError):
+        gpu.Index.build(pts, 2, key_mode="b16")
+
+
+def test_save_load_roundtrip(gpu, O, tmp_path):
+    pts = rand_points(50000, 3, 9)
+    for layout in ("mphr", "bvh"):
+        ix = gpu.Index.build(pts, 3, fanout=64, layout=layout)
+        p = str(tmp_path / ("ix_%s.qcp" % layout))
+        ix.save(p)
+        ix2 = gpu.Index.load(p)
+        q = rand_queries(100, 3, 2, pts)
+        a = ix.search(q, ids=True)
+        b = ix2.search(q, ids=True)
+        assert_results_equal(b, a, "reloaded " + layout)
+        with open(p, "r+b") as f:
+            f.seek(-5, 2)
+            f.write(b"\x00\x01\x02\x03\x04")
+        with pytest.raises(gpu.QcpError) as e:
+            gpu.Index.load(p)
+        assert e.value.status == -5
+
+
+def test_c1_full_size(gpu, O):
+    """C1 at full size: 1,048,576 uniform 2-D points, b16 keys, F=64,
+    1,000 queries of side 0.01 -- counts and sorted ID lists bit-exact."""
+    w = gpu.workload("c1")
+    pts, queries = w["objs"], w["queries"]
+    ix = gpu.Index.build(pts, 2, fanout=64, layout="mphr", key_mode="b16")
+    t = O.build_packed(pts, 2, False, 64, O.KEY_B16)
+    ex = ix.export()
+    assert_bits_equal(ex["keys"], t.keys, "c1 keys")
+    assert_bits_equal(ex["perm"], t.perm, "c1 perm")
+    assert_bits_equal(ex["boxes"], t.boxes, "c1 node MBRs")
+    exp = t.search_dfs(queries, want_ids=True)
+    for mode in ("persistent",):
+        got = ix.search(queries, mode=mode, ids=True)
+        assert_results_equal(got, exp, "c1 " + mode)
+    pl = ix.search(queries, mode="plain")
+    assert_bits_equal(pl["counts"], exp["counts"], "c1 plain counts")
