@@ -48,6 +48,7 @@ namespace {
 constexpr uint64_t kObjSlack = 64 * 16;   // entries read past the end by masked chunks
 constexpr uint64_t kNodeSlack = 256 + 64;
 constexpr uint32_t kMaxScanBlocks = 4096;
+constexpr uint32_t kSlotCap = 1u << 21;   // donated work items per launch
 
 template <typename T>
 qcp_status dalloc(T** p, uint64_t count) {
@@ -114,6 +115,7 @@ struct qcp_batch {
   QueueCtl* d_ctl = nullptr;
   unsigned long long* d_slots = nullptr;
   uint32_t slot_cap = 0;
+  uint64_t slots_dirty = 0;       // slots written by the last donating run
   uint32_t* h_counts = nullptr;   // pinned, plain-form per-batch copies
   uint32_t* h_visits = nullptr;
   hipEvent_t ev[4] = {};
@@ -607,8 +609,8 @@ static SearchArgs make_args(const qcp_batch* b) {
   return a;
 }
 
-static unsigned persistent_grid(const qcp_index* ix, bool ids, unsigned block) {
-  const void* k = search_kernel_ptr(ix->D, ix->box, ix->expl, ids, (int)((ix->F + 63) / 64));
+static unsigned persistent_grid(const qcp_index* ix, int out, unsigned block) {
+  const void* k = search_kernel_ptr(ix->D, ix->box, ix->expl, out, (int)((ix->F + 63) / 64));
   int per_cu = 0;
   if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)block, 0) != hipSuccess ||
       per_cu <= 0)
@@ -669,11 +671,36 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     return QCP_OK;
   }
   // persistent form
+  if (q == 0) {
+    b->total_hits = 0;
+    b->has_offsets = true;
+    if (stats) *stats = st;
+    return QCP_OK;
+  }
   const unsigned block = 256;
-  unsigned grid = cfg.grid_blocks ? cfg.grid_blocks : persistent_grid(ix, false, block);
+  const int out_mode = (cfg.outputs & QCP_OUT_HASH) ? 1 : 0;
+  const int u = cfg.leaf_chunks ? (int)cfg.leaf_chunks : 4;
+  const bool donate = cfg.donate != 0;
+  unsigned grid = cfg.grid_blocks ? cfg.grid_blocks : persistent_grid(ix, out_mode, block);
   SearchArgs a = make_args(b);
+  if (donate) {
+    if (!b->d_slots) {
+      b->slot_cap = kSlotCap;
+      QCP_TRY(dalloc(&b->d_slots, b->slot_cap));
+      b->slots_dirty = b->slot_cap;
+    }
+    a.slots = b->d_slots;
+    a.slot_cap = b->slot_cap;
+    a.donate = 1;
+  }
   HIP_TRY(hipEventRecord(b->ev[0], s));
-  HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, false, grid, block, s));
+  if (donate) {
+    // per-item results are accumulated atomically; donated slots start empty
+    HIP_TRY(launch_fill_u32(b->d_counts, q, 0u, s));
+    HIP_TRY(launch_fill_u32((uint32_t*)b->d_hashes, 2 * q, 0u, s));
+    HIP_TRY(launch_fill_u32((uint32_t*)b->d_slots, 2ull * b->slots_dirty, 0xFFFFFFFFu, s));
+  }
+  HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, out_mode, u, grid, block, s));
   HIP_TRY(hipEventRecord(b->ev[1], s));
   st.launches = 1;
   QueueCtl ctl;
@@ -689,9 +716,10 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
       b->ids_cap = total + 64;
     }
     a.ids = b->d_ids;
-    unsigned grid2 = cfg.grid_blocks ? cfg.grid_blocks : persistent_grid(ix, true, block);
+    a.donate = 0;
+    unsigned grid2 = cfg.grid_blocks ? cfg.grid_blocks : persistent_grid(ix, 2, block);
     HIP_TRY(hipEventRecord(b->ev[2], s));
-    HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, true, grid2, block, s));
+    HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, 2, 4, grid2, block, s));
     HIP_TRY(hipEventRecord(b->ev[3], s));
     st.launches = 2;
     b->has_offsets = true;
@@ -713,6 +741,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   st.total_hits = ctl.total_hits;
   st.donated_items = ctl.donated;
   b->total_hits = ctl.total_hits;
+  if (donate) b->slots_dirty = std::min<uint64_t>(b->slot_cap, std::max<uint64_t>(ctl.tail, 1));
   if (stats) *stats = st;
   return QCP_OK;
 }

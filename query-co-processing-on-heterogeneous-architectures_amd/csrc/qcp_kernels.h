@@ -37,10 +37,12 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s);
 
 // search
-hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, bool ids,
+// out: 0 counts, 1 counts + ID-set hash, 2 sorted ID lists (needs offsets);
+// u: 64-entry leaf chunks in flight per wave (2, 4 or 8)
+hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, int out, int u,
                          unsigned grid, unsigned block, hipStream_t s);
 hipError_t launch_mprs(const SearchArgs& a, unsigned D, bool box, unsigned nblocks,
                        hipStream_t s);
-const void* search_kernel_ptr(unsigned D, bool box, bool expl, bool ids, int mw);
+const void* search_kernel_ptr(unsigned D, bool box, bool expl, int out, int mw);
 
 }  // namespace qcp

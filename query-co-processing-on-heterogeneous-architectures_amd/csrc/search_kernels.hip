@@ -133,10 +133,13 @@ __device__ __forceinline__ void eval_children(const SearchArgs& a, const Ctx<D>&
   }
 }
 
+// Output modes of the persistent form.
+enum { kOutCount = 0, kOutHash = 1, kOutIds = 2 };
+
 // Stream the leaf runs at the bottom of mask m (bits that are also set in lm),
 // U 64-entry chunks in flight, stopping at the first non-leaf set bit.
 // run_range(k, k2, &s, &e) maps a run of children [k, k2) to its rank range.
-template <int D, bool BOX, int MW, bool IDS, int U, typename RunRange>
+template <int D, bool BOX, int MW, int OUT, int U, typename RunRange>
 __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<MW>& m,
                                           const Mask<MW>& lm, RunRange run_range) {
   constexpr int E = BOX ? 2 * D : D;
@@ -179,10 +182,10 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
         bool hit = (lane < len[u]) && overlaps<D>(c.ql, c.qh, lo, hi);
         uint64_t b = ballot(hit);
         uint32_t nb = (uint32_t)__popcll(b);
-        if (IDS) {
+        if (OUT == kOutIds) {
           if (hit) a.ids[c.out_base + rank_below(b)] = (uint32_t)(st[u] + lane + 1);
           c.out_base += nb;
-        } else {
+        } else if (OUT == kOutHash) {
           if (hit) c.h_lane += id_hash((uint32_t)(st[u] + lane + 1));
         }
         c.cnt += nb;
@@ -207,14 +210,76 @@ __device__ __forceinline__ int level_of(const SearchArgs& a, uint32_t node) {
   return l;
 }
 
+__device__ __forceinline__ uint32_t ld_acq(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_acq64(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr unsigned long long kEmptySlot = ~0ull;
+constexpr int kHungerPeriod = 4;   // internal-node descents between queue checks
+
+// Donate the unvisited children of one frame to the device queue: each set bit
+// of `m` becomes an item (query, child node).  Bits that do not fit in the
+// slot array stay in `m`.  pending is raised before the slots are written, and
+// the donor's own item keeps pending above zero meanwhile.
+template <int MW, bool EXPL>
+__device__ __forceinline__ void donate(const SearchArgs& a, uint32_t q, uint32_t node, int level,
+                                       Mask<MW>& m) {
+  const uint32_t lane = lane_id();
+  uint32_t n = 0;
+#pragma unroll
+  for (int w = 0; w < MW; ++w) n += (uint32_t)__popcll(m.w[w]);
+  if (n == 0) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(&a.ctl->tail, n);
+  base = uni(base);
+  const uint32_t ok = base >= a.slot_cap ? 0u : (a.slot_cap - base < n ? a.slot_cap - base : n);
+  if (ok == 0) return;
+  if (lane == 0) {
+    atomicAdd(&a.ctl->pending, ok);
+    atomicAdd(&a.ctl->donated, (unsigned long long)ok);
+  }
+  uint32_t cbase;
+  if (EXPL) {
+    cbase = a.nfirst[node];
+  } else {
+    const uint32_t j = node - a.lvl_off[level];
+    cbase = a.lvl_off[level + 1] + j * a.F;
+  }
+  uint32_t before = 0;
+#pragma unroll
+  for (int w = 0; w < MW; ++w) {
+    const uint64_t bits = m.w[w];
+    const bool mine = (bits >> lane) & 1ull;
+    const uint32_t r = before + rank_below(bits);
+    if (mine && r < ok) {
+      const unsigned long long item =
+          ((unsigned long long)q << 32) | (unsigned long long)(cbase + w * kWave + lane);
+      __hip_atomic_store(&a.slots[base + r], item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // clear the donated bits (the lowest `ok` set bits overall)
+    uint64_t keep = bits;
+    uint32_t take = ok > before ? ok - before : 0;
+    const uint32_t pc = (uint32_t)__popcll(bits);
+    if (take >= pc) {
+      keep = 0;
+    } else {
+      for (uint32_t t = 0; t < take; ++t) keep &= keep - 1;
+    }
+    m.w[w] = keep;
+    before += pc;
+  }
+}
+
 // Canonical top-down traversal of the subtree rooted at `node`.
-template <int D, bool BOX, int MW, bool EXPL, bool IDS, int U>
+template <int D, bool BOX, int MW, bool EXPL, int OUT, int U>
 __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* stk) {
   const uint32_t F = a.F;
   int level = EXPL ? 0 : level_of(a, node);
-  // --- node metadata
-  auto packed_range = [&](int lv, uint32_t j, int k, int k2, uint64_t& s, uint64_t& e) {
-    // children of (lv, j) are leaves: child index j*F+k .. covers ranks
+  auto packed_range = [&](uint32_t j, int k, int k2, uint64_t& s, uint64_t& e) {
+    // children of packed node j at the leaf-parent level are leaves jF+k ..
     uint64_t c0 = (uint64_t)j * F + (uint64_t)k;
     uint64_t c1 = (uint64_t)j * F + (uint64_t)k2;
     s = c0 * F;
@@ -240,7 +305,7 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
     for (int i = 0; i < MW; ++i) { m.w[i] = 0; lm.w[i] = ~0ull; }
     m.w[0] = 1;  // a single pseudo-run covering this leaf's entries
     uint64_t s0 = first, e0 = (uint64_t)first + cnt;
-    scan_runs<D, BOX, MW, IDS, U>(a, c, m, lm,
+    scan_runs<D, BOX, MW, OUT, U>(a, c, m, lm,
                                   [&](int, int, uint64_t& s, uint64_t& e) { s = s0; e = e0; });
     return;
   }
@@ -250,18 +315,19 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
   c.ie += cnt;
   uint32_t cur = node;
   int sp = 0;
+  int since = 0;
   while (true) {
     // leaf children at the bottom of the mask are streamed right away
     if (EXPL) {
       const uint32_t cf = a.nfirst[cur];
-      scan_runs<D, BOX, MW, IDS, U>(a, c, m, lm, [&](int k, int k2, uint64_t& s, uint64_t& e) {
+      scan_runs<D, BOX, MW, OUT, U>(a, c, m, lm, [&](int k, int k2, uint64_t& s, uint64_t& e) {
         s = a.nfirst[cf + k];
         e = (uint64_t)a.nfirst[cf + k2 - 1] + (a.ncount[cf + k2 - 1] & ~kLeafBit);
       });
     } else if (level + 1 == a.h - 1) {
       const uint32_t j = cur - a.lvl_off[level];
-      scan_runs<D, BOX, MW, IDS, U>(a, c, m, lm, [&](int k, int k2, uint64_t& s, uint64_t& e) {
-        packed_range(level, j, k, k2, s, e);
+      scan_runs<D, BOX, MW, OUT, U>(a, c, m, lm, [&](int k, int k2, uint64_t& s, uint64_t& e) {
+        packed_range(j, k, k2, s, e);
       });
     }
     if (m.empty()) {
@@ -275,6 +341,24 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
     }
     const int k = m.lowest();
     m.clear(k);
+    // work stealing: when waves wait for items, hand over the unvisited
+    // siblings of the shallowest frame that has any
+    if (OUT != kOutIds && a.donate && ++since >= kHungerPeriod) {
+      since = 0;
+      const uint32_t head = ld_acq(&a.ctl->head);
+      const uint32_t tail = ld_acq(&a.ctl->tail);
+      if (head > a.nq + tail && tail < a.slot_cap) {
+        int f = 0;
+        while (f < sp && stk[f].m.empty()) ++f;
+        if (f < sp) {
+          Mask<MW> dm = stk[f].m;
+          donate<MW, EXPL>(a, c.q, stk[f].node, (int)stk[f].level, dm);
+          stk[f].m = dm;
+        } else if (!m.empty()) {
+          donate<MW, EXPL>(a, c.q, cur, level, m);
+        }
+      }
+    }
     uint32_t child, cfirst, ccnt;
     if (EXPL) {
       child = a.nfirst[cur] + k;
@@ -307,34 +391,68 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
   }
 }
 
-template <int D, bool BOX, int MW, bool EXPL, bool IDS, int U>
+// Persistent search: tickets below nq are the root items of query t; with
+// donation on, ticket nq+s is donated slot s, waited for until it is written
+// or all work is done.
+template <int D, bool BOX, int MW, bool EXPL, int OUT, int U>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   constexpr int kDepth = EXPL ? kMaxDepth : kMaxLevels;
   __shared__ Frame<MW> stack[256 / kWave][kDepth];
   Frame<MW>* stk = stack[threadIdx.x / kWave];
   const uint32_t lane = lane_id();
   uint64_t ie = 0, le = 0, hits = 0;
+  const bool donating = OUT != kOutIds && a.donate;
   while (true) {
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(&a.ctl->head, 1u);
     t = uni(t);
-    if (t >= a.nq) break;
-    const uint32_t q = t;
+    uint32_t q, node = 0;
+    if (t < a.nq) {
+      q = t;
+    } else {
+      if (!donating) break;
+      const uint32_t s = t - a.nq;
+      unsigned long long item = kEmptySlot;
+      uint32_t nap = 1;
+      while (true) {
+        if (s < a.slot_cap) {
+          item = ld_acq64(&a.slots[s]);
+          if (item != kEmptySlot) break;
+        }
+        if (ld_acq(&a.ctl->done)) break;
+        __builtin_amdgcn_s_sleep(4);
+        if (nap < 64) nap *= 2;
+        for (uint32_t z = 0; z < nap; z += 8) __builtin_amdgcn_s_sleep(8);
+      }
+      item = uni64(item);
+      if (item == kEmptySlot) break;
+      q = (uint32_t)(item >> 32);
+      node = (uint32_t)item;
+    }
     Ctx<D> c;
     c.q = q;
     c.cnt = 0;
     c.h_lane = 0;
     c.ie = 0;
     c.le = 0;
-    c.out_base = IDS ? a.offsets[q] : 0;
+    c.out_base = (OUT == kOutIds) ? a.offsets[q] : 0;
     load_query<D>(a, q, c);
-    if (a.h > 0) walk<D, BOX, MW, EXPL, IDS, U>(a, c, 0u, stk);
-    if (!IDS) {
-      uint64_t hsum = wave_sum_u64(c.h_lane);
+    if (a.h > 0) walk<D, BOX, MW, EXPL, OUT, U>(a, c, node, stk);
+    if (OUT != kOutIds) {
+      uint64_t hsum = OUT == kOutHash ? wave_sum_u64(c.h_lane) : 0;
       if (lane == 0) {
-        a.counts[q] = c.cnt;
-        a.hashes[q] = hsum;
+        if (donating) {
+          if (c.cnt) atomicAdd(&a.counts[q], c.cnt);
+          if (OUT == kOutHash && hsum) atomicAdd(&a.hashes[q], (unsigned long long)hsum);
+        } else {
+          a.counts[q] = c.cnt;
+          if (OUT == kOutHash) a.hashes[q] = hsum;
+        }
       }
+    }
+    if (donating && lane == 0) {
+      if (atomicSub(&a.ctl->pending, 1u) == 1u)
+        __hip_atomic_store(&a.ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     ie += c.ie;
     le += c.le;
@@ -449,7 +567,7 @@ __global__ void k_reset_ctl(QueueCtl* ctl, uint32_t pending) {
     ctl->head = 0;
     ctl->tail = 0;
     ctl->pending = pending;
-    ctl->done = 0;
+    ctl->done = pending == 0 ? 1u : 0u;
     ctl->internal_entries = 0;
     ctl->leaf_entries = 0;
     ctl->donated = 0;
@@ -467,33 +585,51 @@ __global__ void k_reset_ctl(QueueCtl* ctl, uint32_t pending) {
     default: return hipErrorInvalidValue;              \
   }
 
-template <int D, bool BOX, bool EXPL, bool IDS>
-static hipError_t launch_search_t(const SearchArgs& a, int mw, unsigned grid, unsigned block,
-                                  hipStream_t s) {
-  constexpr int U = 4;
-  QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, IDS, U><<<grid, block, 0, s>>>(a));
+// Chunks in flight: 2/4/8 for the packed counting modes, 4 elsewhere.
+#define QCP_U_SWITCH(UV, ...)                          \
+  switch (UV) {                                        \
+    case 2: { constexpr int U = 2; __VA_ARGS__; } break; \
+    case 8: { constexpr int U = 8; __VA_ARGS__; } break; \
+    default: { constexpr int U = 4; __VA_ARGS__; } break; \
+  }
+
+template <int D, bool BOX, bool EXPL, int OUT>
+static hipError_t launch_search_t(const SearchArgs& a, int mw, int u, unsigned grid,
+                                  unsigned block, hipStream_t s) {
+  if (!EXPL && OUT != kOutIds) {
+    QCP_U_SWITCH(u, QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, OUT, U><<<grid, block, 0, s>>>(a)));
+  } else {
+    constexpr int U = 4;
+    QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, OUT, U><<<grid, block, 0, s>>>(a));
+  }
   return hipGetLastError();
 }
 
 template <int D, bool BOX>
-static hipError_t launch_search_d(const SearchArgs& a, bool expl, bool ids, unsigned grid,
+static hipError_t launch_search_d(const SearchArgs& a, bool expl, int out, int u, unsigned grid,
                                   unsigned block, hipStream_t s) {
-  int mw = (int)((a.F + 63) / 64);
+  const int mw = (int)((a.F + 63) / 64);
   if (expl) {
-    return ids ? launch_search_t<D, BOX, true, true>(a, mw, grid, block, s)
-               : launch_search_t<D, BOX, true, false>(a, mw, grid, block, s);
+    switch (out) {
+      case kOutCount: return launch_search_t<D, BOX, true, kOutCount>(a, mw, u, grid, block, s);
+      case kOutHash: return launch_search_t<D, BOX, true, kOutHash>(a, mw, u, grid, block, s);
+      default: return launch_search_t<D, BOX, true, kOutIds>(a, mw, u, grid, block, s);
+    }
   }
-  return ids ? launch_search_t<D, BOX, false, true>(a, mw, grid, block, s)
-             : launch_search_t<D, BOX, false, false>(a, mw, grid, block, s);
+  switch (out) {
+    case kOutCount: return launch_search_t<D, BOX, false, kOutCount>(a, mw, u, grid, block, s);
+    case kOutHash: return launch_search_t<D, BOX, false, kOutHash>(a, mw, u, grid, block, s);
+    default: return launch_search_t<D, BOX, false, kOutIds>(a, mw, u, grid, block, s);
+  }
 }
 
-hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, bool ids,
+hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, int out, int u,
                          unsigned grid, unsigned block, hipStream_t s) {
   k_reset_ctl<<<1, 64, 0, s>>>(a.ctl, a.nq);
-  if (D == 2) return box ? launch_search_d<2, true>(a, expl, ids, grid, block, s)
-                         : launch_search_d<2, false>(a, expl, ids, grid, block, s);
-  if (D == 3) return box ? launch_search_d<3, true>(a, expl, ids, grid, block, s)
-                         : launch_search_d<3, false>(a, expl, ids, grid, block, s);
+  if (D == 2) return box ? launch_search_d<2, true>(a, expl, out, u, grid, block, s)
+                         : launch_search_d<2, false>(a, expl, out, u, grid, block, s);
+  if (D == 3) return box ? launch_search_d<3, true>(a, expl, out, u, grid, block, s)
+                         : launch_search_d<3, false>(a, expl, out, u, grid, block, s);
   return hipErrorInvalidValue;
 }
 
@@ -512,23 +648,28 @@ hipError_t launch_mprs(const SearchArgs& a, unsigned D, bool box,
   return hipGetLastError();
 }
 
-const void* search_kernel_ptr(unsigned D, bool box, bool expl, bool ids, int mw) {
-  // used for occupancy queries
-#define QCP_PTR(DD, BB, EE, II)                                              \
+const void* search_kernel_ptr(unsigned D, bool box, bool expl, int out, int mw) {
+  // the U=4 instantiation, for occupancy queries
+#define QCP_PTR(DD, BB, EE, OO)                                              \
   switch (mw) {                                                              \
-    case 1: return (const void*)&k_search<DD, BB, 1, EE, II, 4>;             \
-    case 2: return (const void*)&k_search<DD, BB, 2, EE, II, 4>;             \
-    case 3: return (const void*)&k_search<DD, BB, 3, EE, II, 4>;             \
-    case 4: return (const void*)&k_search<DD, BB, 4, EE, II, 4>;             \
+    case 1: return (const void*)&k_search<DD, BB, 1, EE, OO, 4>;             \
+    case 2: return (const void*)&k_search<DD, BB, 2, EE, OO, 4>;             \
+    case 3: return (const void*)&k_search<DD, BB, 3, EE, OO, 4>;             \
+    case 4: return (const void*)&k_search<DD, BB, 4, EE, OO, 4>;             \
     default: return nullptr;                                                 \
   }
-#define QCP_PTR_E(DD, BB)                                                        \
-  if (expl) { if (ids) { QCP_PTR(DD, BB, true, true) } else { QCP_PTR(DD, BB, true, false) } } \
-  else { if (ids) { QCP_PTR(DD, BB, false, true) } else { QCP_PTR(DD, BB, false, false) } }
+#define QCP_PTR_O(DD, BB, EE)                                                   \
+  switch (out) {                                                                \
+    case kOutCount: { QCP_PTR(DD, BB, EE, kOutCount) }                          \
+    case kOutHash: { QCP_PTR(DD, BB, EE, kOutHash) }                            \
+    default: { QCP_PTR(DD, BB, EE, kOutIds) }                                   \
+  }
+#define QCP_PTR_E(DD, BB) if (expl) { QCP_PTR_O(DD, BB, true) } else { QCP_PTR_O(DD, BB, false) }
   if (D == 2) { if (box) { QCP_PTR_E(2, true) } else { QCP_PTR_E(2, false) } }
   if (D == 3) { if (box) { QCP_PTR_E(3, true) } else { QCP_PTR_E(3, false) } }
   return nullptr;
 #undef QCP_PTR
+#undef QCP_PTR_O
 #undef QCP_PTR_E
 }
 

@@ -194,10 +194,10 @@ class Index:
         return Batch(self, queries)
 
     def search(self, queries, mode="persistent", ids=False, visits=False, plain_batch=128,
-               grid_blocks=0, donate=0):
+               grid_blocks=0, donate=1, chunks=0, hash=True):
         with self.batch(queries) as b:
             st = b.run(mode=mode, ids=ids, plain_batch=plain_batch, grid_blocks=grid_blocks,
-                       donate=donate)
+                       donate=donate, chunks=chunks, hash=hash)
             res = b.results(ids=ids, visits=visits)
             res["stats"] = st
             return res
@@ -231,10 +231,13 @@ class Batch:
         except Exception:
             pass
 
-    def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=0,
-            hash=True):
+    def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=1,
+            hash=True, chunks=0):
+        """One search of the resident batch.  donate=1 enables subtree donation
+        (work stealing) in the persistent form; chunks = 64-entry leaf chunks in
+        flight per wave (0 = default)."""
         outputs = OUT_COUNTS | (OUT_HASH if hash else 0) | (OUT_IDS if ids else 0)
-        cfg = SearchCfg(MODE[mode], outputs, plain_batch, 0, grid_blocks, 0, donate, 0)
+        cfg = SearchCfg(MODE[mode], outputs, plain_batch, 0, grid_blocks, chunks, donate, 0)
         st = SearchStats()
         check(lib().qcp_batch_run(self._h, ctypes.byref(cfg), ctypes.byref(st)), "qcp_batch_run")
         self._ids = ids

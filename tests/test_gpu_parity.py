@@ -48,6 +48,11 @@ def check_case(gpu, O, objs, D, is_box, F, key_mode="ref", key_src="lo", layout=
     assert st["internal_entries"] == int(exp["int_entries"].sum()), tag + " internal entries"
     assert st["leaf_entries"] == int(exp["leaf_entries"].sum()), tag + " leaf entries"
     assert st["total_hits"] == int(exp["counts"].sum())
+    # the other persistent variants: no donation, counts only, 2 and 8 chunks in flight
+    nd = ix.search(queries, mode="persistent", donate=0, chunks=8)
+    assert_results_equal(nd, exp, tag + " persistent no-donation", ids=False)
+    co = ix.search(queries, mode="persistent", hash=False, chunks=2)
+    assert_bits_equal(co["counts"], exp["counts"], tag + " counts-only")
     if plain and layout != "bvh":
         mp = t.search_mprs(queries)
         assert_bits_equal(mp["counts"], exp["counts"], tag + " oracle MPRS vs DFS")
@@ -97,6 +102,25 @@ def test_edge_sizes(gpu, O, n):
 def test_all_duplicates(gpu, O):
     pts = np.tile(np.array([[0.25, 0.75]], np.float32), (5000, 1))
     check_case(gpu, O, pts, 2, False, 64, nq=32)
+
+
+@pytest.mark.parametrize("layout,F", [("mphr", 16), ("mphr", 64), ("bvh", 128), ("rtree", 32)])
+def test_donation_heavy_queries(gpu, O, layout, F):
+    """A few very large queries and thousands of waves: idle waves take
+    donated subtrees; results must not change."""
+    pts = rand_points(300000, 2, seed=F)
+    rng = np.random.default_rng(F)
+    c = rng.random((6, 2), dtype=np.float32)
+    s = np.float32(0.8)
+    q = np.concatenate([c - s / 2, c + s / 2], 1).astype(np.float32)
+    ix = gpu.Index.build(pts, 2, fanout=F, layout=layout)
+    t = _oracle_tree(O, pts, 2, False, F, "ref", "lo", layout)
+    exp = t.search_dfs(q, want_ids=True)
+    for rep in range(3):
+        got = ix.search(q, donate=1, ids=True)
+        assert_results_equal(got, exp, "donation %s F=%d rep %d" % (layout, F, rep))
+        assert got["stats"]["donated_items"] > 0
+        assert got["stats"]["leaf_entries"] == int(exp["leaf_entries"].sum())
 
 
 def test_empty_index(gpu):
