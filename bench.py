@@ -77,6 +77,39 @@ def cpu_baseline(qcp, ix, queries, got, budget_s, threads):
             "matches_gpu": ok}
 
 
+def serving_bench(qcp, rates, seconds, scale):
+    """The serving config: 33,554,432 uniform 3-D points resident (MPHR, F=64);
+    Poisson arrivals at each rate for `seconds`; each query a box of volume
+    selectivity log-uniform in [1e-7, 1e-3]; continuous batching over 4 slots.
+    Latency = submit -> result on the host."""
+    nmax = int(max(rates) * seconds)
+    w = qcp.workload("c4", scale=scale, nq=nmax)
+    ix = qcp.Index.build(w["objs"], 3, fanout=64)
+    queries = w["queries"]
+    ref = ix.search(queries[:2000], donate=0)
+    out = []
+    for r in rates:
+        n = int(r * seconds)
+        arr = qcp.poisson_arrivals(n, float(r), stream=3)
+        with qcp.Server(ix, slots=4, max_batch=4096) as sv:
+            counts, hashes, lat = sv.replay(queries[:n], arr)
+            st = sv.stats()
+        span = float(np.max(arr + lat * 1e-6))
+        ok = bool(np.array_equal(counts[:2000], ref["counts"][: min(n, 2000)]) and
+                  np.array_equal(hashes[:2000], ref["hashes"][: min(n, 2000)]))
+        out.append({"qps_offered": r, "queries": n, "p50_ms": st["p50_us"] / 1e3,
+                    "p99_ms": st["p99_us"] / 1e3, "max_ms": st["max_us"] / 1e3,
+                    "throughput_qps": n / span, "batches": st["batches"],
+                    "mean_batch": n / max(st["batches"], 1),
+                    "max_inflight": st["max_inflight_seen"], "matches_batch_api": ok})
+        log("serving %6d qps: p50 %.3f ms p99 %.3f ms, %d batches" %
+            (r, st["p50_us"] / 1e3, st["p99_us"] / 1e3, st["batches"]))
+    ix.close()
+    return {"workload": "33,554,432 uniform 3-D points, Poisson arrivals, selectivity "
+                        "log-uniform [1e-7,1e-3], counts+hash", "seconds_per_rate": seconds,
+            "rates": out}
+
+
 def read_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
@@ -98,6 +131,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the box has one); --gpus %d is not "
@@ -166,6 +201,11 @@ def main():
         threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(qcp, ix, queries, got, args.cpu_budget, threads)
         log("cpu baseline: %.3e queries/s (%s)" % (cpu["value"], cpu["sample"]))
+    serving = None
+    if not args.no_serve:
+        b.close()
+        ix.close()
+        serving = serving_bench(qcp, (10000, 50000, 100000), args.serve_seconds, args.scale)
     traffic = read_traffic("traffic_c2_search.json")
     kname = "k_search"
     out = {
@@ -194,9 +234,11 @@ def main():
         "plain": plain,
         "speedup_vs_plain": (value / plain["queries_per_s"]) if plain else None,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
-    b.close()
-    ix.close()
+    if serving is None:
+        b.close()
+        ix.close()
     print(json.dumps(out), flush=True)
 
 

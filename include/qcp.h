@@ -178,6 +178,48 @@ qcp_status qcp_search(qcp_index *ix, const float *boxes, uint64_t q,
                       uint64_t *offsets, uint32_t *ids, uint64_t ids_cap,
                       qcp_search_stats *stats);
 
+/* ---- serving: continuous batching over a resident index -----------------------
+ * New (the reference has no serving path; its closest interface is the
+ * per-batch MPHR::Search loop, src/tree/mphr.cpp:231-258).  A server keeps the
+ * index resident and runs a dispatcher thread: requests are queued as they are
+ * submitted, and whenever one of `slots` in-flight batch slots is free and the
+ * result-buffer budget allows, the waiting requests (up to max_batch) are
+ * admitted as one batch on that slot's HIP stream, so batches that arrive while
+ * others are on the GPU overlap on the device.  Outputs: count + ID-set hash. */
+typedef struct qcp_server qcp_server;
+typedef struct {
+  uint32_t slots;              /* in-flight batches (HIP streams), default 4          */
+  uint32_t max_batch;          /* queries per batch, default 4096                     */
+  uint64_t result_budget;      /* bytes of result buffers in flight, default 64 MiB   */
+  uint32_t max_requests;       /* result-table capacity (tickets), default 1 << 22    */
+  uint32_t grid_blocks;        /* per-batch grid cap (0 = fill the device)            */
+  uint32_t reserved[2];
+} qcp_serve_cfg;
+typedef struct {
+  uint64_t submitted, completed, batches;
+  uint64_t deferred_by_budget; /* admissions postponed because the budget was full   */
+  uint64_t max_batch_seen, max_inflight_seen;
+  double p50_us, p99_us, mean_us, max_us;  /* submit -> result latency           */
+} qcp_serve_stats;
+
+qcp_status qcp_server_start(qcp_index *ix, const qcp_serve_cfg *cfg, qcp_server **out);
+/* Thread-safe, non-blocking: queue one query box; *ticket identifies it. */
+qcp_status qcp_server_submit(qcp_server *sv, const float *box, uint64_t *ticket);
+/* QCP_OK with the result once done, QCP_EBUSY while pending. */
+qcp_status qcp_server_result(qcp_server *sv, uint64_t ticket, uint32_t *count, uint64_t *hash,
+                             double *latency_us);
+/* Block until the ticket is done. */
+qcp_status qcp_server_wait(qcp_server *sv, uint64_t ticket, uint32_t *count, uint64_t *hash,
+                           double *latency_us);
+qcp_status qcp_server_stats(qcp_server *sv, qcp_serve_stats *out);
+/* Drain outstanding work and stop the dispatcher. */
+qcp_status qcp_server_stop(qcp_server *sv);
+/* Load driver: submits query i at start + arrival_s[i] (arrival_s ascending,
+ * seconds) from a native thread, waits for all, and returns per-query results
+ * and submit-to-result latencies. */
+qcp_status qcp_server_replay(qcp_server *sv, const float *boxes, const double *arrival_s,
+                             uint64_t n, uint32_t *counts, uint64_t *hashes, double *latency_us);
+
 /* ---- seeded generator (replaces generator/cuRAND.cpp:17-75, without cuRAND) ----
  * Philox4x32-10 (Salmon et al., SC'11).  Word i of (seed, stream) is output
  * word i%4 of philox(counter = {i/4 low, i/4 high, seed high, 0},

@@ -18,6 +18,7 @@
 
 #include "../../include/qcp.h"
 #include "bvh_topology.h"
+#include "qcp_internal.h"
 #include "qcp_kernels.h"
 
 using namespace qcp;
@@ -67,63 +68,6 @@ void dfree(T*& p) {
 }
 
 }  // namespace
-
-struct qcp_index {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  qcp_build_cfg cfg{};
-  uint32_t D = 0, F = 0, E = 0, nb = 0;
-  bool box = false, expl = false;
-  uint64_t n = 0;
-  int h = 0;
-  std::vector<uint64_t> level_count;  // root first
-  std::vector<uint64_t> level_off;    // h+1 entries
-  uint64_t n_nodes = 0;
-  uint64_t obj_stride = 0, node_stride = 0;
-  // device
-  float* d_obj = nullptr;        // [E][obj_stride]
-  float* d_nbox = nullptr;       // [2D][node_stride]
-  uint32_t* d_nfirst = nullptr;  // explicit layouts
-  uint32_t* d_ncount = nullptr;
-  uint64_t* d_keys = nullptr;    // sorted keys [n]
-  uint32_t* d_perm = nullptr;    // [n]
-  // build scratch
-  uint64_t* d_keys2 = nullptr;
-  uint32_t* d_perm2 = nullptr;
-  uint32_t* d_hist = nullptr;
-  uint32_t* d_hist_scan = nullptr;
-  uint32_t* d_block_sums = nullptr;
-  unsigned long long* d_key_or = nullptr;
-  uint32_t* d_bad = nullptr;
-  float* d_in = nullptr;         // device copy of host input
-  hipEvent_t ev[8] = {};
-  int cus = 0;
-  qcp_index_info info{};
-};
-
-struct qcp_batch {
-  qcp_index* ix = nullptr;
-  uint64_t q = 0;
-  float* d_queries = nullptr;
-  uint32_t* d_counts = nullptr;
-  unsigned long long* d_hashes = nullptr;
-  unsigned long long* d_offsets = nullptr;
-  unsigned long long* d_scan_sums = nullptr;
-  uint32_t* d_ids = nullptr;
-  uint64_t ids_cap = 0;
-  uint32_t* d_visits = nullptr;
-  QueueCtl* d_ctl = nullptr;
-  unsigned long long* d_slots = nullptr;
-  uint32_t slot_cap = 0;
-  uint64_t slots_dirty = 0;       // slots written by the last donating run
-  uint32_t* h_counts = nullptr;   // pinned, plain-form per-batch copies
-  uint32_t* h_visits = nullptr;
-  hipEvent_t ev[4] = {};
-  uint32_t last_outputs = 0;
-  uint32_t last_mode = 0;
-  uint64_t total_hits = 0;
-  bool has_offsets = false;
-};
 
 static qcp_status set_device(int dev) {
   int cnt = 0;
@@ -573,19 +517,21 @@ qcp_status qcp_batch_create_device(qcp_index* ix, const float* d_boxes, uint64_t
   return QCP_OK;
 }
 
-static SearchArgs make_args(const qcp_batch* b) {
-  const qcp_index* ix = b->ix;
+}  // extern "C"
+
+SearchArgs qcp::index_search_args(const qcp_index* ix, const float* d_queries, uint64_t nq,
+                                 uint32_t* d_counts, unsigned long long* d_hashes) {
   SearchArgs a;
   memset(&a, 0, sizeof(a));
   a.obj = ix->d_obj;
   a.nbox = ix->d_nbox;
   a.nfirst = ix->d_nfirst;
   a.ncount = ix->d_ncount;
-  a.queries = b->d_queries;
+  a.queries = d_queries;
   a.obj_stride = ix->obj_stride;
   a.node_stride = ix->node_stride;
   a.n = ix->n;
-  a.nq = (uint32_t)b->q;
+  a.nq = (uint32_t)nq;
   a.F = ix->F;
   a.h = ix->h;
   a.n_nodes = (uint32_t)ix->n_nodes;
@@ -598,8 +544,13 @@ static SearchArgs make_args(const qcp_batch* b) {
       sp = (sp > (1ull << 40)) ? sp : sp * ix->F;
     }
   }
-  a.counts = b->d_counts;
-  a.hashes = b->d_hashes;
+  a.counts = d_counts;
+  a.hashes = d_hashes;
+  return a;
+}
+
+static SearchArgs make_args(const qcp_batch* b) {
+  SearchArgs a = index_search_args(b->ix, b->d_queries, b->q, b->d_counts, b->d_hashes);
   a.offsets = b->d_offsets;
   a.ids = b->d_ids;
   a.visits = b->d_visits;
@@ -609,7 +560,7 @@ static SearchArgs make_args(const qcp_batch* b) {
   return a;
 }
 
-static unsigned persistent_grid(const qcp_index* ix, int out, unsigned block) {
+unsigned qcp::persistent_grid(const qcp_index* ix, int out, unsigned block) {
   const void* k = search_kernel_ptr(ix->D, ix->box, ix->expl, out, (int)((ix->F + 63) / 64));
   int per_cu = 0;
   if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)block, 0) != hipSuccess ||
@@ -617,6 +568,8 @@ static unsigned persistent_grid(const qcp_index* ix, int out, unsigned block) {
     per_cu = 2;
   return (unsigned)(ix->cus * per_cu);
 }
+
+extern "C" {
 
 qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_stats* stats) {
   if (!b) return QCP_EINVAL;

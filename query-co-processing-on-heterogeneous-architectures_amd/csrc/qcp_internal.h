@@ -1,0 +1,77 @@
+// qcp_internal.h -- host-side definitions of the index and batch handles,
+// shared by the C ABI implementation files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/qcp.h"
+#include "qcp_kernels.h"
+
+struct qcp_index {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  qcp_build_cfg cfg{};
+  uint32_t D = 0, F = 0, E = 0, nb = 0;
+  bool box = false, expl = false;
+  uint64_t n = 0;
+  int h = 0;
+  std::vector<uint64_t> level_count;  // root first
+  std::vector<uint64_t> level_off;    // h+1 entries
+  uint64_t n_nodes = 0;
+  uint64_t obj_stride = 0, node_stride = 0;
+  // device
+  float* d_obj = nullptr;        // [E][obj_stride]
+  float* d_nbox = nullptr;       // [2D][node_stride]
+  uint32_t* d_nfirst = nullptr;  // explicit layouts
+  uint32_t* d_ncount = nullptr;
+  uint64_t* d_keys = nullptr;    // sorted keys [n]
+  uint32_t* d_perm = nullptr;    // [n]
+  // build scratch
+  uint64_t* d_keys2 = nullptr;
+  uint32_t* d_perm2 = nullptr;
+  uint32_t* d_hist = nullptr;
+  uint32_t* d_hist_scan = nullptr;
+  uint32_t* d_block_sums = nullptr;
+  unsigned long long* d_key_or = nullptr;
+  uint32_t* d_bad = nullptr;
+  float* d_in = nullptr;         // device copy of host input
+  hipEvent_t ev[8] = {};
+  int cus = 0;
+  qcp_index_info info{};
+};
+
+struct qcp_batch {
+  qcp_index* ix = nullptr;
+  uint64_t q = 0;
+  float* d_queries = nullptr;
+  uint32_t* d_counts = nullptr;
+  unsigned long long* d_hashes = nullptr;
+  unsigned long long* d_offsets = nullptr;
+  unsigned long long* d_scan_sums = nullptr;
+  uint32_t* d_ids = nullptr;
+  uint64_t ids_cap = 0;
+  uint32_t* d_visits = nullptr;
+  qcp::QueueCtl* d_ctl = nullptr;
+  unsigned long long* d_slots = nullptr;
+  uint32_t slot_cap = 0;
+  uint64_t slots_dirty = 0;       // slots written by the last donating run
+  uint32_t* h_counts = nullptr;   // pinned, plain-form per-batch copies
+  uint32_t* h_visits = nullptr;
+  hipEvent_t ev[4] = {};
+  uint32_t last_outputs = 0;
+  uint32_t last_mode = 0;
+  uint64_t total_hits = 0;
+  bool has_offsets = false;
+};
+
+namespace qcp {
+// Kernel arguments of a search over `ix` for `nq` queries at d_queries with
+// the given output buffers (no queue / donation state).
+SearchArgs index_search_args(const qcp_index* ix, const float* d_queries, uint64_t nq,
+                             uint32_t* d_counts, unsigned long long* d_hashes);
+// Workgroups that fill the device with the persistent kernel of `out` mode.
+unsigned persistent_grid(const qcp_index* ix, int out, unsigned block);
+}  // namespace qcp

@@ -64,6 +64,23 @@ class SearchStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServeCfg(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("result_budget", ctypes.c_uint64), ("max_requests", ctypes.c_uint32),
+                ("grid_blocks", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("deferred_by_budget", ctypes.c_uint64),
+                ("max_batch_seen", ctypes.c_uint64), ("max_inflight_seen", ctypes.c_uint64),
+                ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double),
+                ("mean_us", ctypes.c_double), ("max_us", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 P = ctypes.c_void_p
 
@@ -97,6 +114,13 @@ def lib():
         L.qcp_batch_total_hits.restype = ctypes.c_uint64
         L.qcp_search.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(SearchCfg), P, P, P, P,
                                  ctypes.c_uint64, ctypes.POINTER(SearchStats)]
+        L.qcp_server_start.argtypes = [P, ctypes.POINTER(ServeCfg), ctypes.POINTER(P)]
+        L.qcp_server_submit.argtypes = [P, P, ctypes.POINTER(ctypes.c_uint64)]
+        L.qcp_server_result.argtypes = [P, ctypes.c_uint64, P, P, P]
+        L.qcp_server_wait.argtypes = [P, ctypes.c_uint64, P, P, P]
+        L.qcp_server_stats.argtypes = [P, ctypes.POINTER(ServeStats)]
+        L.qcp_server_stop.argtypes = [P]
+        L.qcp_server_replay.argtypes = [P, P, P, ctypes.c_uint64, P, P, P]
         L.qcp_philox4x32_10.argtypes = [P, P, P]
         L.qcp_philox4x32_10.restype = None
         for fn in ("qcp_gen_words", "qcp_gen_uniform", "qcp_gen_normal"):
@@ -270,6 +294,66 @@ class Batch:
         return out
 
 
+class Server:
+    """Continuous-batching server over a resident index (counts + ID-set hash)."""
+
+    def __init__(self, index, slots=4, max_batch=4096, result_budget=0, max_requests=0,
+                 grid_blocks=0):
+        self.index = index
+        cfg = ServeCfg(slots, max_batch, result_budget, max_requests, grid_blocks, (0, 0))
+        self._h = P()
+        check(lib().qcp_server_start(index._h, ctypes.byref(cfg), ctypes.byref(self._h)),
+              "qcp_server_start")
+
+    def submit(self, box):
+        b = np.ascontiguousarray(box, np.float32)
+        t = ctypes.c_uint64()
+        check(lib().qcp_server_submit(self._h, _p(b), ctypes.byref(t)), "qcp_server_submit")
+        return t.value
+
+    def wait(self, ticket):
+        c = ctypes.c_uint32()
+        h = ctypes.c_uint64()
+        lat = ctypes.c_double()
+        check(lib().qcp_server_wait(self._h, ticket, ctypes.byref(c), ctypes.byref(h),
+                                    ctypes.byref(lat)), "qcp_server_wait")
+        return c.value, h.value, lat.value
+
+    def replay(self, boxes, arrival_s):
+        boxes = np.ascontiguousarray(boxes, np.float32)
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        n = len(arr)
+        counts = np.zeros(max(n, 1), np.uint32)
+        hashes = np.zeros(max(n, 1), np.uint64)
+        lat = np.zeros(max(n, 1), np.float64)
+        check(lib().qcp_server_replay(self._h, _p(boxes), _p(arr), n, _p(counts), _p(hashes),
+                                      _p(lat)), "qcp_server_replay")
+        return counts[:n], hashes[:n], lat[:n]
+
+    def stats(self):
+        st = ServeStats()
+        check(lib().qcp_server_stats(self._h, ctypes.byref(st)), "qcp_server_stats")
+        return st.as_dict()
+
+    def stop(self):
+        if self._h:
+            h, self._h = self._h, None
+            check(lib().qcp_server_stop(h), "qcp_server_stop")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()
+
+
+def poisson_arrivals(n, qps, seed=1234, stream=3):
+    """Arrival times (s) of a Poisson process of rate qps: cumulative
+    exponential gaps -ln(1-u)/qps from Philox stream `stream`."""
+    u = gen_uniform(n, seed, stream).astype(np.float64)
+    return np.cumsum(-np.log1p(-u) / qps)
+
+
 # ---------------------------------------------------------------------------
 # seeded generator (Philox4x32-10) and the benchmark workloads
 def philox(ctr, key):
@@ -328,9 +412,10 @@ def mixture_points(n, D, k, sigma, seed=SEED, comp_stream=5, noise_stream=6, cen
     return p, centers
 
 
-def workload(name, scale=1.0, seed=SEED):
+def workload(name, scale=1.0, seed=SEED, nq=None):
     """Synthetic inputs of one benchmark configuration.  `scale` shrinks N and Q
-    proportionally (parity runs); 1.0 is the full size."""
+    proportionally (parity runs); 1.0 is the full size; `nq` overrides the
+    number of queries."""
     if name == "c1":
         n, q = int(1048576 * scale), max(1, int(1000 * min(1.0, scale * 4)))
         pts = uniform_points(n, 2, seed)
@@ -362,6 +447,7 @@ def workload(name, scale=1.0, seed=SEED):
                 "layout": "rtree"}
     if name == "c4":
         n, q = int(33554432 * scale), max(1, int(100000 * scale))
+        q = nq or q
         pts = uniform_points(n, 3, seed)
         cen = gen_uniform(q * 3, seed, 1).reshape(q, 3)
         u = gen_uniform(q, seed, 2).astype(np.float64)
