@@ -656,7 +656,9 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, out_mode, u, grid, block, s));
   HIP_TRY(hipEventRecord(b->ev[1], s));
   st.launches = 1;
+  // queue statistics of the count pass (the ID pass resets the control block)
   QueueCtl ctl;
+  HIP_TRY(hipMemcpyAsync(&ctl, b->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
   if (cfg.outputs & QCP_OUT_IDS) {
     HIP_TRY(scan_u32_u64(b->d_counts, q, b->d_offsets, b->d_scan_sums, kMaxScanBlocks, s));
     unsigned long long total = 0;
@@ -677,7 +679,6 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     st.launches = 2;
     b->has_offsets = true;
   }
-  HIP_TRY(hipMemcpyAsync(&ctl, b->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   float k1 = 0, k2 = 0, tot = 0;
   hipEventElapsedTime(&k1, b->ev[0], b->ev[1]);
