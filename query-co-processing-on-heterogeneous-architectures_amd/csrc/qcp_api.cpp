@@ -625,6 +625,8 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   }
   // persistent form
   if (q == 0) {
+    HIP_TRY(launch_fill_u32((uint32_t*)b->d_offsets, 2, 0u, s));
+    HIP_TRY(hipStreamSynchronize(s));
     b->total_hits = 0;
     b->has_offsets = true;
     if (stats) *stats = st;
