@@ -79,7 +79,8 @@ typedef struct {
   uint32_t grid_blocks;    /* persistent form: workgroups (0 = fill the device)     */
   uint32_t leaf_chunks;    /* persistent form: 64-entry leaf chunks in flight per wave (0 = tuned) */
   uint32_t donate;         /* persistent form: subtree donation (work stealing) on/off */
-  uint32_t reserved;
+  uint32_t order;          /* persistent form: 0 = process queries in Hilbert order of their
+                              centres, split into one range per XCD; 1 = as given */
 } qcp_search_cfg;
 
 typedef struct {

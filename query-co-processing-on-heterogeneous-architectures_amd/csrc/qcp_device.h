@@ -14,11 +14,11 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 
 // Device-wide work queue state of one persistent search launch.
 struct QueueCtl {
-  uint32_t head;      // next ticket (tickets < nq are root items of query t)
+  uint32_t head;      // next donated-slot ticket
   uint32_t tail;      // donated slots claimed
   uint32_t pending;   // items not finished (root items + donations)
   uint32_t done;      // set once pending reaches zero
-  uint32_t spare[4];
+  uint32_t xhead[8];  // next root ticket of each XCD's query range
   unsigned long long internal_entries;
   unsigned long long leaf_entries;
   unsigned long long donated;
@@ -32,6 +32,7 @@ struct SearchArgs {
   const uint32_t* nfirst;    // explicit layouts: first child node / first rank
   const uint32_t* ncount;    // explicit layouts: count | kLeafBit for leaves
   const float* queries;      // [nq][2D] row-major [lo(D), hi(D)]
+  const uint32_t* qperm;     // processing order of the queries (nullptr = as given)
   uint64_t obj_stride;
   uint64_t node_stride;
   uint64_t n;                // objects

@@ -51,6 +51,16 @@ struct qcp_batch {
   unsigned long long* d_hashes = nullptr;
   unsigned long long* d_offsets = nullptr;
   unsigned long long* d_scan_sums = nullptr;
+  // spatial processing order of the queries (Hilbert key of each centre)
+  uint32_t* d_qperm = nullptr;
+  uint32_t* d_qperm2 = nullptr;
+  uint64_t* d_qkeys = nullptr;
+  uint64_t* d_qkeys2 = nullptr;
+  uint32_t* d_qhist = nullptr;
+  uint32_t* d_qhist_scan = nullptr;
+  uint32_t* d_qsums = nullptr;
+  unsigned long long* d_qor = nullptr;
+  uint32_t* d_qbad = nullptr;
   uint32_t* d_ids = nullptr;
   uint64_t ids_cap = 0;
   uint32_t* d_visits = nullptr;

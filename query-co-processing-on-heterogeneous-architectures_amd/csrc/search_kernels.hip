@@ -218,19 +218,23 @@ __device__ __forceinline__ unsigned long long ld_acq64(const unsigned long long*
 }
 
 constexpr unsigned long long kEmptySlot = ~0ull;
-constexpr int kHungerPeriod = 4;   // internal-node descents between queue checks
+constexpr int kHungerPeriod = 4;     // internal-node descents between queue checks
+constexpr uint32_t kMaxWaiters = 256;  // waves kept waiting for donated work
+constexpr int kXcds = 8;             // workgroup i runs on XCD i % 8
 
-// Donate the unvisited children of one frame to the device queue: each set bit
-// of `m` becomes an item (query, child node).  Bits that do not fit in the
+// Donate up to `want` of the unvisited children of one frame to the device
+// queue, taking the highest-numbered ones (the donor keeps the next siblings):
+// each becomes an item (query, child node).  Children that do not fit in the
 // slot array stay in `m`.  pending is raised before the slots are written, and
 // the donor's own item keeps pending above zero meanwhile.
 template <int MW, bool EXPL>
 __device__ __forceinline__ void donate(const SearchArgs& a, uint32_t q, uint32_t node, int level,
-                                       Mask<MW>& m) {
+                                       Mask<MW>& m, uint32_t want) {
   const uint32_t lane = lane_id();
-  uint32_t n = 0;
+  uint32_t avail = 0;
 #pragma unroll
-  for (int w = 0; w < MW; ++w) n += (uint32_t)__popcll(m.w[w]);
+  for (int w = 0; w < MW; ++w) avail += (uint32_t)__popcll(m.w[w]);
+  uint32_t n = avail < want ? avail : want;
   if (n == 0) return;
   uint32_t base = 0;
   if (lane == 0) base = atomicAdd(&a.ctl->tail, n);
@@ -248,28 +252,21 @@ __device__ __forceinline__ void donate(const SearchArgs& a, uint32_t q, uint32_t
     const uint32_t j = node - a.lvl_off[level];
     cbase = a.lvl_off[level + 1] + j * a.F;
   }
-  uint32_t before = 0;
+  const uint64_t below_me = (2ull << lane) - 1ull;  // bits 0..lane (all bits for lane 63)
+  uint32_t higher = 0;                             // set bits in words above w
 #pragma unroll
-  for (int w = 0; w < MW; ++w) {
+  for (int w = MW - 1; w >= 0; --w) {
     const uint64_t bits = m.w[w];
     const bool mine = (bits >> lane) & 1ull;
-    const uint32_t r = before + rank_below(bits);
-    if (mine && r < ok) {
+    const uint32_t r = higher + (uint32_t)__popcll(bits & ~below_me);  // rank from the top
+    const bool give = mine && r < ok;
+    if (give) {
       const unsigned long long item =
           ((unsigned long long)q << 32) | (unsigned long long)(cbase + w * kWave + lane);
       __hip_atomic_store(&a.slots[base + r], item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // clear the donated bits (the lowest `ok` set bits overall)
-    uint64_t keep = bits;
-    uint32_t take = ok > before ? ok - before : 0;
-    const uint32_t pc = (uint32_t)__popcll(bits);
-    if (take >= pc) {
-      keep = 0;
-    } else {
-      for (uint32_t t = 0; t < take; ++t) keep &= keep - 1;
-    }
-    m.w[w] = keep;
-    before += pc;
+    m.w[w] = bits & ~ballot(give);
+    higher += (uint32_t)__popcll(bits);
   }
 }
 
@@ -347,15 +344,16 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
       since = 0;
       const uint32_t head = ld_acq(&a.ctl->head);
       const uint32_t tail = ld_acq(&a.ctl->tail);
-      if (head > a.nq + tail && tail < a.slot_cap) {
+      if (head > tail && tail < a.slot_cap) {
+        const uint32_t waiting = head - tail;
         int f = 0;
         while (f < sp && stk[f].m.empty()) ++f;
         if (f < sp) {
           Mask<MW> dm = stk[f].m;
-          donate<MW, EXPL>(a, c.q, stk[f].node, (int)stk[f].level, dm);
+          donate<MW, EXPL>(a, c.q, stk[f].node, (int)stk[f].level, dm, waiting);
           stk[f].m = dm;
         } else if (!m.empty()) {
-          donate<MW, EXPL>(a, c.q, cur, level, m);
+          donate<MW, EXPL>(a, c.q, cur, level, m, waiting);
         }
       }
     }
@@ -391,9 +389,13 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
   }
 }
 
-// Persistent search: tickets below nq are the root items of query t; with
-// donation on, ticket nq+s is donated slot s, waited for until it is written
-// or all work is done.
+// Persistent search.  Root items (query t, from the root) come from eight
+// per-XCD ticket ranges over the (spatially ordered) query list: a wave takes
+// tickets of its own XCD's range first, so each XCD's L2 sees neighbouring
+// queries, and then the other ranges.  With donation on, a wave that finds all
+// root tickets gone takes a slot ticket from `head` and waits for that slot to
+// be written by a donor, or for `done`; at most kMaxWaiters waves wait, the
+// others exit.
 template <int D, bool BOX, int MW, bool EXPL, int OUT, int U>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   constexpr int kDepth = EXPL ? kMaxDepth : kMaxLevels;
@@ -402,27 +404,45 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   const uint32_t lane = lane_id();
   uint64_t ie = 0, le = 0, hits = 0;
   const bool donating = OUT != kOutIds && a.donate;
+  const uint32_t xcd = blockIdx.x % kXcds;
+  bool roots_left = true;
   while (true) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(&a.ctl->head, 1u);
-    t = uni(t);
-    uint32_t q, node = 0;
-    if (t < a.nq) {
-      q = t;
-    } else {
+    uint32_t q = 0, node = 0;
+    bool got = false;
+    if (roots_left) {
+      for (int k = 0; k < kXcds && !got; ++k) {
+        const uint32_t x = (xcd + k) % kXcds;
+        const uint32_t lo = (uint32_t)(((uint64_t)a.nq * x) / kXcds);
+        const uint32_t hi = (uint32_t)(((uint64_t)a.nq * (x + 1)) / kXcds);
+        if (lo >= hi) continue;
+        if (k > 0 && ld_acq(&a.ctl->xhead[x]) >= hi - lo) continue;
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&a.ctl->xhead[x], 1u);
+        t = uni(t);
+        if (lo + t < hi) {
+          q = a.qperm ? a.qperm[lo + t] : lo + t;
+          got = true;
+        }
+      }
+      roots_left = got;
+    }
+    if (!got) {
       if (!donating) break;
-      const uint32_t s = t - a.nq;
+      // Enough waves already wait: leave.  (Signed: donors may have claimed
+      // more slots than there are tickets, and those slots need takers.)
+      const int32_t waiting = (int32_t)(ld_acq(&a.ctl->head) - ld_acq(&a.ctl->tail));
+      if (waiting >= (int32_t)kMaxWaiters && !ld_acq(&a.ctl->done)) break;
+      uint32_t s = 0;
+      if (lane == 0) s = atomicAdd(&a.ctl->head, 1u);
+      s = uni(s);
       unsigned long long item = kEmptySlot;
-      uint32_t nap = 1;
       while (true) {
         if (s < a.slot_cap) {
           item = ld_acq64(&a.slots[s]);
           if (item != kEmptySlot) break;
         }
         if (ld_acq(&a.ctl->done)) break;
-        __builtin_amdgcn_s_sleep(4);
-        if (nap < 64) nap *= 2;
-        for (uint32_t z = 0; z < nap; z += 8) __builtin_amdgcn_s_sleep(8);
+        for (int z = 0; z < 8; ++z) __builtin_amdgcn_s_sleep(32);
       }
       item = uni64(item);
       if (item == kEmptySlot) break;
@@ -563,6 +583,7 @@ __global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
 }
 
 __global__ void k_reset_ctl(QueueCtl* ctl, uint32_t pending) {
+  if (threadIdx.x < 8) ctl->xhead[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     ctl->head = 0;
     ctl->tail = 0;

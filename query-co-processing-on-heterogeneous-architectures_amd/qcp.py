@@ -51,7 +51,7 @@ class IndexInfo(ctypes.Structure):
 class SearchCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in
                 ("mode", "outputs", "plain_batch", "block_waves", "grid_blocks", "leaf_chunks",
-                 "donate", "reserved")]
+                 "donate", "order")]
 
 
 class SearchStats(ctypes.Structure):
@@ -218,10 +218,10 @@ class Index:
         return Batch(self, queries)
 
     def search(self, queries, mode="persistent", ids=False, visits=False, plain_batch=128,
-               grid_blocks=0, donate=1, chunks=0, hash=True):
+               grid_blocks=0, donate=1, chunks=0, hash=True, order=0):
         with self.batch(queries) as b:
             st = b.run(mode=mode, ids=ids, plain_batch=plain_batch, grid_blocks=grid_blocks,
-                       donate=donate, chunks=chunks, hash=hash)
+                       donate=donate, chunks=chunks, hash=hash, order=order)
             res = b.results(ids=ids, visits=visits)
             res["stats"] = st
             return res
@@ -256,12 +256,13 @@ class Batch:
             pass
 
     def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=1,
-            hash=True, chunks=0):
+            hash=True, chunks=0, order=0):
         """One search of the resident batch.  donate=1 enables subtree donation
         (work stealing) in the persistent form; chunks = 64-entry leaf chunks in
-        flight per wave (0 = default)."""
+        flight per wave (0 = default); order=0 processes the queries in Hilbert
+        order of their centres, one range per XCD, order=1 as given."""
         outputs = OUT_COUNTS | (OUT_HASH if hash else 0) | (OUT_IDS if ids else 0)
-        cfg = SearchCfg(MODE[mode], outputs, plain_batch, 0, grid_blocks, chunks, donate, 0)
+        cfg = SearchCfg(MODE[mode], outputs, plain_batch, 0, grid_blocks, chunks, donate, order)
         st = SearchStats()
         check(lib().qcp_batch_run(self._h, ctypes.byref(cfg), ctypes.byref(st)), "qcp_batch_run")
         self._ids = ids

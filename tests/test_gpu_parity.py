@@ -49,8 +49,8 @@ def check_case(gpu, O, objs, D, is_box, F, key_mode="ref", key_src="lo", layout=
     assert st["leaf_entries"] == int(exp["leaf_entries"].sum()), tag + " leaf entries"
     assert st["total_hits"] == int(exp["counts"].sum())
     # the other persistent variants: no donation, counts only, 2 and 8 chunks in flight
-    nd = ix.search(queries, mode="persistent", donate=0, chunks=8)
-    assert_results_equal(nd, exp, tag + " persistent no-donation", ids=False)
+    nd = ix.search(queries, mode="persistent", donate=0, chunks=8, order=1)
+    assert_results_equal(nd, exp, tag + " persistent no-donation, given order", ids=False)
     co = ix.search(queries, mode="persistent", hash=False, chunks=2)
     assert_bits_equal(co["counts"], exp["counts"], tag + " counts-only")
     if plain and layout != "bvh":
