@@ -81,6 +81,8 @@ typedef struct {
   uint32_t donate;         /* persistent form: subtree donation (work stealing) on/off */
   uint32_t order;          /* persistent form: 0 = process queries in Hilbert order of their
                               centres, split into one range per XCD; 1 = as given */
+  uint32_t max_waiters;    /* donation: waves kept waiting for donated work (0 = 256)  */
+  uint32_t hunger_period;  /* donation: internal-node descents between queue checks (0 = 4) */
 } qcp_search_cfg;
 
 typedef struct {

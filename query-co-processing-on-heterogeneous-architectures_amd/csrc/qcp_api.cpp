@@ -697,6 +697,8 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     a.slots = b->d_slots;
     a.slot_cap = b->slot_cap;
     a.donate = 1;
+    a.max_waiters = cfg.max_waiters ? cfg.max_waiters : 256;
+    a.hunger_period = cfg.hunger_period ? cfg.hunger_period : 4;
   }
   HIP_TRY(hipEventRecord(b->ev[0], s));
   if (donate) {

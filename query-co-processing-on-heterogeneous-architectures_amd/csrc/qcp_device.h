@@ -52,6 +52,8 @@ struct SearchArgs {
   unsigned long long* slots; // donated items (q << 32 | node)
   uint32_t slot_cap;
   uint32_t donate;
+  uint32_t max_waiters;      // waves kept waiting for donated work
+  uint32_t hunger_period;    // internal-node descents between queue checks
   uint32_t q_begin;          // plain form: first query of the launch
 };
 

@@ -218,8 +218,6 @@ __device__ __forceinline__ unsigned long long ld_acq64(const unsigned long long*
 }
 
 constexpr unsigned long long kEmptySlot = ~0ull;
-constexpr int kHungerPeriod = 4;     // internal-node descents between queue checks
-constexpr uint32_t kMaxWaiters = 256;  // waves kept waiting for donated work
 constexpr int kXcds = 8;             // workgroup i runs on XCD i % 8
 
 // Donate up to `want` of the unvisited children of one frame to the device
@@ -340,7 +338,7 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
     m.clear(k);
     // work stealing: when waves wait for items, hand over the unvisited
     // siblings of the shallowest frame that has any
-    if (OUT != kOutIds && a.donate && ++since >= kHungerPeriod) {
+    if (OUT != kOutIds && a.donate && ++since >= (int)a.hunger_period) {
       since = 0;
       const uint32_t head = ld_acq(&a.ctl->head);
       const uint32_t tail = ld_acq(&a.ctl->tail);
@@ -394,7 +392,7 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
 // tickets of its own XCD's range first, so each XCD's L2 sees neighbouring
 // queries, and then the other ranges.  With donation on, a wave that finds all
 // root tickets gone takes a slot ticket from `head` and waits for that slot to
-// be written by a donor, or for `done`; at most kMaxWaiters waves wait, the
+// be written by a donor, or for `done`; at most max_waiters waves wait, the
 // others exit.
 template <int D, bool BOX, int MW, bool EXPL, int OUT, int U>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
@@ -431,7 +429,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       // Enough waves already wait: leave.  (Signed: donors may have claimed
       // more slots than there are tickets, and those slots need takers.)
       const int32_t waiting = (int32_t)(ld_acq(&a.ctl->head) - ld_acq(&a.ctl->tail));
-      if (waiting >= (int32_t)kMaxWaiters && !ld_acq(&a.ctl->done)) break;
+      if (waiting >= (int32_t)a.max_waiters && !ld_acq(&a.ctl->done)) break;
       uint32_t s = 0;
       if (lane == 0) s = atomicAdd(&a.ctl->head, 1u);
       s = uni(s);
