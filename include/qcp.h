@@ -82,7 +82,10 @@ typedef struct {
   uint32_t order;          /* persistent form: 0 = process queries in Hilbert order of their
                               centres, split into one range per XCD; 1 = as given */
   uint32_t max_waiters;    /* donation: waves kept waiting for donated work (0 = 256)  */
-  uint32_t hunger_period;  /* donation: internal-node descents between queue checks (0 = 4) */
+  uint32_t hunger_period;  /* donation: internal-node descents between queue checks (0 = 8) */
+  uint32_t donate_height;  /* donation (packed layouts): minimum height of a donated subtree,
+                              leaves having height 0 and leaf parents 1 (0 = 1) */
+  uint32_t reserved;
 } qcp_search_cfg;
 
 typedef struct {

@@ -698,7 +698,8 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     a.slot_cap = b->slot_cap;
     a.donate = 1;
     a.max_waiters = cfg.max_waiters ? cfg.max_waiters : 256;
-    a.hunger_period = cfg.hunger_period ? cfg.hunger_period : 4;
+    a.hunger_period = cfg.hunger_period ? cfg.hunger_period : 8;
+    a.donate_min_height = cfg.donate_height ? cfg.donate_height : 1;
   }
   HIP_TRY(hipEventRecord(b->ev[0], s));
   if (donate) {

@@ -54,6 +54,7 @@ struct SearchArgs {
   uint32_t donate;
   uint32_t max_waiters;      // waves kept waiting for donated work
   uint32_t hunger_period;    // internal-node descents between queue checks
+  uint32_t donate_min_height;  // packed: donate subtrees this far above leaf parents
   uint32_t q_begin;          // plain form: first query of the launch
 };
 

@@ -337,21 +337,27 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
     const int k = m.lowest();
     m.clear(k);
     // work stealing: when waves wait for items, hand over the unvisited
-    // siblings of the shallowest frame that has any
+    // siblings of the shallowest frame that has any, if those subtrees are at
+    // least donate_min_height levels above the leaf parents (packed layouts)
     if (OUT != kOutIds && a.donate && ++since >= (int)a.hunger_period) {
       since = 0;
-      const uint32_t head = ld_acq(&a.ctl->head);
-      const uint32_t tail = ld_acq(&a.ctl->tail);
-      if (head > tail && tail < a.slot_cap) {
-        const uint32_t waiting = head - tail;
-        int f = 0;
-        while (f < sp && stk[f].m.empty()) ++f;
-        if (f < sp) {
-          Mask<MW> dm = stk[f].m;
-          donate<MW, EXPL>(a, c.q, stk[f].node, (int)stk[f].level, dm, waiting);
-          stk[f].m = dm;
-        } else if (!m.empty()) {
-          donate<MW, EXPL>(a, c.q, cur, level, m, waiting);
+      const int max_level = EXPL ? kMaxDepth : a.h - 2 - (int)a.donate_min_height;
+      int f = 0;
+      while (f < sp && stk[f].m.empty()) ++f;
+      const bool have = (f < sp) ? ((int)stk[f].level <= max_level)
+                                 : (!m.empty() && level <= max_level);
+      if (have) {
+        const uint32_t head = ld_acq(&a.ctl->head);
+        const uint32_t tail = ld_acq(&a.ctl->tail);
+        if (head > tail && tail < a.slot_cap) {
+          const uint32_t waiting = head - tail;
+          if (f < sp) {
+            Mask<MW> dm = stk[f].m;
+            donate<MW, EXPL>(a, c.q, stk[f].node, (int)stk[f].level, dm, waiting);
+            stk[f].m = dm;
+          } else {
+            donate<MW, EXPL>(a, c.q, cur, level, m, waiting);
+          }
         }
       }
     }

@@ -51,7 +51,8 @@ class IndexInfo(ctypes.Structure):
 class SearchCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in
                 ("mode", "outputs", "plain_batch", "block_waves", "grid_blocks", "leaf_chunks",
-                 "donate", "order", "max_waiters", "hunger_period")]
+                 "donate", "order", "max_waiters", "hunger_period", "donate_height",
+                 "reserved")]
 
 
 class SearchStats(ctypes.Structure):
@@ -256,14 +257,14 @@ class Batch:
             pass
 
     def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=1,
-            hash=True, chunks=0, order=0, max_waiters=0, hunger_period=0):
+            hash=True, chunks=0, order=0, max_waiters=0, hunger_period=0, donate_height=0):
         """One search of the resident batch.  donate=1 enables subtree donation
         (work stealing) in the persistent form; chunks = 64-entry leaf chunks in
         flight per wave (0 = default); order=0 processes the queries in Hilbert
         order of their centres, one range per XCD, order=1 as given."""
         outputs = OUT_COUNTS | (OUT_HASH if hash else 0) | (OUT_IDS if ids else 0)
         cfg = SearchCfg(MODE[mode], outputs, plain_batch, 0, grid_blocks, chunks, donate, order,
-                        max_waiters, hunger_period)
+                        max_waiters, hunger_period, donate_height, 0)
         st = SearchStats()
         check(lib().qcp_batch_run(self._h, ctypes.byref(cfg), ctypes.byref(st)), "qcp_batch_run")
         self._ids = ids
