@@ -45,6 +45,7 @@ enum { QCP_LAYOUT_MPHR = 0, QCP_LAYOUT_RTREE = 1, QCP_LAYOUT_BVH = 2 };
 enum { QCP_KEY_REF = 0, QCP_KEY_B16 = 1 };      /* (ll)(1e6f*p) @ 31|20 bits, or 16 bits */
 enum { QCP_SRC_LO = 0, QCP_SRC_CENTER = 1 };    /* key from lower corner or box centre  */
 enum { QCP_OBJ_POINTS = 0, QCP_OBJ_BOXES = 1 };
+enum { QCP_BUILD_GRAPH = 1 };                   /* qcp_build_cfg.flags                  */
 
 enum { QCP_SEARCH_PERSISTENT = 0, QCP_SEARCH_PLAIN = 1 };
 enum { QCP_OUT_COUNTS = 1, QCP_OUT_HASH = 2, QCP_OUT_IDS = 4, QCP_OUT_VISITS = 8 };
@@ -57,7 +58,7 @@ typedef struct {
   uint32_t key_src;   /* QCP_SRC_*                                                  */
   uint32_t obj_kind;  /* QCP_OBJ_*: points are N x D floats, boxes N x 2D [lo, hi] */
   uint32_t device;    /* HIP device ordinal                                         */
-  uint32_t flags;     /* reserved, 0                                                */
+  uint32_t flags;     /* QCP_BUILD_GRAPH: run the build as a captured hipGraph      */
 } qcp_build_cfg;
 
 typedef struct {
@@ -85,7 +86,9 @@ typedef struct {
   uint32_t hunger_period;  /* donation: internal-node descents between queue checks (0 = 8) */
   uint32_t donate_height;  /* donation (packed layouts): minimum height of a donated subtree,
                               leaves having height 0 and leaf parents 1 (0 = 1) */
-  uint32_t reserved;
+  uint32_t graph;          /* persistent form, counts/hash outputs: capture the whole launch
+                              sequence (query ordering, buffer fills, search) in a hipGraph
+                              on first use and replay it while the configuration is unchanged */
 } qcp_search_cfg;
 
 typedef struct {
@@ -125,6 +128,14 @@ qcp_status qcp_build_device(const qcp_build_cfg *cfg, const float *d_objs, uint6
 /* Rebuild an existing index in place from new device-resident objects of the
  * same count and configuration (timed builds). */
 qcp_status qcp_rebuild_device(qcp_index *ix, const float *d_objs);
+
+/* Device buffers for inputs that should be resident before a timed build or
+ * search (the reference's cudaMalloc + cudaMemcpy of DataSet::GetDeviceQuery,
+ * src/io/dataset.cpp:82-88, as a caller-owned buffer). */
+qcp_status qcp_device_alloc(uint32_t device, uint64_t bytes, void **out);
+qcp_status qcp_device_free(void *p);
+qcp_status qcp_copy_to_device(void *dst, const void *src, uint64_t bytes);
+qcp_status qcp_copy_to_host(void *dst, const void *src, uint64_t bytes);
 
 void qcp_index_destroy(qcp_index *ix);
 

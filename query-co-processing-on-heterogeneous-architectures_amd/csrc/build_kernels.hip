@@ -95,7 +95,6 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   __shared__ uint32_t wcnt[W][256];
   const int tid = threadIdx.x;
   const int w = tid / kWave;
-  const uint32_t lane = lane_id();
   run[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) wcnt[ww][tid] = 0;

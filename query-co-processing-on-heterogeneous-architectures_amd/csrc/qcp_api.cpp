@@ -99,6 +99,7 @@ static void index_free(qcp_index* ix) {
   dfree(ix->d_key_or);
   dfree(ix->d_bad);
   dfree(ix->d_in);
+  if (ix->build_exec) hipGraphExecDestroy(ix->build_exec);
   for (auto& e : ix->ev)
     if (e) hipEventDestroy(e);
   if (ix->stream) hipStreamDestroy(ix->stream);
@@ -215,6 +216,105 @@ static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** 
   return QCP_OK;
 }
 
+// Shape fields of the index info (timings cleared).
+static void set_info_shape(qcp_index* ix) {
+  qcp_index_info& in = ix->info;
+  memset(&in, 0, sizeof(in));
+  in.n = ix->n;
+  in.n_nodes = ix->n_nodes;
+  in.height = (uint32_t)ix->h;
+  in.dims = ix->D;
+  in.fanout = ix->F;
+  in.layout = ix->cfg.layout;
+  in.obj_kind = ix->cfg.obj_kind;
+  in.key_mode = ix->cfg.key_mode;
+  in.key_src = ix->cfg.key_src;
+  in.key_bits = ix->nb * ix->D;
+  for (int l = 0; l < ix->h && l < 32; ++l) in.level_count[l] = ix->level_count[l];
+  in.tree_bytes = (uint64_t)ix->E * 4 * ix->n + (uint64_t)2 * ix->D * 4 * ix->n_nodes;
+}
+
+// Sort passes of a graph-captured build: every byte a key can hold (an even
+// count for every mode, so the sorted keys land back in d_keys / d_perm).
+static unsigned graph_sort_passes(const qcp_index* ix) { return (ix->D * ix->nb + 7) / 8; }
+
+// The whole build as one asynchronous sequence (no host synchronisation), for
+// hipGraph capture: keys, a fixed number of sort passes, gather, node boxes.
+static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned passes) {
+  hipStream_t s = ix->stream;
+  const uint64_t n = ix->n;
+  HIP_TRY(launch_fill_u32((uint32_t*)ix->d_key_or, 2, 0u, s));
+  HIP_TRY(launch_fill_u32(ix->d_bad, 1, 0u, s));
+  HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
+                      ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, ix->d_perm,
+                      ix->d_key_or, ix->d_bad, s));
+  uint64_t* kin = ix->d_keys;
+  uint32_t* vin = ix->d_perm;
+  uint64_t* kout = ix->d_keys2;
+  uint32_t* vout = ix->d_perm2;
+  for (unsigned p = 0; p < passes; ++p) {
+    HIP_TRY(launch_sort_pass(kin, vin, kout, vout, n, 8 * p, ix->d_hist, ix->d_hist_scan,
+                             ix->d_block_sums, kMaxScanBlocks, s));
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  if (kin != ix->d_keys) return QCP_EINVAL;  // odd pass counts are not captured
+  HIP_TRY(launch_gather(d_objs, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s));
+  if (!ix->expl) {
+    const int h = ix->h;
+    HIP_TRY(launch_leaf_boxes(ix->d_obj, ix->obj_stride, n, ix->D, ix->box, ix->F,
+                              ix->level_count[h - 1], ix->d_nbox, ix->node_stride,
+                              ix->level_off[h - 1], s));
+    for (int l = h - 2; l >= 0; --l)
+      HIP_TRY(launch_level_boxes(ix->d_nbox, ix->node_stride, ix->D, ix->level_off[l + 1],
+                                 ix->level_count[l + 1], ix->F, ix->level_off[l],
+                                 ix->level_count[l], s));
+  } else {
+    for (int l = ix->h - 1; l >= 0; --l)
+      HIP_TRY(launch_tree_boxes(ix->d_obj, ix->obj_stride, ix->d_nfirst, ix->d_ncount, ix->D,
+                                ix->box, ix->F, ix->level_off[l], ix->level_off[l + 1],
+                                ix->d_nbox, ix->node_stride, s));
+  }
+  return QCP_OK;
+}
+
+// Graph-captured build: captured once per input pointer, replayed after.
+static qcp_status index_run_build_graph(qcp_index* ix, const float* d_objs) {
+  hipStream_t s = ix->stream;
+  if (ix->n == 0) return QCP_OK;
+  if (!ix->build_exec || ix->build_exec_objs != d_objs) {
+    if (ix->build_exec) {
+      hipGraphExecDestroy(ix->build_exec);
+      ix->build_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    qcp_status st = enqueue_build(ix, d_objs, graph_sort_passes(ix));
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (st != QCP_OK) {
+      if (g) hipGraphDestroy(g);
+      return st;
+    }
+    HIP_TRY(e);
+    e = hipGraphInstantiate(&ix->build_exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIP_TRY(e);
+    ix->build_exec_objs = d_objs;
+  }
+  HIP_TRY(hipEventRecord(ix->ev[0], s));
+  HIP_TRY(hipGraphLaunch(ix->build_exec, s));
+  HIP_TRY(hipEventRecord(ix->ev[4], s));
+  uint32_t bad = 0;
+  HIP_TRY(hipMemcpyAsync(&bad, ix->d_bad, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float tot = 0;
+  hipEventElapsedTime(&tot, ix->ev[0], ix->ev[4]);
+  set_info_shape(ix);
+  ix->info.build_ms = tot;
+  ix->info.build_phase_ms[5] = tot;
+  return bad ? QCP_EDOMAIN : QCP_OK;
+}
+
 // The build pipeline on the index's stream.
 static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
   hipStream_t s = ix->stream;
@@ -281,20 +381,8 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
   hipEventElapsedTime(&ms[2], ix->ev[2], ix->ev[3]);
   hipEventElapsedTime(&ms[3], ix->ev[3], ix->ev[4]);
   hipEventElapsedTime(&tot, ix->ev[0], ix->ev[4]);
+  set_info_shape(ix);
   qcp_index_info& in = ix->info;
-  memset(&in, 0, sizeof(in));
-  in.n = n;
-  in.n_nodes = ix->n_nodes;
-  in.height = (uint32_t)ix->h;
-  in.dims = ix->D;
-  in.fanout = ix->F;
-  in.layout = ix->cfg.layout;
-  in.obj_kind = ix->cfg.obj_kind;
-  in.key_mode = ix->cfg.key_mode;
-  in.key_src = ix->cfg.key_src;
-  in.key_bits = ix->nb * ix->D;
-  for (int l = 0; l < ix->h && l < 32; ++l) in.level_count[l] = ix->level_count[l];
-  in.tree_bytes = (uint64_t)ix->E * 4 * n + (uint64_t)2 * ix->D * 4 * ix->n_nodes;
   in.build_ms = tot;
   for (int i = 0; i < 4; ++i) in.build_phase_ms[i] = ms[i];
   in.build_phase_ms[5] = tot;
@@ -330,7 +418,7 @@ qcp_status qcp_build_device(const qcp_build_cfg* cfg, const float* d_objs, uint6
   *out = nullptr;
   qcp_index* ix = nullptr;
   QCP_TRY(index_alloc(cfg, n, &ix));
-  qcp_status s = index_run_build(ix, d_objs);
+  qcp_status s = (cfg->flags & QCP_BUILD_GRAPH) ? index_run_build_graph(ix, d_objs) : index_run_build(ix, d_objs);
   if (s != QCP_OK) {
     index_free(ix);
     return s;
@@ -349,7 +437,8 @@ qcp_status qcp_build(const qcp_build_cfg* cfg, const float* objs, uint64_t n, qc
   if (s == QCP_OK && n &&
       (hipMemcpyAsync(ix->d_in, objs, bytes, hipMemcpyHostToDevice, ix->stream) != hipSuccess))
     s = QCP_EHIP;
-  if (s == QCP_OK) s = index_run_build(ix, ix->d_in);
+  if (s == QCP_OK)
+    s = (cfg->flags & QCP_BUILD_GRAPH) ? index_run_build_graph(ix, ix->d_in) : index_run_build(ix, ix->d_in);
   dfree(ix->d_in);
   if (s != QCP_OK) {
     index_free(ix);
@@ -363,10 +452,39 @@ qcp_status qcp_rebuild_device(qcp_index* ix, const float* d_objs) {
   if (!ix || (ix->n && !d_objs)) return QCP_EINVAL;
   QCP_TRY(set_device(ix->device));
   if (!ix->d_keys2) return QCP_EINVAL;
+  if (ix->cfg.flags & QCP_BUILD_GRAPH) return index_run_build_graph(ix, d_objs);
   return index_run_build(ix, d_objs);
 }
 
 void qcp_index_destroy(qcp_index* ix) { index_free(ix); }
+
+qcp_status qcp_device_alloc(uint32_t device, uint64_t bytes, void** out) {
+  if (!out) return QCP_EINVAL;
+  *out = nullptr;
+  QCP_TRY(set_device((int)device));
+  if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+    *out = nullptr;
+    return QCP_ENOMEM;
+  }
+  return QCP_OK;
+}
+
+qcp_status qcp_device_free(void* p) {
+  if (p) HIP_TRY(hipFree(p));
+  return QCP_OK;
+}
+
+qcp_status qcp_copy_to_device(void* dst, const void* src, uint64_t bytes) {
+  if (bytes && (!dst || !src)) return QCP_EINVAL;
+  if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return QCP_OK;
+}
+
+qcp_status qcp_copy_to_host(void* dst, const void* src, uint64_t bytes) {
+  if (bytes && (!dst || !src)) return QCP_EINVAL;
+  if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return QCP_OK;
+}
 
 qcp_status qcp_index_info_get(const qcp_index* ix, qcp_index_info* out) {
   if (!ix || !out) return QCP_EINVAL;
@@ -460,6 +578,7 @@ void qcp_batch_destroy(qcp_batch* b) {
   dfree(b->d_slots);
   if (b->h_counts) hipHostFree(b->h_counts);
   if (b->h_visits) hipHostFree(b->h_visits);
+  if (b->exec) hipGraphExecDestroy(b->exec);
   for (auto& e : b->ev)
     if (e) hipEventDestroy(e);
   delete b;
@@ -701,19 +820,53 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     a.hunger_period = cfg.hunger_period ? cfg.hunger_period : 8;
     a.donate_min_height = cfg.donate_height ? cfg.donate_height : 1;
   }
-  HIP_TRY(hipEventRecord(b->ev[0], s));
-  if (donate) {
-    // per-item results are accumulated atomically; donated slots start empty
-    HIP_TRY(launch_fill_u32(b->d_counts, q, 0u, s));
-    HIP_TRY(launch_fill_u32((uint32_t*)b->d_hashes, 2 * q, 0u, s));
-    HIP_TRY(launch_fill_u32((uint32_t*)b->d_slots, 2ull * b->slots_dirty, 0xFFFFFFFFu, s));
+  // the device work of one run, all on the stream (graph-capturable)
+  auto enqueue = [&](uint64_t slots_to_clear) -> qcp_status {
+    if (donate) {
+      // per-item results are accumulated atomically; donated slots start empty
+      HIP_TRY(launch_fill_u32(b->d_counts, q, 0u, s));
+      HIP_TRY(launch_fill_u32((uint32_t*)b->d_hashes, 2 * q, 0u, s));
+      HIP_TRY(launch_fill_u32((uint32_t*)b->d_slots, 2ull * slots_to_clear, 0xFFFFFFFFu, s));
+    }
+    if (cfg.order == 0) {
+      QCP_TRY(batch_order(b));
+      a.qperm = b->d_qperm;
+    }
+    HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, out_mode, u, grid, block, s));
+    return QCP_OK;
+  };
+  const bool use_graph = cfg.graph && !(cfg.outputs & QCP_OUT_IDS);
+  if (use_graph) {
+    const uint32_t key[12] = {1u, (uint32_t)out_mode, (uint32_t)u, grid, donate ? 1u : 0u,
+                              cfg.order, a.max_waiters, a.hunger_period, a.donate_min_height,
+                              0u, 0u, 0u};
+    if (!b->exec || memcmp(key, b->exec_key, sizeof(key)) != 0) {
+      if (b->exec) {
+        hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      qcp_status cs = enqueue(b->slot_cap);
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (cs != QCP_OK) {
+        if (g) hipGraphDestroy(g);
+        return cs;
+      }
+      HIP_TRY(e);
+      e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      HIP_TRY(e);
+      memcpy(b->exec_key, key, sizeof(key));
+    }
+    HIP_TRY(hipEventRecord(b->ev[0], s));
+    HIP_TRY(hipGraphLaunch(b->exec, s));
+    HIP_TRY(hipEventRecord(b->ev[1], s));
+  } else {
+    HIP_TRY(hipEventRecord(b->ev[0], s));
+    QCP_TRY(enqueue(b->slots_dirty));
+    HIP_TRY(hipEventRecord(b->ev[1], s));
   }
-  if (cfg.order == 0) {
-    QCP_TRY(batch_order(b));
-    a.qperm = b->d_qperm;
-  }
-  HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, out_mode, u, grid, block, s));
-  HIP_TRY(hipEventRecord(b->ev[1], s));
   st.launches = 1;
   // queue statistics of the count pass (the ID pass resets the control block)
   QueueCtl ctl;
@@ -849,20 +1002,7 @@ __attribute__((visibility("hidden"))) qcp_status qcp_internal_restore(
                        4 * n_nodes, hipMemcpyHostToDevice, s) != hipSuccess)
       return fail(QCP_EHIP);
   if (hipStreamSynchronize(s) != hipSuccess) return fail(QCP_EHIP);
-  qcp_index_info& in = ix->info;
-  memset(&in, 0, sizeof(in));
-  in.n = n;
-  in.n_nodes = ix->n_nodes;
-  in.height = (uint32_t)ix->h;
-  in.dims = ix->D;
-  in.fanout = ix->F;
-  in.layout = ix->cfg.layout;
-  in.obj_kind = ix->cfg.obj_kind;
-  in.key_mode = ix->cfg.key_mode;
-  in.key_src = ix->cfg.key_src;
-  in.key_bits = ix->nb * ix->D;
-  for (int l = 0; l < ix->h && l < 32; ++l) in.level_count[l] = ix->level_count[l];
-  in.tree_bytes = (uint64_t)ix->E * 4 * n + (uint64_t)2 * ix->D * 4 * ix->n_nodes;
+  set_info_shape(ix);
   *out = ix;
   return QCP_OK;
 }

@@ -39,6 +39,8 @@ struct qcp_index {
   uint32_t* d_bad = nullptr;
   float* d_in = nullptr;         // device copy of host input
   hipEvent_t ev[8] = {};
+  hipGraphExec_t build_exec = nullptr;      // captured build (QCP_BUILD_GRAPH)
+  const float* build_exec_objs = nullptr;   // input pointer baked into build_exec
   int cus = 0;
   qcp_index_info info{};
 };
@@ -71,6 +73,8 @@ struct qcp_batch {
   uint32_t* h_counts = nullptr;   // pinned, plain-form per-batch copies
   uint32_t* h_visits = nullptr;
   hipEvent_t ev[4] = {};
+  hipGraphExec_t exec = nullptr;            // captured search (qcp_search_cfg.graph)
+  uint32_t exec_key[12] = {};               // configuration baked into exec
   uint32_t last_outputs = 0;
   uint32_t last_mode = 0;
   uint64_t total_hits = 0;

@@ -52,7 +52,7 @@ class SearchCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in
                 ("mode", "outputs", "plain_batch", "block_waves", "grid_blocks", "leaf_chunks",
                  "donate", "order", "max_waiters", "hunger_period", "donate_height",
-                 "reserved")]
+                 "graph")]
 
 
 class SearchStats(ctypes.Structure):
@@ -115,6 +115,10 @@ def lib():
         L.qcp_batch_total_hits.restype = ctypes.c_uint64
         L.qcp_search.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(SearchCfg), P, P, P, P,
                                  ctypes.c_uint64, ctypes.POINTER(SearchStats)]
+        L.qcp_device_alloc.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.qcp_device_free.argtypes = [P]
+        L.qcp_copy_to_device.argtypes = [P, P, ctypes.c_uint64]
+        L.qcp_copy_to_host.argtypes = [P, P, ctypes.c_uint64]
         L.qcp_server_start.argtypes = [P, ctypes.POINTER(ServeCfg), ctypes.POINTER(P)]
         L.qcp_server_submit.argtypes = [P, P, ctypes.POINTER(ctypes.c_uint64)]
         L.qcp_server_result.argtypes = [P, ctypes.c_uint64, P, P, P]
@@ -146,6 +150,39 @@ def device_count():
 
 
 # ---------------------------------------------------------------------------
+class DeviceBuffer:
+    """A caller-owned HBM buffer holding a copy of a host array."""
+
+    def __init__(self, host, device=0):
+        host = np.ascontiguousarray(host)
+        self.nbytes = host.nbytes
+        self.shape, self.dtype = host.shape, host.dtype
+        self.ptr = P()
+        check(lib().qcp_device_alloc(device, self.nbytes, ctypes.byref(self.ptr)), "qcp_device_alloc")
+        check(lib().qcp_copy_to_device(self.ptr, _p(host), self.nbytes), "qcp_copy_to_device")
+
+    def write(self, host):
+        host = np.ascontiguousarray(host)
+        assert host.nbytes == self.nbytes
+        check(lib().qcp_copy_to_device(self.ptr, _p(host), self.nbytes), "qcp_copy_to_device")
+
+    def read(self):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().qcp_copy_to_host(_p(out), self.ptr, self.nbytes), "qcp_copy_to_host")
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().qcp_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Index:
     """A device-resident tree built by libqcp.so."""
 
@@ -156,16 +193,38 @@ class Index:
 
     @classmethod
     def build(cls, objs, dims, fanout=64, layout="mphr", key_mode="ref", key_src="lo",
-              obj_kind="points", device=0):
+              obj_kind="points", device=0, graph=False):
+        """Build from host objects (N x D points or N x 2D boxes).  With
+        graph=True the device pipeline runs as a captured hipGraph."""
+        if isinstance(objs, DeviceBuffer):
+            return cls.build_device(objs, dims, fanout, layout, key_mode, key_src, obj_kind,
+                                    device, graph)
         objs = np.ascontiguousarray(objs, dtype=np.float32)
         E = dims * (2 if obj_kind == "boxes" else 1)
         n = objs.size // E if objs.size else 0
         cfg = BuildCfg(dims, fanout, LAYOUT[layout], KEY[key_mode], SRC[key_src],
-                       OBJ[obj_kind], device, 0)
+                       OBJ[obj_kind], device, 1 if graph else 0)
         h = P()
         check(lib().qcp_build(ctypes.byref(cfg), _p(objs) if n else None, n, ctypes.byref(h)),
               "qcp_build")
         return cls(h, dims, obj_kind)
+
+    @classmethod
+    def build_device(cls, buf, dims, fanout=64, layout="mphr", key_mode="ref", key_src="lo",
+                     obj_kind="points", device=0, graph=False):
+        """Build from objects already resident in HBM (a DeviceBuffer)."""
+        E = dims * (2 if obj_kind == "boxes" else 1)
+        n = buf.nbytes // (4 * E)
+        cfg = BuildCfg(dims, fanout, LAYOUT[layout], KEY[key_mode], SRC[key_src],
+                       OBJ[obj_kind], device, 1 if graph else 0)
+        h = P()
+        check(lib().qcp_build_device(ctypes.byref(cfg), buf.ptr, n, ctypes.byref(h)),
+              "qcp_build_device")
+        return cls(h, dims, obj_kind)
+
+    def rebuild(self, buf):
+        """Rebuild in place from a DeviceBuffer of the same size (timed builds)."""
+        check(lib().qcp_rebuild_device(self._h, buf.ptr), "qcp_rebuild_device")
 
     @classmethod
     def load(cls, path, device=0):
@@ -257,14 +316,15 @@ class Batch:
             pass
 
     def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=1,
-            hash=True, chunks=0, order=0, max_waiters=0, hunger_period=0, donate_height=0):
+            hash=True, chunks=0, order=0, max_waiters=0, hunger_period=0, donate_height=0,
+            graph=False):
         """One search of the resident batch.  donate=1 enables subtree donation
         (work stealing) in the persistent form; chunks = 64-entry leaf chunks in
         flight per wave (0 = default); order=0 processes the queries in Hilbert
         order of their centres, one range per XCD, order=1 as given."""
         outputs = OUT_COUNTS | (OUT_HASH if hash else 0) | (OUT_IDS if ids else 0)
         cfg = SearchCfg(MODE[mode], outputs, plain_batch, 0, grid_blocks, chunks, donate, order,
-                        max_waiters, hunger_period, donate_height, 0)
+                        max_waiters, hunger_period, donate_height, 1 if graph else 0)
         st = SearchStats()
         check(lib().qcp_batch_run(self._h, ctypes.byref(cfg), ctypes.byref(st)), "qcp_batch_run")
         self._ids = ids
