@@ -110,6 +110,159 @@ def serving_bench(qcp, rates, seconds, scale):
             "rates": out}
 
 
+def oracle_tree_from(ix):
+    """The oracle's view of a GPU-built tree (exported), for the CPU baseline."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    ex = ix.export()
+    inf = ex["info"]
+    t = O.Tree.__new__(O.Tree)
+    t.sobjs = np.ascontiguousarray(ex["sobjs"])
+    t.n = inf["n"]
+    t.D, t.is_box, t.F = inf["dims"], int(inf["obj_kind"] == 1), inf["fanout"]
+    t.layout = "bvh" if inf["layout"] == 2 else "packed"
+    t.n_nodes = inf["n_nodes"]
+    t.first, t.count, t.leaf = ex["first"], ex["count"], ex["leaf"]
+    t.boxes = np.ascontiguousarray(ex["boxes"])
+    return t
+
+
+def cpu_sample(t, queries, got, budget_s, threads):
+    """Time the oracle DFS over a query prefix sized to about budget_s."""
+    probe = min(len(queries), max(threads * 2, 32))
+    t0 = time.perf_counter()
+    t.search_dfs_mt(queries[:probe], threads)
+    per_q = max((time.perf_counter() - t0) / probe, 1e-9)
+    m = int(min(len(queries), max(probe, budget_s / per_q)))
+    t0 = time.perf_counter()
+    c, h = t.search_dfs_mt(queries[:m], threads)
+    dt = time.perf_counter() - t0
+    ok = bool(np.array_equal(c, got["counts"][:m]) and np.array_equal(h, got["hashes"][:m]))
+    return {"value": m / dt, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": "first %d of %d queries, oracle DFS over %d threads, %.1f s" %
+                      (m, len(queries), threads, dt), "matches_gpu": ok}
+
+
+def timed_search(b, steps, **kw):
+    b.run(**kw)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = b.run(**kw)
+        kms.append(st["kernel_ms"])
+    wall = time.perf_counter() - t0
+    return st, wall / steps, float(np.mean(kms))
+
+
+def run_config(qcp, name, args):
+    """Measurement of one of the other configurations; returns a JSON-able dict."""
+    threads = min(os.cpu_count() or 1, 16)
+    out = {"config": name}
+    if name == "c1":
+        w = qcp.workload("c1")
+        pts, q = w["objs"], w["queries"]
+        ix = qcp.Index.build(pts, 2, fanout=64, key_mode="b16")
+        t = oracle_tree_from(ix)
+        exp = t.search_dfs(q, want_ids=True)
+        with ix.batch(q) as b:
+            st, wall, kms = timed_search(b, args.steps, ids=True)
+            got = b.results(ids=True)
+            plain = b.run(mode="plain", plain_batch=128)
+        ok = bool(np.array_equal(got["counts"], exp["counts"]) and
+                  np.array_equal(got["ids"], exp["ids"]))
+        out.update({"workload": "1,048,576 uniform 2-D points, b16 keys, F=64, 1,000 "
+                                "queries of side 0.01, counts + sorted ID lists",
+                    "persistent_qps": len(q) / wall, "persistent_ms": wall * 1e3,
+                    "kernel_ms": kms, "plain_qps": len(q) / (plain["total_ms"] * 1e-3),
+                    "total_hits": int(exp["counts"].sum()), "bit_exact_ids": ok,
+                    "build_ms": ix.info()["build_ms"],
+                    "cpu_baseline": cpu_sample(t, q, got, 5.0, threads)})
+        return out
+    if name == "c3":
+        w = qcp.workload("c3", scale=args.scale)
+        objs, q = w["objs"], w["queries"]
+        buf = qcp.DeviceBuffer(objs)
+        res = {}
+        for layout in ("rtree", "bvh"):
+            log("c3 %s build" % layout)
+            ix = qcp.Index.build(buf, 2, fanout=128, layout=layout, obj_kind="boxes",
+                                 key_src="center", graph=True)
+            ix.rebuild(buf)
+            binfo = ix.info()
+            with ix.batch(q) as b:
+                st, wall, kms = timed_search(b, max(2, args.steps // 3), graph=True)
+                got = b.results()
+            abytes = algorithmic_bytes(st, len(q), 2, 4)
+            t = oracle_tree_from(ix)
+            cpu = cpu_sample(t, q, got, args.cpu_budget, threads)
+            res[layout] = {"queries_per_s": len(q) / wall, "ms_per_step": wall * 1e3,
+                           "kernel_ms": kms, "total_hits": int(st["total_hits"]),
+                           "algorithmic_GBps": abytes / (kms * 1e-3) / 1e9,
+                           "build_ms_graph": binfo["build_ms"],
+                           "nodes": binfo["n_nodes"], "levels": binfo["level_count"],
+                           "cpu_baseline": cpu}
+            log("c3 %s: %.3e q/s, %.1f ms, build %.1f ms, cpu %.3e q/s" %
+                (layout, len(q) / wall, wall * 1e3, binfo["build_ms"], cpu["value"]))
+            ix.close()
+        buf.close()
+        out.update({"workload": "67,108,864 2-D boxes (uniform lower corners, exponential "
+                                "sides mean 1e-4), Hilbert order of centres, F=128; 1,000,000 "
+                                "queries (90%% side 1e-3, 10%% side 0.0316); hipGraph-captured "
+                                "search, counts+hash", "layouts": res})
+        return out
+    if name == "c5":
+        w = qcp.workload("c5", scale=args.scale)
+        pts, q = w["objs"], w["queries"]
+        buf = qcp.DeviceBuffer(pts)
+        log("c5 build")
+        ix = qcp.Index.build(buf, 2, fanout=64)
+        eager = []
+        for _ in range(3):
+            ix.rebuild(buf)
+            eager.append(ix.info())
+        gix = qcp.Index.build(buf, 2, fanout=64, graph=True)
+        graph_ms = []
+        for _ in range(3):
+            gix.rebuild(buf)
+            graph_ms.append(gix.info()["build_ms"])
+        assert np.array_equal(gix.export()["boxes"], ix.export()["boxes"])
+        gix.close()
+        n = len(pts)
+        best = min(eager, key=lambda i: i["build_ms"])
+        res = {"n": n, "build_ms_eager": best["build_ms"],
+               "build_phase_ms": dict(zip(("keys", "sort", "gather", "pack"),
+                                          best["build_phase_ms"][:4])),
+               "build_mkeys_per_s": n / (best["build_ms"] * 1e-3) / 1e6,
+               "build_ms_graph": min(graph_ms),
+               "build_mkeys_per_s_graph": n / (min(graph_ms) * 1e-3) / 1e6}
+        log("c5 build: %.1f ms eager (%.0f Mkeys/s), %.1f ms graph" %
+            (best["build_ms"], res["build_mkeys_per_s"], min(graph_ms)))
+        with ix.batch(q) as b:
+            for donate in (1, 0):
+                st, wall, kms = timed_search(b, 1, donate=donate)
+                res["search_donate%d" % donate] = {
+                    "queries_per_s": len(q) / wall, "ms_per_step": wall * 1e3,
+                    "kernel_ms": kms, "total_hits": int(st["total_hits"]),
+                    "donated": int(st["donated_items"]),
+                    "algorithmic_GBps": algorithmic_bytes(st, len(q), 2, 2) / (kms * 1e-3) / 1e9}
+                log("c5 search donate=%d: %.3e q/s, %.1f ms, %d donated" %
+                    (donate, len(q) / wall, wall * 1e3, st["donated_items"]))
+            got = b.results()
+        t = oracle_tree_from(ix)
+        res["cpu_baseline"] = cpu_sample(t, q, got, args.cpu_budget, threads)
+        with ix.batch(q[:1024]) as pb:
+            pst = pb.run(mode="plain", plain_batch=128)
+        res["plain_qps_first_1024"] = 1024 / (pst["total_ms"] * 1e-3)
+        buf.close()
+        ix.close()
+        out.update({"workload": "134,217,728 2-D points (80%% from 16 Zipf(1.2) hotspots, "
+                                "sigma 0.002; 20%% uniform); 500,000 data-centred queries, "
+                                "sides U(1e-4,1e-2); counts+hash", **res})
+        return out
+    raise SystemExit("unknown config " + name)
+
+
 def read_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
@@ -133,6 +286,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--config", default="c2",
+                    help="c2 (default: the headline line); c1, c3 or c5 print their own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the box has one); --gpus %d is not "
@@ -142,6 +297,9 @@ def main():
     if qcp.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.config != "c2":
+        print(json.dumps(run_config(qcp, args.config, args)), flush=True)
+        return
     log("generating workload c2 (scale %.3f)" % args.scale)
     w = qcp.workload("c2", scale=args.scale)
     pts, queries = w["objs"], w["queries"]
