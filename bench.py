@@ -263,6 +263,33 @@ def run_config(qcp, name, args):
     raise SystemExit("unknown config " + name)
 
 
+def profile_traffic(qcp, args):
+    """A short, fixed run for counter collection: steps persistent searches of
+    the 16M-point config at --fanout (default 64), then one 1 GiB hipMemcpy
+    device-to-device (a copy kernel of known size, to calibrate FETCH_SIZE and
+    WRITE_SIZE).  Prints the algorithmic bytes per search launch."""
+    import ctypes
+
+    F = args.fanout or 64
+    w = qcp.workload("c2", scale=args.scale)
+    ix = qcp.Index.build(w["objs"], 3, fanout=F)
+    q = w["queries"]
+    with ix.batch(q) as b:
+        for _ in range(args.steps):
+            st = b.run()
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 1 << 30
+    src = qcp.DeviceBuffer(np.zeros(nbytes // 4, np.float32))
+    dst = qcp.DeviceBuffer(np.zeros(nbytes // 4, np.float32))
+    rc = hip.hipMemcpy(dst.ptr, src.ptr, ctypes.c_size_t(nbytes), 3)  # device to device
+    hip.hipDeviceSynchronize()
+    print(json.dumps({"fanout": F, "queries": len(q), "steps": args.steps, "copy_bytes": nbytes,
+                      "copy_rc": rc, "algorithmic_bytes": algorithmic_bytes(st, len(q), 3, 3),
+                      "internal_entries": st["internal_entries"],
+                      "leaf_entries": st["leaf_entries"], "kernel_ms": st["kernel_ms"]}),
+          flush=True)
+
+
 def read_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
@@ -286,6 +313,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--profile-traffic", action="store_true",
+                    help="short run for rocprofv3 --pmc: the search at the given fanout, "
+                         "then a 1 GiB device-to-device copy as the calibration kernel")
     ap.add_argument("--config", default="c2",
                     help="c2 (default: the headline line); c1, c3 or c5 print their own line")
     args = ap.parse_args()
@@ -299,6 +329,9 @@ def main():
         sys.exit(1)
     if args.config != "c2":
         print(json.dumps(run_config(qcp, args.config, args)), flush=True)
+        return
+    if args.profile_traffic:
+        profile_traffic(qcp, args)
         return
     log("generating workload c2 (scale %.3f)" % args.scale)
     w = qcp.workload("c2", scale=args.scale)
