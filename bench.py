@@ -274,9 +274,10 @@ def profile_traffic(qcp, args):
     w = qcp.workload("c2", scale=args.scale)
     ix = qcp.Index.build(w["objs"], 3, fanout=F)
     q = w["queries"]
+    kw = {"block_waves": args.block_waves, "chunks": args.chunks, "donate": args.donate}
     with ix.batch(q) as b:
         for _ in range(args.steps):
-            st = b.run()
+            st = b.run(**kw)
     hip = ctypes.CDLL("libamdhip64.so")
     nbytes = 1 << 30
     src = qcp.DeviceBuffer(np.zeros(nbytes // 4, np.float32))
@@ -316,6 +317,9 @@ def main():
     ap.add_argument("--profile-traffic", action="store_true",
                     help="short run for rocprofv3 --pmc: the search at the given fanout, "
                          "then a 1 GiB device-to-device copy as the calibration kernel")
+    ap.add_argument("--block-waves", type=int, default=0, help="--profile-traffic: waves/WG")
+    ap.add_argument("--chunks", type=int, default=0, help="--profile-traffic: leaf chunks")
+    ap.add_argument("--donate", type=int, default=0, help="--profile-traffic: donation")
     ap.add_argument("--config", default="c2",
                     help="c2 (default: the headline line); c1, c3 or c5 print their own line")
     args = ap.parse_args()
@@ -337,34 +341,27 @@ def main():
     w = qcp.workload("c2", scale=args.scale)
     pts, queries = w["objs"], w["queries"]
     n, q, D, E = len(pts), len(queries), 3, 3
-    # autotune fanout on a query sample (untimed setup)
-    cands = [args.fanout] if args.fanout else [32, 64, 128]
-    best = None
-    tune = {}
+    # the library's autotuner on a query sample (untimed setup): fanout x
+    # workgroup size x leaf chunks in flight x donation
+    cands = (args.fanout,) if args.fanout else (32, 64, 128)
     sample = queries[: min(q, 20000)]
-    for F in cands:
-        ix = qcp.Index.build(pts, 3, fanout=F, layout="mphr", key_mode="ref")
-        with ix.batch(sample) as b:
-            b.run()
-            ms = min(b.run()["kernel_ms"] for _ in range(3))
-        tune[F] = ms
-        log("autotune F=%d: %.3f ms for %d queries; build %.2f ms" % (F, ms, len(sample),
-                                                                       ix.info()["build_ms"]))
-        if best is None or ms < tune[best[0]]:
-            if best is not None:
-                best[1].close()
-            best = (F, ix)
-        else:
-            ix.close()
-    F, ix = best
+    ix, kw, table = qcp.autotune(pts, 3, sample, fanouts=cands)
     info = ix.info()
+    F = info["fanout"]
+    best_ms = min(r["kernel_ms"] for r in table)
+    tune = {"best": dict(kw, fanout=F, sample_ms=best_ms), "sample_queries": len(sample),
+            "candidates": len(table),
+            "best_per_fanout_ms": {str(f): min(r["kernel_ms"] for r in table if r["fanout"] == f)
+                                   for f in cands}}
+    log("autotune: F=%d %s (%.3f ms on %d sample queries; %d candidates)" %
+        (F, kw, best_ms, len(sample), len(table)))
     b = ix.batch(queries)
     for _ in range(args.warmup):
-        b.run()
+        b.run(**kw)
     kms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        st = b.run()
+        st = b.run(**kw)
         kms.append(st["kernel_ms"])
     wall = time.perf_counter() - t0
     got = b.results()
@@ -398,7 +395,9 @@ def main():
         ix.close()
         serving = serving_bench(qcp, (10000, 50000, 100000), args.serve_seconds, args.scale)
     traffic = read_traffic("traffic_c2_search.json")
-    kname = "k_search"
+    kname = "k_search<3,false,%d,false,1,%d> (block %d threads, donation %s)" % (
+        (F + 63) // 64, kw["chunks"] or 4, 64 * (kw["block_waves"] or 4),
+        "on" if kw["donate"] else "off")
     out = {
         "metric": "range queries/sec (batched persistent, 16M-point 3-D config)",
         "value": value,

@@ -195,6 +195,26 @@ qcp_status qcp_search(qcp_index *ix, const float *boxes, uint64_t q,
                       uint64_t *offsets, uint32_t *ids, uint64_t ids_cap,
                       qcp_search_stats *stats);
 
+/* ---- autotuner ---------------------------------------------------------------------
+ * Replaces the reference's evaluation-mode parameter sweeps (-e,
+ * src/evaluator/evaluator.cpp:171-280): builds the index at every candidate
+ * fanout (cfg->fanout is ignored) from device-resident objects, and times the
+ * persistent search of q device-resident sample queries for every workgroup
+ * size {1, 2, 4, 8 waves}, leaf-chunk count {2, 4, 8} (packed layouts) and
+ * donation {off, on}.  Returns the index built at the best fanout (owned by the
+ * caller) and the best search configuration; table (optional) receives every
+ * measurement. */
+typedef struct {
+  uint32_t fanout, block_waves, leaf_chunks, donate;
+  double kernel_ms;          /* best of two timed runs of the sample */
+} qcp_tune_entry;
+
+qcp_status qcp_autotune(const qcp_build_cfg *cfg, const float *d_objs, uint64_t n,
+                        const float *d_queries, uint64_t q, const uint32_t *fanouts,
+                        uint32_t n_fanouts, uint32_t outputs, qcp_index **best_index,
+                        qcp_search_cfg *best, qcp_tune_entry *table, uint32_t table_cap,
+                        uint32_t *table_len);
+
 /* ---- serving: continuous batching over a resident index -----------------------
  * New (the reference has no serving path; its closest interface is the
  * per-batch MPHR::Search loop, src/tree/mphr.cpp:231-258).  A server keeps the

@@ -701,9 +701,11 @@ static SearchArgs make_args(const qcp_batch* b) {
 }
 
 unsigned qcp::persistent_grid(const qcp_index* ix, int out, unsigned block) {
-  const void* k = search_kernel_ptr(ix->D, ix->box, ix->expl, out, (int)((ix->F + 63) / 64));
+  const int mw = (int)((ix->F + 63) / 64);
+  const void* k = search_kernel_ptr(ix->D, ix->box, ix->expl, out, mw);
   int per_cu = 0;
-  if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, (int)block, 0) != hipSuccess ||
+  if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, k, (int)block, search_lds_bytes(ix->expl, mw, block)) != hipSuccess ||
       per_cu <= 0)
     per_cu = 2;
   return (unsigned)(ix->cus * per_cu);
@@ -801,7 +803,8 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     if (stats) *stats = st;
     return QCP_OK;
   }
-  const unsigned block = 256;
+  if (cfg.block_waves > 8) return QCP_EINVAL;
+  const unsigned block = cfg.block_waves ? 64u * cfg.block_waves : 256u;
   const int out_mode = (cfg.outputs & QCP_OUT_HASH) ? 1 : 0;
   const int u = cfg.leaf_chunks ? (int)cfg.leaf_chunks : 4;
   const bool donate = cfg.donate != 0;
@@ -839,7 +842,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   if (use_graph) {
     const uint32_t key[12] = {1u, (uint32_t)out_mode, (uint32_t)u, grid, donate ? 1u : 0u,
                               cfg.order, a.max_waiters, a.hunger_period, a.donate_min_height,
-                              0u, 0u, 0u};
+                              block, 0u, 0u};
     if (!b->exec || memcmp(key, b->exec_key, sizeof(key)) != 0) {
       if (b->exec) {
         hipGraphExecDestroy(b->exec);

@@ -44,5 +44,7 @@ hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, i
 hipError_t launch_mprs(const SearchArgs& a, unsigned D, bool box, unsigned nblocks,
                        hipStream_t s);
 const void* search_kernel_ptr(unsigned D, bool box, bool expl, int out, int mw);
+// dynamic LDS bytes of a persistent-search workgroup of `block` threads
+size_t search_lds_bytes(bool expl, int mw, unsigned block);
 
 }  // namespace qcp

@@ -401,10 +401,11 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
 // be written by a donor, or for `done`; at most max_waiters waves wait, the
 // others exit.
 template <int D, bool BOX, int MW, bool EXPL, int OUT, int U>
-__global__ __launch_bounds__(256) void k_search(SearchArgs a) {
+__global__ __launch_bounds__(512) void k_search(SearchArgs a) {
+  // one DFS frame stack per wave in dynamic LDS (blocks of 1 to 8 waves)
   constexpr int kDepth = EXPL ? kMaxDepth : kMaxLevels;
-  __shared__ Frame<MW> stack[256 / kWave][kDepth];
-  Frame<MW>* stk = stack[threadIdx.x / kWave];
+  extern __shared__ unsigned char qcp_lds[];
+  Frame<MW>* stk = reinterpret_cast<Frame<MW>*>(qcp_lds) + (threadIdx.x / kWave) * kDepth;
   const uint32_t lane = lane_id();
   uint64_t ie = 0, le = 0, hits = 0;
   const bool donating = OUT != kOutIds && a.donate;
@@ -618,14 +619,32 @@ __global__ void k_reset_ctl(QueueCtl* ctl, uint32_t pending) {
     default: { constexpr int U = 4; __VA_ARGS__; } break; \
   }
 
+// Dynamic LDS of a persistent-search workgroup: one frame stack per wave.
+template <int MW, bool EXPL>
+static size_t lds_for(unsigned block) {
+  return (size_t)(block / kWave) * (EXPL ? kMaxDepth : kMaxLevels) * sizeof(Frame<MW>);
+}
+
+size_t search_lds_bytes(bool expl, int mw, unsigned block) {
+  switch (mw) {
+    case 1: return expl ? lds_for<1, true>(block) : lds_for<1, false>(block);
+    case 2: return expl ? lds_for<2, true>(block) : lds_for<2, false>(block);
+    case 3: return expl ? lds_for<3, true>(block) : lds_for<3, false>(block);
+    default: return expl ? lds_for<4, true>(block) : lds_for<4, false>(block);
+  }
+}
+
 template <int D, bool BOX, bool EXPL, int OUT>
 static hipError_t launch_search_t(const SearchArgs& a, int mw, int u, unsigned grid,
                                   unsigned block, hipStream_t s) {
+  if (block % kWave || block < kWave || block > 512) return hipErrorInvalidValue;
   if (!EXPL && OUT != kOutIds) {
-    QCP_U_SWITCH(u, QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, OUT, U><<<grid, block, 0, s>>>(a)));
+    QCP_U_SWITCH(u, QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, OUT, U>
+                                  <<<grid, block, (lds_for<MW, EXPL>(block)), s>>>(a)));
   } else {
     constexpr int U = 4;
-    QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, OUT, U><<<grid, block, 0, s>>>(a));
+    QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, OUT, U>
+                      <<<grid, block, (lds_for<MW, EXPL>(block)), s>>>(a));
   }
   return hipGetLastError();
 }

@@ -65,6 +65,12 @@ class SearchStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("fanout", ctypes.c_uint32), ("block_waves", ctypes.c_uint32),
+                ("leaf_chunks", ctypes.c_uint32), ("donate", ctypes.c_uint32),
+                ("kernel_ms", ctypes.c_double)]
+
+
 class ServeCfg(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("result_budget", ctypes.c_uint64), ("max_requests", ctypes.c_uint32),
@@ -115,6 +121,11 @@ def lib():
         L.qcp_batch_total_hits.restype = ctypes.c_uint64
         L.qcp_search.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(SearchCfg), P, P, P, P,
                                  ctypes.c_uint64, ctypes.POINTER(SearchStats)]
+        L.qcp_autotune.argtypes = [ctypes.POINTER(BuildCfg), P, ctypes.c_uint64, P,
+                                   ctypes.c_uint64, P, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.POINTER(P), ctypes.POINTER(SearchCfg),
+                                   ctypes.POINTER(TuneEntry), ctypes.c_uint32,
+                                   ctypes.POINTER(ctypes.c_uint32)]
         L.qcp_device_alloc.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(P)]
         L.qcp_device_free.argtypes = [P]
         L.qcp_copy_to_device.argtypes = [P, P, ctypes.c_uint64]
@@ -278,10 +289,11 @@ class Index:
         return Batch(self, queries)
 
     def search(self, queries, mode="persistent", ids=False, visits=False, plain_batch=128,
-               grid_blocks=0, donate=1, chunks=0, hash=True, order=0):
+               grid_blocks=0, donate=0, chunks=0, hash=True, order=0, block_waves=0):
         with self.batch(queries) as b:
             st = b.run(mode=mode, ids=ids, plain_batch=plain_batch, grid_blocks=grid_blocks,
-                       donate=donate, chunks=chunks, hash=hash, order=order)
+                       donate=donate, chunks=chunks, hash=hash, order=order,
+                       block_waves=block_waves)
             res = b.results(ids=ids, visits=visits)
             res["stats"] = st
             return res
@@ -315,16 +327,18 @@ class Batch:
         except Exception:
             pass
 
-    def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=1,
+    def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=0,
             hash=True, chunks=0, order=0, max_waiters=0, hunger_period=0, donate_height=0,
-            graph=False):
+            graph=False, block_waves=0):
         """One search of the resident batch.  donate=1 enables subtree donation
         (work stealing) in the persistent form; chunks = 64-entry leaf chunks in
         flight per wave (0 = default); order=0 processes the queries in Hilbert
-        order of their centres, one range per XCD, order=1 as given."""
+        order of their centres, one range per XCD, order=1 as given;
+        block_waves = waves per workgroup (0 = 4)."""
         outputs = OUT_COUNTS | (OUT_HASH if hash else 0) | (OUT_IDS if ids else 0)
-        cfg = SearchCfg(MODE[mode], outputs, plain_batch, 0, grid_blocks, chunks, donate, order,
-                        max_waiters, hunger_period, donate_height, 1 if graph else 0)
+        cfg = SearchCfg(MODE[mode], outputs, plain_batch, block_waves, grid_blocks, chunks,
+                        donate, order, max_waiters, hunger_period, donate_height,
+                        1 if graph else 0)
         st = SearchStats()
         check(lib().qcp_batch_run(self._h, ctypes.byref(cfg), ctypes.byref(st)), "qcp_batch_run")
         self._ids = ids
@@ -355,6 +369,37 @@ class Batch:
         if visits:
             out["visits"] = vis[: 2 * q].reshape(q, 2)
         return out
+
+
+def autotune(objs, dims, sample_queries, fanouts=(32, 64, 128), layout="mphr", key_mode="ref",
+             key_src="lo", obj_kind="points", device=0, hash=True):
+    """Run the library's autotuner: returns (Index built at the best fanout,
+    dict of Batch.run keyword arguments, list of all measurements)."""
+    objs_buf = objs if isinstance(objs, DeviceBuffer) else DeviceBuffer(
+        np.ascontiguousarray(objs, np.float32), device)
+    qs = np.ascontiguousarray(sample_queries, np.float32).reshape(-1, 2 * dims)
+    qbuf = DeviceBuffer(qs, device)
+    E = dims * (2 if obj_kind == "boxes" else 1)
+    n = objs_buf.nbytes // (4 * E)
+    cfg = BuildCfg(dims, 0, LAYOUT[layout], KEY[key_mode], SRC[key_src], OBJ[obj_kind], device, 0)
+    fs = np.ascontiguousarray(fanouts, np.uint32)
+    h = P()
+    best = SearchCfg()
+    cap = 4096
+    table = (TuneEntry * cap)()
+    tlen = ctypes.c_uint32()
+    outputs = OUT_COUNTS | (OUT_HASH if hash else 0)
+    check(lib().qcp_autotune(ctypes.byref(cfg), objs_buf.ptr, n, qbuf.ptr, len(qs), _p(fs),
+                             len(fs), outputs, ctypes.byref(h), ctypes.byref(best), table, cap,
+                             ctypes.byref(tlen)), "qcp_autotune")
+    qbuf.close()
+    if objs_buf is not objs:
+        objs_buf.close()
+    ix = Index(h, dims, obj_kind)
+    kw = {"block_waves": best.block_waves, "chunks": best.leaf_chunks, "donate": best.donate}
+    rows = [{"fanout": e.fanout, "block_waves": e.block_waves, "leaf_chunks": e.leaf_chunks,
+             "donate": e.donate, "kernel_ms": e.kernel_ms} for e in table[: tlen.value]]
+    return ix, kw, rows
 
 
 class Server:
