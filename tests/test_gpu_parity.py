@@ -51,8 +51,10 @@ def check_case(gpu, O, objs, D, is_box, F, key_mode="ref", key_src="lo", layout=
     # the other persistent variants: no donation, counts only, 2 and 8 chunks in flight
     nd = ix.search(queries, mode="persistent", donate=0, chunks=8, order=1)
     assert_results_equal(nd, exp, tag + " persistent no-donation, given order", ids=False)
-    co = ix.search(queries, mode="persistent", hash=False, chunks=2)
-    assert_bits_equal(co["counts"], exp["counts"], tag + " counts-only")
+    co = ix.search(queries, mode="persistent", hash=False, chunks=2, block_waves=1)
+    assert_bits_equal(co["counts"], exp["counts"], tag + " counts-only, 1-wave workgroups")
+    w8 = ix.search(queries, mode="persistent", donate=1, block_waves=8)
+    assert_results_equal(w8, exp, tag + " 8-wave workgroups with donation", ids=False)
     if plain and layout != "bvh":
         mp = t.search_mprs(queries)
         assert_bits_equal(mp["counts"], exp["counts"], tag + " oracle MPRS vs DFS")
@@ -121,6 +123,21 @@ def test_donation_heavy_queries(gpu, O, layout, F):
         assert_results_equal(got, exp, "donation %s F=%d rep %d" % (layout, F, rep))
         assert got["stats"]["donated_items"] > 0
         assert got["stats"]["leaf_entries"] == int(exp["leaf_entries"].sum())
+
+
+def test_autotune_returns_consistent_index(gpu, O):
+    pts = rand_points(100000, 3, 77)
+    q = rand_queries(500, 3, 78, pts)
+    ix, kw, table = gpu.autotune(pts, 3, q[:200], fanouts=(16, 64))
+    assert len(table) == 2 * 4 * 3 * 2
+    best = min(table, key=lambda r: r["kernel_ms"])
+    assert ix.info()["fanout"] == best["fanout"]
+    assert kw == {"block_waves": best["block_waves"], "chunks": best["leaf_chunks"],
+                  "donate": best["donate"]}
+    t = O.build_packed(pts, 3, False, best["fanout"], O.KEY_REF)
+    exp = t.search_dfs(q)
+    got = ix.search(q, **kw)
+    assert_results_equal(got, exp, "autotuned", ids=False)
 
 
 def test_empty_index(gpu):
