@@ -156,6 +156,8 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
           int k2 = (m & lm).run_end(k);
           m.clear_range(k, k2);
           run_range(k, k2, cs, ce);
+          cs = uni64(cs);
+          ce = uni64(ce);
           c.le += ce - cs;
         }
       }
@@ -328,10 +330,14 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
     if (m.empty()) {
       if (sp == 0) break;
       --sp;
-      cur = stk[sp].node;
-      level = (int)stk[sp].level;
-      m = stk[sp].m;
-      lm = stk[sp].lm;
+      // frames are wave-uniform: readfirstlane keeps them in SGPRs
+      cur = uni(stk[sp].node);
+      level = (int)uni(stk[sp].level);
+#pragma unroll
+      for (int i = 0; i < MW; ++i) {
+        m.w[i] = uni64(stk[sp].m.w[i]);
+        lm.w[i] = uni64(stk[sp].lm.w[i]);
+      }
       continue;
     }
     const int k = m.lowest();
@@ -343,18 +349,23 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
       since = 0;
       const int max_level = EXPL ? kMaxDepth : a.h - 2 - (int)a.donate_min_height;
       int f = 0;
-      while (f < sp && stk[f].m.empty()) ++f;
-      const bool have = (f < sp) ? ((int)stk[f].level <= max_level)
-                                 : (!m.empty() && level <= max_level);
+      Mask<MW> fm;
+      for (; f < sp; ++f) {
+#pragma unroll
+        for (int i = 0; i < MW; ++i) fm.w[i] = uni64(stk[f].m.w[i]);
+        if (!fm.empty()) break;
+      }
+      const uint32_t f_node = f < sp ? uni(stk[f].node) : cur;
+      const int f_level = f < sp ? (int)uni(stk[f].level) : level;
+      const bool have = (f < sp) ? (f_level <= max_level) : (!m.empty() && level <= max_level);
       if (have) {
         const uint32_t head = ld_acq(&a.ctl->head);
         const uint32_t tail = ld_acq(&a.ctl->tail);
         if (head > tail && tail < a.slot_cap) {
           const uint32_t waiting = head - tail;
           if (f < sp) {
-            Mask<MW> dm = stk[f].m;
-            donate<MW, EXPL>(a, c.q, stk[f].node, (int)stk[f].level, dm, waiting);
-            stk[f].m = dm;
+            donate<MW, EXPL>(a, c.q, f_node, f_level, fm, waiting);
+            stk[f].m = fm;
           } else {
             donate<MW, EXPL>(a, c.q, cur, level, m, waiting);
           }
