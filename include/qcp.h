@@ -12,7 +12,8 @@
  *     for some d, in FP32 (src/node/node_soa.cpp:94-113);
  *   - per-query hit counts (the reference's per-block h_hit[],
  *     src/tree/mphr.cpp:244-252) plus, new here, sorted rank-ID lists and an
- *     order-independent ID-set hash.
+ *     order-independent ID-set hash: the sum over hit rank IDs of
+ *     id*id + 0x9E3779B97F4A7C15*id, mod 2^64.
  * Conventions that differ from the reference on purpose: every error is a
  * status code (nothing calls exit()), every handle owns its device memory and
  * HIP stream (no device globals, no singletons), and inputs are caller-owned.

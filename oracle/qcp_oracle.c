@@ -500,13 +500,13 @@ void orc_tree_boxes(const float *sobjs, unsigned D, int is_box, unsigned F,
 }
 
 /* ---- search ------------------------------------------------------------------------- */
-/* Order-independent set hash of rank IDs: the sum over IDs of the splitmix64
- * finalizer (defined by this build for count+hash result mode). */
+/* Order-independent fingerprint of a set of rank IDs (defined by this build for
+ * the count+hash result mode): the sum over IDs of id^2 + K*id mod 2^64 with
+ * K = 0x9E3779B97F4A7C15, i.e. (sum of squares) + K * (sum).  This is the
+ * per-ID term. */
 uint64_t orc_id_hash(uint32_t id) {
-  u64 z = (u64)id + 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
+  const u64 x = (u64)id;
+  return x * x + 0x9E3779B97F4A7C15ull * x;
 }
 
 /* Node_SOA::IsOverlap / Node::IsOverlap (src/node/node_soa.cpp:94-113,

@@ -58,13 +58,18 @@ struct SearchArgs {
   uint32_t q_begin;          // plain form: first query of the launch
 };
 
-__device__ __forceinline__ uint64_t id_hash(uint32_t id) {
-  // splitmix64 finalizer of the ID: the per-ID term of the order-independent
-  // set hash (sum over IDs, mod 2^64).
-  uint64_t z = (uint64_t)id + 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
+// Order-independent fingerprint of a set of rank IDs: sum of squares plus
+// kHashK times the sum, mod 2^64 (the sum over IDs of id^2 + kHashK*id).
+// Kernels accumulate the two power sums and combine them at the end.
+constexpr uint64_t kHashK = 0x9E3779B97F4A7C15ull;
+
+__device__ __forceinline__ uint64_t set_hash(uint64_t s1, uint64_t s2) { return s2 + kHashK * s1; }
+
+// Power sums of the 64 consecutive IDs first+1 .. first+64 (a fully hit chunk):
+// sum = 64 f + 2080, sum of squares = 64 f^2 + 4160 f + 89440 (mod 2^64).
+__device__ __forceinline__ void full_chunk_sums(uint64_t first, uint64_t& s1, uint64_t& s2) {
+  s1 += 64ull * first + 2080ull;
+  s2 += 64ull * first * first + 4160ull * first + 89440ull;
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }

@@ -94,14 +94,16 @@ struct Frame {
   Mask<MW> lm;
 };
 
-// Per-wave traversal context (all members wave-uniform except h_lane).
+// Per-wave traversal context (all members wave-uniform except the *_lane sums).
 template <int D>
 struct Ctx {
   float ql[D], qh[D];
   uint32_t q;
   uint32_t cnt;           // hits of this item
   uint64_t out_base;      // ID write pass: next output slot
-  uint64_t h_lane;        // per-lane partial hash
+  uint64_t s1, s2;        // power sums of fully hit chunks
+  uint64_t s1_lane;       // per-lane partial sum of hit IDs
+  uint64_t s2_lane;       // per-lane partial sum of squared hit IDs
   uint64_t ie, le;        // canonical entries visited
 };
 
@@ -188,7 +190,13 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
           if (hit) a.ids[c.out_base + rank_below(b)] = (uint32_t)(st[u] + lane + 1);
           c.out_base += nb;
         } else if (OUT == kOutHash) {
-          if (hit) c.h_lane += id_hash((uint32_t)(st[u] + lane + 1));
+          if (b == ~0ull) {
+            full_chunk_sums(st[u], c.s1, c.s2);
+          } else if (hit) {
+            const uint32_t id = (uint32_t)st[u] + lane + 1u;
+            c.s1_lane += id;
+            c.s2_lane += (uint64_t)id * id;
+          }
         }
         c.cnt += nb;
       }
@@ -468,14 +476,18 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
     Ctx<D> c;
     c.q = q;
     c.cnt = 0;
-    c.h_lane = 0;
+    c.s1 = c.s2 = 0;
+    c.s1_lane = c.s2_lane = 0;
     c.ie = 0;
     c.le = 0;
     c.out_base = (OUT == kOutIds) ? a.offsets[q] : 0;
     load_query<D>(a, q, c);
     if (a.h > 0) walk<D, BOX, MW, EXPL, OUT, U>(a, c, node, stk);
     if (OUT != kOutIds) {
-      uint64_t hsum = OUT == kOutHash ? wave_sum_u64(c.h_lane) : 0;
+      // the hash is additive over disjoint ID sets, so donated parts add up
+      uint64_t hsum = 0;
+      if (OUT == kOutHash)
+        hsum = set_hash(c.s1 + wave_sum_u64(c.s1_lane), c.s2 + wave_sum_u64(c.s2_lane));
       if (lane == 0) {
         if (donating) {
           if (c.cnt) atomicAdd(&a.counts[q], c.cnt);
@@ -515,7 +527,7 @@ __global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
   const int h = a.h;
   const uint64_t n = a.n;
   uint32_t cnt = 0, rootv = 0, nodev = 0;
-  uint64_t hl = 0;
+  uint64_t s1 = 0, s2 = 0;
   if (h > 0) {
     auto last_index = [&](int l, uint64_t j) -> uint64_t {
       uint64_t e = (j + 1) * a.span[l];
@@ -571,7 +583,11 @@ __global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
           bool hit = (k < cc) && overlaps<D>(c.ql, c.qh, v, BOX ? v + D : v);
           uint64_t b = ballot(hit);
           cnt += (uint32_t)__popcll(b);
-          if (hit) hl += id_hash((uint32_t)(first + k + 1));
+          if (hit) {
+            const uint64_t id = first + k + 1;
+            s1 += id;
+            s2 += id * id;
+          }
           any = any || (b != 0);
         }
         visited = last_index(lvl, j);
@@ -587,7 +603,7 @@ __global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
       }
     }
   }
-  uint64_t hs = wave_sum_u64(hl);
+  const uint64_t hs = set_hash(wave_sum_u64(s1), wave_sum_u64(s2));
   if (lane == 0) {
     a.counts[q] = cnt;
     a.hashes[q] = hs;

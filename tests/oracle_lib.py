@@ -221,13 +221,10 @@ def search_brute(sobjs, D, is_box, queries, want_ids=False):
 
 
 def id_hash(ids):
-    """numpy restatement of orc_id_hash summed over ids (uint64 wraparound)."""
-    z = ids.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+    """numpy restatement of orc_id_hash per ID (uint64 wraparound): id^2 + K*id."""
+    x = ids.astype(np.uint64)
     with np.errstate(over="ignore"):
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    return z
+        return x * x + np.uint64(0x9E3779B97F4A7C15) * x
 
 
 def build_packed(objs, D, is_box, F, key_mode, key_src=SRC_LO, layout="packed"):

@@ -286,10 +286,13 @@ def test_search_dfs_mprs_brute_agree(O, n, D, box, F, layout):
     for i in range(len(q)):
         ids = b["ids"][b["offsets"][i]:b["offsets"][i + 1]]
         assert (np.diff(ids.astype(np.int64)) > 0).all()
-    # hash closed form: sum of splitmix64(id) mod 2^64
+    # hash closed form in exact integers: (sum of squares + K * sum) mod 2^64
+    K = 0x9E3779B97F4A7C15
     for i in range(0, len(q), 7):
-        ids = b["ids"][b["offsets"][i]:b["offsets"][i + 1]]
-        assert O.id_hash(ids).sum(dtype=np.uint64) == b["hashes"][i]
+        ids = [int(x) for x in b["ids"][b["offsets"][i]:b["offsets"][i + 1]]]
+        want = (sum(x * x for x in ids) + K * sum(ids)) % (1 << 64)
+        assert int(b["hashes"][i]) == want
+        assert O.id_hash(b["ids"][b["offsets"][i]:b["offsets"][i + 1]]).sum(dtype=np.uint64) == want
 
 
 def test_search_numpy_closed_form(O):
