@@ -44,6 +44,37 @@ def algorithmic_bytes(st, q, D, E):
             st["leaf_entries"] * E * 4)
 
 
+def pruned_bytes(st, q, D, E):
+    """Bytes of one search launch with the containment shortcut: the pruned
+    traversal's entries as above plus 8 bytes (a rank range) per contained
+    subtree run.  Reported apart from the canonical figure."""
+    return algorithmic_bytes(st, q, D, E) + 8 * st["contained_runs"]
+
+
+def contained_search(b, q, D, E, steps, warmup, kw, canon):
+    """The same batch with contain=True (subtrees inside a query reported from
+    their rank ranges): time, pruned bytes, and equality with the canonical run."""
+    ckw = dict(kw, contain=True)
+    for _ in range(warmup):
+        b.run(**ckw)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = b.run(**ckw)
+        kms.append(st["kernel_ms"])
+    wall = time.perf_counter() - t0
+    got = b.results()
+    kavg = float(np.mean(kms))
+    pb = pruned_bytes(st, q, D, E)
+    return {"queries_per_s": q * steps / wall, "ms_per_step": wall * 1e3 / steps,
+            "kernel_ms": kavg, "contained_runs": int(st["contained_runs"]),
+            "internal_entries": int(st["internal_entries"]),
+            "leaf_entries": int(st["leaf_entries"]), "pruned_bytes": pb,
+            "pruned_GBps": pb / (kavg * 1e-3) / 1e9,
+            "matches_canonical": bool(np.array_equal(got["counts"], canon["counts"]) and
+                                      np.array_equal(got["hashes"], canon["hashes"]))}
+
+
 def cpu_baseline(qcp, ix, queries, got, budget_s, threads):
     """The oracle's multi-threaded DFS (the reference CPU search's static query
     split, src/tree/bvh.cpp:195-220) on the same tree, over a bounded prefix of
@@ -77,7 +108,7 @@ def cpu_baseline(qcp, ix, queries, got, budget_s, threads):
             "matches_gpu": ok}
 
 
-def serving_bench(qcp, rates, seconds, scale):
+def serving_bench(qcp, rates, seconds, scale, contain_rate=None):
     """The serving config: 33,554,432 uniform 3-D points resident (MPHR, F=64);
     Poisson arrivals at each rate for `seconds`; each query a box of volume
     selectivity log-uniform in [1e-7, 1e-3]; continuous batching over 4 slots.
@@ -104,6 +135,22 @@ def serving_bench(qcp, rates, seconds, scale):
                     "max_inflight": st["max_inflight_seen"], "matches_batch_api": ok})
         log("serving %6d qps: p50 %.3f ms p99 %.3f ms, %d batches" %
             (r, st["p50_us"] / 1e3, st["p99_us"] / 1e3, st["batches"]))
+    if contain_rate:
+        # the same replay with the containment shortcut (reported apart)
+        r = contain_rate
+        n = int(r * seconds)
+        arr = qcp.poisson_arrivals(n, float(r), stream=3)
+        with qcp.Server(ix, slots=4, max_batch=4096, contain=True) as sv:
+            counts, hashes, lat = sv.replay(queries[:n], arr)
+            st = sv.stats()
+        ok = bool(np.array_equal(counts[:2000], ref["counts"][: min(n, 2000)]) and
+                  np.array_equal(hashes[:2000], ref["hashes"][: min(n, 2000)]))
+        out.append({"qps_offered": r, "queries": n, "contain": True,
+                    "p50_ms": st["p50_us"] / 1e3, "p99_ms": st["p99_us"] / 1e3,
+                    "max_ms": st["max_us"] / 1e3, "batches": st["batches"],
+                    "matches_batch_api": ok})
+        log("serving %6d qps, contain: p50 %.3f ms p99 %.3f ms" %
+            (r, st["p50_us"] / 1e3, st["p99_us"] / 1e3))
     ix.close()
     return {"workload": "33,554,432 uniform 3-D points, Poisson arrivals, selectivity "
                         "log-uniform [1e-7,1e-3], counts+hash", "seconds_per_rate": seconds,
@@ -193,6 +240,15 @@ def run_config(qcp, name, args):
             with ix.batch(q) as b:
                 st, wall, kms = timed_search(b, max(2, args.steps // 3), graph=True)
                 got = b.results()
+                cst, cwall, ckms = timed_search(b, max(2, args.steps // 3), graph=True,
+                                                contain=True)
+                cg = b.results()
+                contained = {"queries_per_s": len(q) / cwall, "kernel_ms": ckms,
+                             "contained_runs": int(cst["contained_runs"]),
+                             "pruned_GBps": pruned_bytes(cst, len(q), 2, 4) / (ckms * 1e-3) / 1e9,
+                             "matches_canonical": bool(
+                                 np.array_equal(cg["counts"], got["counts"]) and
+                                 np.array_equal(cg["hashes"], got["hashes"]))}
             abytes = algorithmic_bytes(st, len(q), 2, 4)
             t = oracle_tree_from(ix)
             cpu = cpu_sample(t, q, got, args.cpu_budget, threads)
@@ -201,7 +257,7 @@ def run_config(qcp, name, args):
                            "algorithmic_GBps": abytes / (kms * 1e-3) / 1e9,
                            "build_ms_graph": binfo["build_ms"],
                            "nodes": binfo["n_nodes"], "levels": binfo["level_count"],
-                           "cpu_baseline": cpu}
+                           "contained": contained, "cpu_baseline": cpu}
             log("c3 %s: %.3e q/s, %.1f ms, build %.1f ms, cpu %.3e q/s" %
                 (layout, len(q) / wall, wall * 1e3, binfo["build_ms"], cpu["value"]))
             ix.close()
@@ -249,6 +305,16 @@ def run_config(qcp, name, args):
                 log("c5 search donate=%d: %.3e q/s, %.1f ms, %d donated" %
                     (donate, len(q) / wall, wall * 1e3, st["donated_items"]))
             got = b.results()
+            st, wall, kms = timed_search(b, 1, donate=1, contain=True)
+            cg = b.results()
+            res["search_contained"] = {
+                "queries_per_s": len(q) / wall, "ms_per_step": wall * 1e3, "kernel_ms": kms,
+                "contained_runs": int(st["contained_runs"]),
+                "leaf_entries": int(st["leaf_entries"]),
+                "pruned_GBps": pruned_bytes(st, len(q), 2, 2) / (kms * 1e-3) / 1e9,
+                "matches_canonical": bool(np.array_equal(cg["counts"], got["counts"]) and
+                                          np.array_equal(cg["hashes"], got["hashes"]))}
+            log("c5 search contained: %.3e q/s, %.1f ms" % (len(q) / wall, wall * 1e3))
         t = oracle_tree_from(ix)
         res["cpu_baseline"] = cpu_sample(t, q, got, args.cpu_budget, threads)
         with ix.batch(q[:1024]) as pb:
@@ -313,6 +379,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-contain", action="store_true",
+                    help="skip the separately reported containment-shortcut run")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--profile-traffic", action="store_true",
                     help="short run for rocprofv3 --pmc: the search at the given fanout, "
@@ -374,6 +442,12 @@ def main():
         "int entries %d, leaf entries %d, %.1f GB/s algorithmic" %
         (ms_step, kavg, value, st["total_hits"], st["internal_entries"], st["leaf_entries"],
          achieved))
+    contained = None
+    if not args.no_contain:
+        contained = contained_search(b, q, D, E, args.steps, args.warmup, kw, got)
+        log("contained: %.3e queries/s, %.3f ms kernel, %d runs, leaf entries %d" %
+            (contained["queries_per_s"], contained["kernel_ms"], contained["contained_runs"],
+             contained["leaf_entries"]))
     plain = None
     if not args.no_plain:
         pq = queries[: min(q, 20000)]
@@ -393,7 +467,8 @@ def main():
     if not args.no_serve:
         b.close()
         ix.close()
-        serving = serving_bench(qcp, (10000, 50000, 100000), args.serve_seconds, args.scale)
+        serving = serving_bench(qcp, (10000, 50000, 100000), args.serve_seconds, args.scale,
+                                contain_rate=None if args.no_contain else 50000)
     traffic = read_traffic("traffic_c2_search.json")
     kname = "k_search<3,false,%d,false,1,%d> (block %d threads, donation %s)" % (
         (F + 63) // 64, kw["chunks"] or 4, 64 * (kw["block_waves"] or 4),
@@ -421,6 +496,7 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
                      "kernel": kname, "kernel_ms": kavg, "algorithmic_bytes": abytes},
+        "contained": contained,
         "plain": plain,
         "speedup_vs_plain": (value / plain["queries_per_s"]) if plain else None,
         "cpu_baseline": cpu,

@@ -90,6 +90,11 @@ typedef struct {
   uint32_t graph;          /* persistent form, counts/hash outputs: capture the whole launch
                               sequence (query ordering, buffer fills, search) in a hipGraph
                               on first use and replay it while the configuration is unchanged */
+  uint32_t contain;        /* persistent form: report every child subtree whose node box lies
+                              inside the query box as hits without visiting it (count from
+                              its rank range, hash in closed form, IDs written as a run).
+                              Same results; the stats then count the pruned traversal.  Not
+                              applied to box indexes holding a box with lo > hi. */
 } qcp_search_cfg;
 
 typedef struct {
@@ -100,6 +105,7 @@ typedef struct {
   uint64_t leaf_entries;     /* entries of visited leaf nodes (canonical traversal)     */
   uint64_t donated_items;    /* subtrees handed to other waves                          */
   uint64_t launches;         /* kernel launches issued by the search                    */
+  uint64_t contained_runs;   /* contain: runs of sibling subtrees reported without a visit */
 } qcp_search_stats;
 
 typedef struct qcp_index qcp_index;
@@ -231,7 +237,8 @@ typedef struct {
   uint64_t result_budget;      /* bytes of result buffers in flight, default 64 MiB   */
   uint32_t max_requests;       /* result-table capacity (tickets), default 1 << 22    */
   uint32_t grid_blocks;        /* per-batch grid cap (0 = fill the device)            */
-  uint32_t reserved[2];
+  uint32_t contain;            /* as qcp_search_cfg.contain (same results)            */
+  uint32_t reserved;
 } qcp_serve_cfg;
 typedef struct {
   uint64_t submitted, completed, batches;

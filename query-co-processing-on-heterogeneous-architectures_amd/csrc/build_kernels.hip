@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
                                               unsigned long long* key_or, uint32_t* bad) {
   constexpr int E = BOX ? 2 * D : D;
   uint64_t acc = 0;
-  bool any_bad = false;
+  bool any_bad = false, any_inv = false;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const float* o = objs + i * E;
@@ -48,6 +48,10 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) any_bad |= (v[e] != v[e]);  // NaN objects are rejected
+    if (BOX) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) any_inv |= v[d] > v[D + d];
+    }
     uint64_t k = hilbert_key<D>(c, nb);
     keys[i] = k;
     idx[i] = (uint32_t)i;
@@ -57,7 +61,8 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) acc |= __shfl_xor(acc, off, kWave);
   if (lane_id() == 0 && acc) atomicOr(key_or, (unsigned long long)acc);
-  if (any_bad) atomicOr(bad, 1u);
+  if (any_bad) atomicOr(bad, kBadDomain);
+  if (any_inv) atomicOr(bad, kBadInverted);
 }
 
 // ---------------------------------------------------------------------------

@@ -94,6 +94,8 @@ static void index_free(qcp_index* ix) {
   dfree(ix->d_nbox);
   dfree(ix->d_nfirst);
   dfree(ix->d_ncount);
+  dfree(ix->d_nrlo);
+  dfree(ix->d_nrhi);
   dfree(ix->d_keys);
   dfree(ix->d_perm);
   dfree(ix->d_key_or);
@@ -191,19 +193,30 @@ static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** 
       launch_fill_f32(ix->d_nbox, (uint64_t)2 * ix->D * ix->node_stride, 0.0f, ix->stream) != hipSuccess)
     return fail(QCP_EHIP);
   if (ix->expl) {
-    if ((s = dalloc(&ix->d_nfirst, ix->node_stride)) != QCP_OK) return fail(s);
-    if ((s = dalloc(&ix->d_ncount, ix->node_stride)) != QCP_OK) return fail(s);
-    if (launch_fill_u32(ix->d_nfirst, ix->node_stride, 0u, ix->stream) != hipSuccess ||
-        launch_fill_u32(ix->d_ncount, ix->node_stride, 0u, ix->stream) != hipSuccess)
-      return fail(QCP_EHIP);
-    if (ix->n_nodes) {
-      if (hipMemcpyAsync(ix->d_nfirst, h_first.data(), 4 * ix->n_nodes, hipMemcpyHostToDevice,
-                         ix->stream) != hipSuccess ||
-          hipMemcpyAsync(ix->d_ncount, h_count.data(), 4 * ix->n_nodes, hipMemcpyHostToDevice,
-                         ix->stream) != hipSuccess ||
-          hipStreamSynchronize(ix->stream) != hipSuccess)
+    // rank range under each node: leaves hold [first, first+count); a parent
+    // spans its first child's start to its last child's end (children follow
+    // their parent in BFS order, so one reverse pass suffices)
+    std::vector<uint32_t> rlo(ix->n_nodes), rhi(ix->n_nodes);
+    for (uint64_t i = ix->n_nodes; i-- > 0;) {
+      if (h_count[i] & kLeafBit) {
+        rlo[i] = h_first[i];
+        rhi[i] = h_first[i] + (h_count[i] & ~kLeafBit);
+      } else {
+        rlo[i] = rlo[h_first[i]];
+        rhi[i] = rhi[h_first[i] + h_count[i] - 1];
+      }
+    }
+    uint32_t** arrs[4] = {&ix->d_nfirst, &ix->d_ncount, &ix->d_nrlo, &ix->d_nrhi};
+    const uint32_t* src[4] = {h_first.data(), h_count.data(), rlo.data(), rhi.data()};
+    for (int i = 0; i < 4; ++i) {
+      if ((s = dalloc(arrs[i], ix->node_stride)) != QCP_OK) return fail(s);
+      if (launch_fill_u32(*arrs[i], ix->node_stride, 0u, ix->stream) != hipSuccess)
+        return fail(QCP_EHIP);
+      if (ix->n_nodes && hipMemcpyAsync(*arrs[i], src[i], 4 * ix->n_nodes, hipMemcpyHostToDevice,
+                                        ix->stream) != hipSuccess)
         return fail(QCP_EHIP);
     }
+    if (hipStreamSynchronize(ix->stream) != hipSuccess) return fail(QCP_EHIP);
   }
   // sort scratch
   const uint32_t nt = sort_tiles(n > 0 ? n : 1);
@@ -312,7 +325,8 @@ static qcp_status index_run_build_graph(qcp_index* ix, const float* d_objs) {
   set_info_shape(ix);
   ix->info.build_ms = tot;
   ix->info.build_phase_ms[5] = tot;
-  return bad ? QCP_EDOMAIN : QCP_OK;
+  ix->contain_ok = !(bad & kBadInverted);
+  return (bad & kBadDomain) ? QCP_EDOMAIN : QCP_OK;
 }
 
 // The build pipeline on the index's stream.
@@ -332,7 +346,8 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
     HIP_TRY(hipMemcpyAsync(&bad, ix->d_bad, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipEventRecord(ix->ev[1], s));
     HIP_TRY(hipStreamSynchronize(s));
-    if (bad) return QCP_EDOMAIN;
+    if (bad & kBadDomain) return QCP_EDOMAIN;
+    ix->contain_ok = !(bad & kBadInverted);
     // LSD radix sort over the significant bytes of the keys
     const unsigned bits = key_or ? 64 - __builtin_clzll(key_or) : 0;
     const unsigned passes = (bits + 7) / 8;
@@ -667,6 +682,8 @@ SearchArgs qcp::index_search_args(const qcp_index* ix, const float* d_queries, u
   a.nbox = ix->d_nbox;
   a.nfirst = ix->d_nfirst;
   a.ncount = ix->d_ncount;
+  a.nrlo = ix->d_nrlo;
+  a.nrhi = ix->d_nrhi;
   a.queries = d_queries;
   a.obj_stride = ix->obj_stride;
   a.node_stride = ix->node_stride;
@@ -810,6 +827,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   const bool donate = cfg.donate != 0;
   unsigned grid = cfg.grid_blocks ? cfg.grid_blocks : persistent_grid(ix, out_mode, block);
   SearchArgs a = make_args(b);
+  a.contain = (cfg.contain && ix->contain_ok) ? 1u : 0u;
   if (donate) {
     if (!b->d_slots) {
       b->slot_cap = kSlotCap;
@@ -842,7 +860,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   if (use_graph) {
     const uint32_t key[12] = {1u, (uint32_t)out_mode, (uint32_t)u, grid, donate ? 1u : 0u,
                               cfg.order, a.max_waiters, a.hunger_period, a.donate_min_height,
-                              block, 0u, 0u};
+                              block, a.contain, 0u};
     if (!b->exec || memcmp(key, b->exec_key, sizeof(key)) != 0) {
       if (b->exec) {
         hipGraphExecDestroy(b->exec);
@@ -909,6 +927,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   st.leaf_entries = ctl.leaf_entries;
   st.total_hits = ctl.total_hits;
   st.donated_items = ctl.donated;
+  st.contained_runs = ctl.contained;
   b->total_hits = ctl.total_hits;
   if (donate) b->slots_dirty = std::min<uint64_t>(b->slot_cap, std::max<uint64_t>(ctl.tail, 1));
   if (stats) *stats = st;
@@ -1005,6 +1024,10 @@ __attribute__((visibility("hidden"))) qcp_status qcp_internal_restore(
                        4 * n_nodes, hipMemcpyHostToDevice, s) != hipSuccess)
       return fail(QCP_EHIP);
   if (hipStreamSynchronize(s) != hipSuccess) return fail(QCP_EHIP);
+  if (ix->box)
+    for (uint64_t r = 0; r < n && ix->contain_ok; ++r)
+      for (uint32_t d = 0; d < ix->D; ++d)
+        if (objs[r * ix->E + d] > objs[r * ix->E + ix->D + d]) ix->contain_ok = false;
   set_info_shape(ix);
   *out = ix;
   return QCP_OK;

@@ -23,6 +23,7 @@ struct QueueCtl {
   unsigned long long leaf_entries;
   unsigned long long donated;
   unsigned long long total_hits;
+  unsigned long long contained;  // contained subtree runs reported without a visit
 };
 
 // Tree + query batch as seen by the search kernels.
@@ -31,6 +32,8 @@ struct SearchArgs {
   const float* nbox;         // [2D][node_stride] node MBRs, BFS order, SoA
   const uint32_t* nfirst;    // explicit layouts: first child node / first rank
   const uint32_t* ncount;    // explicit layouts: count | kLeafBit for leaves
+  const uint32_t* nrlo;      // explicit layouts: first rank under each node
+  const uint32_t* nrhi;      // explicit layouts: end rank (exclusive) under each node
   const float* queries;      // [nq][2D] row-major [lo(D), hi(D)]
   const uint32_t* qperm;     // processing order of the queries (nullptr = as given)
   uint64_t obj_stride;
@@ -56,6 +59,7 @@ struct SearchArgs {
   uint32_t hunger_period;    // internal-node descents between queue checks
   uint32_t donate_min_height;  // packed: donate subtrees this far above leaf parents
   uint32_t q_begin;          // plain form: first query of the launch
+  uint32_t contain;          // report subtrees whose box lies inside the query without a visit
 };
 
 // Order-independent fingerprint of a set of rank IDs: sum of squares plus
@@ -64,13 +68,6 @@ struct SearchArgs {
 constexpr uint64_t kHashK = 0x9E3779B97F4A7C15ull;
 
 __device__ __forceinline__ uint64_t set_hash(uint64_t s1, uint64_t s2) { return s2 + kHashK * s1; }
-
-// Power sums of the 64 consecutive IDs first+1 .. first+64 (a fully hit chunk):
-// sum = 64 f + 2080, sum of squares = 64 f^2 + 4160 f + 89440 (mod 2^64).
-__device__ __forceinline__ void full_chunk_sums(uint64_t first, uint64_t& s1, uint64_t& s2) {
-  s1 += 64ull * first + 2080ull;
-  s2 += 64ull * first * first + 4160ull * first + 89440ull;
-}
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -112,6 +109,31 @@ __device__ __forceinline__ float wave_max(float v) {
     v = (v < o) ? o : v;
   }
   return v;
+}
+
+// Box inside the query (closed): q.lo[d] <= lo[d] and hi[d] <= q.hi[d] for all
+// d.  With lo <= hi per dimension for every object under a node, every object
+// under a node whose box is inside the query overlaps the query.
+template <int D>
+__device__ __forceinline__ bool inside(const float* ql, const float* qh, const float* lo,
+                                       const float* hi) {
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) ok = ok && (ql[d] <= lo[d]) && (hi[d] <= qh[d]);
+  return ok;
+}
+
+// Power sums of the IDs s+1 .. e (mod 2^64): sum of i and sum of i^2.  The
+// divisions by 2 and 3 are taken on the factors before the (wrapping) product.
+__device__ __forceinline__ uint64_t psum1(uint64_t m) {  // 1 + .. + m, m < 2^32
+  return (m & 1) ? m * ((m + 1) >> 1) : (m >> 1) * (m + 1);
+}
+__device__ __forceinline__ uint64_t psum2(uint64_t m) {  // 1 + .. + m^2, m < 2^32
+  uint64_t a = m, b = m + 1, c = 2 * m + 1;
+  if (a & 1) b >>= 1; else a >>= 1;
+  const uint64_t r = m % 3;
+  if (r == 0) a /= 3; else if (r == 2) b /= 3; else c /= 3;
+  return a * b * c;
 }
 
 // The reference's closed-interval test: rejected iff q.lo > e.hi or q.hi < e.lo

@@ -27,6 +27,8 @@ struct qcp_index {
   float* d_nbox = nullptr;       // [2D][node_stride]
   uint32_t* d_nfirst = nullptr;  // explicit layouts
   uint32_t* d_ncount = nullptr;
+  uint32_t* d_nrlo = nullptr;    // explicit layouts: rank range under each node
+  uint32_t* d_nrhi = nullptr;
   uint64_t* d_keys = nullptr;    // sorted keys [n]
   uint32_t* d_perm = nullptr;    // [n]
   // build scratch
@@ -42,6 +44,9 @@ struct qcp_index {
   hipGraphExec_t build_exec = nullptr;      // captured build (QCP_BUILD_GRAPH)
   const float* build_exec_objs = nullptr;   // input pointer baked into build_exec
   int cus = 0;
+  // every object box has lo <= hi in each dimension, so a node box inside a
+  // query implies that every object under it overlaps the query
+  bool contain_ok = true;
   qcp_index_info info{};
 };
 

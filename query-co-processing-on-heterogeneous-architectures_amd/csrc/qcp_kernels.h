@@ -9,6 +9,9 @@
 namespace qcp {
 
 // build
+// flag bits of the key kernel's `bad` word
+constexpr uint32_t kBadDomain = 1u;    // NaN, or outside the quantiser's domain
+constexpr uint32_t kBadInverted = 2u;  // a box with lo > hi in some dimension
 hipError_t launch_keys(const float* objs, uint64_t n, unsigned D, bool box, bool ref,
                        bool center, unsigned nb, uint64_t* keys, uint32_t* idx,
                        unsigned long long* key_or, uint32_t* bad, hipStream_t s);

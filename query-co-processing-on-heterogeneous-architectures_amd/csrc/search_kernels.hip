@@ -84,14 +84,27 @@ struct Mask {
     for (int i = 0; i < MW; ++i) r.w[i] = w[i] & o.w[i];
     return r;
   }
+  __device__ __forceinline__ Mask andnot(const Mask& o) const {
+    Mask r;
+#pragma unroll
+    for (int i = 0; i < MW; ++i) r.w[i] = w[i] & ~o.w[i];
+    return r;
+  }
+  __device__ __forceinline__ static Mask none() {
+    Mask r;
+#pragma unroll
+    for (int i = 0; i < MW; ++i) r.w[i] = 0;
+    return r;
+  }
 };
 
 template <int MW>
 struct Frame {
   uint32_t node;
   uint32_t level;
-  Mask<MW> m;
-  Mask<MW> lm;
+  Mask<MW> m;   // children still to visit
+  Mask<MW> lm;  // children that are leaves
+  Mask<MW> im;  // children whose box lies inside the query
 };
 
 // Per-wave traversal context (all members wave-uniform except the *_lane sums).
@@ -101,18 +114,22 @@ struct Ctx {
   uint32_t q;
   uint32_t cnt;           // hits of this item
   uint64_t out_base;      // ID write pass: next output slot
-  uint64_t s1, s2;        // power sums of fully hit chunks
   uint64_t s1_lane;       // per-lane partial sum of hit IDs
   uint64_t s2_lane;       // per-lane partial sum of squared hit IDs
-  uint64_t ie, le;        // canonical entries visited
+  uint64_t s1c, s2c;      // power sums of the IDs of contained subtrees
+  uint64_t ie, le;        // entries visited (the canonical traversal unless contain is on)
+  uint32_t cr;            // contained subtree runs
 };
+
+// Output modes of the persistent form.
+enum { kOutCount = 0, kOutHash = 1, kOutIds = 2 };
 
 // Evaluate the children [cbase, cbase+cc) (node ids) of an internal node
 // against the query; for explicit layouts also their leaf flags.
 template <int D, int MW, bool EXPL>
 __device__ __forceinline__ void eval_children(const SearchArgs& a, const Ctx<D>& c,
                                               uint32_t cbase, uint32_t cc, Mask<MW>& m,
-                                              Mask<MW>& lm) {
+                                              Mask<MW>& lm, Mask<MW>& im) {
   const uint32_t lane = lane_id();
   float lo[MW][D], hi[MW][D];
   uint32_t cw[MW];
@@ -132,36 +149,83 @@ __device__ __forceinline__ void eval_children(const SearchArgs& a, const Ctx<D>&
     bool hit = (k < cc) && overlaps<D>(c.ql, c.qh, lo[w], hi[w]);
     m.w[w] = ballot(hit);
     lm.w[w] = EXPL ? ballot((cw[w] & kLeafBit) != 0) : ~0ull;
+    im.w[w] = a.contain ? ballot(hit && inside<D>(c.ql, c.qh, lo[w], hi[w])) : 0ull;
   }
 }
 
-// Output modes of the persistent form.
-enum { kOutCount = 0, kOutHash = 1, kOutIds = 2 };
+// Report the ranks [s, e) as hits without reading them (a run of children
+// whose boxes lie inside the query).
+template <int D, int OUT>
+__device__ __forceinline__ void emit_range(const SearchArgs& a, Ctx<D>& c, uint64_t s,
+                                           uint64_t e) {
+  const uint64_t len = e - s;
+  c.cnt += (uint32_t)len;
+  c.cr += 1;
+  if (OUT == kOutHash) {
+    c.s1c += psum1(e) - psum1(s);
+    c.s2c += psum2(e) - psum2(s);
+  } else if (OUT == kOutIds) {
+    const uint32_t lane = lane_id();
+    uint32_t* dst = a.ids + c.out_base;
+    // dword stores up to a 16-byte boundary, then 16-byte stores, then the tail
+    const uint64_t head = ((4 - (c.out_base & 3)) & 3) < len ? ((4 - (c.out_base & 3)) & 3) : len;
+    if (lane < head) dst[lane] = (uint32_t)(s + 1 + lane);
+    const uint64_t body = (len - head) & ~3ull;
+    uint4* dst4 = reinterpret_cast<uint4*>(dst + head);
+    const uint32_t b0 = (uint32_t)(s + 1 + head);
+    for (uint64_t i = lane; i < body / 4; i += kWave) {
+      const uint32_t v = b0 + (uint32_t)(4 * i);
+      dst4[i] = make_uint4(v, v + 1, v + 2, v + 3);
+    }
+    const uint64_t t0 = head + body;
+    if (t0 + lane < len) dst[t0 + lane] = (uint32_t)(s + 1 + t0 + lane);
+    c.out_base += len;
+  }
+}
 
 // Stream the leaf runs at the bottom of mask m (bits that are also set in lm),
-// U 64-entry chunks in flight, stopping at the first non-leaf set bit.
-// run_range(k, k2, &s, &e) maps a run of children [k, k2) to its rank range.
-template <int D, bool BOX, int MW, int OUT, int U, typename RunRange>
+// U 64-entry chunks in flight, stopping at the first set bit that is neither a
+// leaf nor inside the query.  Runs of children inside the query (im) are
+// reported whole without a read.  run_range(k, k2, &s, &e) maps a run of leaf
+// children [k, k2) to its rank range, sub_range a run of any children.
+template <int D, bool BOX, int MW, int OUT, int U, typename RunRange, typename SubRange>
 __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<MW>& m,
-                                          const Mask<MW>& lm, RunRange run_range) {
+                                          const Mask<MW>& lm, const Mask<MW>& im,
+                                          RunRange run_range, SubRange sub_range) {
   constexpr int E = BOX ? 2 * D : D;
   const uint32_t lane = lane_id();
   uint64_t cs = 0, ce = 0;
   while (true) {
     uint64_t st[U];
     uint32_t len[U];
+    bool blocked = false;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (cs >= ce) {
-        int k = m.lowest();
-        if (k >= 0 && lm.test(k)) {
-          int k2 = (m & lm).run_end(k);
+      while (cs >= ce && !blocked) {
+        const int k = m.lowest();
+        if (k < 0) break;
+        if (im.test(k)) {
+          // IDs are written in rank order: chunks already in flight go first
+          if (OUT == kOutIds && u > 0) {
+            blocked = true;
+            break;
+          }
+          const int k2 = (m & im).run_end(k);
+          m.clear_range(k, k2);
+          uint64_t s, e;
+          sub_range(k, k2, s, e);
+          emit_range<D, OUT>(a, c, uni64(s), uni64(e));
+          continue;
+        }
+        if (lm.test(k)) {
+          const int k2 = (m & lm).andnot(im).run_end(k);
           m.clear_range(k, k2);
           run_range(k, k2, cs, ce);
           cs = uni64(cs);
           ce = uni64(ce);
           c.le += ce - cs;
         }
+        break;
       }
       uint64_t rem = ce > cs ? ce - cs : 0;
       len[u] = (uint32_t)(rem < kWave ? rem : kWave);
@@ -190,13 +254,11 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
           if (hit) a.ids[c.out_base + rank_below(b)] = (uint32_t)(st[u] + lane + 1);
           c.out_base += nb;
         } else if (OUT == kOutHash) {
-          if (b == ~0ull) {
-            full_chunk_sums(st[u], c.s1, c.s2);
-          } else if (hit) {
-            const uint32_t id = (uint32_t)st[u] + lane + 1u;
-            c.s1_lane += id;
-            c.s2_lane += (uint64_t)id * id;
-          }
+          // branch-free: a select, a 64-bit add and one v_mad_u64_u32 per chunk
+          const uint32_t id = (uint32_t)st[u] + lane + 1u;
+          const uint32_t x = hit ? id : 0u;
+          c.s1_lane += x;
+          c.s2_lane += (uint64_t)x * x;
         }
         c.cnt += nb;
       }
@@ -304,36 +366,56 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
     uint64_t below = is_leaf ? a.n : a.lvl_cnt[level + 1];
     cnt = (uint32_t)((below - first) < F ? (below - first) : F);
   }
+  // rank range of the children [k, k2) of node `nd` at `lvl` (contained runs)
+  auto sub_range = [&](uint32_t nd, int lvl, int k, int k2, uint64_t& s, uint64_t& e) {
+    if (EXPL) {
+      const uint32_t cf = a.nfirst[nd];
+      s = a.nrlo[cf + k];
+      e = a.nrhi[cf + k2 - 1];
+    } else {
+      const uint64_t j = nd - a.lvl_off[lvl];
+      const uint64_t sp = a.span[lvl + 1];
+      s = (j * F + (uint64_t)k) * sp;
+      const uint64_t e1 = (j * F + (uint64_t)k2) * sp;
+      e = e1 < a.n ? e1 : a.n;
+    }
+  };
   if (is_leaf) {
     Mask<MW> m, lm;
 #pragma unroll
     for (int i = 0; i < MW; ++i) { m.w[i] = 0; lm.w[i] = ~0ull; }
     m.w[0] = 1;  // a single pseudo-run covering this leaf's entries
     uint64_t s0 = first, e0 = (uint64_t)first + cnt;
-    scan_runs<D, BOX, MW, OUT, U>(a, c, m, lm,
-                                  [&](int, int, uint64_t& s, uint64_t& e) { s = s0; e = e0; });
+    scan_runs<D, BOX, MW, OUT, U>(a, c, m, lm, Mask<MW>::none(),
+                                  [&](int, int, uint64_t& s, uint64_t& e) { s = s0; e = e0; },
+                                  [&](int, int, uint64_t& s, uint64_t& e) { s = e = 0; });
     return;
   }
-  Mask<MW> m, lm;
+  Mask<MW> m, lm, im;
   uint32_t cbase = EXPL ? first : a.lvl_off[level + 1] + first;
-  eval_children<D, MW, EXPL>(a, c, cbase, cnt, m, lm);
+  eval_children<D, MW, EXPL>(a, c, cbase, cnt, m, lm, im);
   c.ie += cnt;
   uint32_t cur = node;
   int sp = 0;
   int since = 0;
   while (true) {
-    // leaf children at the bottom of the mask are streamed right away
+    // leaf children and children inside the query at the bottom of the mask
+    // are streamed (or reported whole) right away
+    auto subs = [&](int k, int k2, uint64_t& s, uint64_t& e) { sub_range(cur, level, k, k2, s, e); };
     if (EXPL) {
       const uint32_t cf = a.nfirst[cur];
-      scan_runs<D, BOX, MW, OUT, U>(a, c, m, lm, [&](int k, int k2, uint64_t& s, uint64_t& e) {
+      scan_runs<D, BOX, MW, OUT, U>(a, c, m, lm, im, [&](int k, int k2, uint64_t& s, uint64_t& e) {
         s = a.nfirst[cf + k];
         e = (uint64_t)a.nfirst[cf + k2 - 1] + (a.ncount[cf + k2 - 1] & ~kLeafBit);
-      });
+      }, subs);
     } else if (level + 1 == a.h - 1) {
       const uint32_t j = cur - a.lvl_off[level];
-      scan_runs<D, BOX, MW, OUT, U>(a, c, m, lm, [&](int k, int k2, uint64_t& s, uint64_t& e) {
+      scan_runs<D, BOX, MW, OUT, U>(a, c, m, lm, im, [&](int k, int k2, uint64_t& s, uint64_t& e) {
         packed_range(j, k, k2, s, e);
-      });
+      }, subs);
+    } else if (a.contain) {
+      scan_runs<D, BOX, MW, OUT, U>(a, c, m, Mask<MW>::none(), im,
+                                    [&](int, int, uint64_t& s, uint64_t& e) { s = e = 0; }, subs);
     }
     if (m.empty()) {
       if (sp == 0) break;
@@ -345,6 +427,7 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
       for (int i = 0; i < MW; ++i) {
         m.w[i] = uni64(stk[sp].m.w[i]);
         lm.w[i] = uni64(stk[sp].lm.w[i]);
+        im.w[i] = uni64(stk[sp].im.w[i]);
       }
       continue;
     }
@@ -394,8 +477,8 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
       ccnt = (uint32_t)((below - cfirst) < F ? (below - cfirst) : F);
       cfirst += a.lvl_off[level + 2];
     }
-    Mask<MW> cm, clm;
-    eval_children<D, MW, EXPL>(a, c, cfirst, ccnt, cm, clm);
+    Mask<MW> cm, clm, cim;
+    eval_children<D, MW, EXPL>(a, c, cfirst, ccnt, cm, clm, cim);
     c.ie += ccnt;
     if (cm.empty()) continue;
     if (!m.empty()) {
@@ -403,12 +486,14 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
       stk[sp].level = (uint32_t)level;
       stk[sp].m = m;
       stk[sp].lm = lm;
+      stk[sp].im = im;
       ++sp;
     }
     cur = child;
     level += 1;
     m = cm;
     lm = clm;
+    im = cim;
   }
 }
 
@@ -426,7 +511,7 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
   extern __shared__ unsigned char qcp_lds[];
   Frame<MW>* stk = reinterpret_cast<Frame<MW>*>(qcp_lds) + (threadIdx.x / kWave) * kDepth;
   const uint32_t lane = lane_id();
-  uint64_t ie = 0, le = 0, hits = 0;
+  uint64_t ie = 0, le = 0, hits = 0, cruns = 0;
   const bool donating = OUT != kOutIds && a.donate;
   const uint32_t xcd = blockIdx.x % kXcds;
   bool roots_left = true;
@@ -476,8 +561,9 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
     Ctx<D> c;
     c.q = q;
     c.cnt = 0;
-    c.s1 = c.s2 = 0;
     c.s1_lane = c.s2_lane = 0;
+    c.s1c = c.s2c = 0;
+    c.cr = 0;
     c.ie = 0;
     c.le = 0;
     c.out_base = (OUT == kOutIds) ? a.offsets[q] : 0;
@@ -487,7 +573,7 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
       // the hash is additive over disjoint ID sets, so donated parts add up
       uint64_t hsum = 0;
       if (OUT == kOutHash)
-        hsum = set_hash(c.s1 + wave_sum_u64(c.s1_lane), c.s2 + wave_sum_u64(c.s2_lane));
+        hsum = set_hash(c.s1c + wave_sum_u64(c.s1_lane), c.s2c + wave_sum_u64(c.s2_lane));
       if (lane == 0) {
         if (donating) {
           if (c.cnt) atomicAdd(&a.counts[q], c.cnt);
@@ -505,11 +591,13 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
     ie += c.ie;
     le += c.le;
     hits += c.cnt;
+    cruns += c.cr;
   }
   if (lane == 0) {
     atomicAdd(&a.ctl->internal_entries, (unsigned long long)ie);
     atomicAdd(&a.ctl->leaf_entries, (unsigned long long)le);
     atomicAdd(&a.ctl->total_hits, (unsigned long long)hits);
+    if (cruns) atomicAdd(&a.ctl->contained, (unsigned long long)cruns);
   }
 }
 
@@ -625,6 +713,7 @@ __global__ void k_reset_ctl(QueueCtl* ctl, uint32_t pending) {
     ctl->leaf_entries = 0;
     ctl->donated = 0;
     ctl->total_hits = 0;
+    ctl->contained = 0;
   }
 }
 

@@ -127,6 +127,7 @@ static qcp_status slot_launch(qcp_server* sv, Slot& s, uint32_t n) {
   a.max_waiters = 64;
   a.hunger_period = 8;
   a.donate_min_height = 1;
+  a.contain = (sv->cfg.contain && ix->contain_ok) ? 1u : 0u;
   if (launch_fill_u32(s.d_counts, n, 0u, st) != hipSuccess ||
       launch_fill_u32((uint32_t*)s.d_hashes, 2ull * n, 0u, st) != hipSuccess ||
       launch_fill_u32((uint32_t*)s.d_slots, 2ull * kServerSlotCap, 0xFFFFFFFFu, st) != hipSuccess)

@@ -52,14 +52,14 @@ class SearchCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in
                 ("mode", "outputs", "plain_batch", "block_waves", "grid_blocks", "leaf_chunks",
                  "donate", "order", "max_waiters", "hunger_period", "donate_height",
-                 "graph")]
+                 "graph", "contain")]
 
 
 class SearchStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("total_hits", ctypes.c_uint64), ("internal_entries", ctypes.c_uint64),
                 ("leaf_entries", ctypes.c_uint64), ("donated_items", ctypes.c_uint64),
-                ("launches", ctypes.c_uint64)]
+                ("launches", ctypes.c_uint64), ("contained_runs", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -74,7 +74,8 @@ class TuneEntry(ctypes.Structure):
 class ServeCfg(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("result_budget", ctypes.c_uint64), ("max_requests", ctypes.c_uint32),
-                ("grid_blocks", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+                ("grid_blocks", ctypes.c_uint32), ("contain", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class ServeStats(ctypes.Structure):
@@ -289,11 +290,12 @@ class Index:
         return Batch(self, queries)
 
     def search(self, queries, mode="persistent", ids=False, visits=False, plain_batch=128,
-               grid_blocks=0, donate=0, chunks=0, hash=True, order=0, block_waves=0):
+               grid_blocks=0, donate=0, chunks=0, hash=True, order=0, block_waves=0,
+               contain=False):
         with self.batch(queries) as b:
             st = b.run(mode=mode, ids=ids, plain_batch=plain_batch, grid_blocks=grid_blocks,
                        donate=donate, chunks=chunks, hash=hash, order=order,
-                       block_waves=block_waves)
+                       block_waves=block_waves, contain=contain)
             res = b.results(ids=ids, visits=visits)
             res["stats"] = st
             return res
@@ -329,16 +331,17 @@ class Batch:
 
     def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=0,
             hash=True, chunks=0, order=0, max_waiters=0, hunger_period=0, donate_height=0,
-            graph=False, block_waves=0):
+            graph=False, block_waves=0, contain=False):
         """One search of the resident batch.  donate=1 enables subtree donation
         (work stealing) in the persistent form; chunks = 64-entry leaf chunks in
         flight per wave (0 = default); order=0 processes the queries in Hilbert
         order of their centres, one range per XCD, order=1 as given;
-        block_waves = waves per workgroup (0 = 4)."""
+        block_waves = waves per workgroup (0 = 4); contain=True reports subtrees
+        whose box lies inside the query without visiting them (same results)."""
         outputs = OUT_COUNTS | (OUT_HASH if hash else 0) | (OUT_IDS if ids else 0)
         cfg = SearchCfg(MODE[mode], outputs, plain_batch, block_waves, grid_blocks, chunks,
                         donate, order, max_waiters, hunger_period, donate_height,
-                        1 if graph else 0)
+                        1 if graph else 0, 1 if contain else 0)
         st = SearchStats()
         check(lib().qcp_batch_run(self._h, ctypes.byref(cfg), ctypes.byref(st)), "qcp_batch_run")
         self._ids = ids
@@ -406,9 +409,10 @@ class Server:
     """Continuous-batching server over a resident index (counts + ID-set hash)."""
 
     def __init__(self, index, slots=4, max_batch=4096, result_budget=0, max_requests=0,
-                 grid_blocks=0):
+                 grid_blocks=0, contain=False):
         self.index = index
-        cfg = ServeCfg(slots, max_batch, result_budget, max_requests, grid_blocks, (0, 0))
+        cfg = ServeCfg(slots, max_batch, result_budget, max_requests, grid_blocks,
+                       1 if contain else 0, 0)
         self._h = P()
         check(lib().qcp_server_start(index._h, ctypes.byref(cfg), ctypes.byref(self._h)),
               "qcp_server_start")

@@ -55,6 +55,11 @@ def check_case(gpu, O, objs, D, is_box, F, key_mode="ref", key_src="lo", layout=
     assert_bits_equal(co["counts"], exp["counts"], tag + " counts-only, 1-wave workgroups")
     w8 = ix.search(queries, mode="persistent", donate=1, block_waves=8)
     assert_results_equal(w8, exp, tag + " 8-wave workgroups with donation", ids=False)
+    # containment shortcut: subtrees inside the query reported from their rank ranges
+    ct = ix.search(queries, mode="persistent", ids=True, contain=True)
+    assert_results_equal(ct, exp, tag + " contained subtrees")
+    cd = ix.search(queries, mode="persistent", donate=1, contain=True, chunks=2)
+    assert_results_equal(cd, exp, tag + " contained subtrees with donation", ids=False)
     if plain and layout != "bvh":
         mp = t.search_mprs(queries)
         assert_bits_equal(mp["counts"], exp["counts"], tag + " oracle MPRS vs DFS")
@@ -123,6 +128,47 @@ def test_donation_heavy_queries(gpu, O, layout, F):
         assert_results_equal(got, exp, "donation %s F=%d rep %d" % (layout, F, rep))
         assert got["stats"]["donated_items"] > 0
         assert got["stats"]["leaf_entries"] == int(exp["leaf_entries"].sum())
+    got = ix.search(q, donate=1, contain=True)
+    assert_results_equal(got, exp, "donation+contain %s F=%d" % (layout, F), ids=False)
+
+
+@pytest.mark.parametrize("layout", ["mphr", "rtree", "bvh"])
+@pytest.mark.parametrize("D", [2, 3])
+def test_containment_large_queries(gpu, O, layout, D):
+    """Queries covering large parts of the space: most hits come from subtrees
+    reported whole (rank ranges, closed-form hashes, ID runs at unaligned
+    output offsets); everything must equal the canonical traversal."""
+    n = 200000
+    pts = rand_points(n, D, seed=31 + D)
+    rng = np.random.default_rng(D)
+    c = rng.random((40, D), dtype=np.float32)
+    s = rng.uniform(0.05, 0.9, (40, 1)).astype(np.float32)
+    q = np.concatenate([c - s / 2, c + s / 2], 1).astype(np.float32)
+    F = 64
+    ix = gpu.Index.build(pts, D, fanout=F, layout=layout)
+    t = _oracle_tree(O, pts, D, False, F, "ref", "lo", layout)
+    exp = t.search_dfs(q, want_ids=True)
+    got = ix.search(q, ids=True, contain=True)
+    assert_results_equal(got, exp, "contain %s D=%d" % (layout, D))
+    assert got["stats"]["contained_runs"] > 0
+    assert got["stats"]["leaf_entries"] < int(exp["leaf_entries"].sum())
+    dn = ix.search(q, donate=1, contain=True)
+    assert_results_equal(dn, exp, "contain+donate %s D=%d" % (layout, D), ids=False)
+
+
+def test_containment_off_for_inverted_boxes(gpu, O):
+    """A box index holding a box with lo > hi: a node box inside the query no
+    longer implies that its objects overlap, so containment is not applied."""
+    objs = rand_boxes(30000, 2, seed=5, mean_side=0.01)
+    objs[123, [0, 2]] = objs[123, [2, 0]] + np.float32(0.001)  # lo > hi in dimension 0
+    assert objs[123, 0] > objs[123, 2]
+    q = np.array([[0.0, 0.0, 1.0, 1.0], [0.1, 0.1, 0.9, 0.9]], np.float32)
+    ix = gpu.Index.build(objs, 2, fanout=64, layout="rtree", obj_kind="boxes")
+    t = _oracle_tree(O, objs, 2, True, 64, "ref", "lo", "rtree")
+    exp = t.search_dfs(q, want_ids=True)
+    got = ix.search(q, ids=True, contain=True)
+    assert_results_equal(got, exp, "inverted box index")
+    assert got["stats"]["contained_runs"] == 0
 
 
 def test_autotune_returns_consistent_index(gpu, O):

@@ -67,3 +67,13 @@ def test_budget_defers_admission(gpu, served):
     assert st["deferred_by_budget"] > 0, st
     assert_bits_equal(counts, exp["counts"], "budget counts")
     assert_bits_equal(hashes, exp["hashes"], "budget hashes")
+
+
+def test_served_with_containment(gpu, served):
+    """Subtrees inside a query reported from their rank ranges: same results."""
+    ix, q, exp = served
+    arr = gpu.poisson_arrivals(len(q), 50000.0, seed=6)
+    with gpu.Server(ix, slots=4, max_batch=256, contain=True) as sv:
+        counts, hashes, _ = sv.replay(q, arr)
+    assert_bits_equal(counts, exp["counts"], "served counts, contain")
+    assert_bits_equal(hashes, exp["hashes"], "served hashes, contain")
