@@ -136,7 +136,7 @@ static std::vector<uint64_t> packed_levels(uint64_t n, uint32_t F) {
 // Allocate the index and its topology for n objects.
 static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** out) {
   QCP_TRY(validate_cfg(cfg));
-  if (n >= 0xFFFFFFF0ull) return QCP_EINVAL;  // ranks and node ids are 32-bit
+  if (n >= 0xFFFFFE00ull) return QCP_EINVAL;  // ranks (+ one fanout) and node ids are 32-bit
   QCP_TRY(set_device((int)cfg->device));
   qcp_index* ix = new (std::nothrow) qcp_index();
   if (!ix) return QCP_ENOMEM;
@@ -923,12 +923,14 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   }
   st.kernel_ms = k1 + k2;
   st.total_ms = tot;
-  st.internal_entries = ctl.internal_entries;
-  st.leaf_entries = ctl.leaf_entries;
-  st.total_hits = ctl.total_hits;
+  for (int x = 0; x < 8; ++x) {
+    st.internal_entries += ctl.internal_entries[x];
+    st.leaf_entries += ctl.leaf_entries[x];
+    st.total_hits += ctl.total_hits[x];
+    st.contained_runs += ctl.contained[x];
+  }
   st.donated_items = ctl.donated;
-  st.contained_runs = ctl.contained;
-  b->total_hits = ctl.total_hits;
+  b->total_hits = st.total_hits;
   if (donate) b->slots_dirty = std::min<uint64_t>(b->slot_cap, std::max<uint64_t>(ctl.tail, 1));
   if (stats) *stats = st;
   return QCP_OK;

@@ -19,11 +19,13 @@ struct QueueCtl {
   uint32_t pending;   // items not finished (root items + donations)
   uint32_t done;      // set once pending reaches zero
   uint32_t xhead[8];  // next root ticket of each XCD's query range
-  unsigned long long internal_entries;
-  unsigned long long leaf_entries;
   unsigned long long donated;
-  unsigned long long total_hits;
-  unsigned long long contained;  // contained subtree runs reported without a visit
+  // statistics, one slot per XCD (summed on the host): spreads the end-of-
+  // kernel atomics over eight addresses
+  unsigned long long internal_entries[8];
+  unsigned long long leaf_entries[8];
+  unsigned long long total_hits[8];
+  unsigned long long contained[8];  // contained subtree runs reported without a visit
 };
 
 // Tree + query batch as seen by the search kernels.
@@ -71,7 +73,8 @@ __device__ __forceinline__ uint64_t set_hash(uint64_t s1, uint64_t s2) { return 
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// the wave's predicate mask straight from the compare (no int round trip)
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // popcount of the ballot bits below this lane
 __device__ __forceinline__ uint32_t rank_below(uint64_t b) {
@@ -119,21 +122,23 @@ __device__ __forceinline__ bool inside(const float* ql, const float* qh, const f
                                        const float* hi) {
   bool ok = true;
 #pragma unroll
-  for (int d = 0; d < D; ++d) ok = ok && (ql[d] <= lo[d]) && (hi[d] <= qh[d]);
+  for (int d = 0; d < D; ++d) ok = ok & (ql[d] <= lo[d]) & (hi[d] <= qh[d]);
   return ok;
 }
 
-// Power sums of the IDs s+1 .. e (mod 2^64): sum of i and sum of i^2.  The
-// divisions by 2 and 3 are taken on the factors before the (wrapping) product.
-__device__ __forceinline__ uint64_t psum1(uint64_t m) {  // 1 + .. + m, m < 2^32
-  return (m & 1) ? m * ((m + 1) >> 1) : (m >> 1) * (m + 1);
+// Power sums 1 + .. + m and 1 + .. + m^2 (mod 2^64) for m < 2^32 - 512 (the
+// rank bound), all divisions exact and in 32 bits: one of m, m+1 is even, and
+// 3 divides m, m+1 or 2m+1 (for m = 3t+1, (2m+1)/3 = 2t+1).
+__device__ __forceinline__ uint64_t psum1(uint32_t m) {
+  return (m & 1u) ? (uint64_t)m * ((m + 1u) >> 1) : (uint64_t)(m >> 1) * (m + 1u);
 }
-__device__ __forceinline__ uint64_t psum2(uint64_t m) {  // 1 + .. + m^2, m < 2^32
-  uint64_t a = m, b = m + 1, c = 2 * m + 1;
-  if (a & 1) b >>= 1; else a >>= 1;
-  const uint64_t r = m % 3;
-  if (r == 0) a /= 3; else if (r == 2) b /= 3; else c /= 3;
-  return a * b * c;
+__device__ __forceinline__ uint64_t psum2(uint32_t m) {
+  uint32_t a = m, b = m + 1u;
+  if (m & 1u) b >>= 1; else a >>= 1;
+  const uint32_t t = m / 3u, r = m - 3u * t;
+  uint64_t c = 2ull * m + 1ull;
+  if (r == 0u) a /= 3u; else if (r == 2u) b /= 3u; else c = 2ull * t + 1ull;
+  return (uint64_t)a * b * c;
 }
 
 // The reference's closed-interval test: rejected iff q.lo > e.hi or q.hi < e.lo
@@ -141,10 +146,25 @@ __device__ __forceinline__ uint64_t psum2(uint64_t m) {  // 1 + .. + m^2, m < 2^
 template <int D>
 __device__ __forceinline__ bool overlaps(const float* ql, const float* qh,
                                          const float* lo, const float* hi) {
-  bool ok = true;
+  bool ok = true;  // bitwise: no short-circuit branches
 #pragma unroll
-  for (int d = 0; d < D; ++d) ok = ok && !(ql[d] > hi[d] || qh[d] < lo[d]);
+  for (int d = 0; d < D; ++d) ok = ok & !(ql[d] > hi[d]) & !(qh[d] < lo[d]);
   return ok;
+}
+
+// The same test as overlaps(), in vector arithmetic instead of per-dimension
+// compare masks (which the scalar unit would have to combine): an entry is
+// rejected iff max_d(max(q.lo[d] - hi[d], lo[d] - q.hi[d])) > 0.  With IEEE
+// subtraction and denormals kept, x - y > 0 exactly when x > y (the difference
+// of distinct floats never rounds to zero, and its sign is exact), and fmaxf
+// drops a NaN operand, so a NaN coordinate rejects nothing, as in overlaps().
+template <int D>
+__device__ __forceinline__ bool overlap_test(const float* ql, const float* qh, const float* lo,
+                                             const float* hi) {
+  float t = fmaxf(ql[0] - hi[0], lo[0] - qh[0]);
+#pragma unroll
+  for (int d = 1; d < D; ++d) t = fmaxf(t, fmaxf(ql[d] - hi[d], lo[d] - qh[d]));
+  return !(t > 0.0f);
 }
 
 }  // namespace qcp

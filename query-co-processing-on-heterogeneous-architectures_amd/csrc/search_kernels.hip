@@ -124,6 +124,10 @@ struct Ctx {
 // Output modes of the persistent form.
 enum { kOutCount = 0, kOutHash = 1, kOutIds = 2 };
 
+// FP32 in the global address space (a pointer rebuilt from an integer is
+// otherwise generic, which would turn its loads into flat loads)
+typedef __attribute__((address_space(1))) float GlobalF32;
+
 // Evaluate the children [cbase, cbase+cc) (node ids) of an internal node
 // against the query; for explicit layouts also their leaf flags.
 template <int D, int MW, bool EXPL>
@@ -162,8 +166,8 @@ __device__ __forceinline__ void emit_range(const SearchArgs& a, Ctx<D>& c, uint6
   c.cnt += (uint32_t)len;
   c.cr += 1;
   if (OUT == kOutHash) {
-    c.s1c += psum1(e) - psum1(s);
-    c.s2c += psum2(e) - psum2(s);
+    c.s1c += psum1((uint32_t)e) - psum1((uint32_t)s);
+    c.s2c += psum2((uint32_t)e) - psum2((uint32_t)s);
   } else if (OUT == kOutIds) {
     const uint32_t lane = lane_id();
     uint32_t* dst = a.ids + c.out_base;
@@ -183,6 +187,26 @@ __device__ __forceinline__ void emit_range(const SearchArgs& a, Ctx<D>& c, uint6
   }
 }
 
+// Results of one 64-entry chunk whose first rank is st: `hit` is this lane's
+// entry overlapping the query.
+template <int D, int OUT>
+__device__ __forceinline__ void chunk_out(const SearchArgs& a, Ctx<D>& c, uint32_t st, bool hit) {
+  const uint32_t lane = lane_id();
+  const uint64_t b = ballot(hit);
+  const uint32_t nb = (uint32_t)__popcll(b);
+  if (OUT == kOutIds) {
+    if (hit) (a.ids + c.out_base)[rank_below(b)] = st + lane + 1u;
+    c.out_base += nb;
+  } else if (OUT == kOutHash) {
+    // branch-free: a select, a 64-bit add and one v_mad_u64_u32 per chunk
+    const uint32_t id = st + lane + 1u;
+    const uint32_t x = hit ? id : 0u;
+    c.s1_lane += x;
+    c.s2_lane += (uint64_t)x * x;
+  }
+  c.cnt += nb;
+}
+
 // Stream the leaf runs at the bottom of mask m (bits that are also set in lm),
 // U 64-entry chunks in flight, stopping at the first set bit that is neither a
 // leaf nor inside the query.  Runs of children inside the query (im) are
@@ -194,14 +218,38 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
                                           RunRange run_range, SubRange sub_range) {
   constexpr int E = BOX ? 2 * D : D;
   const uint32_t lane = lane_id();
-  uint64_t cs = 0, ce = 0;
+  // Ranks fit in 32 bits (IDs are u32), so the run bookkeeping stays in SGPRs
+  // with scalar compares, and each chunk load is a uniform base pointer plus
+  // the lane offset (no per-lane 64-bit address arithmetic).
+  const float* col[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) col[e] = a.obj + (uint64_t)e * a.obj_stride;
+  uint32_t cs = 0, ce = 0;
   while (true) {
-    uint64_t st[U];
+    // fast path: whole groups of U chunks inside the current run, with no
+    // per-chunk bounds or refill logic and one base pointer per column per
+    // group (the chunks are immediate offsets from it)
+    while (ce - cs >= (uint32_t)(U * kWave)) {
+      float w[U][E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const GlobalF32* base = reinterpret_cast<const GlobalF32*>(
+            uni64(reinterpret_cast<uint64_t>(col[e] + cs)));
+#pragma unroll
+        for (int u = 0; u < U; ++u) w[u][e] = base[u * kWave + lane];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        chunk_out<D, OUT>(a, c, cs + (uint32_t)(u * kWave),
+                          overlap_test<D>(c.ql, c.qh, w[u], BOX ? w[u] + D : w[u]));
+      cs += (uint32_t)(U * kWave);
+    }
+    uint32_t st[U];
     uint32_t len[U];
     bool blocked = false;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      while (cs >= ce && !blocked) {
+      while (cs == ce && !blocked) {
         const int k = m.lowest();
         if (k < 0) break;
         if (im.test(k)) {
@@ -220,15 +268,16 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
         if (lm.test(k)) {
           const int k2 = (m & lm).andnot(im).run_end(k);
           m.clear_range(k, k2);
-          run_range(k, k2, cs, ce);
-          cs = uni64(cs);
-          ce = uni64(ce);
+          uint64_t s, e;
+          run_range(k, k2, s, e);
+          cs = uni((uint32_t)s);
+          ce = uni((uint32_t)e);
           c.le += ce - cs;
         }
         break;
       }
-      uint64_t rem = ce > cs ? ce - cs : 0;
-      len[u] = (uint32_t)(rem < kWave ? rem : kWave);
+      const uint32_t rem = ce - cs;  // cs <= ce: chunks never pass the run's end
+      len[u] = rem < (uint32_t)kWave ? rem : (uint32_t)kWave;
       st[u] = cs;
       cs += len[u];
     }
@@ -237,9 +286,14 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (len[u]) {
-        const uint64_t i = st[u] + lane;
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[u][e] = a.obj[(uint64_t)e * a.obj_stride + i];
+        for (int e = 0; e < E; ++e) {
+          // the chunk's base pointer is made explicitly wave-uniform so the
+          // load takes the SGPR-base + 32-bit lane-offset form
+          const GlobalF32* base = reinterpret_cast<const GlobalF32*>(
+              uni64(reinterpret_cast<uint64_t>(col[e] + st[u])));
+          v[u][e] = base[lane];
+        }
       }
     }
 #pragma unroll
@@ -247,20 +301,7 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
       if (len[u]) {
         const float* lo = v[u];
         const float* hi = BOX ? v[u] + D : v[u];
-        bool hit = (lane < len[u]) && overlaps<D>(c.ql, c.qh, lo, hi);
-        uint64_t b = ballot(hit);
-        uint32_t nb = (uint32_t)__popcll(b);
-        if (OUT == kOutIds) {
-          if (hit) a.ids[c.out_base + rank_below(b)] = (uint32_t)(st[u] + lane + 1);
-          c.out_base += nb;
-        } else if (OUT == kOutHash) {
-          // branch-free: a select, a 64-bit add and one v_mad_u64_u32 per chunk
-          const uint32_t id = (uint32_t)st[u] + lane + 1u;
-          const uint32_t x = hit ? id : 0u;
-          c.s1_lane += x;
-          c.s2_lane += (uint64_t)x * x;
-        }
-        c.cnt += nb;
+        chunk_out<D, OUT>(a, c, st[u], (lane < len[u]) & overlap_test<D>(c.ql, c.qh, lo, hi));
       }
     }
   }
@@ -346,11 +387,13 @@ __device__ void walk(const SearchArgs& a, Ctx<D>& c, uint32_t node, Frame<MW>* s
   const uint32_t F = a.F;
   int level = EXPL ? 0 : level_of(a, node);
   auto packed_range = [&](uint32_t j, int k, int k2, uint64_t& s, uint64_t& e) {
-    // children of packed node j at the leaf-parent level are leaves jF+k ..
-    uint64_t c0 = (uint64_t)j * F + (uint64_t)k;
-    uint64_t c1 = (uint64_t)j * F + (uint64_t)k2;
+    // children of packed node j at the leaf-parent level are leaves jF+k ..;
+    // 32-bit (scalar) arithmetic: the leaves' ranks end below n + F < 2^32
+    const uint32_t c0 = j * F + (uint32_t)k;
+    const uint32_t c1 = j * F + (uint32_t)k2;
+    const uint32_t n32 = (uint32_t)a.n;
     s = c0 * F;
-    e = c1 * F < a.n ? c1 * F : a.n;
+    e = c1 * F < n32 ? c1 * F : n32;
   };
   bool is_leaf;
   uint32_t first, cnt;
@@ -511,10 +554,17 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
   extern __shared__ unsigned char qcp_lds[];
   Frame<MW>* stk = reinterpret_cast<Frame<MW>*>(qcp_lds) + (threadIdx.x / kWave) * kDepth;
   const uint32_t lane = lane_id();
+  __shared__ unsigned long long wg_stats[4];
+  if (threadIdx.x < 4) wg_stats[threadIdx.x] = 0;
+  __syncthreads();
   uint64_t ie = 0, le = 0, hits = 0, cruns = 0;
   const bool donating = OUT != kOutIds && a.donate;
   const uint32_t xcd = blockIdx.x % kXcds;
   bool roots_left = true;
+  // root items finished by this wave and not yet taken off `pending`: they
+  // are reported in one atomic when the wave runs out of root tickets (one
+  // same-address atomic per item would serialise)
+  uint32_t roots_done = 0;
   while (true) {
     uint32_t q = 0, node = 0;
     bool got = false;
@@ -537,6 +587,13 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
     }
     if (!got) {
       if (!donating) break;
+      if (roots_done) {
+        uint32_t prev = 0;
+        if (lane == 0) prev = atomicSub(&a.ctl->pending, roots_done);
+        if (uni(prev) == roots_done && lane == 0)
+          __hip_atomic_store(&a.ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        roots_done = 0;
+      }
       // Enough waves already wait: leave.  (Signed: donors may have claimed
       // more slots than there are tickets, and those slots need takers.)
       const int32_t waiting = (int32_t)(ld_acq(&a.ctl->head) - ld_acq(&a.ctl->tail));
@@ -584,20 +641,32 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
         }
       }
     }
-    if (donating && lane == 0) {
-      if (atomicSub(&a.ctl->pending, 1u) == 1u)
-        __hip_atomic_store(&a.ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (donating) {
+      if (got) {
+        ++roots_done;
+      } else if (lane == 0) {
+        if (atomicSub(&a.ctl->pending, 1u) == 1u)
+          __hip_atomic_store(&a.ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
     ie += c.ie;
     le += c.le;
     hits += c.cnt;
     cruns += c.cr;
   }
+  // statistics: waves -> workgroup (LDS) -> this XCD's slot
   if (lane == 0) {
-    atomicAdd(&a.ctl->internal_entries, (unsigned long long)ie);
-    atomicAdd(&a.ctl->leaf_entries, (unsigned long long)le);
-    atomicAdd(&a.ctl->total_hits, (unsigned long long)hits);
-    if (cruns) atomicAdd(&a.ctl->contained, (unsigned long long)cruns);
+    atomicAdd(&wg_stats[0], (unsigned long long)ie);
+    atomicAdd(&wg_stats[1], (unsigned long long)le);
+    atomicAdd(&wg_stats[2], (unsigned long long)hits);
+    atomicAdd(&wg_stats[3], (unsigned long long)cruns);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&a.ctl->internal_entries[xcd], wg_stats[0]);
+    atomicAdd(&a.ctl->leaf_entries[xcd], wg_stats[1]);
+    atomicAdd(&a.ctl->total_hits[xcd], wg_stats[2]);
+    if (wg_stats[3]) atomicAdd(&a.ctl->contained[xcd], wg_stats[3]);
   }
 }
 
@@ -703,17 +772,20 @@ __global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
 }
 
 __global__ void k_reset_ctl(QueueCtl* ctl, uint32_t pending) {
-  if (threadIdx.x < 8) ctl->xhead[threadIdx.x] = 0;
-  if (threadIdx.x == 0) {
+  const uint32_t t = threadIdx.x;
+  if (t < 8) {
+    ctl->xhead[t] = 0;
+    ctl->internal_entries[t] = 0;
+    ctl->leaf_entries[t] = 0;
+    ctl->total_hits[t] = 0;
+    ctl->contained[t] = 0;
+  }
+  if (t == 0) {
     ctl->head = 0;
     ctl->tail = 0;
     ctl->pending = pending;
     ctl->done = pending == 0 ? 1u : 0u;
-    ctl->internal_entries = 0;
-    ctl->leaf_entries = 0;
     ctl->donated = 0;
-    ctl->total_hits = 0;
-    ctl->contained = 0;
   }
 }
 
