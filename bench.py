@@ -329,43 +329,72 @@ def run_config(qcp, name, args):
     raise SystemExit("unknown config " + name)
 
 
+TUNE_FANOUTS = (32, 64, 128)
+TUNE_WAVES = (1, 2, 4, 8)
+TUNE_CHUNKS = (2, 4, 8)
+
+
 def profile_traffic(qcp, args):
-    """A short, fixed run for counter collection: steps persistent searches of
-    the 16M-point config at --fanout (default 64), then one 1 GiB hipMemcpy
-    device-to-device (a copy kernel of known size, to calibrate FETCH_SIZE and
-    WRITE_SIZE).  Prints the algorithmic bytes per search launch."""
+    """A short, fixed run for counter collection on the 16M-point config: with
+    --sweep, `steps` persistent searches (counts+hash) of every configuration
+    the autotuner can pick (fanout x waves per workgroup x leaf chunks x
+    donation, in that nesting order), else of the one given by --fanout /
+    --block-waves / --chunks / --donate; then one 1 GiB device-to-device
+    hipMemcpy (a copy kernel of known size, to calibrate FETCH_SIZE and
+    WRITE_SIZE).  Prints one JSON line: the configuration of each k_search
+    dispatch in launch order and the algorithmic bytes of each."""
     import ctypes
 
-    F = args.fanout or 64
     w = qcp.workload("c2", scale=args.scale)
-    ix = qcp.Index.build(w["objs"], 3, fanout=F)
     q = w["queries"]
-    kw = {"block_waves": args.block_waves, "chunks": args.chunks, "donate": args.donate}
-    with ix.batch(q) as b:
+    if args.sweep:
+        cfgs = [(F, bw, u, d) for F in TUNE_FANOUTS for bw in TUNE_WAVES for u in TUNE_CHUNKS
+                for d in (0, 1)]
+    else:
+        cfgs = [(args.fanout or 64, args.block_waves, args.chunks, args.donate)]
+    launches = []
+    cur_f, ix, b = None, None, None
+    for F, bw, u, d in cfgs:
+        if F != cur_f:
+            if b:
+                b.close()
+                ix.close()
+            ix = qcp.Index.build(w["objs"], 3, fanout=F)
+            b = ix.batch(q)
+            cur_f = F
         for _ in range(args.steps):
-            st = b.run(**kw)
+            st = b.run(block_waves=bw, chunks=u, donate=d)
+            launches.append({"fanout": F, "block_waves": bw, "chunks": u, "donate": d,
+                             "algorithmic_bytes": algorithmic_bytes(st, len(q), 3, 3),
+                             "kernel_ms": st["kernel_ms"]})
+    b.close()
+    ix.close()
     hip = ctypes.CDLL("libamdhip64.so")
     nbytes = 1 << 30
     src = qcp.DeviceBuffer(np.zeros(nbytes // 4, np.float32))
     dst = qcp.DeviceBuffer(np.zeros(nbytes // 4, np.float32))
     rc = hip.hipMemcpy(dst.ptr, src.ptr, ctypes.c_size_t(nbytes), 3)  # device to device
     hip.hipDeviceSynchronize()
-    print(json.dumps({"fanout": F, "queries": len(q), "steps": args.steps, "copy_bytes": nbytes,
-                      "copy_rc": rc, "algorithmic_bytes": algorithmic_bytes(st, len(q), 3, 3),
-                      "internal_entries": st["internal_entries"],
-                      "leaf_entries": st["leaf_entries"], "kernel_ms": st["kernel_ms"]}),
-          flush=True)
+    print(json.dumps({"queries": len(q), "steps": args.steps, "copy_bytes": nbytes,
+                      "copy_rc": rc, "launches": launches}), flush=True)
 
 
-def read_traffic(name):
+def read_traffic(name, F, kw):
+    """HBM bytes per launch measured for this search configuration (counts +
+    hash), from the committed counter summary, or None if it was not measured."""
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
-            return json.load(f)
+            t = json.load(f)
     except Exception:
         return None
+    key = (F, kw["block_waves"], kw["chunks"], kw["donate"])
+    for e in t.get("configs", []):
+        if (e["fanout"], e["block_waves"], e["chunks"], e["donate"]) == key:
+            return e
+    return None
 
 
 def main():
@@ -388,6 +417,8 @@ def main():
     ap.add_argument("--block-waves", type=int, default=0, help="--profile-traffic: waves/WG")
     ap.add_argument("--chunks", type=int, default=0, help="--profile-traffic: leaf chunks")
     ap.add_argument("--donate", type=int, default=0, help="--profile-traffic: donation")
+    ap.add_argument("--sweep", action="store_true",
+                    help="--profile-traffic: every configuration the autotuner can pick")
     ap.add_argument("--config", default="c2",
                     help="c2 (default: the headline line); c1, c3 or c5 print their own line")
     args = ap.parse_args()
@@ -411,7 +442,7 @@ def main():
     n, q, D, E = len(pts), len(queries), 3, 3
     # the library's autotuner on a query sample (untimed setup): fanout x
     # workgroup size x leaf chunks in flight x donation
-    cands = (args.fanout,) if args.fanout else (32, 64, 128)
+    cands = (args.fanout,) if args.fanout else TUNE_FANOUTS
     sample = queries[: min(q, 20000)]
     ix, kw, table = qcp.autotune(pts, 3, sample, fanouts=cands)
     info = ix.info()
@@ -426,18 +457,20 @@ def main():
     b = ix.batch(queries)
     for _ in range(args.warmup):
         b.run(**kw)
-    kms = []
+    kms, sms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st = b.run(**kw)
         kms.append(st["kernel_ms"])
+        sms.append(st["search_ms"])
     wall = time.perf_counter() - t0
     got = b.results()
     ms_step = wall * 1000.0 / args.steps
     value = q * args.steps / wall
-    kavg = float(np.mean(kms))
+    kavg = float(np.mean(kms))     # query ordering + search, per step
+    savg = float(np.mean(sms))     # the search kernel alone (HIP events on its stream)
     abytes = algorithmic_bytes(st, q, D, E)
-    achieved = abytes / (kavg * 1e-3) / 1e9
+    achieved = abytes / (savg * 1e-3) / 1e9
     log("persistent: %.3f ms/step wall, %.3f ms kernel, %.3e queries/s, hits %d, "
         "int entries %d, leaf entries %d, %.1f GB/s algorithmic" %
         (ms_step, kavg, value, st["total_hits"], st["internal_entries"], st["leaf_entries"],
@@ -469,7 +502,7 @@ def main():
         ix.close()
         serving = serving_bench(qcp, (10000, 50000, 100000), args.serve_seconds, args.scale,
                                 contain_rate=None if args.no_contain else 50000)
-    traffic = read_traffic("traffic_c2_search.json")
+    traffic = read_traffic("traffic_c2_search.json", F, kw)
     kname = "k_search<3,false,%d,false,1,%d> (block %d threads, donation %s)" % (
         (F + 63) // 64, kw["chunks"] or 4, 64 * (kw["block_waves"] or 4),
         "on" if kw["donate"] else "off")
@@ -494,8 +527,11 @@ def main():
                    "total_hits": int(st["total_hits"]), "outputs": "counts+hash"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel": kname, "kernel_ms": kavg, "algorithmic_bytes": abytes},
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "kernel": kname, "kernel_ms": savg, "device_ms_per_step": kavg,
+                     "algorithmic_bytes": abytes,
+                     "hbm_GBps": (traffic["bytes_per_launch"] / (savg * 1e-3) / 1e9
+                                  if traffic else None)},
         "contained": contained,
         "plain": plain,
         "speedup_vs_plain": (value / plain["queries_per_s"]) if plain else None,

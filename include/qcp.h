@@ -106,6 +106,9 @@ typedef struct {
   uint64_t donated_items;    /* subtrees handed to other waves                          */
   uint64_t launches;         /* kernel launches issued by the search                    */
   uint64_t contained_runs;   /* contain: runs of sibling subtrees reported without a visit */
+  double search_ms;          /* persistent form: device time of the (first) search kernel
+                                alone, without query ordering and buffer fills (whole
+                                graph when graph-captured)                              */
 } qcp_search_stats;
 
 typedef struct qcp_index qcp_index;

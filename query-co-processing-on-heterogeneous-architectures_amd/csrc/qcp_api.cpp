@@ -842,7 +842,8 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     a.donate_min_height = cfg.donate_height ? cfg.donate_height : 1;
   }
   // the device work of one run, all on the stream (graph-capturable)
-  auto enqueue = [&](uint64_t slots_to_clear) -> qcp_status {
+  // timed: HIP events around the search kernel alone (not inside a capture)
+  auto enqueue = [&](uint64_t slots_to_clear, bool timed) -> qcp_status {
     if (donate) {
       // per-item results are accumulated atomically; donated slots start empty
       HIP_TRY(launch_fill_u32(b->d_counts, q, 0u, s));
@@ -853,7 +854,9 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
       QCP_TRY(batch_order(b));
       a.qperm = b->d_qperm;
     }
+    if (timed) HIP_TRY(hipEventRecord(b->ev[4], s));
     HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, out_mode, u, grid, block, s));
+    if (timed) HIP_TRY(hipEventRecord(b->ev[5], s));
     return QCP_OK;
   };
   const bool use_graph = cfg.graph && !(cfg.outputs & QCP_OUT_IDS);
@@ -868,7 +871,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
       }
       hipGraph_t g = nullptr;
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      qcp_status cs = enqueue(b->slot_cap);
+      qcp_status cs = enqueue(b->slot_cap, false);
       hipError_t e = hipStreamEndCapture(s, &g);
       if (cs != QCP_OK) {
         if (g) hipGraphDestroy(g);
@@ -885,7 +888,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     HIP_TRY(hipEventRecord(b->ev[1], s));
   } else {
     HIP_TRY(hipEventRecord(b->ev[0], s));
-    QCP_TRY(enqueue(b->slots_dirty));
+    QCP_TRY(enqueue(b->slots_dirty, true));
     HIP_TRY(hipEventRecord(b->ev[1], s));
   }
   st.launches = 1;
@@ -923,6 +926,12 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   }
   st.kernel_ms = k1 + k2;
   st.total_ms = tot;
+  st.search_ms = k1;
+  if (!use_graph) {
+    float ks = 0;
+    hipEventElapsedTime(&ks, b->ev[4], b->ev[5]);
+    st.search_ms = ks;
+  }
   for (int x = 0; x < 8; ++x) {
     st.internal_entries += ctl.internal_entries[x];
     st.leaf_entries += ctl.leaf_entries[x];

@@ -77,7 +77,7 @@ struct qcp_batch {
   uint64_t slots_dirty = 0;       // slots written by the last donating run
   uint32_t* h_counts = nullptr;   // pinned, plain-form per-batch copies
   uint32_t* h_visits = nullptr;
-  hipEvent_t ev[4] = {};
+  hipEvent_t ev[6] = {};
   hipGraphExec_t exec = nullptr;            // captured search (qcp_search_cfg.graph)
   uint32_t exec_key[12] = {};               // configuration baked into exec
   uint32_t last_outputs = 0;

@@ -59,7 +59,8 @@ class SearchStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("total_hits", ctypes.c_uint64), ("internal_entries", ctypes.c_uint64),
                 ("leaf_entries", ctypes.c_uint64), ("donated_items", ctypes.c_uint64),
-                ("launches", ctypes.c_uint64), ("contained_runs", ctypes.c_uint64)]
+                ("launches", ctypes.c_uint64), ("contained_runs", ctypes.c_uint64),
+                ("search_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
