@@ -82,7 +82,8 @@ typedef struct {
   uint32_t leaf_chunks;    /* persistent form: 64-entry leaf chunks in flight per wave (0 = tuned) */
   uint32_t donate;         /* persistent form: subtree donation (work stealing) on/off */
   uint32_t order;          /* persistent form: 0 = process queries in Hilbert order of their
-                              centres, split into one range per XCD; 1 = as given */
+                              centres, split into one range per XCD (batches of 8192 queries
+                              or more; smaller ones as given); 1 = as given */
   uint32_t max_waiters;    /* donation: waves kept waiting for donated work (0 = 256)  */
   uint32_t hunger_period;  /* donation: internal-node descents between queue checks (0 = 8) */
   uint32_t donate_height;  /* donation (packed layouts): minimum height of a donated subtree,

@@ -50,6 +50,7 @@ constexpr uint64_t kObjSlack = 64 * 16;   // entries read past the end by masked
 constexpr uint64_t kNodeSlack = 256 + 64;
 constexpr uint32_t kMaxScanBlocks = 4096;
 constexpr uint32_t kSlotCap = 1u << 21;   // donated work items per launch
+constexpr uint64_t kOrderMinQueries = 8192;  // smaller batches are searched as given
 
 template <typename T>
 qcp_status dalloc(T** p, uint64_t count) {
@@ -825,7 +826,17 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   const int out_mode = (cfg.outputs & QCP_OUT_HASH) ? 1 : 0;
   const int u = cfg.leaf_chunks ? (int)cfg.leaf_chunks : 4;
   const bool donate = cfg.donate != 0;
-  unsigned grid = cfg.grid_blocks ? cfg.grid_blocks : persistent_grid(ix, out_mode, block);
+  // Small batches: a wave per query is enough (no donation) and a spatial
+  // order buys nothing, so the grid is sized to the batch and ordering skipped.
+  const unsigned wpb = block / 64u;
+  auto fit_grid = [&](unsigned full) -> unsigned {
+    if (cfg.grid_blocks) return cfg.grid_blocks;
+    if (donate) return full;
+    const uint64_t need = (q + wpb - 1) / wpb;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(full, need));
+  };
+  const bool order_queries = cfg.order == 0 && q >= kOrderMinQueries;
+  unsigned grid = fit_grid(persistent_grid(ix, out_mode, block));
   SearchArgs a = make_args(b);
   a.contain = (cfg.contain && ix->contain_ok) ? 1u : 0u;
   if (donate) {
@@ -850,7 +861,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
       HIP_TRY(launch_fill_u32((uint32_t*)b->d_hashes, 2 * q, 0u, s));
       HIP_TRY(launch_fill_u32((uint32_t*)b->d_slots, 2ull * slots_to_clear, 0xFFFFFFFFu, s));
     }
-    if (cfg.order == 0) {
+    if (order_queries) {
       QCP_TRY(batch_order(b));
       a.qperm = b->d_qperm;
     }
@@ -908,7 +919,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     }
     a.ids = b->d_ids;
     a.donate = 0;
-    unsigned grid2 = cfg.grid_blocks ? cfg.grid_blocks : persistent_grid(ix, 2, block);
+    unsigned grid2 = fit_grid(persistent_grid(ix, 2, block));
     HIP_TRY(hipEventRecord(b->ev[2], s));
     HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, 2, 4, grid2, block, s));
     HIP_TRY(hipEventRecord(b->ev[3], s));

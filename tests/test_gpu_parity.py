@@ -171,6 +171,23 @@ def test_containment_off_for_inverted_boxes(gpu, O):
     assert got["stats"]["contained_runs"] == 0
 
 
+@pytest.mark.parametrize("layout", ["mphr", "bvh"])
+def test_ordered_large_batch(gpu, O, layout):
+    """Batches of 8192+ queries are processed in Hilbert order of their centres
+    (per-XCD ranges); smaller ones as given.  Both must give the oracle's results."""
+    pts = rand_points(150000, 3, seed=91)
+    q = rand_queries(12000, 3, 92, pts)
+    ix = gpu.Index.build(pts, 3, fanout=64, layout=layout)
+    t = _oracle_tree(O, pts, 3, False, 64, "ref", "lo", layout)
+    exp = t.search_dfs(q, want_ids=True)
+    got = ix.search(q, ids=True)
+    assert_results_equal(got, exp, "ordered batch " + layout)
+    given = ix.search(q, order=1, chunks=2)
+    assert_results_equal(given, exp, "given order " + layout, ids=False)
+    small = ix.search(q[:3000])
+    assert_bits_equal(small["hashes"], exp["hashes"][:3000], "small batch " + layout)
+
+
 def test_autotune_returns_consistent_index(gpu, O):
     pts = rand_points(100000, 3, 77)
     q = rand_queries(500, 3, 78, pts)
