@@ -75,6 +75,45 @@ def contained_search(b, q, D, E, steps, warmup, kw, canon):
                                       np.array_equal(got["hashes"], canon["hashes"]))}
 
 
+def build_figures(qcp, ix, pts, D, threads, with_cpu, key_mode="ref"):
+    """GPU build time of the index's configuration (best of 3 in-place rebuilds
+    from device-resident input) and, with_cpu, the oracle's threaded CPU build
+    (the reference's CPU build path with its thread fan-outs) on the same
+    points, checked equal to the GPU's keys, permutation and node boxes."""
+    buf = qcp.DeviceBuffer(pts)
+    best = None
+    for _ in range(3):
+        ix.rebuild(buf)
+        inf = ix.info()
+        if best is None or inf["build_ms"] < best["build_ms"]:
+            best = inf
+    buf.close()
+    n = len(pts)
+    out = {"n": n, "gpu_ms": best["build_ms"],
+           "gpu_mkeys_per_s": n / (best["build_ms"] * 1e-3) / 1e6,
+           "gpu_phase_ms": dict(zip(("keys", "sort", "gather", "pack"),
+                                    best["build_phase_ms"][:4]))}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+
+        F = best["fanout"]
+        t0 = time.perf_counter()
+        k, perm, sobjs, boxes, ms, bad = O.build_mt(
+            pts, D, False, F, O.KEY_REF if key_mode == "ref" else O.KEY_B16, nthreads=threads)
+        dt = time.perf_counter() - t0
+        ex = ix.export()
+        ok = bool(bad == 0 and np.array_equal(k, ex["keys"]) and np.array_equal(perm, ex["perm"])
+                  and np.array_equal(boxes.view(np.uint32), ex["boxes"].view(np.uint32)))
+        out["cpu"] = {"value": n / dt / 1e6, "unit": "Mkeys/s", "cores": threads, "kind": "port",
+                      "sample": "all %d points, oracle build over %d threads, %.1f s" %
+                                (n, threads, dt),
+                      "phase_ms": dict(zip(("keys", "sort", "gather", "pack"),
+                                           [float(x) for x in ms])),
+                      "matches_gpu": ok}
+    return out
+
+
 def cpu_baseline(qcp, ix, queries, got, budget_s, threads):
     """The oracle's multi-threaded DFS (the reference CPU search's static query
     split, src/tree/bvh.cpp:195-220) on the same tree, over a bounded prefix of
@@ -102,10 +141,24 @@ def cpu_baseline(qcp, ix, queries, got, budget_s, threads):
     c, h = t.search_dfs_mt(queries[:m], threads)
     dt = time.perf_counter() - t0
     ok = bool(np.array_equal(c, got["counts"][:m]) and np.array_equal(h, got["hashes"][:m]))
+    # the serial MPRS restatement, one query at a time per thread (the CPU
+    # counterpart of the plain form), over a prefix of about a third of the budget
+    probe = min(len(queries), max(threads * 4, 64))
+    t0 = time.perf_counter()
+    t.search_mprs_mt(queries[:probe], threads)
+    per_q = max((time.perf_counter() - t0) / probe, 1e-9)
+    mm = int(min(len(queries), max(probe, budget_s / 3 / per_q)))
+    t0 = time.perf_counter()
+    mc, mh = t.search_mprs_mt(queries[:mm], threads)
+    mdt = time.perf_counter() - t0
+    mok = bool(np.array_equal(mc, got["counts"][:mm]) and np.array_equal(mh, got["hashes"][:mm]))
     return {"value": m / dt, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": "first %d of %d queries, oracle DFS over %d threads, %.1f s" %
                       (m, len(queries), threads, dt),
-            "matches_gpu": ok}
+            "matches_gpu": ok,
+            "mprs": {"value": mm / mdt, "unit": "queries/s", "cores": threads,
+                     "sample": "first %d queries, oracle MPRS over %d threads, %.1f s" %
+                               (mm, threads, mdt), "matches_gpu": mok}}
 
 
 def serving_bench(qcp, rates, seconds, scale, contain_rate=None):
@@ -326,6 +379,9 @@ def run_config(qcp, name, args):
             log("c5 search contained: %.3e q/s, %.1f ms" % (len(q) / wall, wall * 1e3))
         t = oracle_tree_from(ix)
         res["cpu_baseline"] = cpu_sample(t, q, got, args.cpu_budget, threads)
+        cb = build_figures(qcp, ix, pts, 2, threads, True)
+        res["cpu_build"] = cb["cpu"]
+        log("c5 cpu build: %.1f Mkeys/s" % cb["cpu"]["value"])
         with ix.batch(q[:1024]) as pb:
             pst = pb.run(mode="plain", plain_batch=128)
         res["plain_qps_first_1024"] = 1024 / (pst["total_ms"] * 1e-3)
@@ -501,10 +557,15 @@ def main():
                  "matches_persistent": bool(np.array_equal(pres["counts"], got["counts"][: len(pq)]))}
         log("plain: %.3e queries/s over %d queries" % (plain["queries_per_s"], len(pq)))
     cpu = None
+    threads = min(os.cpu_count() or 1, 16)
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(qcp, ix, queries, got, args.cpu_budget, threads)
-        log("cpu baseline: %.3e queries/s (%s)" % (cpu["value"], cpu["sample"]))
+        log("cpu baseline: %.3e queries/s (%s); MPRS %.3e queries/s" %
+            (cpu["value"], cpu["sample"], cpu["mprs"]["value"]))
+    build = build_figures(qcp, ix, pts, D, threads, not args.no_cpu)
+    log("build: GPU %.2f ms (%.0f Mkeys/s)%s" % (
+        build["gpu_ms"], build["gpu_mkeys_per_s"],
+        ", CPU %.1f Mkeys/s" % build["cpu"]["value"] if "cpu" in build else ""))
     serving = None
     if not args.no_serve:
         b.close()
@@ -545,6 +606,7 @@ def main():
         "plain": plain,
         "speedup_vs_plain": (value / plain["queries_per_s"]) if plain else None,
         "cpu_baseline": cpu,
+        "build": build,
         "serving": serving,
     }
     if serving is None:

@@ -7,11 +7,13 @@
  * so every FP32 operation here is one IEEE single-precision operation, as in
  * the reference's host code.
  */
+#define _POSIX_C_SOURCE 200809L
 #include "qcp_oracle.h"
 
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 typedef uint64_t u64;
 typedef int64_t i64;
@@ -747,4 +749,189 @@ void orc_search_brute(const float *sobjs, uint64_t n, unsigned D, int is_box,
     if (counts) counts[qi] = cnt;
     if (hashes) hashes[qi] = hash;
   }
+}
+
+/* ---- multi-threaded CPU baselines ------------------------------------------- */
+/* The reference's CPU build with its thread fan-outs: key mapping over static
+ * chunks of std::thread x cores (Tree::Thread_Mapping, src/tree/tree.cpp:
+ * 596-639), a parallel sort (TBB parallel_sort, src/sort/parallel_sorter.cpp:
+ * 22-51; substituted by a pthreads merge sort that is stable, TBB being absent),
+ * the rank-ordered gather, and the bottom-up node boxes (BottomUpBuildonCPU,
+ * src/tree/tree.cpp:839-928, with the leaf level split over the threads).
+ * Results equal orc_keys + orc_sort_stable + orc_pack.  ms[4] receives the
+ * wall time of keys, sort, gather and pack. */
+typedef struct {
+  const float *objs; uint64_t s, e; unsigned D; int is_box, key_mode, key_src;
+  uint64_t *keys; int bad;
+} keys_arg;
+
+static void *keys_worker(void *p) {
+  keys_arg *a = (keys_arg *)p;
+  unsigned const E = a->is_box ? 2 * a->D : a->D;
+  a->bad = orc_keys(a->objs + a->s * E, a->e - a->s, a->D, a->is_box, a->key_mode,
+                    a->key_src, a->keys + a->s);
+  return NULL;
+}
+
+typedef struct {
+  const uint64_t *keys; uint32_t *src, *dst; uint64_t lo, mid, hi;
+} merge_arg;
+
+/* stable merge of src[lo,mid) and src[mid,hi) into dst[lo,hi) by (key, index) */
+static void *merge_worker(void *p) {
+  merge_arg *m = (merge_arg *)p;
+  uint64_t a = m->lo, b = m->mid, k = m->lo;
+  while (a < m->mid && b < m->hi) {
+    if (m->keys[m->src[b]] < m->keys[m->src[a]]) m->dst[k++] = m->src[b++];
+    else m->dst[k++] = m->src[a++];
+  }
+  while (a < m->mid) m->dst[k++] = m->src[a++];
+  while (b < m->hi) m->dst[k++] = m->src[b++];
+  return NULL;
+}
+
+typedef struct {
+  const uint64_t *keys; uint32_t *perm; uint64_t s, e;
+} sort_arg;
+
+static void *sort_worker(void *p) {
+  sort_arg *a = (sort_arg *)p;
+  uint64_t n = a->e - a->s;
+  uint32_t *local = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t *src = local, *dst = tmp;
+  for (uint64_t i = 0; i < n; ++i) local[i] = (uint32_t)(a->s + i);
+  for (uint64_t w = 1; w < n; w *= 2) {
+    for (uint64_t lo = 0; lo < n; lo += 2 * w) {
+      merge_arg m = {a->keys, src, dst, lo, lo + w < n ? lo + w : n, lo + 2 * w < n ? lo + 2 * w : n};
+      merge_worker(&m);
+    }
+    uint32_t *t = src; src = dst; dst = t;
+  }
+  memcpy(a->perm + a->s, src, sizeof(uint32_t) * n);
+  free(local);
+  free(tmp);
+  return NULL;
+}
+
+typedef struct {
+  const float *objs; const uint32_t *perm; float *sobjs; uint64_t s, e; unsigned E;
+} gather_arg;
+
+static void *gather_worker(void *p) {
+  gather_arg *a = (gather_arg *)p;
+  for (uint64_t r = a->s; r < a->e; ++r)
+    memcpy(a->sobjs + r * a->E, a->objs + (uint64_t)a->perm[r] * a->E, sizeof(float) * a->E);
+  return NULL;
+}
+
+static double now_ms(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+int orc_build_mt(const float *objs, uint64_t n, unsigned D, int is_box, int key_mode,
+                 int key_src, unsigned F, unsigned nthreads, uint64_t *keys, uint32_t *perm,
+                 float *sobjs, float *node_boxes, double *ms) {
+  if (nthreads < 1) nthreads = 1;
+  unsigned const E = is_box ? 2 * D : D;
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+  int bad = 0;
+  /* keys */
+  double t0 = now_ms();
+  keys_arg *ka = (keys_arg *)malloc(sizeof(keys_arg) * nthreads);
+  for (unsigned t = 0; t < nthreads; ++t) {
+    keys_arg a = {objs, n * t / nthreads, n * (t + 1) / nthreads, D, is_box, key_mode, key_src,
+                  keys, 0};
+    ka[t] = a;
+    pthread_create(&th[t], NULL, keys_worker, &ka[t]);
+  }
+  for (unsigned t = 0; t < nthreads; ++t) { pthread_join(th[t], NULL); bad |= ka[t].bad; }
+  free(ka);
+  double t1 = now_ms();
+  /* sort: per-thread stable merge sorts, then rounds of pairwise merges */
+  sort_arg *sa = (sort_arg *)malloc(sizeof(sort_arg) * nthreads);
+  for (unsigned t = 0; t < nthreads; ++t) {
+    sort_arg a = {keys, perm, n * t / nthreads, n * (t + 1) / nthreads};
+    sa[t] = a;
+    pthread_create(&th[t], NULL, sort_worker, &sa[t]);
+  }
+  for (unsigned t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+  free(sa);
+  {
+    uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t *src = perm, *dst = tmp;
+    merge_arg *ma = (merge_arg *)malloc(sizeof(merge_arg) * nthreads);
+    for (unsigned w = 1; w < nthreads; w *= 2) {
+      unsigned jobs = 0;
+      for (unsigned t = 0; t < nthreads; t += 2 * w) {
+        uint64_t lo = n * t / nthreads;
+        uint64_t mid = n * (t + w < nthreads ? t + w : nthreads) / nthreads;
+        uint64_t hi = n * (t + 2 * w < nthreads ? t + 2 * w : nthreads) / nthreads;
+        merge_arg m = {keys, src, dst, lo, mid, hi};
+        ma[jobs] = m;
+        pthread_create(&th[jobs], NULL, merge_worker, &ma[jobs]);
+        ++jobs;
+      }
+      for (unsigned j = 0; j < jobs; ++j) pthread_join(th[j], NULL);
+      uint32_t *x = src; src = dst; dst = x;
+    }
+    if (src != perm) memcpy(perm, src, sizeof(uint32_t) * n);
+    free(ma);
+    free(tmp);
+  }
+  /* the sorted keys */
+  {
+    uint64_t *sk = (uint64_t *)malloc(sizeof(uint64_t) * (n ? n : 1));
+    for (uint64_t r = 0; r < n; ++r) sk[r] = keys[perm[r]];
+    memcpy(keys, sk, sizeof(uint64_t) * n);
+    free(sk);
+  }
+  double t2 = now_ms();
+  gather_arg *ga = (gather_arg *)malloc(sizeof(gather_arg) * nthreads);
+  for (unsigned t = 0; t < nthreads; ++t) {
+    gather_arg a = {objs, perm, sobjs, n * t / nthreads, n * (t + 1) / nthreads, E};
+    ga[t] = a;
+    pthread_create(&th[t], NULL, gather_worker, &ga[t]);
+  }
+  for (unsigned t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+  free(ga);
+  double t3 = now_ms();
+  orc_pack(sobjs, n, D, is_box, F, node_boxes);
+  double t4 = now_ms();
+  if (ms) { ms[0] = t1 - t0; ms[1] = t2 - t1; ms[2] = t3 - t2; ms[3] = t4 - t3; }
+  free(th);
+  return bad;
+}
+
+typedef struct {
+  const float *sobjs; uint64_t n; unsigned D; int is_box; unsigned F; const float *nb;
+  const float *queries; uint64_t q0, q1; uint32_t *counts; uint64_t *hashes;
+} mprs_arg;
+
+static void *mprs_worker(void *p) {
+  mprs_arg *a = (mprs_arg *)p;
+  orc_search_mprs(a->sobjs, a->n, a->D, a->is_box, a->F, a->nb, a->queries + a->q0 * 2 * a->D,
+                  a->q1 - a->q0, a->counts + a->q0, a->hashes + a->q0, NULL, NULL, NULL);
+  return NULL;
+}
+
+/* The serial MPRS restatement run over static query chunks, one query at a
+ * time per thread (the CPU counterpart of the reference's plain GPU form). */
+void orc_search_mprs_mt(const float *sobjs, uint64_t n, unsigned D, int is_box, unsigned F,
+                        const float *node_boxes, const float *queries, uint64_t nq,
+                        unsigned nthreads, uint32_t *counts, uint64_t *hashes) {
+  if (nthreads < 1) nthreads = 1;
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+  mprs_arg *args = (mprs_arg *)malloc(sizeof(mprs_arg) * nthreads);
+  for (unsigned t = 0; t < nthreads; ++t) {
+    mprs_arg a = {sobjs, n, D, is_box, F, node_boxes, queries, nq * t / nthreads,
+                  nq * (t + 1) / nthreads, counts, hashes};
+    args[t] = a;
+    pthread_create(&th[t], NULL, mprs_worker, &args[t]);
+  }
+  for (unsigned t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+  free(th);
+  free(args);
 }

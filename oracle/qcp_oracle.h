@@ -112,6 +112,19 @@ void     orc_search_mprs(const float *sobjs, uint64_t n, unsigned D, int is_box,
                          uint32_t *counts, uint64_t *hashes,
                          const uint64_t *offsets, uint32_t *ids,
                          uint32_t *visits);
+/* The build over nthreads POSIX threads (the reference's threaded CPU build:
+ * key mapping chunks, a parallel sort, gather, bottom-up boxes), for the CPU
+ * baseline.  Outputs equal orc_keys + orc_sort_stable + orc_pack: keys[n] are
+ * the sorted keys, perm[n] the input index of each rank, sobjs[n*E] the
+ * rank-ordered objects, node_boxes the packed tree's MBRs.  ms[4] = wall ms of
+ * keys, sort, gather, pack.  Returns nonzero on a domain error. */
+int      orc_build_mt(const float *objs, uint64_t n, unsigned D, int is_box, int key_mode,
+                      int key_src, unsigned F, unsigned nthreads, uint64_t *keys,
+                      uint32_t *perm, float *sobjs, float *node_boxes, double *ms);
+/* orc_search_mprs over nthreads POSIX threads, static query chunks. */
+void     orc_search_mprs_mt(const float *sobjs, uint64_t n, unsigned D, int is_box, unsigned F,
+                            const float *node_boxes, const float *queries, uint64_t nq,
+                            unsigned nthreads, uint32_t *counts, uint64_t *hashes);
 /* Closed-form brute force: every object against every query. */
 void     orc_search_brute(const float *sobjs, uint64_t n, unsigned D, int is_box,
                           const float *queries, uint64_t nq,

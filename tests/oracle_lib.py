@@ -73,6 +73,9 @@ def _declare(L):
     L.orc_search_dfs_mt.argtypes = [P, UNS, INT, I64, P, P, P, P, P, U64, UNS, P, P]
     L.orc_search_mprs.argtypes = [P, U64, UNS, INT, UNS, P, P, U64, P, P, P, P, P]
     L.orc_search_brute.argtypes = [P, U64, UNS, INT, P, U64, P, P, P, P]
+    L.orc_build_mt.argtypes = [P, U64, UNS, INT, INT, INT, UNS, UNS, P, P, P, P, P]
+    L.orc_build_mt.restype = INT
+    L.orc_search_mprs_mt.argtypes = [P, U64, UNS, INT, UNS, P, P, U64, UNS, P, P]
 
 
 def ptr(a):
@@ -177,6 +180,16 @@ class Tree:
         lib().orc_search_dfs_mt(*self._args(), ptr(q), nq, nthreads, ptr(counts), ptr(hashes))
         return counts, hashes
 
+    def search_mprs_mt(self, queries, nthreads):
+        assert self.layout == "packed"
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        counts = np.zeros(nq, dtype=np.uint32)
+        hashes = np.zeros(nq, dtype=np.uint64)
+        lib().orc_search_mprs_mt(ptr(self.sobjs), self.n, self.D, self.is_box, self.F,
+                                 ptr(self.boxes), ptr(q), nq, nthreads, ptr(counts), ptr(hashes))
+        return counts, hashes
+
     def search_mprs(self, queries, want_ids=False):
         assert self.layout == "packed"
         q = np.ascontiguousarray(queries, dtype=np.float32)
@@ -225,6 +238,23 @@ def id_hash(ids):
     x = ids.astype(np.uint64)
     with np.errstate(over="ignore"):
         return x * x + np.uint64(0x9E3779B97F4A7C15) * x
+
+
+def build_mt(objs, D, is_box, F, key_mode, key_src=SRC_LO, nthreads=8):
+    """The threaded CPU build (keys, parallel stable sort, gather, packed boxes);
+    returns (keys, perm, sobjs, boxes, phase ms, domain error)."""
+    objs = np.ascontiguousarray(objs, dtype=np.float32)
+    n = objs.shape[0]
+    E = 2 * D if is_box else D
+    keys = np.zeros(max(n, 1), np.uint64)
+    perm = np.zeros(max(n, 1), np.uint32)
+    sobjs = np.zeros((max(n, 1), E), np.float32)
+    m = sum(level_counts(n, F)) if n else 0
+    boxes = np.zeros((max(m, 1), 2 * D), np.float32)
+    ms = np.zeros(4, np.float64)
+    bad = lib().orc_build_mt(ptr(objs), n, D, int(is_box), key_mode, key_src, F, nthreads,
+                             ptr(keys), ptr(perm), ptr(sobjs), ptr(boxes), ptr(ms))
+    return keys[:n], perm[:n], sobjs[:n], boxes[:m], ms, bad
 
 
 def build_packed(objs, D, is_box, F, key_mode, key_src=SRC_LO, layout="packed"):

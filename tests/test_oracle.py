@@ -336,3 +336,27 @@ def test_dfs_mt_matches_serial(O):
     a = t.search_dfs(q)
     c, h = t.search_dfs_mt(q, 5)
     assert np.array_equal(a["counts"], c) and np.array_equal(a["hashes"], h)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("D,is_box", [(2, False), (3, False), (2, True)])
+def test_threaded_cpu_build_equals_serial(O, threads, D, is_box):
+    """The threaded CPU build (the CPU baseline) gives the serial oracle's keys,
+    permutation, rank-ordered objects and node boxes, duplicates included."""
+    objs = rand_boxes(20011, D, 3 + D, mean_side=0.01) if is_box else rand_points(20011, D, 3 + D, dup_frac=0.2)
+    k, perm, sobjs, boxes, ms, bad = O.build_mt(objs, D, is_box, 64, O.KEY_REF, nthreads=threads)
+    t = O.build_packed(objs, D, is_box, 64, O.KEY_REF)
+    assert bad == 0
+    assert np.array_equal(k, t.keys) and np.array_equal(perm, t.perm)
+    assert np.array_equal(sobjs.view(np.uint32), t.sobjs.view(np.uint32))
+    assert np.array_equal(boxes.view(np.uint32), t.boxes.view(np.uint32))
+    assert (ms >= 0).all()
+
+
+def test_threaded_mprs_equals_serial(O):
+    pts = rand_points(30000, 3, 21)
+    q = rand_queries(300, 3, 22, pts)
+    t = O.build_packed(pts, 3, False, 32, O.KEY_REF)
+    ref = t.search_mprs(q)
+    c, h = t.search_mprs_mt(q, 5)
+    assert np.array_equal(c, ref["counts"]) and np.array_equal(h, ref["hashes"])
