@@ -48,7 +48,7 @@ enum { QCP_SRC_LO = 0, QCP_SRC_CENTER = 1 };    /* key from lower corner or box 
 enum { QCP_OBJ_POINTS = 0, QCP_OBJ_BOXES = 1 };
 enum { QCP_BUILD_GRAPH = 1 };                   /* qcp_build_cfg.flags                  */
 
-enum { QCP_SEARCH_PERSISTENT = 0, QCP_SEARCH_PLAIN = 1 };
+enum { QCP_SEARCH_PERSISTENT = 0, QCP_SEARCH_PLAIN = 1, QCP_SEARCH_HYBRID = 2 };
 enum { QCP_OUT_COUNTS = 1, QCP_OUT_HASH = 2, QCP_OUT_IDS = 4, QCP_OUT_VISITS = 8 };
 
 typedef struct {
@@ -96,6 +96,11 @@ typedef struct {
                               its rank range, hash in closed form, IDs written as a run).
                               Same results; the stats then count the pruned traversal.  Not
                               applied to box indexes holding a box with lo > hi. */
+  uint32_t cpu_threads;    /* hybrid form: host threads walking the upper tree (0 = all cores) */
+  uint32_t handoff_level;  /* hybrid form: tree level whose overlapping nodes the host hands
+                              to the GPU (1 = the root's children; 0 or beyond = the leaves,
+                              the reference's CPU-internal / GPU-leaf split)              */
+  uint32_t hybrid_batch;   /* hybrid form: queries per pipeline block (0 = 16384)          */
 } qcp_search_cfg;
 
 typedef struct {
@@ -186,9 +191,12 @@ void qcp_batch_destroy(qcp_batch *b);
  * cfg->mode = QCP_SEARCH_PLAIN: one workgroup per query, plain_batch queries per
  * launch, a synchronous count copy after each launch) and, new here, the
  * persistent work-stealing form (QCP_SEARCH_PERSISTENT: all queries in one
- * launch).  With QCP_OUT_IDS the sorted rank-ID lists are produced by a count
- * pass, a device prefix sum and an ordered write pass.  cfg may be NULL
- * (persistent, counts+hash, tuned). */
+ * launch).  QCP_SEARCH_HYBRID is the CPU-GPU co-processing search of
+ * Hybrid::Search (src/tree/hybrid.cpp:451-969): host threads walk the upper
+ * levels and hand the overlapping nodes of cfg->handoff_level to the GPU, in
+ * double-buffered blocks of queries (counts and hash only).  With QCP_OUT_IDS
+ * the sorted rank-ID lists are produced by a count pass, a device prefix sum
+ * and an ordered write pass.  cfg may be NULL (persistent, counts+hash, tuned). */
 qcp_status qcp_batch_run(qcp_batch *b, const qcp_search_cfg *cfg, qcp_search_stats *stats);
 
 /* Results of the last run (host copies).  counts[q], hashes[q],

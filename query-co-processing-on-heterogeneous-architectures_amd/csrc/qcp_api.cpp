@@ -180,6 +180,8 @@ static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** 
     for (int l = 0; l < ix->h; ++l) ix->level_count.push_back(ix->level_off[l + 1] - ix->level_off[l]);
     h_first.swap(t.first);
     h_count.swap(t.count);
+    ix->h_first = h_first;
+    ix->h_count = h_count;
   }
   ix->obj_stride = ((n + kObjSlack + 63) / 64) * 64;
   ix->node_stride = ((ix->n_nodes + kNodeSlack + 63) / 64) * 64;
@@ -295,6 +297,7 @@ static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned pas
 // Graph-captured build: captured once per input pointer, replayed after.
 static qcp_status index_run_build_graph(qcp_index* ix, const float* d_objs) {
   hipStream_t s = ix->stream;
+  ix->h_nbox.clear();
   if (ix->n == 0) return QCP_OK;
   if (!ix->build_exec || ix->build_exec_objs != d_objs) {
     if (ix->build_exec) {
@@ -333,6 +336,7 @@ static qcp_status index_run_build_graph(qcp_index* ix, const float* d_objs) {
 // The build pipeline on the index's stream.
 static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
   hipStream_t s = ix->stream;
+  ix->h_nbox.clear();
   const uint64_t n = ix->n;
   HIP_TRY(hipEventRecord(ix->ev[0], s));
   uint64_t key_or = 0;
@@ -767,9 +771,15 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   cfg.mode = QCP_SEARCH_PERSISTENT;
   cfg.outputs = QCP_OUT_COUNTS | QCP_OUT_HASH;
   if (cfg_in) cfg = *cfg_in;
-  if (cfg.mode > QCP_SEARCH_PLAIN) return QCP_EINVAL;
+  if (cfg.mode > QCP_SEARCH_HYBRID) return QCP_EINVAL;
   qcp_index* ix = b->ix;
   QCP_TRY(set_device(ix->device));
+  if (cfg.mode == QCP_SEARCH_HYBRID) {
+    b->has_offsets = false;
+    b->last_mode = cfg.mode;
+    b->last_outputs = cfg.outputs;
+    return hybrid_run(b, cfg, stats);
+  }
   hipStream_t s = ix->stream;
   const uint64_t q = b->q;
   qcp_search_stats st;

@@ -62,6 +62,10 @@ struct SearchArgs {
   uint32_t donate_min_height;  // packed: donate subtrees this far above leaf parents
   uint32_t q_begin;          // plain form: first query of the launch
   uint32_t contain;          // report subtrees whose box lies inside the query without a visit
+  // optional root items (query << 32 | subtree root node) instead of one root
+  // item per query; results then accumulate (the caller zeroes them)
+  const unsigned long long* items;
+  uint32_t n_items;
 };
 
 // Order-independent fingerprint of a set of rank IDs: sum of squares plus

@@ -47,6 +47,11 @@ struct qcp_index {
   // every object box has lo <= hi in each dimension, so a node box inside a
   // query implies that every object under it overlaps the query
   bool contain_ok = true;
+  // host copies for the co-processing search: explicit topology (kept from the
+  // build) and the node boxes [node][lo(D), hi(D)] (mirrored on first use,
+  // cleared by every build)
+  std::vector<uint32_t> h_first, h_count;
+  std::vector<float> h_nbox;
   qcp_index_info info{};
 };
 
@@ -71,6 +76,7 @@ struct qcp_batch {
   uint32_t* d_ids = nullptr;
   uint64_t ids_cap = 0;
   uint32_t* d_visits = nullptr;
+  std::vector<float> h_queries;   // host copy of the queries (co-processing search)
   qcp::QueueCtl* d_ctl = nullptr;
   unsigned long long* d_slots = nullptr;
   uint32_t slot_cap = 0;
@@ -93,4 +99,6 @@ SearchArgs index_search_args(const qcp_index* ix, const float* d_queries, uint64
                              uint32_t* d_counts, unsigned long long* d_hashes);
 // Workgroups that fill the device with the persistent kernel of `out` mode.
 unsigned persistent_grid(const qcp_index* ix, int out, unsigned block);
+// the co-processing search (QCP_SEARCH_HYBRID), in hybrid.cpp
+qcp_status hybrid_run(qcp_batch* b, const qcp_search_cfg& cfg, qcp_search_stats* stats);
 }  // namespace qcp

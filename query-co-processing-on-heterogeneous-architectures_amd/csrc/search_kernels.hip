@@ -559,6 +559,10 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
   __syncthreads();
   uint64_t ie = 0, le = 0, hits = 0, cruns = 0;
   const bool donating = OUT != kOutIds && a.donate;
+  // root items: the queries (from the root), or a caller's (query, subtree) list
+  const uint32_t nroot = a.items ? a.n_items : a.nq;
+  // several items of one query (donated subtrees, or an item list) add up
+  const bool accumulate = donating || a.items != nullptr;
   const uint32_t xcd = blockIdx.x % kXcds;
   bool roots_left = true;
   // root items finished by this wave and not yet taken off `pending`: they
@@ -571,15 +575,21 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
     if (roots_left) {
       for (int k = 0; k < kXcds && !got; ++k) {
         const uint32_t x = (xcd + k) % kXcds;
-        const uint32_t lo = (uint32_t)(((uint64_t)a.nq * x) / kXcds);
-        const uint32_t hi = (uint32_t)(((uint64_t)a.nq * (x + 1)) / kXcds);
+        const uint32_t lo = (uint32_t)(((uint64_t)nroot * x) / kXcds);
+        const uint32_t hi = (uint32_t)(((uint64_t)nroot * (x + 1)) / kXcds);
         if (lo >= hi) continue;
         if (k > 0 && ld_acq(&a.ctl->xhead[x]) >= hi - lo) continue;
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(&a.ctl->xhead[x], 1u);
         t = uni(t);
         if (lo + t < hi) {
-          q = a.qperm ? a.qperm[lo + t] : lo + t;
+          if (a.items) {
+            const unsigned long long it = a.items[lo + t];
+            q = uni((uint32_t)(it >> 32));
+            node = uni((uint32_t)it);
+          } else {
+            q = a.qperm ? a.qperm[lo + t] : lo + t;
+          }
           got = true;
         }
       }
@@ -632,7 +642,7 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
       if (OUT == kOutHash)
         hsum = set_hash(c.s1c + wave_sum_u64(c.s1_lane), c.s2c + wave_sum_u64(c.s2_lane));
       if (lane == 0) {
-        if (donating) {
+        if (accumulate) {
           if (c.cnt) atomicAdd(&a.counts[q], c.cnt);
           if (OUT == kOutHash && hsum) atomicAdd(&a.hashes[q], (unsigned long long)hsum);
         } else {
@@ -857,7 +867,7 @@ static hipError_t launch_search_d(const SearchArgs& a, bool expl, int out, int u
 
 hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, int out, int u,
                          unsigned grid, unsigned block, hipStream_t s) {
-  k_reset_ctl<<<1, 64, 0, s>>>(a.ctl, a.nq);
+  k_reset_ctl<<<1, 64, 0, s>>>(a.ctl, a.items ? a.n_items : a.nq);
   if (D == 2) return box ? launch_search_d<2, true>(a, expl, out, u, grid, block, s)
                          : launch_search_d<2, false>(a, expl, out, u, grid, block, s);
   if (D == 3) return box ? launch_search_d<3, true>(a, expl, out, u, grid, block, s)

@@ -20,7 +20,7 @@ LAYOUT = {"mphr": 0, "rtree": 1, "bvh": 2}
 KEY = {"ref": 0, "b16": 1}
 SRC = {"lo": 0, "center": 1}
 OBJ = {"points": 0, "boxes": 1}
-MODE = {"persistent": 0, "plain": 1}
+MODE = {"persistent": 0, "plain": 1, "hybrid": 2}
 OUT_COUNTS, OUT_HASH, OUT_IDS, OUT_VISITS = 1, 2, 4, 8
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP runtime error",
@@ -52,7 +52,7 @@ class SearchCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in
                 ("mode", "outputs", "plain_batch", "block_waves", "grid_blocks", "leaf_chunks",
                  "donate", "order", "max_waiters", "hunger_period", "donate_height",
-                 "graph", "contain")]
+                 "graph", "contain", "cpu_threads", "handoff_level", "hybrid_batch")]
 
 
 class SearchStats(ctypes.Structure):
@@ -292,11 +292,11 @@ class Index:
 
     def search(self, queries, mode="persistent", ids=False, visits=False, plain_batch=128,
                grid_blocks=0, donate=0, chunks=0, hash=True, order=0, block_waves=0,
-               contain=False):
+               contain=False, **hybrid):
         with self.batch(queries) as b:
             st = b.run(mode=mode, ids=ids, plain_batch=plain_batch, grid_blocks=grid_blocks,
                        donate=donate, chunks=chunks, hash=hash, order=order,
-                       block_waves=block_waves, contain=contain)
+                       block_waves=block_waves, contain=contain, **hybrid)
             res = b.results(ids=ids, visits=visits)
             res["stats"] = st
             return res
@@ -332,17 +332,21 @@ class Batch:
 
     def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=0,
             hash=True, chunks=0, order=0, max_waiters=0, hunger_period=0, donate_height=0,
-            graph=False, block_waves=0, contain=False):
+            graph=False, block_waves=0, contain=False, cpu_threads=0, handoff_level=0,
+            hybrid_batch=0):
         """One search of the resident batch.  donate=1 enables subtree donation
         (work stealing) in the persistent form; chunks = 64-entry leaf chunks in
         flight per wave (0 = default); order=0 processes the queries in Hilbert
         order of their centres, one range per XCD, order=1 as given;
         block_waves = waves per workgroup (0 = 4); contain=True reports subtrees
-        whose box lies inside the query without visiting them (same results)."""
+        whose box lies inside the query without visiting them (same results).
+        mode="hybrid": host threads walk the levels above handoff_level (0 = the
+        leaves) and the GPU searches the subtrees they hand over."""
         outputs = OUT_COUNTS | (OUT_HASH if hash else 0) | (OUT_IDS if ids else 0)
         cfg = SearchCfg(MODE[mode], outputs, plain_batch, block_waves, grid_blocks, chunks,
                         donate, order, max_waiters, hunger_period, donate_height,
-                        1 if graph else 0, 1 if contain else 0)
+                        1 if graph else 0, 1 if contain else 0, cpu_threads, handoff_level,
+                        hybrid_batch)
         st = SearchStats()
         check(lib().qcp_batch_run(self._h, ctypes.byref(cfg), ctypes.byref(st)), "qcp_batch_run")
         self._ids = ids
