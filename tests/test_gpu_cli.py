@@ -1,0 +1,67 @@
+"""The command-line driver (reference-flag compatible): generated data and
+queries, every index type, the plain / persistent / co-processing forms, index
+save + load, and the autotuner, all checked against the CPU oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "query-co-processing-on-heterogeneous-architectures_amd", "_build", "qcp_cli")
+
+
+def run(*args):
+    print("cli", " ".join(args), flush=True)
+    r = subprocess.run([CLI] + list(args), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def read_out(path):
+    a = np.loadtxt(path, dtype=np.uint64, ndmin=2)
+    return a[:, 0].astype(np.uint32), a[:, 1]
+
+
+@pytest.fixture(scope="module")
+def files(tmp_path_factory):
+    d = tmp_path_factory.mktemp("cli")
+    data, queries = str(d / "data.bin"), str(d / "queries.bin")
+    run("--gen-data", data, "-d", "120000", "--dims", "3", "--seed", "7")
+    run("--gen-queries", queries, "-q", "1500", "-s", "1e-3", "--dims", "3", "--seed", "7")
+    pts = np.fromfile(data, np.float32).reshape(-1, 3)
+    q = np.fromfile(queries, np.float32).reshape(-1, 6)
+    return d, data, queries, pts, q
+
+
+@pytest.mark.parametrize("itype,layout,extra", [
+    ("m", "packed", []), ("m", "packed", ["--mode", "plain", "-b", "64"]),
+    ("r", "packed", []), ("b", "bvh", []), ("h", "packed", ["-p", "4", "-c", "400"]),
+    ("hb", "bvh", ["-l", "2"]), ("m", "packed", ["--contain"])])
+def test_cli_search_matches_oracle(gpu, O, files, itype, layout, extra):
+    d, data, queries, pts, q = files
+    out = str(d / ("out_%s_%d.txt" % (itype, len(extra))))
+    log = run("-i", itype, "--data", data, "--queries", queries, "--dims", "3", "--fanout", "64",
+              "--out", out, "-r", "2", *extra)
+    assert "Hit :" in log
+    t = O.build_packed(pts, 3, False, 64, O.KEY_REF, layout=layout)
+    exp = t.search_dfs(q)
+    c, h = read_out(out)
+    assert np.array_equal(c, exp["counts"]) and np.array_equal(h, exp["hashes"])
+
+
+def test_cli_index_file_and_autotune(gpu, O, files):
+    d, data, queries, pts, q = files
+    ixf = str(d / "ix.qcp")
+    out1, out2 = str(d / "o1.txt"), str(d / "o2.txt")
+    log = run("-i", "m", "--data", data, "--queries", queries, "--dims", "3", "-e", "1",
+              "--index", ixf, "--out", out1, "--json")
+    assert "autotune:" in log and os.path.exists(ixf)
+    # second run loads the saved index (no --data needed)
+    run("-i", "m", "--queries", queries, "--dims", "3", "--index", ixf, "--out", out2)
+    exp = O.build_packed(pts, 3, False, 64, O.KEY_REF).search_dfs(q)
+    for o in (out1, out2):
+        c, h = read_out(o)
+        assert np.array_equal(c, exp["counts"]) and np.array_equal(h, exp["hashes"])

@@ -120,3 +120,19 @@ def test_workload_shapes_small(qcp):
         assert np.isfinite(w["objs"]).all() and np.isfinite(w["queries"]).all()
         if w["obj_kind"] == "points":
             assert w["objs"].min() >= 0.0 and w["objs"].max() < 1.0
+
+
+def test_cli_generators_match_library(qcp, tmp_path):
+    """qcp_cli --gen-data / --gen-queries write the library's Philox streams
+    (data stream 0; query centres stream 1, boxes of side selectivity^(1/D))."""
+    import subprocess
+    cli = os.path.join(os.path.dirname(qcp.LIB_PATH), "qcp_cli")
+    data, queries = str(tmp_path / "d.bin"), str(tmp_path / "q.bin")
+    subprocess.run([cli, "--gen-data", data, "-d", "1000", "--dims", "2", "--seed", "5"], check=True)
+    subprocess.run([cli, "--gen-queries", queries, "-q", "10", "-s", "0.01", "--dims", "2",
+                    "--seed", "5"], check=True)
+    assert np.array_equal(np.fromfile(data, np.float32), qcp.gen_uniform(2000, 5, 0))
+    q = np.fromfile(queries, np.float32).reshape(10, 4)
+    cen = qcp.gen_uniform(20, 5, 1).reshape(10, 2)
+    h = np.float32(0.5) * np.float32(0.1)
+    assert np.array_equal(q[:, :2], cen - h) and np.array_equal(q[:, 2:], cen + h)
