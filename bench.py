@@ -473,6 +473,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-hybrid", action="store_true")
     ap.add_argument("--no-contain", action="store_true",
                     help="skip the separately reported containment-shortcut run")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
@@ -546,6 +547,22 @@ def main():
         log("contained: %.3e queries/s, %.3f ms kernel, %d runs, leaf entries %d" %
             (contained["queries_per_s"], contained["kernel_ms"], contained["contained_runs"],
              contained["leaf_entries"]))
+    # the CPU-GPU co-processing form: host threads walk the upper levels, the
+    # GPU searches the subtrees of the hand-off level (leaves = the reference's split)
+    hybrid = []
+    if not args.no_hybrid:
+        hthreads = min(os.cpu_count() or 1, 16)
+        for level in (0, 1):
+            b.run(mode="hybrid", handoff_level=level, cpu_threads=hthreads)
+            hst = b.run(mode="hybrid", handoff_level=level, cpu_threads=hthreads)
+            hres = b.results()
+            hybrid.append({"handoff_level": level or "leaves", "queries_per_s": q / (hst["total_ms"] * 1e-3),
+                           "total_ms": hst["total_ms"], "device_ms": hst["kernel_ms"],
+                           "launches": hst["launches"], "cpu_threads": hthreads,
+                           "matches_persistent": bool(np.array_equal(hres["counts"], got["counts"]) and
+                                                      np.array_equal(hres["hashes"], got["hashes"]))})
+            log("hybrid (hand-off %s): %.3e queries/s, %.1f ms" %
+                (level or "leaves", q / (hst["total_ms"] * 1e-3), hst["total_ms"]))
     plain = None
     if not args.no_plain:
         pq = queries[: min(q, 20000)]
@@ -603,6 +620,7 @@ def main():
                      "hbm_GBps": (traffic["bytes_per_launch"] / (savg * 1e-3) / 1e9
                                   if traffic else None)},
         "contained": contained,
+        "hybrid": hybrid,
         "plain": plain,
         "speedup_vs_plain": (value / plain["queries_per_s"]) if plain else None,
         "cpu_baseline": cpu,

@@ -177,6 +177,23 @@ qcp_status qcp_export(const qcp_index *ix, uint64_t *sorted_keys, uint32_t *perm
 qcp_status qcp_save(const qcp_index *ix, const char *path);
 qcp_status qcp_load(const char *path, uint32_t device, qcp_index **out);
 
+/* Build from objects already in rank order (entry i gets rank ID i+1): no keys
+ * are computed and nothing is sorted; node boxes as in qcp_build.  For data
+ * whose order was fixed elsewhere, such as the ranks of a reference dump. */
+qcp_status qcp_build_ranked(const qcp_build_cfg *cfg, const float *objs, uint64_t n,
+                            qcp_index **out);
+
+/* Read a file written by the reference's MPHR::DumpToFile (src/tree/mphr.cpp:
+ * 167-194) and recover its objects in the reference's rank order (its leaf
+ * entries' indexes), as points (lo corner, D floats each) or boxes (2D).
+ * dims and leaf_degree are the reference build's GetNumberOfDims() and
+ * GetNumberOfLeafNodeDegrees() (3 and 192 as shipped).  objs == NULL returns
+ * the object count only; QCP_EFORMAT if the size, a node's count or the ranks
+ * do not form a valid dump.  Host only; follow with qcp_build_ranked. */
+qcp_status qcp_read_reference_mphr(const char *path, uint32_t dims, uint32_t leaf_degree,
+                                   uint32_t obj_kind, float *objs, uint64_t cap,
+                                   uint64_t *n_out);
+
 /* Upload a query set to HBM.  Replaces io::DataSet::GetDeviceQuery
  * (src/io/dataset.cpp:82-88); queries are Q x 2D floats [lo(D), hi(D)]. */
 qcp_status qcp_batch_create(qcp_index *ix, const float *boxes, uint64_t q,

@@ -150,3 +150,71 @@ qcp_status qcp_load(const char* path, uint32_t device, qcp_index** out) {
 }
 
 }  // extern "C"
+
+// ---- the reference's MPHR dump ------------------------------------------------
+// MPHR::DumpToFile (src/tree/mphr.cpp:167-194) writes u32 partitions, i64 root
+// offset per partition, u32 node count, then the Node_SOA array as raw memory
+// (src/node/node_soa.h:40-55): points[2*D*F] floats (dimension-major, lo
+// dimensions then hi), index[F] and child_offset[F] as i64, then node_type,
+// level (i32) and branch_count (u32), padded to 8 bytes; D and F are the
+// reference's compile-time GetNumberOfDims() and GetNumberOfLeafNodeDegrees()
+// (src/common/config.h:4-8).  Leaf nodes (node_type 1) hold the entries, and
+// an entry's index is its 1-based Hilbert rank (src/sort/thrust_sorter.cpp:
+// 39-47) in every partition, so the objects are recovered in rank order.
+extern "C" qcp_status qcp_read_reference_mphr(const char* path, uint32_t dims,
+                                              uint32_t leaf_degree, uint32_t obj_kind,
+                                              float* objs, uint64_t cap, uint64_t* n_out) {
+  if (!path || !n_out || (dims != 2 && dims != 3) || leaf_degree == 0 ||
+      leaf_degree > (1u << 16) || obj_kind > QCP_OBJ_BOXES)
+    return QCP_EINVAL;
+  const uint32_t D = dims, F = leaf_degree;
+  const uint32_t E = obj_kind == QCP_OBJ_BOXES ? 2 * D : D;
+  const uint64_t raw = (uint64_t)2 * D * F * 4 + 16ull * F + 12;
+  const uint64_t node_bytes = (raw + 7) / 8 * 8;
+  FILE* f = fopen(path, "rb");
+  if (!f) return QCP_EIO;
+  FileCloser fc{f};
+  if (fseek(f, 0, SEEK_END) != 0) return QCP_EIO;
+  const long fsize = ftell(f);
+  if (fsize < 8 || fseek(f, 0, SEEK_SET) != 0) return QCP_EFORMAT;
+  uint32_t parts = 0, count = 0;
+  if (fread(&parts, 4, 1, f) != 1 || parts == 0 || parts > (1u << 16)) return QCP_EFORMAT;
+  if (fseek(f, 8l * parts, SEEK_CUR) != 0 || fread(&count, 4, 1, f) != 1) return QCP_EFORMAT;
+  if ((uint64_t)fsize != 8ull + 8ull * parts + node_bytes * count) return QCP_EFORMAT;
+  std::vector<uint8_t> node(node_bytes);
+  const float* pts = reinterpret_cast<const float*>(node.data());
+  // first pass: the number of leaf entries (= N)
+  const long body = ftell(f);
+  uint64_t n = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (fread(node.data(), 1, node_bytes, f) != node_bytes) return QCP_EFORMAT;
+    int32_t type;
+    uint32_t bc;
+    memcpy(&type, node.data() + raw - 12, 4);
+    memcpy(&bc, node.data() + raw - 4, 4);
+    if (bc > F) return QCP_EFORMAT;
+    if (type == 1) n += bc;
+  }
+  *n_out = n;
+  if (!objs) return QCP_OK;
+  if (cap < n) return QCP_EBUSY;
+  // second pass: every entry to its rank, each rank exactly once
+  std::vector<uint8_t> seen(n, 0);
+  if (fseek(f, body, SEEK_SET) != 0) return QCP_EIO;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (fread(node.data(), 1, node_bytes, f) != node_bytes) return QCP_EFORMAT;
+    int32_t type;
+    uint32_t bc;
+    memcpy(&type, node.data() + raw - 12, 4);
+    memcpy(&bc, node.data() + raw - 4, 4);
+    if (type != 1) continue;
+    const int64_t* idx = reinterpret_cast<const int64_t*>(node.data() + (uint64_t)2 * D * F * 4);
+    for (uint32_t k = 0; k < bc; ++k) {
+      const int64_t r = idx[k];
+      if (r < 1 || (uint64_t)r > n || seen[r - 1]) return QCP_EFORMAT;
+      seen[r - 1] = 1;
+      for (uint32_t e = 0; e < E; ++e) objs[(uint64_t)(r - 1) * E + e] = pts[(uint64_t)e * F + k];
+    }
+  }
+  return QCP_OK;
+}

@@ -250,6 +250,28 @@ static void set_info_shape(qcp_index* ix) {
   in.tree_bytes = (uint64_t)ix->E * 4 * ix->n + (uint64_t)2 * ix->D * 4 * ix->n_nodes;
 }
 
+// Node boxes of the rank-ordered objects in d_obj: bottom-up over the packed
+// levels, or level by level of an explicit tree.
+static qcp_status enqueue_boxes(qcp_index* ix) {
+  hipStream_t s = ix->stream;
+  if (!ix->expl) {
+    const int h = ix->h;
+    HIP_TRY(launch_leaf_boxes(ix->d_obj, ix->obj_stride, ix->n, ix->D, ix->box, ix->F,
+                              ix->level_count[h - 1], ix->d_nbox, ix->node_stride,
+                              ix->level_off[h - 1], s));
+    for (int l = h - 2; l >= 0; --l)
+      HIP_TRY(launch_level_boxes(ix->d_nbox, ix->node_stride, ix->D, ix->level_off[l + 1],
+                                 ix->level_count[l + 1], ix->F, ix->level_off[l],
+                                 ix->level_count[l], s));
+  } else {
+    for (int l = ix->h - 1; l >= 0; --l)
+      HIP_TRY(launch_tree_boxes(ix->d_obj, ix->obj_stride, ix->d_nfirst, ix->d_ncount, ix->D,
+                                ix->box, ix->F, ix->level_off[l], ix->level_off[l + 1],
+                                ix->d_nbox, ix->node_stride, s));
+  }
+  return QCP_OK;
+}
+
 // Sort passes of a graph-captured build: every byte a key can hold (an even
 // count for every mode, so the sorted keys land back in d_keys / d_perm).
 static unsigned graph_sort_passes(const qcp_index* ix) { return (ix->D * ix->nb + 7) / 8; }
@@ -276,22 +298,7 @@ static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned pas
   }
   if (kin != ix->d_keys) return QCP_EINVAL;  // odd pass counts are not captured
   HIP_TRY(launch_gather(d_objs, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s));
-  if (!ix->expl) {
-    const int h = ix->h;
-    HIP_TRY(launch_leaf_boxes(ix->d_obj, ix->obj_stride, n, ix->D, ix->box, ix->F,
-                              ix->level_count[h - 1], ix->d_nbox, ix->node_stride,
-                              ix->level_off[h - 1], s));
-    for (int l = h - 2; l >= 0; --l)
-      HIP_TRY(launch_level_boxes(ix->d_nbox, ix->node_stride, ix->D, ix->level_off[l + 1],
-                                 ix->level_count[l + 1], ix->F, ix->level_off[l],
-                                 ix->level_count[l], s));
-  } else {
-    for (int l = ix->h - 1; l >= 0; --l)
-      HIP_TRY(launch_tree_boxes(ix->d_obj, ix->obj_stride, ix->d_nfirst, ix->d_ncount, ix->D,
-                                ix->box, ix->F, ix->level_off[l], ix->level_off[l + 1],
-                                ix->d_nbox, ix->node_stride, s));
-  }
-  return QCP_OK;
+  return enqueue_boxes(ix);
 }
 
 // Graph-captured build: captured once per input pointer, replayed after.
@@ -373,21 +380,7 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
     HIP_TRY(hipEventRecord(ix->ev[2], s));
     HIP_TRY(launch_gather(d_objs, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s));
     HIP_TRY(hipEventRecord(ix->ev[3], s));
-    if (!ix->expl) {
-      const int h = ix->h;
-      HIP_TRY(launch_leaf_boxes(ix->d_obj, ix->obj_stride, n, ix->D, ix->box, ix->F,
-                                ix->level_count[h - 1], ix->d_nbox, ix->node_stride,
-                                ix->level_off[h - 1], s));
-      for (int l = h - 2; l >= 0; --l)
-        HIP_TRY(launch_level_boxes(ix->d_nbox, ix->node_stride, ix->D, ix->level_off[l + 1],
-                                   ix->level_count[l + 1], ix->F, ix->level_off[l],
-                                   ix->level_count[l], s));
-    } else {
-      for (int l = ix->h - 1; l >= 0; --l)
-        HIP_TRY(launch_tree_boxes(ix->d_obj, ix->obj_stride, ix->d_nfirst, ix->d_ncount, ix->D,
-                                  ix->box, ix->F, ix->level_off[l], ix->level_off[l + 1],
-                                  ix->d_nbox, ix->node_stride, s));
-    }
+    QCP_TRY(enqueue_boxes(ix));
   } else {
     HIP_TRY(hipEventRecord(ix->ev[1], s));
     HIP_TRY(hipEventRecord(ix->ev[2], s));
@@ -443,6 +436,54 @@ qcp_status qcp_build_device(const qcp_build_cfg* cfg, const float* d_objs, uint6
     index_free(ix);
     return s;
   }
+  *out = ix;
+  return QCP_OK;
+}
+
+qcp_status qcp_build_ranked(const qcp_build_cfg* cfg, const float* objs, uint64_t n,
+                            qcp_index** out) {
+  if (!out || (n && !objs)) return QCP_EINVAL;
+  *out = nullptr;
+  qcp_index* ix = nullptr;
+  QCP_TRY(index_alloc(cfg, n, &ix));
+  auto fail = [&](qcp_status st) {
+    index_free(ix);
+    return st;
+  };
+  hipStream_t s = ix->stream;
+  for (uint64_t i = 0; i < (uint64_t)ix->E * n; ++i)
+    if (objs[i] != objs[i]) return fail(QCP_EDOMAIN);  // NaN objects are rejected
+  HIP_TRY(hipEventRecord(ix->ev[0], s));
+  if (n) {
+    // the given order is the rank order: keys and permutation are the identity
+    std::vector<uint64_t> keys(n);
+    std::vector<uint32_t> perm(n);
+    for (uint64_t i = 0; i < n; ++i) {
+      keys[i] = i;
+      perm[i] = (uint32_t)i;
+    }
+    qcp_status st = dalloc(&ix->d_in, (uint64_t)ix->E * n);
+    if (st != QCP_OK) return fail(st);
+    if (hipMemcpyAsync(ix->d_in, objs, (uint64_t)ix->E * 4 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(ix->d_keys, keys.data(), 8 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(ix->d_perm, perm.data(), 4 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        launch_gather(ix->d_in, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s) != hipSuccess)
+      return fail(QCP_EHIP);
+    if ((st = enqueue_boxes(ix)) != QCP_OK) return fail(st);
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(QCP_EHIP);
+    dfree(ix->d_in);
+  }
+  if (hipEventRecord(ix->ev[4], s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return fail(QCP_EHIP);
+  float tot = 0;
+  hipEventElapsedTime(&tot, ix->ev[0], ix->ev[4]);
+  set_info_shape(ix);
+  ix->info.build_ms = tot;
+  ix->info.build_phase_ms[5] = tot;
+  if (ix->box)
+    for (uint64_t r = 0; r < n && ix->contain_ok; ++r)
+      for (uint32_t d = 0; d < ix->D; ++d)
+        if (objs[r * ix->E + d] > objs[r * ix->E + ix->D + d]) ix->contain_ok = false;
   *out = ix;
   return QCP_OK;
 }

@@ -28,6 +28,8 @@
 //   --key ref|b16 --mode persistent|plain|hybrid --index FILE (load/save)
 //   --out FILE (per-query count and hash, text) --contain --json
 //   --gen-data FILE --gen-queries FILE --seed S
+//   --ref-dump FILE [--ref-degree F]: index the objects of a reference MPHR
+//             dump (MPHR::DumpToFile) in its rank order, F = its leaf degree
 #include <getopt.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -55,6 +57,8 @@ struct Opts {
   bool boxes = false, center = false, b16 = false, contain = false, json = false;
   std::string mode;
   uint64_t seed = 1234;
+  std::string ref_dump;
+  uint32_t ref_degree = 192;
 };
 
 uint64_t parse_count(const char* s) {
@@ -118,6 +122,7 @@ int main(int argc, char** argv) {
       {"contain", no_argument, nullptr, 1010},        {"json", no_argument, nullptr, 1011},
       {"gen-data", required_argument, nullptr, 1012}, {"gen-queries", required_argument, nullptr, 1013},
       {"seed", required_argument, nullptr, 1014},     {"help", no_argument, nullptr, 'h'},
+      {"ref-dump", required_argument, nullptr, 1015}, {"ref-degree", required_argument, nullptr, 1016},
       {nullptr, 0, nullptr, 0}};
   // the reference's option string: every letter in both cases
   static const char* shortopts = "c:C:i:I:d:D:q:Q:b:B:p:P:s:S:l:L:r:R:e:E:t:T:y:Y:u:U:fFh";
@@ -152,6 +157,8 @@ int main(int argc, char** argv) {
         case 1012: o.gen_data = optarg; break;
         case 1013: o.gen_queries = optarg; break;
         case 1014: o.seed = std::stoull(optarg); break;
+        case 1015: o.ref_dump = optarg; break;
+        case 1016: o.ref_degree = (uint32_t)atoi(optarg); break;
         default: usage(); return 2;
       }
     }
@@ -218,6 +225,19 @@ int main(int argc, char** argv) {
   const bool have_index = !o.index_file.empty() && stat(o.index_file.c_str(), &sb) == 0;
   if (have_index && !o.force) {
     if ((st = qcp_load(o.index_file.c_str(), 0, &ix)) != QCP_OK) return fail("load", st);
+  } else if (!o.ref_dump.empty()) {
+    uint64_t n = 0;
+    const uint32_t kind = o.boxes ? (uint32_t)QCP_OBJ_BOXES : (uint32_t)QCP_OBJ_POINTS;
+    if ((st = qcp_read_reference_mphr(o.ref_dump.c_str(), D, o.ref_degree, kind, nullptr, 0, &n)) != QCP_OK)
+      return fail("reference dump", st);
+    std::vector<float> objs((size_t)n * E);
+    if ((st = qcp_read_reference_mphr(o.ref_dump.c_str(), D, o.ref_degree, kind, objs.data(), n,
+                                      &n)) != QCP_OK)
+      return fail("reference dump", st);
+    qcp_build_cfg cfg = {D, o.fanout, layout, QCP_KEY_REF, QCP_SRC_LO, kind, 0, 0};
+    if ((st = qcp_build_ranked(&cfg, objs.data(), n, &ix)) != QCP_OK) return fail("build", st);
+    if (!o.index_file.empty() && (st = qcp_save(ix, o.index_file.c_str())) != QCP_OK)
+      return fail("save", st);
   } else {
     if (o.data.empty()) { usage(); return 2; }
     if (!o.n) o.n = file_floats(o.data) / E;

@@ -106,6 +106,10 @@ def lib():
         L.qcp_device_count.restype = ctypes.c_int
         L.qcp_build.argtypes = [ctypes.POINTER(BuildCfg), P, ctypes.c_uint64, ctypes.POINTER(P)]
         L.qcp_build_device.argtypes = [ctypes.POINTER(BuildCfg), P, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.qcp_build_ranked.argtypes = [ctypes.POINTER(BuildCfg), P, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.qcp_read_reference_mphr.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32,
+                                              ctypes.c_uint32, P, ctypes.c_uint64,
+                                              ctypes.POINTER(ctypes.c_uint64)]
         L.qcp_rebuild_device.argtypes = [P, P]
         L.qcp_index_destroy.argtypes = [P]
         L.qcp_index_destroy.restype = None
@@ -160,6 +164,18 @@ def check(st, what):
 
 def device_count():
     return lib().qcp_device_count()
+
+
+def read_reference_mphr(path, dims=3, leaf_degree=192, obj_kind="points"):
+    """Objects of a reference MPHR dump in its rank order (host only)."""
+    n = ctypes.c_uint64()
+    check(lib().qcp_read_reference_mphr(path.encode(), dims, leaf_degree, OBJ[obj_kind], None, 0,
+                                        ctypes.byref(n)), "qcp_read_reference_mphr")
+    E = dims * (2 if obj_kind == "boxes" else 1)
+    out = np.zeros((max(n.value, 1), E), np.float32)
+    check(lib().qcp_read_reference_mphr(path.encode(), dims, leaf_degree, OBJ[obj_kind], _p(out),
+                                        n.value, ctypes.byref(n)), "qcp_read_reference_mphr")
+    return out[: n.value]
 
 
 # ---------------------------------------------------------------------------
@@ -220,6 +236,18 @@ class Index:
         h = P()
         check(lib().qcp_build(ctypes.byref(cfg), _p(objs) if n else None, n, ctypes.byref(h)),
               "qcp_build")
+        return cls(h, dims, obj_kind)
+
+    @classmethod
+    def build_ranked(cls, objs, dims, fanout=64, layout="mphr", obj_kind="points", device=0):
+        """Build from objects already in rank order (entry i gets rank ID i+1)."""
+        objs = np.ascontiguousarray(objs, dtype=np.float32)
+        E = dims * (2 if obj_kind == "boxes" else 1)
+        n = objs.size // E if objs.size else 0
+        cfg = BuildCfg(dims, fanout, LAYOUT[layout], 0, 0, OBJ[obj_kind], device, 0)
+        h = P()
+        check(lib().qcp_build_ranked(ctypes.byref(cfg), _p(objs) if n else None, n,
+                                     ctypes.byref(h)), "qcp_build_ranked")
         return cls(h, dims, obj_kind)
 
     @classmethod

@@ -65,3 +65,16 @@ def test_cli_index_file_and_autotune(gpu, O, files):
     for o in (out1, out2):
         c, h = read_out(o)
         assert np.array_equal(c, exp["counts"]) and np.array_equal(h, exp["hashes"])
+
+
+def test_cli_reference_dump(gpu, O, files):
+    """--ref-dump indexes a reference MPHR dump's objects in its rank order."""
+    import refdump
+    d, data, queries, pts, q = files
+    dump, out = str(d / "ref.idx"), str(d / "oref.txt")
+    refdump.write(dump, pts, 3, F=192, partitions=3)
+    run("-i", "m", "--ref-dump", dump, "--queries", queries, "--dims", "3", "--fanout", "192",
+        "--out", out)
+    exp = O.Tree(pts, 3, False, 192).search_dfs(q)
+    c, h = read_out(out)
+    assert np.array_equal(c, exp["counts"]) and np.array_equal(h, exp["hashes"])

@@ -136,3 +136,27 @@ def test_cli_generators_match_library(qcp, tmp_path):
     cen = qcp.gen_uniform(20, 5, 1).reshape(10, 2)
     h = np.float32(0.5) * np.float32(0.1)
     assert np.array_equal(q[:, :2], cen - h) and np.array_equal(q[:, 2:], cen + h)
+
+
+@pytest.mark.parametrize("D,parts,boxes", [(3, 1, False), (2, 3, False), (2, 2, True)])
+def test_read_reference_mphr_dump(qcp, tmp_path, D, parts, boxes):
+    """Objects come back in the dump's rank order from every partition."""
+    import refdump
+    from helpers import rand_boxes, rand_points
+    objs = rand_boxes(5000, D, 3, mean_side=0.01) if boxes else rand_points(5000, D, 3)
+    p = str(tmp_path / "mphr.idx")
+    refdump.write(p, objs, D, partitions=parts, boxes=boxes)
+    got = qcp.read_reference_mphr(p, D, 192, "boxes" if boxes else "points")
+    assert got.shape == objs.shape
+    assert np.array_equal(got.view(np.uint32), objs.view(np.uint32))
+
+
+@pytest.mark.parametrize("bad", ["duplicate", "count", "truncated"])
+def test_read_reference_mphr_rejects_malformed(qcp, tmp_path, bad):
+    import refdump
+    from helpers import rand_points
+    p = str(tmp_path / "bad.idx")
+    refdump.write(p, rand_points(1000, 3, 4), 3, corrupt=bad)
+    with pytest.raises(qcp.QcpError) as e:
+        qcp.read_reference_mphr(p, 3, 192)
+    assert e.value.status == -5
