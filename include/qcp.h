@@ -216,6 +216,16 @@ void qcp_batch_destroy(qcp_batch *b);
  * and an ordered write pass.  cfg may be NULL (persistent, counts+hash, tuned). */
 qcp_status qcp_batch_run(qcp_batch *b, const qcp_search_cfg *cfg, qcp_search_stats *stats);
 
+/* The hipGraph-captured build+query pipeline: one graph holding the rebuild of
+ * ix from device objects d_objs (same count and configuration as its build)
+ * followed by the persistent search of batch b (counts + hash), captured on
+ * the first call for a given d_objs and search configuration and replayed
+ * after.  New (the reference builds and searches in separate host-driven
+ * steps, src/evaluator/evaluator.cpp:121-280).  QCP_EDOMAIN after the run if
+ * the new objects are outside the quantiser's domain (results then invalid). */
+qcp_status qcp_pipeline_run(qcp_index *ix, const float *d_objs, qcp_batch *b,
+                            const qcp_search_cfg *cfg, qcp_search_stats *stats);
+
 /* Results of the last run (host copies).  counts[q], hashes[q],
  * offsets[q+1] (exclusive prefix of counts), ids[offsets[Q]], visits[2q]
  * (root visits, node visits: plain form only).  Each pointer may be NULL. */

@@ -925,15 +925,18 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   if (use_graph) {
     const uint32_t key[12] = {1u, (uint32_t)out_mode, (uint32_t)u, grid, donate ? 1u : 0u,
                               cfg.order, a.max_waiters, a.hunger_period, a.donate_min_height,
-                              block, a.contain, 0u};
-    if (!b->exec || memcmp(key, b->exec_key, sizeof(key)) != 0) {
+                              block, a.contain, b->pipe_objs ? 1u : 0u};
+    if (!b->exec || memcmp(key, b->exec_key, sizeof(key)) != 0 || b->exec_objs != b->pipe_objs) {
       if (b->exec) {
         hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
       }
       hipGraph_t g = nullptr;
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      qcp_status cs = enqueue(b->slot_cap, false);
+      qcp_status cs = QCP_OK;
+      // build+query pipeline: the index's rebuild from device objects first
+      if (b->pipe_objs && ix->n) cs = enqueue_build(ix, b->pipe_objs, graph_sort_passes(ix));
+      if (cs == QCP_OK) cs = enqueue(b->slot_cap, false);
       hipError_t e = hipStreamEndCapture(s, &g);
       if (cs != QCP_OK) {
         if (g) hipGraphDestroy(g);
@@ -944,6 +947,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
       hipGraphDestroy(g);
       HIP_TRY(e);
       memcpy(b->exec_key, key, sizeof(key));
+      b->exec_objs = b->pipe_objs;
     }
     HIP_TRY(hipEventRecord(b->ev[0], s));
     HIP_TRY(hipGraphLaunch(b->exec, s));
@@ -1005,6 +1009,28 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   if (donate) b->slots_dirty = std::min<uint64_t>(b->slot_cap, std::max<uint64_t>(ctl.tail, 1));
   if (stats) *stats = st;
   return QCP_OK;
+}
+
+qcp_status qcp_pipeline_run(qcp_index* ix, const float* d_objs, qcp_batch* b,
+                            const qcp_search_cfg* cfg_in, qcp_search_stats* stats) {
+  if (!ix || !b || b->ix != ix || (ix->n && !d_objs)) return QCP_EINVAL;
+  qcp_search_cfg cfg;
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.outputs = QCP_OUT_COUNTS | QCP_OUT_HASH;
+  if (cfg_in) cfg = *cfg_in;
+  if (cfg.mode != QCP_SEARCH_PERSISTENT || (cfg.outputs & QCP_OUT_IDS)) return QCP_EINVAL;
+  if (cfg.contain && ix->box) return QCP_EINVAL;  // box validity is known only after the build
+  QCP_TRY(set_device(ix->device));
+  cfg.graph = 1;
+  b->pipe_objs = d_objs;
+  qcp_status st = qcp_batch_run(b, &cfg, stats);
+  b->pipe_objs = nullptr;
+  if (st != QCP_OK) return st;
+  uint32_t bad = 0;
+  HIP_TRY(hipMemcpy(&bad, ix->d_bad, 4, hipMemcpyDeviceToHost));
+  ix->h_nbox.clear();
+  ix->contain_ok = !(bad & kBadInverted);
+  return (bad & kBadDomain) ? QCP_EDOMAIN : QCP_OK;
 }
 
 uint64_t qcp_batch_total_hits(const qcp_batch* b) { return b ? b->total_hits : 0; }

@@ -86,6 +86,8 @@ struct qcp_batch {
   hipEvent_t ev[6] = {};
   hipGraphExec_t exec = nullptr;            // captured search (qcp_search_cfg.graph)
   uint32_t exec_key[12] = {};               // configuration baked into exec
+  const float* exec_objs = nullptr;         // rebuild input baked into exec (pipeline)
+  const float* pipe_objs = nullptr;         // set while qcp_pipeline_run captures / runs
   uint32_t last_outputs = 0;
   uint32_t last_mode = 0;
   uint64_t total_hits = 0;

@@ -122,6 +122,8 @@ def lib():
         L.qcp_batch_destroy.argtypes = [P]
         L.qcp_batch_destroy.restype = None
         L.qcp_batch_run.argtypes = [P, ctypes.POINTER(SearchCfg), ctypes.POINTER(SearchStats)]
+        L.qcp_pipeline_run.argtypes = [P, P, P, ctypes.POINTER(SearchCfg),
+                                       ctypes.POINTER(SearchStats)]
         L.qcp_batch_results.argtypes = [P, P, P, P, P, ctypes.c_uint64, P]
         L.qcp_batch_total_hits.argtypes = [P]
         L.qcp_batch_total_hits.restype = ctypes.c_uint64
@@ -379,6 +381,18 @@ class Batch:
         check(lib().qcp_batch_run(self._h, ctypes.byref(cfg), ctypes.byref(st)), "qcp_batch_run")
         self._ids = ids
         self._mode = mode
+        return st.as_dict()
+
+    def pipeline(self, buf, block_waves=0, chunks=0, donate=0, order=0, contain=False):
+        """Rebuild the index from a DeviceBuffer and search this batch as one
+        captured hipGraph (replayed while buf and the configuration stay)."""
+        cfg = SearchCfg(MODE["persistent"], OUT_COUNTS | OUT_HASH, 0, block_waves, 0, chunks,
+                        donate, order, 0, 0, 0, 1, 1 if contain else 0)
+        st = SearchStats()
+        check(lib().qcp_pipeline_run(self.index._h, buf.ptr, self._h, ctypes.byref(cfg),
+                                     ctypes.byref(st)), "qcp_pipeline_run")
+        self._ids = False
+        self._mode = "persistent"
         return st.as_dict()
 
     def results(self, ids=False, visits=False):

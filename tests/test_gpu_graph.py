@@ -55,3 +55,28 @@ def test_graph_search_replays(gpu, O, donate):
             b.run(donate=1 - donate, chunks=8)
             assert_results_equal(b.results(), exp, "eager between replays %d" % rep, ids=False)
             ix.search(q[:100], ids=True)
+
+
+@pytest.mark.parametrize("layout,donate", [("mphr", 0), ("bvh", 1)])
+def test_graph_build_query_pipeline(gpu, O, layout, donate):
+    """Rebuild + search captured as one hipGraph: each replay (objects changed
+    in place between replays) equals the eager build and search of those objects."""
+    a_objs, b_objs = rand_points(80000, 3, 21), rand_points(80000, 3, 22)
+    q = rand_queries(9000, 3, 23, a_objs)
+    exps = [O.build_packed(o, 3, False, 64, O.KEY_REF,
+                           layout="bvh" if layout == "bvh" else "packed").search_dfs(q)
+            for o in (a_objs, b_objs)]
+    buf = gpu.DeviceBuffer(a_objs)
+    ix = gpu.Index.build(buf, 3, fanout=64, layout=layout)
+    with ix.batch(q) as b:
+        for rep in range(5):
+            host = a_objs if rep % 2 == 0 else b_objs
+            buf.write(host)
+            st = b.pipeline(buf, donate=donate)
+            r = b.results()
+            assert_results_equal(r, exps[rep % 2], "pipeline replay %d" % rep, ids=False)
+            assert st["total_hits"] == int(exps[rep % 2]["counts"].sum())
+            # eager work between replays, and the rebuilt index seen from outside
+            assert_bits_equal(ix.search(q[:200])["counts"], exps[rep % 2]["counts"][:200],
+                              "eager search after replay %d" % rep)
+    buf.close()
