@@ -580,6 +580,20 @@ def main():
         log("cpu baseline: %.3e queries/s (%s); MPRS %.3e queries/s" %
             (cpu["value"], cpu["sample"], cpu["mprs"]["value"]))
     build = build_figures(qcp, ix, pts, D, threads, not args.no_cpu)
+    # the build+query pipeline as one captured hipGraph (rebuild, then search)
+    pbuf = qcp.DeviceBuffer(pts)
+    b.pipeline(pbuf, **kw)
+    pms = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        pst = b.pipeline(pbuf, **kw)
+        pms.append((time.perf_counter() - t0) * 1e3)
+    pres = b.results()
+    pbuf.close()
+    build["pipeline_graph"] = {"wall_ms": min(pms), "device_ms": pst["kernel_ms"],
+                               "matches": bool(np.array_equal(pres["counts"], got["counts"]) and
+                                               np.array_equal(pres["hashes"], got["hashes"]))}
+    log("build+query graph: %.2f ms wall, %.2f ms device" % (min(pms), pst["kernel_ms"]))
     log("build: GPU %.2f ms (%.0f Mkeys/s)%s" % (
         build["gpu_ms"], build["gpu_mkeys_per_s"],
         ", CPU %.1f Mkeys/s" % build["cpu"]["value"] if "cpu" in build else ""))

@@ -47,6 +47,7 @@ enum { QCP_KEY_REF = 0, QCP_KEY_B16 = 1 };      /* (ll)(1e6f*p) @ 31|20 bits, or
 enum { QCP_SRC_LO = 0, QCP_SRC_CENTER = 1 };    /* key from lower corner or box centre  */
 enum { QCP_OBJ_POINTS = 0, QCP_OBJ_BOXES = 1 };
 enum { QCP_BUILD_GRAPH = 1 };                   /* qcp_build_cfg.flags                  */
+enum { QCP_ORDER_HILBERT = 0, QCP_ORDER_KMEANS_HILBERT = 1 };
 
 enum { QCP_SEARCH_PERSISTENT = 0, QCP_SEARCH_PLAIN = 1, QCP_SEARCH_HYBRID = 2 };
 enum { QCP_OUT_COUNTS = 1, QCP_OUT_HASH = 2, QCP_OUT_IDS = 4, QCP_OUT_VISITS = 8 };
@@ -60,6 +61,12 @@ typedef struct {
   uint32_t obj_kind;  /* QCP_OBJ_*: points are N x D floats, boxes N x 2D [lo, hi] */
   uint32_t device;    /* HIP device ordinal                                         */
   uint32_t flags;     /* QCP_BUILD_GRAPH: run the build as a captured hipGraph      */
+  uint32_t order;     /* QCP_ORDER_*: Hilbert order of the objects' keys, or k-means
+                         clusters ranked by their centres' keys, then Hilbert order
+                         within a cluster (KmeansMapper, src/mapper/kmeans_mapper.cpp:
+                         201-259; not with QCP_BUILD_GRAPH)                        */
+  uint32_t clusters;  /* k-means order: clusters (0 = 64, at most 4096)             */
+  uint32_t kmeans_iters; /* k-means order: Lloyd iterations at most (0 = 20)         */
 } qcp_build_cfg;
 
 typedef struct {

@@ -935,3 +935,97 @@ void orc_search_mprs_mt(const float *sobjs, uint64_t n, unsigned D, int is_box, 
   free(th);
   free(args);
 }
+
+/* ---- k-means + Hilbert order --------------------------------------------- */
+/* KmeansMapper::ClusteringBranches / cuda_kmeans (src/mapper/kmeans_mapper.cpp:
+ * 15-259) with the defined variant of this build: K seeds given as object
+ * indexes; each iteration assigns every key point (lower corner, or box centre
+ * (lo+hi)*0.5f) to the nearest centre by FP32 squared distance summed over
+ * d = 0..D-1, ties to the lower cluster, stops when no assignment changed
+ * (after the first), else moves each non-empty centre to the mean of its key
+ * points: FP64 sums in index order within chunks of 65536 objects, chunk sums
+ * added in chunk order, rounded to FP32.  Clusters are ranked by (Hilbert key
+ * of the centre, cluster number) and the sort key is
+ * rank << (key bits - s) | key >> s, s = max(0, key bits + kbits - 64),
+ * kbits = ceil(log2 K).  keys[n] receives the sort keys, cid[n] the clusters. */
+void orc_kmeans_keys(const float *objs, uint64_t n, unsigned D, int is_box, int key_mode,
+                     int key_src, unsigned K, unsigned iters, const uint32_t *init_idx,
+                     uint64_t *keys, uint32_t *cid) {
+  unsigned const E = is_box ? 2 * D : D;
+  unsigned const nb = key_mode == ORC_KEY_REF ? orc_ref_bits(D) : 16u;
+  const uint64_t CH = 65536;
+  float *pts = (float *)malloc(sizeof(float) * D * (n ? n : 1));
+  float *cen = (float *)malloc(sizeof(float) * D * (K ? K : 1));
+  orc_keys(objs, n, D, is_box, key_mode, key_src, keys);
+  for (uint64_t i = 0; i < n; ++i)
+    for (unsigned d = 0; d < D; ++d) {
+      float p = objs[i * E + d];
+      if (is_box && key_src == ORC_SRC_CENTER) p = (p + objs[i * E + D + d]) * 0.5f;
+      pts[i * D + d] = p;
+    }
+  for (unsigned k = 0; k < K; ++k)
+    for (unsigned d = 0; d < D; ++d) cen[k * D + d] = pts[(uint64_t)init_idx[k] * D + d];
+  for (uint64_t i = 0; i < n; ++i) cid[i] = 0xFFFFFFFFu;
+  for (unsigned it = 0; it < iters; ++it) {
+    uint64_t changes = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      float best = 0.0f;
+      uint32_t bk = 0;
+      for (unsigned k = 0; k < K; ++k) {
+        float dd = 0.0f;
+        for (unsigned d = 0; d < D; ++d) {
+          float const t = pts[i * D + d] - cen[k * D + d];
+          dd = dd + t * t;
+        }
+        if (k == 0 || dd < best) { best = dd; bk = k; }
+      }
+      if (cid[i] != bk) { cid[i] = bk; ++changes; }
+    }
+    if (it > 0 && changes == 0) break;
+    uint64_t nch = (n + CH - 1) / CH;
+    for (unsigned k = 0; k < K; ++k) {
+      double sum[3] = {0, 0, 0};
+      uint64_t total = 0;
+      for (uint64_t c = 0; c < nch; ++c) {
+        double part[3] = {0, 0, 0};
+        uint64_t e = (c + 1) * CH < n ? (c + 1) * CH : n;
+        for (uint64_t i = c * CH; i < e; ++i) {
+          if (cid[i] != k) continue;
+          ++total;
+          for (unsigned d = 0; d < D; ++d) part[d] += (double)pts[i * D + d];
+        }
+        for (unsigned d = 0; d < D; ++d) sum[d] += part[d];
+      }
+      if (total)
+        for (unsigned d = 0; d < D; ++d) cen[k * D + d] = (float)(sum[d] / (double)total);
+    }
+  }
+  /* cluster ranks */
+  uint64_t *ck = (uint64_t *)malloc(sizeof(uint64_t) * (K ? K : 1));
+  uint32_t *ord = (uint32_t *)malloc(sizeof(uint32_t) * (K ? K : 1));
+  uint32_t *rank = (uint32_t *)malloc(sizeof(uint32_t) * (K ? K : 1));
+  for (unsigned k = 0; k < K; ++k) {
+    uint64_t c[4] = {0, 0, 0, 0};
+    for (unsigned d = 0; d < D; ++d) {
+      float const p = cen[k * D + d];
+      if (key_mode == ORC_KEY_REF) {
+        int64_t q = (p >= 0.0f) ? orc_quant_ref(p) : -1;
+        c[d] = (q < 0 || q >= ((int64_t)1 << nb)) ? 0 : (uint64_t)q;
+      } else {
+        c[d] = orc_quant_b16(p);
+      }
+    }
+    ck[k] = orc_hilbert_c2i(D, nb, c);
+  }
+  orc_sort_stable(ck, K, ord); /* ties by cluster number */
+  for (unsigned r = 0; r < K; ++r) rank[ord[r]] = r;
+  unsigned kbits = 0;
+  while (((uint64_t)1 << kbits) < K) ++kbits;
+  unsigned const keybits = D * nb;
+  unsigned const s = keybits + kbits > 64 ? keybits + kbits - 64 : 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    uint64_t hi = kbits ? ((uint64_t)rank[cid[i]] << (keybits - s)) : 0;
+    keys[i] = hi | (keys[i] >> s);
+  }
+  free(ck); free(ord); free(rank); free(pts); free(cen);
+}

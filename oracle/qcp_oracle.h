@@ -125,6 +125,13 @@ int      orc_build_mt(const float *objs, uint64_t n, unsigned D, int is_box, int
 void     orc_search_mprs_mt(const float *sobjs, uint64_t n, unsigned D, int is_box, unsigned F,
                             const float *node_boxes, const float *queries, uint64_t nq,
                             unsigned nthreads, uint32_t *counts, uint64_t *hashes);
+/* k-means + Hilbert sort keys (this build's defined variant of the reference's
+ * KmeansMapper, src/mapper/kmeans_mapper.cpp:15-259): K clusters seeded at
+ * init_idx[K], at most iters Lloyd iterations; keys[n] = composite sort keys,
+ * cid[n] = final clusters.  Follow with orc_sort_stable. */
+void     orc_kmeans_keys(const float *objs, uint64_t n, unsigned D, int is_box, int key_mode,
+                         int key_src, unsigned K, unsigned iters, const uint32_t *init_idx,
+                         uint64_t *keys, uint32_t *cid);
 /* Closed-form brute force: every object against every query. */
 void     orc_search_brute(const float *sobjs, uint64_t n, unsigned D, int is_box,
                           const float *queries, uint64_t nq,

@@ -451,6 +451,90 @@ hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, un
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// k-means + Hilbert order (the assignment step of KmeansMapper::cuda_kmeans,
+// find_nearest_cluster, src/mapper/kmeans_mapper.cpp:100-126,280-370): each
+// object's key point goes to the nearest centre by FP32 squared distance
+// summed over d = 0..D-1 (no contraction), ties to the lower cluster.
+// Centres sit in LDS; `changes` counts objects whose cluster changed.
+template <int D, bool BOX, bool CENTER>
+__global__ __launch_bounds__(256) void k_assign(const float* __restrict__ objs, uint64_t n,
+                                                const float* __restrict__ centers, uint32_t K,
+                                                uint32_t* __restrict__ cid, uint32_t* changes) {
+  constexpr int E = BOX ? 2 * D : D;
+  extern __shared__ float qcp_centres[];
+  for (uint32_t t = threadIdx.x; t < K * D; t += blockDim.x) qcp_centres[t] = centers[t];
+  __syncthreads();
+  uint32_t changed = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    float p[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      p[d] = objs[i * E + d];
+      if (CENTER && BOX) p[d] = (p[d] + objs[i * E + D + d]) * 0.5f;
+    }
+    float best = 0.0f;
+    uint32_t bk = 0;
+    for (uint32_t k = 0; k < K; ++k) {
+      float dd = 0.0f;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const float t = p[d] - qcp_centres[k * D + d];
+        dd = dd + t * t;
+      }
+      if (k == 0 || dd < best) {
+        best = dd;
+        bk = k;
+      }
+    }
+    if (cid[i] != bk) {
+      cid[i] = bk;
+      ++changed;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) changed += __shfl_xor(changed, off, kWave);
+  if (lane_id() == 0 && changed) atomicAdd(changes, changed);
+}
+
+// keys[i] = cluster rank << shift_rank | keys[i] >> shift_key
+__global__ void k_compose(uint64_t* keys, const uint32_t* __restrict__ cid,
+                          const uint32_t* __restrict__ rank, uint64_t n, unsigned shift_key,
+                          unsigned shift_rank) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t hi = shift_rank < 64 ? ((uint64_t)rank[cid[i]] << shift_rank) : 0ull;
+    keys[i] = hi | (keys[i] >> shift_key);
+  }
+}
+
+hipError_t launch_assign(const float* objs, uint64_t n, unsigned D, bool box, bool center,
+                         const float* centers, uint32_t K, uint32_t* cid, uint32_t* changes,
+                         hipStream_t s) {
+  const unsigned g = grid_for(n, 256, 8192);
+  const size_t lds = (size_t)K * D * sizeof(float);
+#define QCP_ASSIGN(DD, BB, CC) \
+  k_assign<DD, BB, CC><<<g, 256, lds, s>>>(objs, n, centers, K, cid, changes)
+  if (D == 2) {
+    if (box) { if (center) QCP_ASSIGN(2, true, true); else QCP_ASSIGN(2, true, false); }
+    else QCP_ASSIGN(2, false, false);
+  } else if (D == 3) {
+    if (box) { if (center) QCP_ASSIGN(3, true, true); else QCP_ASSIGN(3, true, false); }
+    else QCP_ASSIGN(3, false, false);
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef QCP_ASSIGN
+  return hipGetLastError();
+}
+
+hipError_t launch_compose(uint64_t* keys, const uint32_t* cid, const uint32_t* rank, uint64_t n,
+                          unsigned shift_key, unsigned shift_rank, hipStream_t s) {
+  k_compose<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, cid, rank, n, shift_key, shift_rank);
+  return hipGetLastError();
+}
+
 #define QCP_MW_SWITCH(MWV, ...)                        \
   switch (MWV) {                                       \
     case 1: { constexpr int MW = 1; __VA_ARGS__; } break; \

@@ -117,6 +117,8 @@ static qcp_status validate_cfg(const qcp_build_cfg* cfg) {
   if (cfg->key_mode > QCP_KEY_B16 || cfg->key_src > QCP_SRC_CENTER) return QCP_EINVAL;
   if (cfg->obj_kind > QCP_OBJ_BOXES) return QCP_EINVAL;
   if (cfg->key_src == QCP_SRC_CENTER && cfg->obj_kind != QCP_OBJ_BOXES) return QCP_EINVAL;
+  if (cfg->order > QCP_ORDER_KMEANS_HILBERT || cfg->clusters > 4096) return QCP_EINVAL;
+  if (cfg->order == QCP_ORDER_KMEANS_HILBERT && (cfg->flags & QCP_BUILD_GRAPH)) return QCP_EINVAL;
   return QCP_OK;
 }
 
@@ -361,7 +363,12 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
     if (bad & kBadDomain) return QCP_EDOMAIN;
     ix->contain_ok = !(bad & kBadInverted);
     // LSD radix sort over the significant bytes of the keys
-    const unsigned bits = key_or ? 64 - __builtin_clzll(key_or) : 0;
+    unsigned bits = key_or ? 64 - __builtin_clzll(key_or) : 0;
+    if (ix->cfg.order == QCP_ORDER_KMEANS_HILBERT) {
+      uint64_t kb = 0;
+      QCP_TRY(kmeans_keys(ix, d_objs, &kb));
+      bits = (unsigned)kb;
+    }
     const unsigned passes = (bits + 7) / 8;
     uint64_t* kin = ix->d_keys;
     uint32_t* vin = ix->d_perm;

@@ -20,9 +20,12 @@
 //   -r R      repeats
 //   -e E      evaluation mode: nonzero runs the autotuner (the reference's sweep)
 //   -s S      selectivity of --gen-queries (volume fraction)
-//   -t T, -y Y, -u U, -f   accepted; partitions, data type and cluster type
-//             have no counterpart (one tree per index, data from --data,
-//             Hilbert order); -f rebuilds even when --index exists
+//   -u U      cluster type (GetClusterType, 431-446): h|hilbert (default),
+//             k|kmeans (k-means clusters ranked by their centres' Hilbert keys,
+//             --clusters K), o|original (the input order is the rank order)
+//   -t T, -y Y, -f   accepted; partitions and data type have no counterpart
+//             (one tree per index, data from --data); -f rebuilds even when
+//             --index exists
 // Options of this driver:
 //   --data FILE --queries FILE --dims D --fanout F --boxes --center
 //   --key ref|b16 --mode persistent|plain|hybrid --index FILE (load/save)
@@ -59,6 +62,8 @@ struct Opts {
   uint64_t seed = 1234;
   std::string ref_dump;
   uint32_t ref_degree = 192;
+  std::string cluster = "h";
+  uint32_t clusters = 0;
 };
 
 uint64_t parse_count(const char* s) {
@@ -123,6 +128,7 @@ int main(int argc, char** argv) {
       {"gen-data", required_argument, nullptr, 1012}, {"gen-queries", required_argument, nullptr, 1013},
       {"seed", required_argument, nullptr, 1014},     {"help", no_argument, nullptr, 'h'},
       {"ref-dump", required_argument, nullptr, 1015}, {"ref-degree", required_argument, nullptr, 1016},
+      {"clusters", required_argument, nullptr, 1017},
       {nullptr, 0, nullptr, 0}};
   // the reference's option string: every letter in both cases
   static const char* shortopts = "c:C:i:I:d:D:q:Q:b:B:p:P:s:S:l:L:r:R:e:E:t:T:y:Y:u:U:fFh";
@@ -140,7 +146,8 @@ int main(int argc, char** argv) {
         case 'l': case 'L': o.scan_level = (uint32_t)atoi(optarg); break;
         case 'r': case 'R': o.repeat = (uint32_t)atoi(optarg); break;
         case 'e': case 'E': o.eval = (uint32_t)atoi(optarg); break;
-        case 't': case 'T': case 'y': case 'Y': case 'u': case 'U': break;
+        case 'u': case 'U': o.cluster = optarg; break;
+        case 't': case 'T': case 'y': case 'Y': break;
         case 'f': case 'F': o.force = true; break;
         case 1000: o.data = optarg; break;
         case 1001: o.queries = optarg; break;
@@ -159,6 +166,7 @@ int main(int argc, char** argv) {
         case 1014: o.seed = std::stoull(optarg); break;
         case 1015: o.ref_dump = optarg; break;
         case 1016: o.ref_degree = (uint32_t)atoi(optarg); break;
+        case 1017: o.clusters = (uint32_t)atoi(optarg); break;
         default: usage(); return 2;
       }
     }
@@ -245,7 +253,13 @@ int main(int argc, char** argv) {
     if (!read_floats(o.data, o.n * E, objs)) return fail("read data", QCP_EIO);
     qcp_build_cfg cfg = {D, o.fanout, layout, o.b16 ? (uint32_t)QCP_KEY_B16 : (uint32_t)QCP_KEY_REF,
                          o.center ? (uint32_t)QCP_SRC_CENTER : (uint32_t)QCP_SRC_LO,
-                         o.boxes ? (uint32_t)QCP_OBJ_BOXES : (uint32_t)QCP_OBJ_POINTS, 0, 0};
+                         o.boxes ? (uint32_t)QCP_OBJ_BOXES : (uint32_t)QCP_OBJ_POINTS, 0, 0,
+                         QCP_ORDER_HILBERT, o.clusters, 0};
+    std::string ct = o.cluster;
+    for (auto& ch : ct) ch = (char)tolower(ch);
+    const bool original = ct == "o" || ct == "original";
+    if (ct == "k" || ct == "kmeans") cfg.order = QCP_ORDER_KMEANS_HILBERT;
+    else if (!original && ct != "h" && ct != "hilbert") { usage(); return 2; }
     qcp_search_cfg tuned;
     memset(&tuned, 0, sizeof(tuned));
     if (o.eval) {
@@ -273,7 +287,8 @@ int main(int argc, char** argv) {
       printf("autotune: fanout %u, %u waves per workgroup, %u leaf chunks, donation %s "
              "(%u configurations)\n", inf.fanout, tuned.block_waves, tuned.leaf_chunks,
              tuned.donate ? "on" : "off", tl);
-    } else if ((st = qcp_build(&cfg, objs.data(), o.n, &ix)) != QCP_OK) {
+    } else if ((st = original ? qcp_build_ranked(&cfg, objs.data(), o.n, &ix)
+                              : qcp_build(&cfg, objs.data(), o.n, &ix)) != QCP_OK) {
       return fail("build", st);
     }
     if (!o.index_file.empty() && (st = qcp_save(ix, o.index_file.c_str())) != QCP_OK)

@@ -16,6 +16,11 @@ hipError_t launch_keys(const float* objs, uint64_t n, unsigned D, bool box, bool
                        bool center, unsigned nb, uint64_t* keys, uint32_t* idx,
                        unsigned long long* key_or, uint32_t* bad, hipStream_t s);
 uint32_t sort_tiles(uint64_t n);
+hipError_t launch_assign(const float* objs, uint64_t n, unsigned D, bool box, bool center,
+                         const float* centers, uint32_t K, uint32_t* cid, uint32_t* changes,
+                         hipStream_t s);
+hipError_t launch_compose(uint64_t* keys, const uint32_t* cid, const uint32_t* rank, uint64_t n,
+                          unsigned shift_key, unsigned shift_rank, hipStream_t s);
 hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
                             uint32_t* vout, uint64_t n, unsigned shift, uint32_t* hist,
                             uint32_t* hist_scan, uint32_t* block_sums, uint32_t max_blocks,

@@ -36,7 +36,10 @@ class QcpError(RuntimeError):
 
 class BuildCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in
-                ("dims", "fanout", "layout", "key_mode", "key_src", "obj_kind", "device", "flags")]
+                ("dims", "fanout", "layout", "key_mode", "key_src", "obj_kind", "device", "flags",
+                 "order", "clusters", "kmeans_iters")]
+
+ORDER = {"hilbert": 0, "kmeans": 1}
 
 
 class IndexInfo(ctypes.Structure):
@@ -224,9 +227,11 @@ class Index:
 
     @classmethod
     def build(cls, objs, dims, fanout=64, layout="mphr", key_mode="ref", key_src="lo",
-              obj_kind="points", device=0, graph=False):
+              obj_kind="points", device=0, graph=False, order="hilbert", clusters=0,
+              kmeans_iters=0):
         """Build from host objects (N x D points or N x 2D boxes).  With
-        graph=True the device pipeline runs as a captured hipGraph."""
+        graph=True the device pipeline runs as a captured hipGraph; order="kmeans"
+        ranks k-means clusters by their centres' Hilbert keys first."""
         if isinstance(objs, DeviceBuffer):
             return cls.build_device(objs, dims, fanout, layout, key_mode, key_src, obj_kind,
                                     device, graph)
@@ -234,7 +239,8 @@ class Index:
         E = dims * (2 if obj_kind == "boxes" else 1)
         n = objs.size // E if objs.size else 0
         cfg = BuildCfg(dims, fanout, LAYOUT[layout], KEY[key_mode], SRC[key_src],
-                       OBJ[obj_kind], device, 1 if graph else 0)
+                       OBJ[obj_kind], device, 1 if graph else 0, ORDER[order], clusters,
+                       kmeans_iters)
         h = P()
         check(lib().qcp_build(ctypes.byref(cfg), _p(objs) if n else None, n, ctypes.byref(h)),
               "qcp_build")

@@ -76,6 +76,7 @@ def _declare(L):
     L.orc_build_mt.argtypes = [P, U64, UNS, INT, INT, INT, UNS, UNS, P, P, P, P, P]
     L.orc_build_mt.restype = INT
     L.orc_search_mprs_mt.argtypes = [P, U64, UNS, INT, UNS, P, P, U64, UNS, P, P]
+    L.orc_kmeans_keys.argtypes = [P, U64, UNS, INT, INT, INT, UNS, UNS, P, P, P]
 
 
 def ptr(a):
@@ -255,6 +256,45 @@ def build_mt(objs, D, is_box, F, key_mode, key_src=SRC_LO, nthreads=8):
     bad = lib().orc_build_mt(ptr(objs), n, D, int(is_box), key_mode, key_src, F, nthreads,
                              ptr(keys), ptr(perm), ptr(sobjs), ptr(boxes), ptr(ms))
     return keys[:n], perm[:n], sobjs[:n], boxes[:m], ms, bad
+
+
+def kmeans_seeds(words, n, K):
+    """Seed objects of the k-means order: (w * n) >> 32 over the Philox words in
+    order, repeats skipped."""
+    out, seen = [], set()
+    for w in words:
+        i = (int(w) * n) >> 32
+        if i in seen:
+            continue
+        seen.add(i)
+        out.append(i)
+        if len(out) == K:
+            return np.array(out, np.uint32)
+    raise ValueError("not enough words")
+
+
+def kmeans_keys(objs, D, is_box, key_mode, K, iters, init_idx, key_src=SRC_LO):
+    objs = np.ascontiguousarray(objs, dtype=np.float32)
+    n = objs.shape[0]
+    keys = np.zeros(max(n, 1), np.uint64)
+    cid = np.zeros(max(n, 1), np.uint32)
+    init = np.ascontiguousarray(init_idx, np.uint32)
+    lib().orc_kmeans_keys(ptr(objs), n, D, int(is_box), key_mode, key_src, K, iters, ptr(init),
+                          ptr(keys), ptr(cid))
+    return keys[:n], cid[:n]
+
+
+def build_kmeans(objs, D, is_box, F, key_mode, K, iters, init_idx, key_src=SRC_LO,
+                 layout="packed"):
+    """Oracle build in the k-means + Hilbert order."""
+    objs = np.ascontiguousarray(objs, dtype=np.float32)
+    k, cid = kmeans_keys(objs, D, is_box, key_mode, K, iters, init_idx, key_src)
+    perm = sort_stable(k)
+    t = Tree(objs[perm], D, is_box, F, layout=layout)
+    t.keys = k[perm]
+    t.perm = perm
+    t.cid = cid
+    return t
 
 
 def build_packed(objs, D, is_box, F, key_mode, key_src=SRC_LO, layout="packed"):

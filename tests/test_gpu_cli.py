@@ -78,3 +78,16 @@ def test_cli_reference_dump(gpu, O, files):
     exp = O.Tree(pts, 3, False, 192).search_dfs(q)
     c, h = read_out(out)
     assert np.array_equal(c, exp["counts"]) and np.array_equal(h, exp["hashes"])
+
+
+@pytest.mark.parametrize("u", ["k", "o"])
+def test_cli_cluster_types(gpu, O, files, u):
+    """-u k (k-means + Hilbert order) and -u o (input order) give the same
+    search results as any order (counts; rank IDs differ by construction)."""
+    d, data, queries, pts, q = files
+    out = str(d / ("ou_%s.txt" % u))
+    run("-i", "m", "-u", u, "--clusters", "16", "--data", data, "--queries", queries, "--dims", "3",
+        "--out", out)
+    exp = O.build_packed(pts, 3, False, 64, O.KEY_REF).search_dfs(q)
+    c, _ = read_out(out)
+    assert np.array_equal(c, exp["counts"])

@@ -263,3 +263,25 @@ def test_c1_full_size(gpu, O):
         assert_results_equal(got, exp, "c1 " + mode)
     pl = ix.search(queries, mode="plain")
     assert_bits_equal(pl["counts"], exp["counts"], "c1 plain counts")
+
+
+@pytest.mark.parametrize("D,K,layout,key", [(2, 16, "mphr", "ref"), (3, 64, "mphr", "b16"),
+                                            (3, 7, "bvh", "ref")])
+def test_kmeans_hilbert_order(gpu, O, D, K, layout, key):
+    """k-means + Hilbert order: GPU assignment with host centre updates gives
+    the oracle's clusters, composite keys, ranks, tree and search results."""
+    pts = rand_points(70000, D, 80 + K)
+    n = len(pts)
+    init = O.kmeans_seeds(gpu.gen_words(4096, 1234, 5), n, K)
+    km = O.KEY_REF if key == "ref" else O.KEY_B16
+    t = O.build_kmeans(pts, D, False, 64, km, K, 20, init,
+                       layout="bvh" if layout == "bvh" else "packed")
+    ix = gpu.Index.build(pts, D, fanout=64, layout=layout, key_mode=key, order="kmeans",
+                         clusters=K)
+    ex = ix.export()
+    assert_bits_equal(ex["keys"], t.keys, "k-means keys")
+    assert_bits_equal(ex["perm"], t.perm, "k-means permutation")
+    assert_bits_equal(ex["boxes"], t.boxes[: t.n_nodes], "k-means node boxes")
+    q = rand_queries(300, D, 81, pts)
+    exp = t.search_dfs(q, want_ids=True)
+    assert_results_equal(ix.search(q, ids=True), exp, "k-means order search")

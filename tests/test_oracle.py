@@ -360,3 +360,61 @@ def test_threaded_mprs_equals_serial(O):
     ref = t.search_mprs(q)
     c, h = t.search_mprs_mt(q, 5)
     assert np.array_equal(c, ref["counts"]) and np.array_equal(h, ref["hashes"])
+
+
+def _kmeans_numpy(pts, K, iters, init):
+    """Independent numpy restatement of the k-means rule (FP32 distances summed
+    over d in order, first minimum wins; FP64 means summed sequentially within
+    chunks of 65536, chunks in order)."""
+    n, D = pts.shape
+    cen = pts[init].copy()
+    cid = np.full(n, 0xFFFFFFFF, np.uint64)
+    for it in range(iters):
+        dd = np.zeros((n, K), np.float32)
+        for d in range(D):
+            t = pts[:, d:d + 1] - cen[None, :, d]
+            dd = dd + t * t
+        new = np.argmin(dd, axis=1).astype(np.uint64)
+        changes = int((new != cid).sum())
+        cid = new
+        if it > 0 and changes == 0:
+            break
+        for k in range(K):
+            tot, s = 0, np.zeros(D)
+            for c0 in range(0, n, 65536):
+                sel = pts[c0:c0 + 65536][cid[c0:c0 + 65536] == k].astype(np.float64)
+                tot += len(sel)
+                if len(sel):
+                    s = s + np.cumsum(sel, axis=0)[-1]
+            if tot:
+                cen[k] = (s / tot).astype(np.float32)
+    return cid.astype(np.uint32), cen
+
+
+@pytest.mark.parametrize("D,K", [(2, 5), (3, 8)])
+def test_kmeans_order_against_numpy(O, D, K):
+    pts = rand_points(3000, D, 60 + D)
+    init = np.arange(0, 3000, 3000 // K, dtype=np.uint32)[:K]
+    keys, cid = O.kmeans_keys(pts, D, False, O.KEY_REF, K, 12, init)
+    cid_np, cen = _kmeans_numpy(pts, K, 12, init)
+    assert np.array_equal(cid, cid_np)
+    # composite key: cluster rank (by centre key) above the object's key
+    obj_keys, _ = O.keys(pts, D, False, O.KEY_REF)
+    nb = 31 if D == 2 else 20
+    ck = np.array([O.c2i(D, nb, [int(np.float32(1e6) * np.float32(c)) for c in cen[k]])
+                   for k in range(K)], np.uint64)
+    rank = np.argsort(np.argsort(ck, kind="stable"), kind="stable")
+    kbits = int(np.ceil(np.log2(K)))
+    s = max(0, D * nb + kbits - 64)
+    want = (rank[cid].astype(np.uint64) << np.uint64(D * nb - s)) | (obj_keys >> np.uint64(s))
+    assert np.array_equal(keys, want)
+    # the sorted order groups clusters by rank, Hilbert order inside
+    perm = O.sort_stable(keys)
+    assert (np.diff(rank[cid[perm]].astype(np.int64)) >= 0).all()
+
+
+def test_kmeans_seeds_rule(O, qcp):
+    w = qcp.gen_words(64, 1234, 5)
+    s = O.kmeans_seeds(w, 1000, 10)
+    assert len(set(s.tolist())) == 10
+    assert s[0] == (int(w[0]) * 1000) >> 32
