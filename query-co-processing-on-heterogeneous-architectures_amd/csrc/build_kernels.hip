@@ -91,54 +91,6 @@ __global__ __launch_bounds__(kSortThreads) void k_hist(const uint64_t* __restric
   hist[(uint64_t)tid * ntiles + blockIdx.x] = cnt[tid];
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_scatter(
-    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned shift,
-    const uint32_t* __restrict__ offs, uint32_t ntiles) {
-  constexpr int W = kSortThreads / kWave;
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wcnt[W][256];
-  const int tid = threadIdx.x;
-  const int w = tid / kWave;
-  run[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
-#pragma unroll
-  for (int ww = 0; ww < W; ++ww) wcnt[ww][tid] = 0;
-  __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-  for (int it = 0; it < kSortKPT; ++it) {
-    uint64_t i = base + (uint64_t)it * kSortThreads + tid;
-    bool valid = i < n;
-    uint64_t k = valid ? kin[i] : 0;
-    uint32_t v = valid ? vin[i] : 0;
-    uint32_t dig = (uint32_t)(k >> shift) & 0xFF;
-    // wave-level multisplit: lanes holding the same digit
-    uint64_t peers = ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      uint64_t bb = ballot((dig >> b) & 1u);
-      peers &= ((dig >> b) & 1u) ? bb : ~bb;
-    }
-    uint32_t r = rank_below(peers);
-    if (valid && r == 0) wcnt[w][dig] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = run[dig] + r;
-      for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][dig];
-      kout[pos] = k;
-      vout[pos] = v;
-    }
-    __syncthreads();
-    uint32_t tot = 0;
-#pragma unroll
-    for (int ww = 0; ww < W; ++ww) {
-      tot += wcnt[ww][tid];
-      wcnt[ww][tid] = 0;
-    }
-    run[tid] += tot;
-    __syncthreads();
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Device-wide exclusive scan (three phases).  Each block owns `per_block`
 // consecutive elements, processed 2048 at a time.
@@ -426,6 +378,76 @@ hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
   return scan_impl<uint32_t, unsigned long long>(in, n, out, block_sums, max_blocks, s);
 }
 
+// Stable scatter of one 4096-key tile for one 8-bit digit: the tile is first
+// ranked into LDS in (digit, index) order -- wave multisplit (8 ballots) per
+// 256 keys, buckets placed at the tile's own digit offsets -- and then written
+// out in that order, so the keys of one bucket go to consecutive global
+// positions (runs of ~16 keys) instead of one scattered 8-byte store each.
+__global__ __launch_bounds__(kSortThreads) void k_scatter(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned shift,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ offs, uint32_t ntiles) {
+  constexpr int W = kSortThreads / kWave;
+  __shared__ uint64_t sk[kSortTile];
+  __shared__ uint32_t sv[kSortTile];
+  __shared__ uint32_t gstart[256];  // global position of this tile's bucket
+  __shared__ uint32_t lstart[256];  // the bucket's start inside the tile
+  __shared__ uint32_t lrun[256];    // keys of the bucket placed so far
+  __shared__ uint32_t scan_sh[kScanThreads];
+  __shared__ uint32_t wcnt[W][256];
+  const int tid = threadIdx.x;
+  const int w = tid / kWave;
+  gstart[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
+  uint32_t tot_tile;
+  lstart[tid] = block_excl_scan<uint32_t>(hist[(uint64_t)tid * ntiles + blockIdx.x], scan_sh,
+                                          &tot_tile);
+  lrun[tid] = 0;
+#pragma unroll
+  for (int ww = 0; ww < W; ++ww) wcnt[ww][tid] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+  for (int it = 0; it < kSortKPT; ++it) {
+    const uint64_t i = base + (uint64_t)it * kSortThreads + tid;
+    const bool valid = i < n;
+    const uint64_t k = valid ? kin[i] : 0;
+    const uint32_t v = valid ? vin[i] : 0;
+    const uint32_t dig = (uint32_t)(k >> shift) & 0xFF;
+    // wave-level multisplit: lanes holding the same digit
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bb = ballot((dig >> b) & 1u);
+      peers &= ((dig >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t r = rank_below(peers);
+    if (valid && r == 0) wcnt[w][dig] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = lstart[dig] + lrun[dig] + r;
+      for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][dig];
+      sk[pos] = k;
+      sv[pos] = v;
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+#pragma unroll
+    for (int ww = 0; ww < W; ++ww) {
+      tot += wcnt[ww][tid];
+      wcnt[ww][tid] = 0;
+    }
+    lrun[tid] += tot;
+    __syncthreads();
+  }
+  const uint32_t cnt = (uint32_t)(n - base < (uint64_t)kSortTile ? n - base : kSortTile);
+  for (uint32_t p = tid; p < cnt; p += kSortThreads) {
+    const uint64_t k = sk[p];
+    const uint32_t dig = (uint32_t)(k >> shift) & 0xFF;
+    const uint32_t pos = gstart[dig] + (p - lstart[dig]);
+    kout[pos] = k;
+    vout[pos] = sv[p];
+  }
+}
+
 hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
                             uint32_t* vout, uint64_t n, unsigned shift, uint32_t* hist,
                             uint32_t* hist_scan, uint32_t* block_sums, uint32_t max_blocks,
@@ -434,7 +456,7 @@ hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* 
   k_hist<<<nt, kSortThreads, 0, s>>>(kin, n, shift, hist, nt);
   hipError_t e = scan_u32_u32(hist, (uint64_t)nt * 256, hist_scan, block_sums, max_blocks, s);
   if (e != hipSuccess) return e;
-  k_scatter<<<nt, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, hist_scan, nt);
+  k_scatter<<<nt, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, hist, hist_scan, nt);
   return hipGetLastError();
 }
 
