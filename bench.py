@@ -474,6 +474,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-hybrid", action="store_true")
+    ap.add_argument("--no-ids", action="store_true")
     ap.add_argument("--no-contain", action="store_true",
                     help="skip the separately reported containment-shortcut run")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
@@ -547,6 +548,28 @@ def main():
         log("contained: %.3e queries/s, %.3f ms kernel, %d runs, leaf entries %d" %
             (contained["queries_per_s"], contained["kernel_ms"], contained["contained_runs"],
              contained["leaf_entries"]))
+    # full sorted rank-ID lists (1.7e10 IDs, 68 GB, stay in HBM); the first
+    # queries' lists are checked against the oracle
+    id_runs = []
+    if not args.no_ids:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        t_small = oracle_tree_from(ix)
+        exp_small = t_small.search_dfs(queries[:40], want_ids=True)
+        for contain in (False, True):
+            b.run(ids=True, contain=contain, **kw)
+            t0 = time.perf_counter()
+            ist = b.run(ids=True, contain=contain, **kw)
+            iwall = time.perf_counter() - t0
+            off, idl = b.ids_range(0, 40)
+            ok = bool(np.array_equal(off, exp_small["offsets"]) and
+                      np.array_equal(idl, exp_small["ids"]))
+            id_runs.append({"contain": contain, "queries_per_s": q / iwall, "wall_ms": iwall * 1e3,
+                            "device_ms": ist["kernel_ms"], "ids": int(ist["total_hits"]),
+                            "id_bytes": 4 * int(ist["total_hits"]), "first_40_match_oracle": ok})
+            log("IDs%s: %.3e queries/s, %.1f ms wall, %.1f ms device, %d IDs" %
+                (" (contain)" if contain else "", q / iwall, iwall * 1e3, ist["kernel_ms"],
+                 ist["total_hits"]))
+        b.run(**kw)  # counts + hashes again for the checks below
     # the CPU-GPU co-processing form: host threads walk the upper levels, the
     # GPU searches the subtrees of the hand-off level (leaves = the reference's split)
     hybrid = []
@@ -635,6 +658,7 @@ def main():
                                   if traffic else None)},
         "contained": contained,
         "hybrid": hybrid,
+        "id_lists": id_runs,
         "plain": plain,
         "speedup_vs_plain": (value / plain["queries_per_s"]) if plain else None,
         "cpu_baseline": cpu,

@@ -223,6 +223,13 @@ void qcp_batch_destroy(qcp_batch *b);
  * and an ordered write pass.  cfg may be NULL (persistent, counts+hash, tuned). */
 qcp_status qcp_batch_run(qcp_batch *b, const qcp_search_cfg *cfg, qcp_search_stats *stats);
 
+/* The sorted rank-ID lists of queries [q0, q1) of the last run with
+ * QCP_OUT_IDS: offsets[q1-q0+1] relative to the first (may be NULL), ids
+ * (ids_cap entries; QCP_EBUSY if too small).  For result sets too large to copy
+ * whole (pagination of the reference's per-query results). */
+qcp_status qcp_batch_ids(const qcp_batch *b, uint64_t q0, uint64_t q1, uint64_t *offsets,
+                         uint32_t *ids, uint64_t ids_cap);
+
 /* The hipGraph-captured build+query pipeline: one graph holding the rebuild of
  * ix from device objects d_objs (same count and configuration as its build)
  * followed by the persistent search of batch b (counts + hash), captured on

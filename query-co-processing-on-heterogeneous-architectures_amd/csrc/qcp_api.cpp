@@ -1071,6 +1071,28 @@ qcp_status qcp_batch_results(const qcp_batch* b, uint32_t* counts, uint64_t* has
   return QCP_OK;
 }
 
+qcp_status qcp_batch_ids(const qcp_batch* b, uint64_t q0, uint64_t q1, uint64_t* offsets,
+                         uint32_t* ids, uint64_t ids_cap) {
+  if (!b || q0 > q1 || q1 > b->q) return QCP_EINVAL;
+  if (!b->has_offsets) return QCP_EINVAL;
+  const qcp_index* ix = b->ix;
+  QCP_TRY(set_device(ix->device));
+  hipStream_t s = ix->stream;
+  std::vector<unsigned long long> off(q1 - q0 + 1);
+  HIP_TRY(hipMemcpyAsync(off.data(), b->d_offsets + q0, 8 * off.size(), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const uint64_t total = off.back() - off.front();
+  if (offsets)
+    for (size_t i = 0; i < off.size(); ++i) offsets[i] = off[i] - off.front();
+  if (ids) {
+    if (ids_cap < total) return QCP_EBUSY;
+    if (total)
+      HIP_TRY(hipMemcpyAsync(ids, b->d_ids + off.front(), 4 * total, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  return QCP_OK;
+}
+
 qcp_status qcp_search(qcp_index* ix, const float* boxes, uint64_t q, const qcp_search_cfg* cfg,
                       uint32_t* counts, uint64_t* hashes, uint64_t* offsets, uint32_t* ids,
                       uint64_t ids_cap, qcp_search_stats* stats) {

@@ -179,7 +179,10 @@ __device__ __forceinline__ void emit_range(const SearchArgs& a, Ctx<D>& c, uint6
     const uint32_t b0 = (uint32_t)(s + 1 + head);
     for (uint64_t i = lane; i < body / 4; i += kWave) {
       const uint32_t v = b0 + (uint32_t)(4 * i);
-      dst4[i] = make_uint4(v, v + 1, v + 2, v + 3);
+      __builtin_nontemporal_store(v, &dst4[i].x);
+      __builtin_nontemporal_store(v + 1, &dst4[i].y);
+      __builtin_nontemporal_store(v + 2, &dst4[i].z);
+      __builtin_nontemporal_store(v + 3, &dst4[i].w);
     }
     const uint64_t t0 = head + body;
     if (t0 + lane < len) dst[t0 + lane] = (uint32_t)(s + 1 + t0 + lane);
@@ -195,7 +198,9 @@ __device__ __forceinline__ void chunk_out(const SearchArgs& a, Ctx<D>& c, uint32
   const uint64_t b = ballot(hit);
   const uint32_t nb = (uint32_t)__popcll(b);
   if (OUT == kOutIds) {
-    if (hit) (a.ids + c.out_base)[rank_below(b)] = st + lane + 1u;
+    // the ID stream is written once and read back only by the host:
+    // nontemporal (streaming) stores keep it from displacing the tree in L2
+    if (hit) __builtin_nontemporal_store(st + lane + 1u, (a.ids + c.out_base) + rank_below(b));
     c.out_base += nb;
   } else if (OUT == kOutHash) {
     // branch-free: a select, a 64-bit add and one v_mad_u64_u32 per chunk

@@ -128,6 +128,7 @@ def lib():
         L.qcp_pipeline_run.argtypes = [P, P, P, ctypes.POINTER(SearchCfg),
                                        ctypes.POINTER(SearchStats)]
         L.qcp_batch_results.argtypes = [P, P, P, P, P, ctypes.c_uint64, P]
+        L.qcp_batch_ids.argtypes = [P, ctypes.c_uint64, ctypes.c_uint64, P, P, ctypes.c_uint64]
         L.qcp_batch_total_hits.argtypes = [P]
         L.qcp_batch_total_hits.restype = ctypes.c_uint64
         L.qcp_search.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(SearchCfg), P, P, P, P,
@@ -400,6 +401,14 @@ class Batch:
         self._ids = False
         self._mode = "persistent"
         return st.as_dict()
+
+    def ids_range(self, q0, q1):
+        """Offsets (relative) and sorted rank-ID lists of queries [q0, q1)."""
+        off = np.zeros(q1 - q0 + 1, np.uint64)
+        check(lib().qcp_batch_ids(self._h, q0, q1, _p(off), None, 0), "qcp_batch_ids")
+        ids = np.zeros(max(int(off[-1]), 1), np.uint32)
+        check(lib().qcp_batch_ids(self._h, q0, q1, None, _p(ids), int(off[-1])), "qcp_batch_ids")
+        return off, ids[: int(off[-1])]
 
     def results(self, ids=False, visits=False):
         q = self.q

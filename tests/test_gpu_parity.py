@@ -285,3 +285,19 @@ def test_kmeans_hilbert_order(gpu, O, D, K, layout, key):
     q = rand_queries(300, D, 81, pts)
     exp = t.search_dfs(q, want_ids=True)
     assert_results_equal(ix.search(q, ids=True), exp, "k-means order search")
+
+
+def test_ids_range_pagination(gpu, O):
+    pts = rand_points(50000, 2, 90)
+    q = rand_queries(200, 2, 91, pts)
+    ix = gpu.Index.build(pts, 2, fanout=32)
+    exp = O.build_packed(pts, 2, False, 32, O.KEY_REF).search_dfs(q, want_ids=True)
+    with ix.batch(q) as b:
+        b.run(ids=True)
+        for q0, q1 in ((0, 200), (5, 17), (199, 200), (30, 30)):
+            off, ids = b.ids_range(q0, q1)
+            eo = exp["offsets"][q0:q1 + 1]
+            assert_bits_equal(off, eo - eo[0], "offsets %d..%d" % (q0, q1))
+            assert_bits_equal(ids, exp["ids"][eo[0]:eo[-1]], "ids %d..%d" % (q0, q1))
+        with pytest.raises(gpu.QcpError):
+            b.ids_range(10, 201)
