@@ -7,6 +7,7 @@
 // chosen by timing the persistent search of a query sample on the GPU itself.
 #include <string.h>
 
+#include <new>
 #include <vector>
 
 #include "../../include/qcp.h"
@@ -39,7 +40,7 @@ extern "C" qcp_status qcp_autotune(const qcp_build_cfg* cfg, const float* d_objs
                                    const float* d_queries, uint64_t q, const uint32_t* fanouts,
                                    uint32_t n_fanouts, uint32_t outputs, qcp_index** best_index,
                                    qcp_search_cfg* best, qcp_tune_entry* table, uint32_t table_cap,
-                                   uint32_t* table_len) {
+                                   uint32_t* table_len) try {
   if (!cfg || !best_index || !best || !fanouts || n_fanouts == 0 || (q && !d_queries) ||
       (n && !d_objs))
     return QCP_EINVAL;
@@ -115,4 +116,8 @@ extern "C" qcp_status qcp_autotune(const qcp_build_cfg* cfg, const float* d_objs
   }
   *best_index = best_ix;
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }

@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <new>
 #include <vector>
 
 #include "../../include/qcp.h"
@@ -41,7 +42,7 @@ struct FileCloser {
 
 extern "C" {
 
-qcp_status qcp_save(const qcp_index* ix, const char* path) {
+qcp_status qcp_save(const qcp_index* ix, const char* path) try {
   if (!ix || !path) return QCP_EINVAL;
   qcp_index_info in;
   qcp_status s = qcp_index_info_get(ix, &in);
@@ -97,6 +98,10 @@ qcp_status qcp_save(const qcp_index* ix, const char* path) {
             fwrite(&pbytes, 8, 1, fc.f) == 1 && fwrite(&sum, 8, 1, fc.f) == 1 &&
             (pbytes == 0 || fwrite(payload.data(), 1, pbytes, fc.f) == pbytes);
   return ok ? QCP_OK : QCP_EIO;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 // Implemented in qcp_api.cpp: rebuild an index's device state from exported arrays.
@@ -105,7 +110,7 @@ __attribute__((visibility("hidden"))) qcp_status qcp_internal_restore(const qcp_
                                 const uint32_t* first, const uint32_t* count, uint64_t n_nodes,
                                 qcp_index** out);
 
-qcp_status qcp_load(const char* path, uint32_t device, qcp_index** out) {
+qcp_status qcp_load(const char* path, uint32_t device, qcp_index** out) try {
   if (!path || !out) return QCP_EINVAL;
   *out = nullptr;
   FileCloser fc{fopen(path, "rb")};
@@ -147,6 +152,10 @@ qcp_status qcp_load(const char* path, uint32_t device, qcp_index** out) {
   const uint32_t* count = expl ? (const uint32_t*)(p + 4 * m) : nullptr;
   cfg.device = device;
   return qcp_internal_restore(&cfg, n, keys, perm, objs, boxes, first, count, m, out);
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 }  // extern "C"
@@ -163,7 +172,7 @@ qcp_status qcp_load(const char* path, uint32_t device, qcp_index** out) {
 // 39-47) in every partition, so the objects are recovered in rank order.
 extern "C" qcp_status qcp_read_reference_mphr(const char* path, uint32_t dims,
                                               uint32_t leaf_degree, uint32_t obj_kind,
-                                              float* objs, uint64_t cap, uint64_t* n_out) {
+                                              float* objs, uint64_t cap, uint64_t* n_out) try {
   if (!path || !n_out || (dims != 2 && dims != 3) || leaf_degree == 0 ||
       leaf_degree > (1u << 16) || obj_kind > QCP_OBJ_BOXES)
     return QCP_EINVAL;
@@ -217,4 +226,8 @@ extern "C" qcp_status qcp_read_reference_mphr(const char* path, uint32_t dims,
     }
   }
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }

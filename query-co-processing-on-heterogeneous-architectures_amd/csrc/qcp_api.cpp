@@ -433,7 +433,7 @@ int qcp_device_count(void) {
 }
 
 qcp_status qcp_build_device(const qcp_build_cfg* cfg, const float* d_objs, uint64_t n,
-                            qcp_index** out) {
+                            qcp_index** out) try {
   if (!out || (n && !d_objs)) return QCP_EINVAL;
   *out = nullptr;
   qcp_index* ix = nullptr;
@@ -445,10 +445,14 @@ qcp_status qcp_build_device(const qcp_build_cfg* cfg, const float* d_objs, uint6
   }
   *out = ix;
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 qcp_status qcp_build_ranked(const qcp_build_cfg* cfg, const float* objs, uint64_t n,
-                            qcp_index** out) {
+                            qcp_index** out) try {
   if (!out || (n && !objs)) return QCP_EINVAL;
   *out = nullptr;
   qcp_index* ix = nullptr;
@@ -493,9 +497,13 @@ qcp_status qcp_build_ranked(const qcp_build_cfg* cfg, const float* objs, uint64_
         if (objs[r * ix->E + d] > objs[r * ix->E + ix->D + d]) ix->contain_ok = false;
   *out = ix;
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
-qcp_status qcp_build(const qcp_build_cfg* cfg, const float* objs, uint64_t n, qcp_index** out) {
+qcp_status qcp_build(const qcp_build_cfg* cfg, const float* objs, uint64_t n, qcp_index** out) try {
   if (!out || (n && !objs)) return QCP_EINVAL;
   *out = nullptr;
   qcp_index* ix = nullptr;
@@ -514,14 +522,22 @@ qcp_status qcp_build(const qcp_build_cfg* cfg, const float* objs, uint64_t n, qc
   }
   *out = ix;
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
-qcp_status qcp_rebuild_device(qcp_index* ix, const float* d_objs) {
+qcp_status qcp_rebuild_device(qcp_index* ix, const float* d_objs) try {
   if (!ix || (ix->n && !d_objs)) return QCP_EINVAL;
   QCP_TRY(set_device(ix->device));
   if (!ix->d_keys2) return QCP_EINVAL;
   if (ix->cfg.flags & QCP_BUILD_GRAPH) return index_run_build_graph(ix, d_objs);
   return index_run_build(ix, d_objs);
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 void qcp_index_destroy(qcp_index* ix) { index_free(ix); }
@@ -562,7 +578,7 @@ qcp_status qcp_index_info_get(const qcp_index* ix, qcp_index_info* out) {
 
 qcp_status qcp_export(const qcp_index* ix, uint64_t* sorted_keys, uint32_t* perm, float* sobjs,
                       float* node_boxes, uint32_t* node_first, uint32_t* node_count,
-                      uint8_t* node_leaf) {
+                      uint8_t* node_leaf) try {
   if (!ix) return QCP_EINVAL;
   QCP_TRY(set_device(ix->device));
   const uint64_t n = ix->n;
@@ -617,6 +633,10 @@ qcp_status qcp_export(const qcp_index* ix, uint64_t* sorted_keys, uint32_t* perm
     }
   }
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 // ---------------------------------------------------------------------------
@@ -688,7 +708,7 @@ static qcp_status batch_alloc(qcp_index* ix, uint64_t q, qcp_batch** out) {
   return QCP_OK;
 }
 
-qcp_status qcp_batch_create(qcp_index* ix, const float* boxes, uint64_t q, qcp_batch** out) {
+qcp_status qcp_batch_create(qcp_index* ix, const float* boxes, uint64_t q, qcp_batch** out) try {
   if (!ix || !out || (q && !boxes)) return QCP_EINVAL;
   *out = nullptr;
   QCP_TRY(set_device(ix->device));
@@ -704,10 +724,14 @@ qcp_status qcp_batch_create(qcp_index* ix, const float* boxes, uint64_t q, qcp_b
   }
   *out = b;
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 qcp_status qcp_batch_create_device(qcp_index* ix, const float* d_boxes, uint64_t q,
-                                   qcp_batch** out) {
+                                   qcp_batch** out) try {
   if (!ix || !out || (q && !d_boxes)) return QCP_EINVAL;
   *out = nullptr;
   QCP_TRY(set_device(ix->device));
@@ -723,6 +747,10 @@ qcp_status qcp_batch_create_device(qcp_index* ix, const float* d_boxes, uint64_t
   }
   *out = b;
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 }  // extern "C"
@@ -812,7 +840,7 @@ static qcp_status batch_order(qcp_batch* b) {
   return QCP_OK;
 }
 
-qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_stats* stats) {
+qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_stats* stats) try {
   if (!b) return QCP_EINVAL;
   qcp_search_cfg cfg;
   memset(&cfg, 0, sizeof(cfg));
@@ -1016,10 +1044,14 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   if (donate) b->slots_dirty = std::min<uint64_t>(b->slot_cap, std::max<uint64_t>(ctl.tail, 1));
   if (stats) *stats = st;
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 qcp_status qcp_pipeline_run(qcp_index* ix, const float* d_objs, qcp_batch* b,
-                            const qcp_search_cfg* cfg_in, qcp_search_stats* stats) {
+                            const qcp_search_cfg* cfg_in, qcp_search_stats* stats) try {
   if (!ix || !b || b->ix != ix || (ix->n && !d_objs)) return QCP_EINVAL;
   qcp_search_cfg cfg;
   memset(&cfg, 0, sizeof(cfg));
@@ -1038,6 +1070,10 @@ qcp_status qcp_pipeline_run(qcp_index* ix, const float* d_objs, qcp_batch* b,
   ix->h_nbox.clear();
   ix->contain_ok = !(bad & kBadInverted);
   return (bad & kBadDomain) ? QCP_EDOMAIN : QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 uint64_t qcp_batch_total_hits(const qcp_batch* b) { return b ? b->total_hits : 0; }
@@ -1072,7 +1108,7 @@ qcp_status qcp_batch_results(const qcp_batch* b, uint32_t* counts, uint64_t* has
 }
 
 qcp_status qcp_batch_ids(const qcp_batch* b, uint64_t q0, uint64_t q1, uint64_t* offsets,
-                         uint32_t* ids, uint64_t ids_cap) {
+                         uint32_t* ids, uint64_t ids_cap) try {
   if (!b || q0 > q1 || q1 > b->q) return QCP_EINVAL;
   if (!b->has_offsets) return QCP_EINVAL;
   const qcp_index* ix = b->ix;
@@ -1091,11 +1127,15 @@ qcp_status qcp_batch_ids(const qcp_batch* b, uint64_t q0, uint64_t q1, uint64_t*
     HIP_TRY(hipStreamSynchronize(s));
   }
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 qcp_status qcp_search(qcp_index* ix, const float* boxes, uint64_t q, const qcp_search_cfg* cfg,
                       uint32_t* counts, uint64_t* hashes, uint64_t* offsets, uint32_t* ids,
-                      uint64_t ids_cap, qcp_search_stats* stats) {
+                      uint64_t ids_cap, qcp_search_stats* stats) try {
   qcp_batch* b = nullptr;
   QCP_TRY(qcp_batch_create(ix, boxes, q, &b));
   qcp_status s = qcp_batch_run(b, cfg, stats);
@@ -1106,6 +1146,10 @@ qcp_status qcp_search(qcp_index* ix, const float* boxes, uint64_t q, const qcp_s
   }
   qcp_batch_destroy(b);
   return s;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 // Device state of an index from host arrays (the payload of a saved index).

@@ -229,7 +229,7 @@ static void dispatcher(qcp_server* sv) {
 
 extern "C" {
 
-qcp_status qcp_server_start(qcp_index* ix, const qcp_serve_cfg* cfg_in, qcp_server** out) {
+qcp_status qcp_server_start(qcp_index* ix, const qcp_serve_cfg* cfg_in, qcp_server** out) try {
   if (!ix || !out) return QCP_EINVAL;
   *out = nullptr;
   qcp_serve_cfg cfg;
@@ -270,9 +270,13 @@ qcp_status qcp_server_start(qcp_index* ix, const qcp_serve_cfg* cfg_in, qcp_serv
   sv->worker = std::thread(dispatcher, sv);
   *out = sv;
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
-qcp_status qcp_server_submit(qcp_server* sv, const float* box, uint64_t* ticket) {
+qcp_status qcp_server_submit(qcp_server* sv, const float* box, uint64_t* ticket) try {
   if (!sv || !box || !ticket) return QCP_EINVAL;
   const uint32_t D = sv->ix->D;
   {
@@ -289,6 +293,10 @@ qcp_status qcp_server_submit(qcp_server* sv, const float* box, uint64_t* ticket)
   }
   sv->cv_submit.notify_one();
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 qcp_status qcp_server_result(qcp_server* sv, uint64_t ticket, uint32_t* count, uint64_t* hash,
@@ -357,7 +365,7 @@ qcp_status qcp_server_stop(qcp_server* sv) {
 }
 
 qcp_status qcp_server_replay(qcp_server* sv, const float* boxes, const double* arrival_s,
-                             uint64_t n, uint32_t* counts, uint64_t* hashes, double* latency_us) {
+                             uint64_t n, uint32_t* counts, uint64_t* hashes, double* latency_us) try {
   if (!sv || (n && (!boxes || !arrival_s))) return QCP_EINVAL;
   const uint32_t D = sv->ix->D;
   std::vector<uint64_t> tk(n);
@@ -380,6 +388,10 @@ qcp_status qcp_server_replay(qcp_server* sv, const float* boxes, const double* a
     if (latency_us) latency_us[i] = l;
   }
   return QCP_OK;
+} catch (const std::bad_alloc&) {
+  return QCP_ENOMEM;  // host allocation failed
+} catch (...) {
+  return QCP_EINVAL;  // no exception crosses the C ABI
 }
 
 }  // extern "C"
