@@ -391,6 +391,10 @@ def run_config(qcp, name, args):
                                 "sigma 0.002; 20%% uniform); 500,000 data-centred queries, "
                                 "sides U(1e-4,1e-2); counts+hash", **res})
         return out
+    if name == "c4":
+        out.update(serving_bench(qcp, (10000, 50000, 100000), args.serve_seconds, args.scale,
+                                 contain_rate=50000))
+        return out
     raise SystemExit("unknown config " + name)
 
 
@@ -487,7 +491,8 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="--profile-traffic: every configuration the autotuner can pick")
     ap.add_argument("--config", default="c2",
-                    help="c2 (default: the headline line); c1, c3 or c5 print their own line")
+                    help="c2 (default: the headline line); c1, c3, c4 (serving, "
+                         "--serve-seconds per rate) or c5 print their own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the box has one); --gpus %d is not "
