@@ -265,6 +265,25 @@ def test_c1_full_size(gpu, O):
     assert_bits_equal(pl["counts"], exp["counts"], "c1 plain counts")
 
 
+def test_kmeans_small_and_boxes(gpu, O):
+    """Fewer objects than clusters (K becomes N), and boxes keyed by centre."""
+    pts = rand_points(5, 2, 3)
+    ix = gpu.Index.build(pts, 2, fanout=16, order="kmeans", clusters=64)
+    init = O.kmeans_seeds(gpu.gen_words(4096, 1234, 5), 5, 5)
+    t = O.build_kmeans(pts, 2, False, 16, O.KEY_REF, 5, 20, init)
+    assert_bits_equal(ix.export()["keys"], t.keys, "k-means keys, K = N")
+    boxes = rand_boxes(30000, 2, 5, mean_side=0.01)
+    init = O.kmeans_seeds(gpu.gen_words(4096, 1234, 5), len(boxes), 12)
+    t = O.build_kmeans(boxes, 2, True, 64, O.KEY_REF, 12, 20, init, key_src=O.SRC_CENTER)
+    ix = gpu.Index.build(boxes, 2, fanout=64, layout="rtree", obj_kind="boxes", key_src="center",
+                         order="kmeans", clusters=12)
+    ex = ix.export()
+    assert_bits_equal(ex["keys"], t.keys, "k-means box keys")
+    assert_bits_equal(ex["perm"], t.perm, "k-means box permutation")
+    q = rand_queries(200, 2, 6, boxes)
+    assert_results_equal(ix.search(q, ids=True), t.search_dfs(q, want_ids=True), "k-means boxes")
+
+
 @pytest.mark.parametrize("D,K,layout,key", [(2, 16, "mphr", "ref"), (3, 64, "mphr", "b16"),
                                             (3, 7, "bvh", "ref")])
 def test_kmeans_hilbert_order(gpu, O, D, K, layout, key):
