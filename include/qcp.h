@@ -42,7 +42,14 @@ typedef enum {
   QCP_ENODEV = -8    /* no usable GPU                                         */
 } qcp_status;
 
-enum { QCP_LAYOUT_MPHR = 0, QCP_LAYOUT_RTREE = 1, QCP_LAYOUT_BVH = 2 };
+/* Node layouts: MPHR and RTREE are Hilbert-packed (F consecutive ranks per
+ * leaf); BVH splits rank ranges at common-prefix positions; GUTTMAN inserts the
+ * Hilbert-ordered objects one by one on the host (quadratic split, minimum
+ * fill F/2; the reference RTree, src/tree/tree.cpp:126-185) and numbers the
+ * entries depth-first (src/tree/tree.cpp:244-256), so its entry IDs are
+ * depth-first positions rather than Hilbert ranks; fanout >= 4, not with
+ * QCP_BUILD_GRAPH or qcp_pipeline_run. */
+enum { QCP_LAYOUT_MPHR = 0, QCP_LAYOUT_RTREE = 1, QCP_LAYOUT_BVH = 2, QCP_LAYOUT_GUTTMAN = 3 };
 enum { QCP_KEY_REF = 0, QCP_KEY_B16 = 1 };      /* (ll)(1e6f*p) @ 31|20 bits, or 16 bits */
 enum { QCP_SRC_LO = 0, QCP_SRC_CENTER = 1 };    /* key from lower corner or box centre  */
 enum { QCP_OBJ_POINTS = 0, QCP_OBJ_BOXES = 1 };
@@ -54,7 +61,7 @@ enum { QCP_OUT_COUNTS = 1, QCP_OUT_HASH = 2, QCP_OUT_IDS = 4, QCP_OUT_VISITS = 8
 
 typedef struct {
   uint32_t dims;      /* 2 or 3                                                     */
-  uint32_t fanout;    /* entries per node: 16..256 (MPHR/RTREE), 16..256 (BVH: reference 128) */
+  uint32_t fanout;    /* entries per node, 2..256 (reference: 192 MPHR, 128 RTREE/BVH/GUTTMAN) */
   uint32_t layout;    /* QCP_LAYOUT_*                                               */
   uint32_t key_mode;  /* QCP_KEY_*                                                  */
   uint32_t key_src;   /* QCP_SRC_*                                                  */
@@ -77,7 +84,8 @@ typedef struct {
   uint64_t level_count[32];   /* nodes per level, root first              */
   uint64_t tree_bytes;        /* device bytes of node boxes + objects     */
   double build_ms;            /* device time of the last build (HIP events) */
-  double build_phase_ms[6];   /* keys, sort, gather, pack, topology, total */
+  double build_phase_ms[6];   /* keys, sort, gather, pack, topology (GUTTMAN: host
+                                 insertion, wall ms), total */
 } qcp_index_info;
 
 typedef struct {

@@ -21,6 +21,9 @@
  *   orc_bvh_build        child ranges partition [0,N), leaf size <= F,
  *                        children <= F-1, split positions match a brute-force
  *                        common-prefix rule.
+ *   orc_guttman_build    fill bounds (F/2..F below the root), one leaf level,
+ *                        exact covers, a permutation of the inputs, and
+ *                        hand-derived splits and descents at small F.
  *   searches             DFS == MPRS == brute-force closed-interval filter.
  */
 #ifndef QCP_ORACLE_H
@@ -69,7 +72,8 @@ uint64_t orc_pack(const float *sobjs, uint64_t n, unsigned D, int is_box,
 int64_t  orc_bvh_topology(uint64_t n, unsigned F, int64_t cap,
                           uint32_t *node_first, uint32_t *node_count,
                           uint8_t *node_leaf, uint8_t *node_level);
-/* node boxes of an explicit tree, padded-min/max rule (src/node/node.cpp:19-70). */
+/* node boxes of an explicit tree, padded-min/max rule (src/node/node.cpp:19-70);
+ * F = 0 gives exact covers (no padding). */
 void     orc_tree_boxes(const float *sobjs, unsigned D, int is_box, unsigned F,
                         int64_t n_nodes, const uint32_t *node_first,
                         const uint32_t *node_count, const uint8_t *node_leaf,
@@ -132,6 +136,16 @@ void     orc_search_mprs_mt(const float *sobjs, uint64_t n, unsigned D, int is_b
 void     orc_kmeans_keys(const float *objs, uint64_t n, unsigned D, int is_box, int key_mode,
                          int key_src, unsigned K, unsigned iters, const uint32_t *init_idx,
                          uint64_t *keys, uint32_t *cid);
+/* Guttman R-tree (src/tree/rtree.h; Tree::RTree_Top_Down and SetNodeIndex,
+ * src/tree/tree.cpp:126-185,244-256): inserts objs[i] in index order (node
+ * capacity F, minimum fill F/2, quadratic split, spherical-volume metric in
+ * FP32), then writes the BFS node arrays as orc_bvh_topology does, leaves
+ * holding depth-first positions, and order[p] = index of the object at
+ * depth-first position p.  Returns the node count, or -1 if cap is too small.
+ * Node boxes: orc_tree_boxes with F = 0 (exact covers, no padding). */
+int64_t  orc_guttman_build(const float *objs, uint64_t n, unsigned D, int is_box, unsigned F,
+                           int64_t cap, uint32_t *node_first, uint32_t *node_count,
+                           uint8_t *node_leaf, uint8_t *node_level, uint32_t *order);
 /* Closed-form brute force: every object against every query. */
 void     orc_search_brute(const float *sobjs, uint64_t n, unsigned D, int is_box,
                           const float *queries, uint64_t nq,

@@ -47,7 +47,7 @@ extern "C" qcp_status qcp_autotune(const qcp_build_cfg* cfg, const float* d_objs
   if (outputs & QCP_OUT_IDS) return QCP_EINVAL;  // tuned on the counting pass
   *best_index = nullptr;
   if (table_len) *table_len = 0;
-  const bool packed = cfg->layout != QCP_LAYOUT_BVH;
+  const bool packed = cfg->layout < QCP_LAYOUT_BVH;
   const uint32_t waves[] = {1, 2, 4, 8};
   const uint32_t chunks_packed[] = {2, 4, 8};
   const uint32_t chunks_expl[] = {4};

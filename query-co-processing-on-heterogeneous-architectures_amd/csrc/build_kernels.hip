@@ -1,7 +1,7 @@
 // build_kernels.hip -- GPU bulk-load pipeline for gfx950:
 //   Hilbert keys -> stable LSD radix sort (8-bit digits, LDS histograms and
 //   wave-level multisplit ranking) -> gather into rank-ordered SoA objects ->
-//   node MBRs (packed levels bottom-up, or explicit BVH levels deepest first).
+//   node MBRs (packed levels bottom-up, or explicit BVH / Guttman levels deepest first).
 //
 // Reference steps replaced: Tree::AssignHilbertIndexToBranches
 // (src/tree/tree.cpp:596-639), Sorter::Sort + ReassignHilbertIndexes
@@ -206,7 +206,8 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ objs,
 }
 
 // Box of `cnt` entries with the reference's padding rule: when cnt < F the
-// reduction also sees lower 1.0f / upper 0.0f (src/tree/tree.cpp:1070-1075).
+// reduction also sees lower 1.0f / upper 0.0f (src/tree/tree.cpp:1070-1075);
+// F = 0 gives the exact cover.
 // One wave per node; lane handles entries lane + 64*w.
 template <int D, int MW>
 __device__ __forceinline__ void wave_box(const float* __restrict__ src, uint64_t stride,
@@ -281,14 +282,14 @@ __global__ __launch_bounds__(256) void k_level_boxes(float* __restrict__ nbox,
                   parent_off + wave);
 }
 
-// Explicit tree (BVH): nodes [lo, hi) of one depth; leaves box their objects,
+// Explicit tree (BVH, Guttman): nodes [lo, hi) of one depth; leaves box their objects,
 // internal nodes their children (which live at the next depth, already done).
 template <int D, bool BOX, int MW>
 __global__ __launch_bounds__(256) void k_tree_boxes(const float* __restrict__ obj,
                                                     uint64_t ostride,
                                                     const uint32_t* __restrict__ nfirst,
                                                     const uint32_t* __restrict__ ncount,
-                                                    uint32_t F, uint64_t lo, uint64_t hi,
+                                                    uint32_t pad, uint64_t lo, uint64_t hi,
                                                     float* __restrict__ nbox,
                                                     uint64_t nstride) {
   const uint64_t node = lo + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -297,9 +298,9 @@ __global__ __launch_bounds__(256) void k_tree_boxes(const float* __restrict__ ob
   const uint32_t cw = ncount[node];
   const uint32_t cnt = cw & ~kLeafBit;
   if (cw & kLeafBit)
-    wave_box<D, MW>(obj, ostride, BOX ? D : 0, first, cnt, F, nbox, nstride, node);
+    wave_box<D, MW>(obj, ostride, BOX ? D : 0, first, cnt, pad, nbox, nstride, node);
   else
-    wave_box<D, MW>(nbox, nstride, D, first, cnt, F, nbox, nstride, node);
+    wave_box<D, MW>(nbox, nstride, D, first, cnt, pad, nbox, nstride, node);
 }
 
 // Fill helpers (kernels, so that build and search stay graph-capturable).
@@ -600,17 +601,17 @@ hipError_t launch_level_boxes(float* nbox, uint64_t nstride, unsigned D, uint64_
 
 hipError_t launch_tree_boxes(const float* obj, uint64_t ostride, const uint32_t* nfirst,
                              const uint32_t* ncount, unsigned D, bool box, uint32_t F,
-                             uint64_t lo, uint64_t hi, float* nbox, uint64_t nstride,
-                             hipStream_t s) {
+                             uint32_t pad, uint64_t lo, uint64_t hi, float* nbox,
+                             uint64_t nstride, hipStream_t s) {
   if (hi <= lo) return hipSuccess;
   unsigned g = (unsigned)(((hi - lo) * kWave + 255) / 256);
   int mwv = (int)((F + 63) / 64);
   if (D == 2) {
-    if (box) { QCP_MW_SWITCH(mwv, k_tree_boxes<2, true, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, F, lo, hi, nbox, nstride)) }
-    else { QCP_MW_SWITCH(mwv, k_tree_boxes<2, false, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, F, lo, hi, nbox, nstride)) }
+    if (box) { QCP_MW_SWITCH(mwv, k_tree_boxes<2, true, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, pad, lo, hi, nbox, nstride)) }
+    else { QCP_MW_SWITCH(mwv, k_tree_boxes<2, false, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, pad, lo, hi, nbox, nstride)) }
   } else if (D == 3) {
-    if (box) { QCP_MW_SWITCH(mwv, k_tree_boxes<3, true, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, F, lo, hi, nbox, nstride)) }
-    else { QCP_MW_SWITCH(mwv, k_tree_boxes<3, false, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, F, lo, hi, nbox, nstride)) }
+    if (box) { QCP_MW_SWITCH(mwv, k_tree_boxes<3, true, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, pad, lo, hi, nbox, nstride)) }
+    else { QCP_MW_SWITCH(mwv, k_tree_boxes<3, false, MW><<<g, 256, 0, s>>>(obj, ostride, nfirst, ncount, pad, lo, hi, nbox, nstride)) }
   } else {
     return hipErrorInvalidValue;
   }

@@ -50,7 +50,7 @@ qcp_status qcp_save(const qcp_index* ix, const char* path) try {
   const uint64_t n = in.n, m = in.n_nodes;
   const uint32_t D = in.dims;
   const uint32_t E = in.obj_kind == QCP_OBJ_BOXES ? 2 * D : D;
-  const bool expl = in.layout == QCP_LAYOUT_BVH;
+  const bool expl = in.layout == QCP_LAYOUT_BVH || in.layout == QCP_LAYOUT_GUTTMAN;
   std::vector<uint64_t> keys(n);
   std::vector<uint32_t> perm(n);
   std::vector<float> objs((size_t)E * n), boxes((size_t)2 * D * m);
@@ -133,7 +133,7 @@ qcp_status qcp_load(const char* path, uint32_t device, qcp_index** out) try {
   if (fread(&pbytes, 8, 1, fc.f) != 1 || fread(&sum, 8, 1, fc.f) != 1) return QCP_EFORMAT;
   const uint32_t D = cfg.dims;
   const uint32_t E = cfg.obj_kind == QCP_OBJ_BOXES ? 2 * D : D;
-  const bool expl = cfg.layout == QCP_LAYOUT_BVH;
+  const bool expl = cfg.layout == QCP_LAYOUT_BVH || cfg.layout == QCP_LAYOUT_GUTTMAN;
   const uint64_t expect = 8 * n + 4 * n + 4ull * E * n + 8ull * D * m + (expl ? 8 * m : 0);
   if (pbytes != expect) return QCP_EFORMAT;
   std::vector<uint8_t> payload(pbytes);

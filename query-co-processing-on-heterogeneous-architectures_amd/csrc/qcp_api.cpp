@@ -13,11 +13,13 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <new>
 #include <vector>
 
 #include "../../include/qcp.h"
 #include "bvh_topology.h"
+#include "guttman.h"
 #include "qcp_internal.h"
 #include "qcp_kernels.h"
 
@@ -113,7 +115,10 @@ static qcp_status validate_cfg(const qcp_build_cfg* cfg) {
   if (!cfg) return QCP_EINVAL;
   if (cfg->dims != 2 && cfg->dims != 3) return QCP_EINVAL;
   if (cfg->fanout < 2 || cfg->fanout > 256) return QCP_EINVAL;
-  if (cfg->layout > QCP_LAYOUT_BVH) return QCP_EINVAL;
+  if (cfg->layout > QCP_LAYOUT_GUTTMAN) return QCP_EINVAL;
+  if (cfg->layout == QCP_LAYOUT_GUTTMAN && (cfg->flags & QCP_BUILD_GRAPH)) return QCP_EINVAL;
+  // a minimum fill of 1 (F < 4) lets single-child chains grow without bound
+  if (cfg->layout == QCP_LAYOUT_GUTTMAN && cfg->fanout < 4) return QCP_EINVAL;
   if (cfg->key_mode > QCP_KEY_B16 || cfg->key_src > QCP_SRC_CENTER) return QCP_EINVAL;
   if (cfg->obj_kind > QCP_OBJ_BOXES) return QCP_EINVAL;
   if (cfg->key_src == QCP_SRC_CENTER && cfg->obj_kind != QCP_OBJ_BOXES) return QCP_EINVAL;
@@ -136,6 +141,57 @@ static std::vector<uint64_t> packed_levels(uint64_t n, uint32_t F) {
   return lv;
 }
 
+// Explicit layouts: adopt a BFS topology (first child / first position, count
+// with kLeafBit on leaves, BFS offset of each depth).  The node arrays are
+// (re)allocated when the node count outgrows them; the rank range under each
+// node is derived here: leaves hold [first, first+count), a parent spans its
+// first child's start to its last child's end (children follow their parent
+// in BFS order, so one reverse pass suffices).
+static qcp_status attach_topology(qcp_index* ix, std::vector<uint32_t>&& first,
+                                  std::vector<uint32_t>&& count,
+                                  const std::vector<uint64_t>& level_off) {
+  const uint64_t m = first.size();
+  if (count.size() != m || level_off.empty() || level_off.back() != m) return QCP_EINVAL;
+  const uint64_t stride = ((m + kNodeSlack + 63) / 64) * 64;
+  qcp_status s;
+  if (!ix->d_nbox || stride > ix->node_stride) {
+    for (uint32_t** a : {&ix->d_nfirst, &ix->d_ncount, &ix->d_nrlo, &ix->d_nrhi}) dfree(*a);
+    dfree(ix->d_nbox);
+    ix->node_stride = stride;
+    if ((s = dalloc(&ix->d_nbox, (uint64_t)2 * ix->D * stride)) != QCP_OK) return s;
+    if (launch_fill_f32(ix->d_nbox, (uint64_t)2 * ix->D * stride, 0.0f, ix->stream) != hipSuccess)
+      return QCP_EHIP;
+    for (uint32_t** a : {&ix->d_nfirst, &ix->d_ncount, &ix->d_nrlo, &ix->d_nrhi}) {
+      if ((s = dalloc(a, stride)) != QCP_OK) return s;
+      if (launch_fill_u32(*a, stride, 0u, ix->stream) != hipSuccess) return QCP_EHIP;
+    }
+  }
+  std::vector<uint32_t> rlo(m), rhi(m);
+  for (uint64_t i = m; i-- > 0;) {
+    if (count[i] & kLeafBit) {
+      rlo[i] = first[i];
+      rhi[i] = first[i] + (count[i] & ~kLeafBit);
+    } else {
+      rlo[i] = rlo[first[i]];
+      rhi[i] = rhi[first[i] + count[i] - 1];
+    }
+  }
+  uint32_t* dst[4] = {ix->d_nfirst, ix->d_ncount, ix->d_nrlo, ix->d_nrhi};
+  const uint32_t* src[4] = {first.data(), count.data(), rlo.data(), rhi.data()};
+  for (int i = 0; i < 4 && m; ++i)
+    HIP_TRY(hipMemcpyAsync(dst[i], src[i], 4 * m, hipMemcpyHostToDevice, ix->stream));
+  HIP_TRY(hipStreamSynchronize(ix->stream));
+  ix->n_nodes = m;
+  ix->h = (int)level_off.size() - 1;
+  ix->level_off.assign(level_off.begin(), level_off.end());
+  ix->level_count.clear();
+  for (int l = 0; l < ix->h; ++l) ix->level_count.push_back(level_off[l + 1] - level_off[l]);
+  ix->h_first = std::move(first);
+  ix->h_count = std::move(count);
+  ix->h_nbox.clear();
+  return QCP_OK;
+}
+
 // Allocate the index and its topology for n objects.
 static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** out) {
   QCP_TRY(validate_cfg(cfg));
@@ -149,7 +205,7 @@ static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** 
   ix->F = cfg->fanout;
   ix->box = cfg->obj_kind == QCP_OBJ_BOXES;
   ix->E = ix->box ? 2 * ix->D : ix->D;
-  ix->expl = cfg->layout == QCP_LAYOUT_BVH;
+  ix->expl = cfg->layout == QCP_LAYOUT_BVH || cfg->layout == QCP_LAYOUT_GUTTMAN;
   ix->nb = cfg->key_mode == QCP_KEY_REF ? (ix->D > 2 ? 20u : 31u) : 16u;
   ix->n = n;
   auto fail = [&](qcp_status s) {
@@ -163,7 +219,7 @@ static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** 
   hipDeviceGetAttribute(&ix->cus, hipDeviceAttributeMultiprocessorCount, ix->device);
   if (ix->cus <= 0) ix->cus = 256;
 
-  std::vector<uint32_t> h_first, h_count;
+  qcp_status s;
   if (!ix->expl) {
     ix->level_count = packed_levels(n, ix->F);
     ix->h = (int)ix->level_count.size();
@@ -171,58 +227,27 @@ static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** 
     ix->level_off.assign(ix->h + 1, 0);
     for (int l = 0; l < ix->h; ++l) ix->level_off[l + 1] = ix->level_off[l] + ix->level_count[l];
     ix->n_nodes = ix->level_off[ix->h];
-  } else {
+    ix->node_stride = ((ix->n_nodes + kNodeSlack + 63) / 64) * 64;
+    if ((s = dalloc(&ix->d_nbox, (uint64_t)2 * ix->D * ix->node_stride)) != QCP_OK) return fail(s);
+    if (launch_fill_f32(ix->d_nbox, (uint64_t)2 * ix->D * ix->node_stride, 0.0f, ix->stream) != hipSuccess)
+      return fail(QCP_EHIP);
+  } else if (cfg->layout == QCP_LAYOUT_BVH) {
     BvhTopology t;
     if (!bvh_topology(n, ix->F, kMaxDepth, &t)) return fail(QCP_EINVAL);
-    ix->n_nodes = t.first.size();
-    ix->h = (int)t.level_off.size() - 1;
-    if (ix->h < 0) ix->h = 0;
-    ix->level_off.assign(t.level_off.begin(), t.level_off.end());
-    ix->level_count.clear();
-    for (int l = 0; l < ix->h; ++l) ix->level_count.push_back(ix->level_off[l + 1] - ix->level_off[l]);
-    h_first.swap(t.first);
-    h_count.swap(t.count);
-    ix->h_first = h_first;
-    ix->h_count = h_count;
+    std::vector<uint64_t> lo(t.level_off.begin(), t.level_off.end());
+    if ((s = attach_topology(ix, std::move(t.first), std::move(t.count), lo)) != QCP_OK) return fail(s);
+  } else {
+    // Guttman: the topology follows from the objects; empty until the build
+    if ((s = attach_topology(ix, {}, {}, {0})) != QCP_OK) return fail(s);
   }
   ix->obj_stride = ((n + kObjSlack + 63) / 64) * 64;
-  ix->node_stride = ((ix->n_nodes + kNodeSlack + 63) / 64) * 64;
-  qcp_status s;
   if ((s = dalloc(&ix->d_obj, (uint64_t)ix->E * ix->obj_stride)) != QCP_OK) return fail(s);
-  if ((s = dalloc(&ix->d_nbox, (uint64_t)2 * ix->D * ix->node_stride)) != QCP_OK) return fail(s);
   if ((s = dalloc(&ix->d_keys, n)) != QCP_OK) return fail(s);
   if ((s = dalloc(&ix->d_perm, n)) != QCP_OK) return fail(s);
   if ((s = dalloc(&ix->d_key_or, 1)) != QCP_OK) return fail(s);
   if ((s = dalloc(&ix->d_bad, 1)) != QCP_OK) return fail(s);
-  if (launch_fill_f32(ix->d_obj, (uint64_t)ix->E * ix->obj_stride, 0.0f, ix->stream) != hipSuccess ||
-      launch_fill_f32(ix->d_nbox, (uint64_t)2 * ix->D * ix->node_stride, 0.0f, ix->stream) != hipSuccess)
+  if (launch_fill_f32(ix->d_obj, (uint64_t)ix->E * ix->obj_stride, 0.0f, ix->stream) != hipSuccess)
     return fail(QCP_EHIP);
-  if (ix->expl) {
-    // rank range under each node: leaves hold [first, first+count); a parent
-    // spans its first child's start to its last child's end (children follow
-    // their parent in BFS order, so one reverse pass suffices)
-    std::vector<uint32_t> rlo(ix->n_nodes), rhi(ix->n_nodes);
-    for (uint64_t i = ix->n_nodes; i-- > 0;) {
-      if (h_count[i] & kLeafBit) {
-        rlo[i] = h_first[i];
-        rhi[i] = h_first[i] + (h_count[i] & ~kLeafBit);
-      } else {
-        rlo[i] = rlo[h_first[i]];
-        rhi[i] = rhi[h_first[i] + h_count[i] - 1];
-      }
-    }
-    uint32_t** arrs[4] = {&ix->d_nfirst, &ix->d_ncount, &ix->d_nrlo, &ix->d_nrhi};
-    const uint32_t* src[4] = {h_first.data(), h_count.data(), rlo.data(), rhi.data()};
-    for (int i = 0; i < 4; ++i) {
-      if ((s = dalloc(arrs[i], ix->node_stride)) != QCP_OK) return fail(s);
-      if (launch_fill_u32(*arrs[i], ix->node_stride, 0u, ix->stream) != hipSuccess)
-        return fail(QCP_EHIP);
-      if (ix->n_nodes && hipMemcpyAsync(*arrs[i], src[i], 4 * ix->n_nodes, hipMemcpyHostToDevice,
-                                        ix->stream) != hipSuccess)
-        return fail(QCP_EHIP);
-    }
-    if (hipStreamSynchronize(ix->stream) != hipSuccess) return fail(QCP_EHIP);
-  }
   // sort scratch
   const uint32_t nt = sort_tiles(n > 0 ? n : 1);
   if ((s = dalloc(&ix->d_keys2, n)) != QCP_OK) return fail(s);
@@ -252,6 +277,48 @@ static void set_info_shape(qcp_index* ix) {
   in.tree_bytes = (uint64_t)ix->E * 4 * ix->n + (uint64_t)2 * ix->D * 4 * ix->n_nodes;
 }
 
+// Guttman layout: insert the objects on the host in their sorted order
+// (sobjs: n rows of E floats, perm / keys of each sorted position), adopt the
+// tree, and store perm / keys in depth-first order.  *host_ms: insertion wall
+// time.
+static qcp_status guttman_adopt(qcp_index* ix, const float* sobjs, const uint32_t* perm,
+                                const uint64_t* keys, double* host_ms) {
+  const uint64_t n = ix->n;
+  const auto t0 = std::chrono::steady_clock::now();
+  GuttmanTree t;
+  if (!guttman_build(sobjs, n, ix->D, ix->box, ix->F, &t)) return QCP_EINVAL;
+  if (t.level_off.size() > (size_t)kMaxDepth) return QCP_EINVAL;  // deeper than a wave's stack
+  std::vector<uint32_t> perm2(n);
+  std::vector<uint64_t> keys2(n);
+  for (uint64_t p = 0; p < n; ++p) {
+    perm2[p] = perm[t.order[p]];
+    keys2[p] = keys[t.order[p]];
+  }
+  *host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (n) {
+    HIP_TRY(hipMemcpyAsync(ix->d_perm, perm2.data(), 4 * n, hipMemcpyHostToDevice, ix->stream));
+    HIP_TRY(hipMemcpyAsync(ix->d_keys, keys2.data(), 8 * n, hipMemcpyHostToDevice, ix->stream));
+  }
+  return attach_topology(ix, std::move(t.first), std::move(t.count), t.level_off);
+}
+
+// The same from the device state after the key sort (d_perm / d_keys sorted).
+static qcp_status guttman_from_sorted(qcp_index* ix, const float* d_objs, double* host_ms) {
+  const uint64_t n = ix->n;
+  const uint32_t E = ix->E;
+  hipStream_t s = ix->stream;
+  std::vector<float> objs((size_t)E * n), sobjs((size_t)E * n);
+  std::vector<uint32_t> perm(n);
+  std::vector<uint64_t> keys(n);
+  HIP_TRY(hipMemcpyAsync(objs.data(), d_objs, 4ull * E * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(perm.data(), ix->d_perm, 4 * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(keys.data(), ix->d_keys, 8 * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (uint64_t r = 0; r < n; ++r)
+    memcpy(sobjs.data() + r * E, objs.data() + (uint64_t)perm[r] * E, 4ull * E);
+  return guttman_adopt(ix, sobjs.data(), perm.data(), keys.data(), host_ms);
+}
+
 // Node boxes of the rank-ordered objects in d_obj: bottom-up over the packed
 // levels, or level by level of an explicit tree.
 static qcp_status enqueue_boxes(qcp_index* ix) {
@@ -268,8 +335,9 @@ static qcp_status enqueue_boxes(qcp_index* ix) {
   } else {
     for (int l = ix->h - 1; l >= 0; --l)
       HIP_TRY(launch_tree_boxes(ix->d_obj, ix->obj_stride, ix->d_nfirst, ix->d_ncount, ix->D,
-                                ix->box, ix->F, ix->level_off[l], ix->level_off[l + 1],
-                                ix->d_nbox, ix->node_stride, s));
+                                ix->box, ix->F, ix->cfg.layout == QCP_LAYOUT_GUTTMAN ? 0 : ix->F,
+                                ix->level_off[l], ix->level_off[l + 1], ix->d_nbox,
+                                ix->node_stride, s));
   }
   return QCP_OK;
 }
@@ -347,6 +415,8 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
   hipStream_t s = ix->stream;
   ix->h_nbox.clear();
   const uint64_t n = ix->n;
+  const bool guttman = ix->cfg.layout == QCP_LAYOUT_GUTTMAN;
+  double host_ms = 0.0;
   HIP_TRY(hipEventRecord(ix->ev[0], s));
   uint64_t key_or = 0;
   uint32_t bad = 0;
@@ -385,10 +455,15 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
       std::swap(ix->d_perm, ix->d_perm2);
     }
     HIP_TRY(hipEventRecord(ix->ev[2], s));
+    if (guttman) {
+      QCP_TRY(guttman_from_sorted(ix, d_objs, &host_ms));
+      HIP_TRY(hipEventRecord(ix->ev[6], s));
+    }
     HIP_TRY(launch_gather(d_objs, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s));
     HIP_TRY(hipEventRecord(ix->ev[3], s));
     QCP_TRY(enqueue_boxes(ix));
   } else {
+    if (guttman) QCP_TRY(guttman_adopt(ix, nullptr, nullptr, nullptr, &host_ms));
     HIP_TRY(hipEventRecord(ix->ev[1], s));
     HIP_TRY(hipEventRecord(ix->ev[2], s));
     HIP_TRY(hipEventRecord(ix->ev[3], s));
@@ -398,13 +473,14 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
   float ms[4] = {0, 0, 0, 0}, tot = 0;
   hipEventElapsedTime(&ms[0], ix->ev[0], ix->ev[1]);
   hipEventElapsedTime(&ms[1], ix->ev[1], ix->ev[2]);
-  hipEventElapsedTime(&ms[2], ix->ev[2], ix->ev[3]);
+  hipEventElapsedTime(&ms[2], ix->ev[guttman && n ? 6 : 2], ix->ev[3]);
   hipEventElapsedTime(&ms[3], ix->ev[3], ix->ev[4]);
   hipEventElapsedTime(&tot, ix->ev[0], ix->ev[4]);
   set_info_shape(ix);
   qcp_index_info& in = ix->info;
   in.build_ms = tot;
   for (int i = 0; i < 4; ++i) in.build_phase_ms[i] = ms[i];
+  in.build_phase_ms[4] = host_ms;
   in.build_phase_ms[5] = tot;
   return QCP_OK;
 }
@@ -464,6 +540,7 @@ qcp_status qcp_build_ranked(const qcp_build_cfg* cfg, const float* objs, uint64_
   hipStream_t s = ix->stream;
   for (uint64_t i = 0; i < (uint64_t)ix->E * n; ++i)
     if (objs[i] != objs[i]) return fail(QCP_EDOMAIN);  // NaN objects are rejected
+  double host_ms = 0.0;
   HIP_TRY(hipEventRecord(ix->ev[0], s));
   if (n) {
     // the given order is the rank order: keys and permutation are the identity
@@ -475,9 +552,14 @@ qcp_status qcp_build_ranked(const qcp_build_cfg* cfg, const float* objs, uint64_
     }
     qcp_status st = dalloc(&ix->d_in, (uint64_t)ix->E * n);
     if (st != QCP_OK) return fail(st);
+    if (ix->cfg.layout == QCP_LAYOUT_GUTTMAN) {
+      // the given order is the insertion order
+      if ((st = guttman_adopt(ix, objs, perm.data(), keys.data(), &host_ms)) != QCP_OK) return fail(st);
+    } else if (hipMemcpyAsync(ix->d_keys, keys.data(), 8 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+               hipMemcpyAsync(ix->d_perm, perm.data(), 4 * n, hipMemcpyHostToDevice, s) != hipSuccess) {
+      return fail(QCP_EHIP);
+    }
     if (hipMemcpyAsync(ix->d_in, objs, (uint64_t)ix->E * 4 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(ix->d_keys, keys.data(), 8 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(ix->d_perm, perm.data(), 4 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
         launch_gather(ix->d_in, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s) != hipSuccess)
       return fail(QCP_EHIP);
     if ((st = enqueue_boxes(ix)) != QCP_OK) return fail(st);
@@ -490,6 +572,7 @@ qcp_status qcp_build_ranked(const qcp_build_cfg* cfg, const float* objs, uint64_
   hipEventElapsedTime(&tot, ix->ev[0], ix->ev[4]);
   set_info_shape(ix);
   ix->info.build_ms = tot;
+  ix->info.build_phase_ms[4] = host_ms;
   ix->info.build_phase_ms[5] = tot;
   if (ix->box)
     for (uint64_t r = 0; r < n && ix->contain_ok; ++r)
@@ -1059,6 +1142,7 @@ qcp_status qcp_pipeline_run(qcp_index* ix, const float* d_objs, qcp_batch* b,
   if (cfg_in) cfg = *cfg_in;
   if (cfg.mode != QCP_SEARCH_PERSISTENT || (cfg.outputs & QCP_OUT_IDS)) return QCP_EINVAL;
   if (cfg.contain && ix->box) return QCP_EINVAL;  // box validity is known only after the build
+  if (ix->cfg.layout == QCP_LAYOUT_GUTTMAN) return QCP_EINVAL;  // the build has a host step
   QCP_TRY(set_device(ix->device));
   cfg.graph = 1;
   b->pipe_objs = d_objs;
@@ -1164,9 +1248,18 @@ __attribute__((visibility("hidden"))) qcp_status qcp_internal_restore(
     index_free(ix);
     return s;
   };
-  if (ix->n_nodes != n_nodes) return fail(QCP_EFORMAT);
   hipStream_t s = ix->stream;
-  if (ix->expl && n_nodes) {
+  if (cfg->layout == QCP_LAYOUT_GUTTMAN) {
+    // the stored topology is the tree: check its shape, then adopt it
+    std::vector<uint64_t> lo;
+    if (!first || !count || !guttman_check(first, count, n_nodes, n, ix->F, kMaxDepth, &lo))
+      return fail(QCP_EFORMAT);
+    qcp_status st = attach_topology(ix, std::vector<uint32_t>(first, first + n_nodes),
+                                    std::vector<uint32_t>(count, count + n_nodes), lo);
+    if (st != QCP_OK) return fail(st);
+  }
+  if (ix->n_nodes != n_nodes) return fail(QCP_EFORMAT);
+  if (ix->cfg.layout == QCP_LAYOUT_BVH && n_nodes) {
     std::vector<uint32_t> f(n_nodes), c(n_nodes);
     if (hipMemcpy(f.data(), ix->d_nfirst, 4 * n_nodes, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(c.data(), ix->d_ncount, 4 * n_nodes, hipMemcpyDeviceToHost) != hipSuccess)

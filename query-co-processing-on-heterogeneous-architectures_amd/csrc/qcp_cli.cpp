@@ -9,8 +9,10 @@
 // place of generator/cuRAND.cpp).
 //
 // Reference flags (upper- and lower-case alike, as in the reference):
-//   -i TYPE   m|mphr, r|rtree, b|bvh, h|hybrid (MPHR, co-processing search),
-//             hr (R-tree, co-processing), hb (BVH, co-processing)
+//   -i TYPE   m|mphr, r|rtree (the reference RTree: Guttman insertion), b|bvh,
+//             h|hybrid (MPHR, co-processing search), hr (Guttman R-tree,
+//             co-processing), hb (BVH, co-processing); pr|packed_rtree selects
+//             this build's Hilbert-packed R-tree
 //   -d N      number of data objects (a trailing "m" multiplies by 10^6)
 //   -q Q      number of queries
 //   -b B      queries per launch of the plain form (number_of_cuda_blocks)
@@ -106,7 +108,7 @@ int fail(const char* what, qcp_status s) {
 
 void usage() {
   fprintf(stderr,
-          "usage: qcp_cli -i m|r|b|h|hr|hb -d N -q Q [-b B -p P -l L -c C -r R -e E -s S]\n"
+          "usage: qcp_cli -i m|r|pr|b|h|hr|hb -d N -q Q [-b B -p P -l L -c C -r R -e E -s S]\n"
           "               --data FILE --queries FILE [--dims D --fanout F --boxes --center\n"
           "               --key ref|b16 --mode persistent|plain|hybrid --index FILE\n"
           "               --out FILE --contain --json]\n"
@@ -209,10 +211,11 @@ int main(int argc, char** argv) {
   uint32_t layout = QCP_LAYOUT_MPHR;
   std::string mode = o.mode;
   if (t == "m" || t == "mphr") layout = QCP_LAYOUT_MPHR;
-  else if (t == "r" || t == "rtree") layout = QCP_LAYOUT_RTREE;
+  else if (t == "r" || t == "rtree") layout = QCP_LAYOUT_GUTTMAN;
+  else if (t == "pr" || t == "packed_rtree") layout = QCP_LAYOUT_RTREE;
   else if (t == "b" || t == "bvh") layout = QCP_LAYOUT_BVH;
   else if (t == "h" || t == "hybrid") { layout = QCP_LAYOUT_MPHR; if (mode.empty()) mode = "hybrid"; }
-  else if (t == "hr") { layout = QCP_LAYOUT_RTREE; if (mode.empty()) mode = "hybrid"; }
+  else if (t == "hr") { layout = QCP_LAYOUT_GUTTMAN; if (mode.empty()) mode = "hybrid"; }
   else if (t == "hb") { layout = QCP_LAYOUT_BVH; if (mode.empty()) mode = "hybrid"; }
   else {
     fprintf(stderr, "qcp_cli: index type %s has no counterpart here\n", o.index_type.c_str());

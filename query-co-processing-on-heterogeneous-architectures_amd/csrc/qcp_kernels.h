@@ -37,10 +37,12 @@ hipError_t launch_leaf_boxes(const float* obj, uint64_t ostride, uint64_t n, uns
 hipError_t launch_level_boxes(float* nbox, uint64_t nstride, unsigned D, uint64_t child_off,
                               uint64_t child_cnt, uint32_t F, uint64_t parent_off,
                               uint64_t parent_cnt, hipStream_t s);
+// F sizes the per-node loop (counts <= F); nodes with fewer than pad entries
+// are padded 1.0f / 0.0f (pad = F for the BVH, 0 for exact covers).
 hipError_t launch_tree_boxes(const float* obj, uint64_t ostride, const uint32_t* nfirst,
                              const uint32_t* ncount, unsigned D, bool box, uint32_t F,
-                             uint64_t lo, uint64_t hi, float* nbox, uint64_t nstride,
-                             hipStream_t s);
+                             uint32_t pad, uint64_t lo, uint64_t hi, float* nbox,
+                             uint64_t nstride, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s);
 

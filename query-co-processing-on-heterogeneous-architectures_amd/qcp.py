@@ -16,7 +16,7 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "_build", "libqcp.so")
 
-LAYOUT = {"mphr": 0, "rtree": 1, "bvh": 2}
+LAYOUT = {"mphr": 0, "rtree": 1, "bvh": 2, "guttman": 3}
 KEY = {"ref": 0, "b16": 1}
 SRC = {"lo": 0, "center": 1}
 OBJ = {"points": 0, "boxes": 1}

@@ -77,6 +77,8 @@ def _declare(L):
     L.orc_build_mt.restype = INT
     L.orc_search_mprs_mt.argtypes = [P, U64, UNS, INT, UNS, P, P, U64, UNS, P, P]
     L.orc_kmeans_keys.argtypes = [P, U64, UNS, INT, INT, INT, UNS, UNS, P, P, P]
+    L.orc_guttman_build.argtypes = [P, U64, UNS, INT, UNS, I64, P, P, P, P, P]
+    L.orc_guttman_build.restype = I64
 
 
 def ptr(a):
@@ -121,8 +123,26 @@ def level_counts(n, F):
     return [int(x) for x in out[:h]]
 
 
+def guttman(objs, D, is_box, F):
+    """Guttman insertion of objs in index order: (first, count, leaf, level, order)."""
+    objs = np.ascontiguousarray(objs, dtype=np.float32)
+    n = objs.shape[0]
+    cap = max(16, 2 * n + 16)
+    nf = np.zeros(cap, dtype=np.uint32)
+    nc = np.zeros(cap, dtype=np.uint32)
+    nl = np.zeros(cap, dtype=np.uint8)
+    lv = np.zeros(cap, dtype=np.uint8)
+    order = np.zeros(max(n, 1), dtype=np.uint32)
+    m = lib().orc_guttman_build(ptr(objs), n, D, int(bool(is_box)), F, cap, ptr(nf), ptr(nc),
+                                ptr(nl), ptr(lv), ptr(order))
+    assert m >= 0
+    return nf[:m].copy(), nc[:m].copy(), nl[:m].copy(), lv[:m].copy(), order[:n].copy()
+
+
 class Tree:
-    """An explicit tree (BFS order) over rank-sorted objects, as the oracle builds it."""
+    """An explicit tree (BFS order) over rank-sorted objects, as the oracle builds it.
+    layout "guttman" inserts the rank-sorted objects one by one and then holds
+    them in depth-first leaf order (self.order maps position -> rank)."""
 
     def __init__(self, sobjs, D, is_box, F, layout="packed"):
         self.sobjs = np.ascontiguousarray(sobjs, dtype=np.float32)
@@ -134,7 +154,12 @@ class Tree:
         nl = np.zeros(cap, dtype=np.uint8)
         lv = np.zeros(cap, dtype=np.uint8)
         L = lib()
-        if layout == "bvh":
+        if layout == "guttman":
+            f, c, lf, lvl, self.order = guttman(self.sobjs, D, is_box, F)
+            m = len(f)
+            nf[:m], nc[:m], nl[:m], lv[:m] = f, c, lf, lvl
+            self.sobjs = np.ascontiguousarray(self.sobjs[self.order])
+        elif layout == "bvh":
             m = L.orc_bvh_topology(self.n, F, cap, ptr(nf), ptr(nc), ptr(nl), ptr(lv))
         else:
             m = L.orc_packed_topology(self.n, F, cap, ptr(nf), ptr(nc), ptr(nl), ptr(lv))
@@ -142,9 +167,9 @@ class Tree:
         self.n_nodes = m
         self.first, self.count, self.leaf, self.level = nf[:m].copy(), nc[:m].copy(), nl[:m].copy(), lv[:m].copy()
         self.boxes = np.zeros((max(m, 1), 2 * D), dtype=np.float32)
-        if layout == "bvh":
-            L.orc_tree_boxes(ptr(self.sobjs), D, self.is_box, F, m, ptr(self.first),
-                             ptr(self.count), ptr(self.leaf), ptr(self.boxes))
+        if layout in ("bvh", "guttman"):
+            L.orc_tree_boxes(ptr(self.sobjs), D, self.is_box, 0 if layout == "guttman" else F, m,
+                             ptr(self.first), ptr(self.count), ptr(self.leaf), ptr(self.boxes))
         else:
             got = L.orc_pack(ptr(self.sobjs), self.n, D, self.is_box, F, ptr(self.boxes))
             assert got == m
@@ -291,6 +316,8 @@ def build_kmeans(objs, D, is_box, F, key_mode, K, iters, init_idx, key_src=SRC_L
     k, cid = kmeans_keys(objs, D, is_box, key_mode, K, iters, init_idx, key_src)
     perm = sort_stable(k)
     t = Tree(objs[perm], D, is_box, F, layout=layout)
+    if layout == "guttman":
+        perm = perm[t.order]
     t.keys = k[perm]
     t.perm = perm
     t.cid = cid
@@ -298,12 +325,14 @@ def build_kmeans(objs, D, is_box, F, key_mode, K, iters, init_idx, key_src=SRC_L
 
 
 def build_packed(objs, D, is_box, F, key_mode, key_src=SRC_LO, layout="packed"):
-    """Full oracle build: keys -> stable sort -> packed or BVH tree."""
+    """Full oracle build: keys -> stable sort -> packed, BVH or Guttman tree."""
     objs = np.ascontiguousarray(objs, dtype=np.float32)
     k, bad = keys(objs, D, is_box, key_mode, key_src)
     perm = sort_stable(k)
     sobjs = objs[perm]
     t = Tree(sobjs, D, is_box, F, layout=layout)
+    if layout == "guttman":
+        perm = perm[t.order]
     t.keys = k[perm]
     t.perm = perm
     t.domain_error = bad

@@ -1,6 +1,7 @@
 """The command-line driver (reference-flag compatible): generated data and
-queries, every index type, the plain / persistent / co-processing forms, index
-save + load, and the autotuner, all checked against the CPU oracle."""
+queries, every index type (-i r is the reference's Guttman R-tree), the plain /
+persistent / co-processing forms, index save + load, and the autotuner, all
+checked against the CPU oracle."""
 import os
 import subprocess
 
@@ -38,8 +39,9 @@ def files(tmp_path_factory):
 
 @pytest.mark.parametrize("itype,layout,extra", [
     ("m", "packed", []), ("m", "packed", ["--mode", "plain", "-b", "64"]),
-    ("r", "packed", []), ("b", "bvh", []), ("h", "packed", ["-p", "4", "-c", "400"]),
-    ("hb", "bvh", ["-l", "2"]), ("m", "packed", ["--contain"])])
+    ("r", "guttman", []), ("pr", "packed", []), ("b", "bvh", []),
+    ("h", "packed", ["-p", "4", "-c", "400"]), ("hb", "bvh", ["-l", "2"]),
+    ("hr", "guttman", ["-l", "1"]), ("m", "packed", ["--contain"])])
 def test_cli_search_matches_oracle(gpu, O, files, itype, layout, extra):
     d, data, queries, pts, q = files
     out = str(d / ("out_%s_%d.txt" % (itype, len(extra))))
