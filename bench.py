@@ -221,7 +221,7 @@ def oracle_tree_from(ix):
     t.sobjs = np.ascontiguousarray(ex["sobjs"])
     t.n = inf["n"]
     t.D, t.is_box, t.F = inf["dims"], int(inf["obj_kind"] == 1), inf["fanout"]
-    t.layout = "bvh" if inf["layout"] == 2 else "packed"
+    t.layout = {2: "bvh", 3: "guttman"}.get(inf["layout"], "packed")
     t.n_nodes = inf["n_nodes"]
     t.first, t.count, t.leaf = ex["first"], ex["count"], ex["leaf"]
     t.boxes = np.ascontiguousarray(ex["boxes"])
@@ -287,6 +287,38 @@ def run_config(qcp, name, args):
                     "total_hits": int(exp["counts"].sum()), "bit_exact_ids": ok,
                     "build_ms": ix.info()["build_ms"],
                     "cpu_baseline": cpu_sample(t, q, got, 5.0, threads)})
+        ix.close()
+        # the reference's R-tree: Guttman insertion (host) at its degree 128, GPU search
+        log("c1 guttman build")
+        gx = qcp.Index.build(pts, 2, fanout=128, layout="guttman")
+        ginf = gx.info()
+        gt = oracle_tree_from(gx)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+
+        ot = O.build_packed(pts, 2, False, 128, O.KEY_REF, layout="guttman")
+        gexp = ot.search_dfs(q, want_ids=True)
+        with gx.batch(q) as b:
+            gst, gwall, gkms = timed_search(b, args.steps, ids=True)
+            ggot = b.results(ids=True)
+            hst, hwall, hkms = timed_search(b, args.steps)
+        same_tree = bool(np.array_equal(gt.first, ot.first) and np.array_equal(gt.count, ot.count)
+                         and np.array_equal(gt.sobjs, ot.sobjs))
+        gok = bool(np.array_equal(ggot["counts"], gexp["counts"]) and
+                   np.array_equal(ggot["ids"], gexp["ids"]))
+        out["guttman"] = {
+            "workload": "the same points and queries, Guttman R-tree (REF keys, capacity 128, "
+                        "minimum fill 64), depth-first IDs",
+            "nodes": ginf["n_nodes"], "height": ginf["height"],
+            "build_ms": ginf["build_ms"], "host_insertion_ms": ginf["build_phase_ms"][4],
+            "persistent_ids_qps": len(q) / gwall, "persistent_ids_device_ms": gkms,
+            "persistent_counts_qps": len(q) / hwall, "persistent_counts_device_ms": hkms,
+            "tree_matches_oracle": same_tree, "bit_exact_ids": gok,
+            "cpu_baseline": cpu_sample(gt, q, ggot, 5.0, threads)}
+        log("c1 guttman: build %.0f ms (host %.0f ms), %.3g q/s ids, %.3g q/s counts, "
+            "tree %s, ids %s" % (ginf["build_ms"], ginf["build_phase_ms"][4], len(q) / gwall,
+                                 len(q) / hwall, same_tree, gok))
+        gx.close()
         return out
     if name == "c3":
         w = qcp.workload("c3", scale=args.scale)
