@@ -11,8 +11,12 @@
 // Reference flags (upper- and lower-case alike, as in the reference):
 //   -i TYPE   m|mphr, r|rtree (the reference RTree: Guttman insertion), b|bvh,
 //             h|hybrid (MPHR, co-processing search), hr (Guttman R-tree,
-//             co-processing), hb (BVH, co-processing); pr|packed_rtree selects
-//             this build's Hilbert-packed R-tree
+//             co-processing), hb (BVH, co-processing), rls|rtree_ls (RTree_LS,
+//             src/tree/rtree_ls.cpp: a Guttman tree over the objects in input
+//             order -- RTree_LS::Build never sorts -- whose level-1 nodes are
+//             its large leaves, searched by host threads down to the leaves and
+//             GPU leaf scans); pr|packed_rtree selects this build's
+//             Hilbert-packed R-tree
 //   -d N      number of data objects (a trailing "m" multiplies by 10^6)
 //   -q Q      number of queries
 //   -b B      queries per launch of the plain form (number_of_cuda_blocks)
@@ -108,7 +112,7 @@ int fail(const char* what, qcp_status s) {
 
 void usage() {
   fprintf(stderr,
-          "usage: qcp_cli -i m|r|pr|b|h|hr|hb -d N -q Q [-b B -p P -l L -c C -r R -e E -s S]\n"
+          "usage: qcp_cli -i m|r|pr|b|h|hr|hb|rls -d N -q Q [-b B -p P -l L -c C -r R -e E -s S]\n"
           "               --data FILE --queries FILE [--dims D --fanout F --boxes --center\n"
           "               --key ref|b16 --mode persistent|plain|hybrid --index FILE\n"
           "               --out FILE --contain --json]\n"
@@ -209,6 +213,7 @@ int main(int argc, char** argv) {
   std::string t = o.index_type;
   for (auto& ch : t) ch = (char)tolower(ch);
   uint32_t layout = QCP_LAYOUT_MPHR;
+  bool input_order = false;  // index the objects in the given order (no keys, no sort)
   std::string mode = o.mode;
   if (t == "m" || t == "mphr") layout = QCP_LAYOUT_MPHR;
   else if (t == "r" || t == "rtree") layout = QCP_LAYOUT_GUTTMAN;
@@ -217,6 +222,11 @@ int main(int argc, char** argv) {
   else if (t == "h" || t == "hybrid") { layout = QCP_LAYOUT_MPHR; if (mode.empty()) mode = "hybrid"; }
   else if (t == "hr") { layout = QCP_LAYOUT_GUTTMAN; if (mode.empty()) mode = "hybrid"; }
   else if (t == "hb") { layout = QCP_LAYOUT_BVH; if (mode.empty()) mode = "hybrid"; }
+  else if (t == "rls" || t == "rtree_ls") {
+    layout = QCP_LAYOUT_GUTTMAN;
+    input_order = true;
+    if (mode.empty()) mode = "hybrid";
+  }
   else {
     fprintf(stderr, "qcp_cli: index type %s has no counterpart here\n", o.index_type.c_str());
     return 2;
@@ -260,7 +270,7 @@ int main(int argc, char** argv) {
                          QCP_ORDER_HILBERT, o.clusters, 0};
     std::string ct = o.cluster;
     for (auto& ch : ct) ch = (char)tolower(ch);
-    const bool original = ct == "o" || ct == "original";
+    const bool original = input_order || ct == "o" || ct == "original";
     if (ct == "k" || ct == "kmeans") cfg.order = QCP_ORDER_KMEANS_HILBERT;
     else if (!original && ct != "h" && ct != "hilbert") { usage(); return 2; }
     qcp_search_cfg tuned;

@@ -41,14 +41,18 @@ def files(tmp_path_factory):
     ("m", "packed", []), ("m", "packed", ["--mode", "plain", "-b", "64"]),
     ("r", "guttman", []), ("pr", "packed", []), ("b", "bvh", []),
     ("h", "packed", ["-p", "4", "-c", "400"]), ("hb", "bvh", ["-l", "2"]),
-    ("hr", "guttman", ["-l", "1"]), ("m", "packed", ["--contain"])])
+    ("hr", "guttman", ["-l", "1"]), ("rls", "guttman-input", ["-p", "4"]),
+    ("m", "packed", ["--contain"])])
 def test_cli_search_matches_oracle(gpu, O, files, itype, layout, extra):
     d, data, queries, pts, q = files
     out = str(d / ("out_%s_%d.txt" % (itype, len(extra))))
     log = run("-i", itype, "--data", data, "--queries", queries, "--dims", "3", "--fanout", "64",
               "--out", out, "-r", "2", *extra)
     assert "Hit :" in log
-    t = O.build_packed(pts, 3, False, 64, O.KEY_REF, layout=layout)
+    if layout == "guttman-input":  # RTree_LS: Guttman insertion in the input order
+        t = O.Tree(pts, 3, False, 64, layout="guttman")
+    else:
+        t = O.build_packed(pts, 3, False, 64, O.KEY_REF, layout=layout)
     exp = t.search_dfs(q)
     c, h = read_out(out)
     assert np.array_equal(c, exp["counts"]) and np.array_equal(h, exp["hashes"])
