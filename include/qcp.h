@@ -144,7 +144,8 @@ int qcp_device_count(void);
 
 /* Build an index on the GPU: Hilbert keys, stable radix sort by (key, input
  * index), rank-ordered SoA objects, bottom-up (MPHR/RTREE) or rank-prefix
- * (BVH) node boxes, all resident in HBM.  Replaces tree::Tree::Build
+ * (BVH) node boxes, all resident in HBM (GUTTMAN: the host inserts the sorted
+ * objects and the tree is then uploaded in depth-first order).  Replaces tree::Tree::Build
  * (src/tree/tree.h:24) as implemented by MPHR::Build (src/tree/mphr.cpp:35-136),
  * BVH::Build (src/tree/bvh.cpp:25-63) and RTree::Build (src/tree/rtree.cpp:25-63),
  * together with the upload of manager::ChunkManager::Init/CopyNode

@@ -1,0 +1,53 @@
+"""The C++ tree::Tree mirror (include/qcp_tree.hpp): a compiled program
+(tests/cpp/test_tree_api.cpp) builds every tree type from a raw data file the
+way the reference's Evaluator does, searches with repeats, dumps and reloads,
+and exercises the error paths; its per-query counts and ID-set hashes are
+compared here with the oracle's build of the same layout."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import rand_points, rand_queries
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PROG = os.path.join(ROOT, "query-co-processing-on-heterogeneous-architectures_amd", "_build",
+                    "test_tree_api")
+
+
+def _read(path):
+    a = np.loadtxt(path, dtype=np.uint64, ndmin=2)
+    return a[:, 0].astype(np.uint32), a[:, 1]
+
+
+def test_tree_api_program(gpu, O, tmp_path):
+    D, n, q = 3, 50000, 700
+    pts = rand_points(n, D, 500, dup_frac=0.1)
+    qs = rand_queries(q, D, 501, pts)
+    data, queries = str(tmp_path / "data.bin"), str(tmp_path / "q.bin")
+    pts.astype(np.float32).tofile(data)
+    qs.astype(np.float32).tofile(queries)
+    print("run", PROG, flush=True)
+    r = subprocess.run([PROG, data, queries, str(D), str(n), str(q), str(tmp_path)],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    oracles = {
+        "m": O.build_packed(pts, D, False, 192, O.KEY_REF),
+        "r": O.build_packed(pts, D, False, 128, O.KEY_REF, layout="guttman"),
+        "b": O.build_packed(pts, D, False, 128, O.KEY_REF, layout="bvh"),
+        "pr": O.build_packed(pts, D, False, 128, O.KEY_REF),
+        "rls": O.Tree(pts, D, False, 128, layout="guttman"),  # the given order
+    }
+    exp = {k: t.search_dfs(qs) for k, t in oracles.items()}
+    exp["h"], exp["hr"], exp["hb"] = exp["m"], exp["r"], exp["b"]
+    exp["m_plain"], exp["hb1"] = exp["m"], exp["b"]
+    for ty, e in exp.items():
+        c, h = _read(str(tmp_path / (ty + ".txt")))
+        assert np.array_equal(c, e["counts"]), ty + " counts"
+        assert np.array_equal(h, e["hashes"]), ty + " hashes"
+    for ty in ("m", "r", "b", "h", "hr", "hb", "rls", "pr"):
+        assert "%s: Hit : %d" % (ty, int(exp[ty]["counts"].sum())) in r.stdout
