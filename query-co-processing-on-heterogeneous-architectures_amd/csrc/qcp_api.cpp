@@ -886,7 +886,7 @@ unsigned qcp::persistent_grid(const qcp_index* ix, int out, unsigned block) {
   const void* k = search_kernel_ptr(ix->D, ix->box, ix->expl, out, mw);
   int per_cu = 0;
   if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, k, (int)block, search_lds_bytes(ix->expl, mw, block)) != hipSuccess ||
+                &per_cu, k, (int)block, search_lds_bytes(ix->expl, mw, out, block)) != hipSuccess ||
       per_cu <= 0)
     per_cu = 2;
   return (unsigned)(ix->cus * per_cu);
@@ -1094,7 +1094,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     a.donate = 0;
     unsigned grid2 = fit_grid(persistent_grid(ix, 2, block));
     HIP_TRY(hipEventRecord(b->ev[2], s));
-    HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, 2, 4, grid2, block, s));
+    HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, 2, u >= 8 ? 8 : 4, grid2, block, s));
     HIP_TRY(hipEventRecord(b->ev[3], s));
     st.launches = 2;
     b->has_offsets = true;

@@ -55,6 +55,6 @@ hipError_t launch_mprs(const SearchArgs& a, unsigned D, bool box, unsigned nbloc
                        hipStream_t s);
 const void* search_kernel_ptr(unsigned D, bool box, bool expl, int out, int mw);
 // dynamic LDS bytes of a persistent-search workgroup of `block` threads
-size_t search_lds_bytes(bool expl, int mw, unsigned block);
+size_t search_lds_bytes(bool expl, int mw, int out, unsigned block);
 
 }  // namespace qcp

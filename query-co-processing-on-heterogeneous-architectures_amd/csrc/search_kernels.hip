@@ -114,6 +114,13 @@ struct Ctx {
   uint32_t q;
   uint32_t cnt;           // hits of this item
   uint64_t out_base;      // ID write pass: next output slot
+  // ID write pass: IDs are staged in a per-wave LDS window of 2 x 64 slots
+  // holding output slots [wbase, wbase + 128), wbase a multiple of 64; this
+  // query owns the slots from wlo on.  Whole aligned 64-slot blocks are
+  // written with one aligned 256-byte store each, so only the blocks at a
+  // query's two ends are partial.
+  uint32_t* idb;
+  uint64_t wbase, wlo;
   uint64_t s1_lane;       // per-lane partial sum of hit IDs
   uint64_t s2_lane;       // per-lane partial sum of squared hit IDs
   uint64_t s1c, s2c;      // power sums of the IDs of contained subtrees
@@ -157,6 +164,33 @@ __device__ __forceinline__ void eval_children(const SearchArgs& a, const Ctx<D>&
   }
 }
 
+// ID write pass: write the staged slots [wlo, out_base) of the window's first
+// block (a query's last block, or the block before a directly written run).
+template <int D>
+__device__ __forceinline__ void flush_partial(const SearchArgs& a, Ctx<D>& c) {
+  const uint32_t lane = lane_id();
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t v = c.idb[lane];
+  const uint64_t pos = c.wbase + lane;
+  if (pos >= c.wlo && pos < c.out_base) __builtin_nontemporal_store(v, a.ids + pos);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// ID write pass: the window's first block is complete; write it (from wlo on)
+// and slide the window by one block.
+template <int D>
+__device__ __forceinline__ void flush_block(const SearchArgs& a, Ctx<D>& c) {
+  const uint32_t lane = lane_id();
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t v = c.idb[lane];
+  const uint32_t v2 = c.idb[kWave + lane];
+  if (c.wbase + lane >= c.wlo) __builtin_nontemporal_store(v, a.ids + c.wbase + lane);
+  __builtin_amdgcn_wave_barrier();
+  c.idb[lane] = v2;
+  c.wbase += kWave;
+  c.wlo = c.wbase;
+}
+
 // Report the ranks [s, e) as hits without reading them (a run of children
 // whose boxes lie inside the query).
 template <int D, int OUT>
@@ -170,6 +204,7 @@ __device__ __forceinline__ void emit_range(const SearchArgs& a, Ctx<D>& c, uint6
     c.s2c += psum2((uint32_t)e) - psum2((uint32_t)s);
   } else if (OUT == kOutIds) {
     const uint32_t lane = lane_id();
+    flush_partial<D>(a, c);  // staged IDs before the run, then the run directly
     uint32_t* dst = a.ids + c.out_base;
     // dword stores up to a 16-byte boundary, then 16-byte stores, then the tail
     const uint64_t head = ((4 - (c.out_base & 3)) & 3) < len ? ((4 - (c.out_base & 3)) & 3) : len;
@@ -187,6 +222,8 @@ __device__ __forceinline__ void emit_range(const SearchArgs& a, Ctx<D>& c, uint6
     const uint64_t t0 = head + body;
     if (t0 + lane < len) dst[t0 + lane] = (uint32_t)(s + 1 + t0 + lane);
     c.out_base += len;
+    c.wbase = c.out_base & ~(uint64_t)(kWave - 1);
+    c.wlo = c.out_base;
   }
 }
 
@@ -198,10 +235,12 @@ __device__ __forceinline__ void chunk_out(const SearchArgs& a, Ctx<D>& c, uint32
   const uint64_t b = ballot(hit);
   const uint32_t nb = (uint32_t)__popcll(b);
   if (OUT == kOutIds) {
-    // the ID stream is written once and read back only by the host:
-    // nontemporal (streaming) stores keep it from displacing the tree in L2
-    if (hit) __builtin_nontemporal_store(st + lane + 1u, (a.ids + c.out_base) + rank_below(b));
+    // staged in LDS, written in aligned blocks (nontemporal: the ID stream is
+    // written once and read back only by the host, so it should not displace
+    // the tree in L2)
+    if (hit) c.idb[(uint32_t)(c.out_base - c.wbase) + rank_below(b)] = st + lane + 1u;
     c.out_base += nb;
+    if (c.out_base - c.wbase >= kWave) flush_block<D>(a, c);
   } else if (OUT == kOutHash) {
     // branch-free: a select, a 64-bit add and one v_mad_u64_u32 per chunk
     const uint32_t id = st + lane + 1u;
@@ -558,6 +597,10 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
   constexpr int kDepth = EXPL ? kMaxDepth : kMaxLevels;
   extern __shared__ unsigned char qcp_lds[];
   Frame<MW>* stk = reinterpret_cast<Frame<MW>*>(qcp_lds) + (threadIdx.x / kWave) * kDepth;
+  // ID pass: each wave's 2 x 64-slot output window follows all frame stacks
+  uint32_t* idb = reinterpret_cast<uint32_t*>(reinterpret_cast<Frame<MW>*>(qcp_lds) +
+                                              (blockDim.x / kWave) * kDepth) +
+                  (threadIdx.x / kWave) * (2 * kWave);
   const uint32_t lane = lane_id();
   __shared__ unsigned long long wg_stats[4];
   if (threadIdx.x < 4) wg_stats[threadIdx.x] = 0;
@@ -639,8 +682,12 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
     c.ie = 0;
     c.le = 0;
     c.out_base = (OUT == kOutIds) ? a.offsets[q] : 0;
+    c.idb = idb;
+    c.wbase = c.out_base & ~(uint64_t)(kWave - 1);
+    c.wlo = c.out_base;
     load_query<D>(a, q, c);
     if (a.h > 0) walk<D, BOX, MW, EXPL, OUT, U>(a, c, node, stk);
+    if (OUT == kOutIds) flush_partial<D>(a, c);
     if (OUT != kOutIds) {
       // the hash is additive over disjoint ID sets, so donated parts add up
       uint64_t hsum = 0;
@@ -822,18 +869,21 @@ __global__ void k_reset_ctl(QueueCtl* ctl, uint32_t pending) {
     default: { constexpr int U = 4; __VA_ARGS__; } break; \
   }
 
-// Dynamic LDS of a persistent-search workgroup: one frame stack per wave.
-template <int MW, bool EXPL>
+// Dynamic LDS of a persistent-search workgroup: one frame stack per wave,
+// and for the ID pass one 2 x 64-slot output window per wave after them.
+template <int MW, bool EXPL, int OUT>
 static size_t lds_for(unsigned block) {
-  return (size_t)(block / kWave) * (EXPL ? kMaxDepth : kMaxLevels) * sizeof(Frame<MW>);
+  return (size_t)(block / kWave) * ((EXPL ? kMaxDepth : kMaxLevels) * sizeof(Frame<MW>) +
+                                    (OUT == kOutIds ? 2 * kWave * sizeof(uint32_t) : 0));
 }
 
-size_t search_lds_bytes(bool expl, int mw, unsigned block) {
+size_t search_lds_bytes(bool expl, int mw, int out, unsigned block) {
+  const size_t ids = out == kOutIds ? (size_t)(block / kWave) * 2 * kWave * sizeof(uint32_t) : 0;
   switch (mw) {
-    case 1: return expl ? lds_for<1, true>(block) : lds_for<1, false>(block);
-    case 2: return expl ? lds_for<2, true>(block) : lds_for<2, false>(block);
-    case 3: return expl ? lds_for<3, true>(block) : lds_for<3, false>(block);
-    default: return expl ? lds_for<4, true>(block) : lds_for<4, false>(block);
+    case 1: return ids + (expl ? lds_for<1, true, 0>(block) : lds_for<1, false, 0>(block));
+    case 2: return ids + (expl ? lds_for<2, true, 0>(block) : lds_for<2, false, 0>(block));
+    case 3: return ids + (expl ? lds_for<3, true, 0>(block) : lds_for<3, false, 0>(block));
+    default: return ids + (expl ? lds_for<4, true, 0>(block) : lds_for<4, false, 0>(block));
   }
 }
 
@@ -843,11 +893,15 @@ static hipError_t launch_search_t(const SearchArgs& a, int mw, int u, unsigned g
   if (block % kWave || block < kWave || block > 512) return hipErrorInvalidValue;
   if (!EXPL && OUT != kOutIds) {
     QCP_U_SWITCH(u, QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, OUT, U>
-                                  <<<grid, block, (lds_for<MW, EXPL>(block)), s>>>(a)));
+                                  <<<grid, block, (lds_for<MW, EXPL, OUT>(block)), s>>>(a)));
+  } else if (!EXPL && u >= 8) {  // ID pass, packed layouts: 4 or 8 chunks in flight
+    constexpr int U = 8;
+    QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, OUT, U>
+                      <<<grid, block, (lds_for<MW, EXPL, OUT>(block)), s>>>(a));
   } else {
     constexpr int U = 4;
     QCP_MW_SWITCH(mw, k_search<D, BOX, MW, EXPL, OUT, U>
-                      <<<grid, block, (lds_for<MW, EXPL>(block)), s>>>(a));
+                      <<<grid, block, (lds_for<MW, EXPL, OUT>(block)), s>>>(a));
   }
   return hipGetLastError();
 }
