@@ -110,7 +110,9 @@ typedef struct {
                               inside the query box as hits without visiting it (count from
                               its rank range, hash in closed form, IDs written as a run).
                               Same results; the stats then count the pruned traversal.  Not
-                              applied to box indexes holding a box with lo > hi. */
+                              applied to box indexes holding a box with lo > hi.  (The count
+                              pass of an ID search, which only sizes the output, always uses
+                              it; an ID search's stats describe its ID pass.) */
   uint32_t cpu_threads;    /* hybrid form: host threads walking the upper tree (0 = all cores) */
   uint32_t handoff_level;  /* hybrid form: tree level whose overlapping nodes the host hands
                               to the GPU (1 = the root's children; 0 or beyond = the leaves,

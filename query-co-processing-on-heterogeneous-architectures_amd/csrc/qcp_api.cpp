@@ -1007,7 +1007,12 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   const bool order_queries = cfg.order == 0 && q >= kOrderMinQueries;
   unsigned grid = fit_grid(persistent_grid(ix, out_mode, block));
   SearchArgs a = make_args(b);
-  a.contain = (cfg.contain && ix->contain_ok) ? 1u : 0u;
+  const bool want_ids = (cfg.outputs & QCP_OUT_IDS) != 0;
+  // The count pass of an ID search only sizes the output: the containment
+  // shortcut gives the same counts and hashes with fewer reads, so it is always
+  // on there; the traversal statistics then come from the ID pass, which
+  // follows the caller's choice.
+  a.contain = ((cfg.contain || want_ids) && ix->contain_ok) ? 1u : 0u;
   if (donate) {
     if (!b->d_slots) {
       b->slot_cap = kSlotCap;
@@ -1077,7 +1082,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   }
   st.launches = 1;
   // queue statistics of the count pass (the ID pass resets the control block)
-  QueueCtl ctl;
+  QueueCtl ctl, ctl_ids;
   HIP_TRY(hipMemcpyAsync(&ctl, b->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
   if (cfg.outputs & QCP_OUT_IDS) {
     HIP_TRY(scan_u32_u64(b->d_counts, q, b->d_offsets, b->d_scan_sums, kMaxScanBlocks, s));
@@ -1092,10 +1097,12 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     }
     a.ids = b->d_ids;
     a.donate = 0;
+    a.contain = (cfg.contain && ix->contain_ok) ? 1u : 0u;
     unsigned grid2 = fit_grid(persistent_grid(ix, 2, block));
     HIP_TRY(hipEventRecord(b->ev[2], s));
     HIP_TRY(launch_search(a, ix->D, ix->box, ix->expl, 2, u >= 8 ? 8 : 4, grid2, block, s));
     HIP_TRY(hipEventRecord(b->ev[3], s));
+    HIP_TRY(hipMemcpyAsync(&ctl_ids, b->d_ctl, sizeof(ctl_ids), hipMemcpyDeviceToHost, s));
     st.launches = 2;
     b->has_offsets = true;
   }
@@ -1117,10 +1124,11 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     st.search_ms = ks;
   }
   for (int x = 0; x < 8; ++x) {
-    st.internal_entries += ctl.internal_entries[x];
-    st.leaf_entries += ctl.leaf_entries[x];
-    st.total_hits += ctl.total_hits[x];
-    st.contained_runs += ctl.contained[x];
+    const QueueCtl& tc = want_ids ? ctl_ids : ctl;  // the traversal the caller chose
+    st.internal_entries += tc.internal_entries[x];
+    st.leaf_entries += tc.leaf_entries[x];
+    st.total_hits += tc.total_hits[x];
+    st.contained_runs += tc.contained[x];
   }
   st.donated_items = ctl.donated;
   b->total_hits = st.total_hits;
