@@ -435,6 +435,28 @@ TUNE_WAVES = (1, 2, 4, 8)
 TUNE_CHUNKS = (2, 4, 8)
 
 
+def profile_ids(qcp, args):
+    """A short, fixed run of the ID-list search on the 16M-point config (for
+    kernel traces and counters): F=128, 8-wave workgroups, 8 leaf chunks,
+    `steps` searches with sorted ID lists without, then with, the containment
+    shortcut.  Prints one JSON line with each search's pass times."""
+    w = qcp.workload("c2", scale=args.scale)
+    ix = qcp.Index.build(w["objs"], 3, fanout=128)
+    runs = []
+    with ix.batch(w["queries"]) as b:
+        for contain in (False, True):
+            for _ in range(args.steps):
+                st = b.run(ids=True, contain=contain, block_waves=8, chunks=8)
+                runs.append({"contain": contain, "kernel_ms": st["kernel_ms"],
+                             "count_pass_ms": st["search_ms"],
+                             "id_pass_ms": st["kernel_ms"] - st["search_ms"],
+                             "ids": int(st["total_hits"])})
+                log("ids contain=%d: count pass %.3f ms, ID pass %.3f ms" %
+                    (contain, st["search_ms"], st["kernel_ms"] - st["search_ms"]))
+    ix.close()
+    print(json.dumps({"profile": "id lists", "runs": runs}), flush=True)
+
+
 def profile_traffic(qcp, args):
     """A short, fixed run for counter collection on the 16M-point config: with
     --sweep, `steps` persistent searches (counts+hash) of every configuration
@@ -517,6 +539,8 @@ def main():
     ap.add_argument("--profile-traffic", action="store_true",
                     help="short run for rocprofv3 --pmc: the search at the given fanout, "
                          "then a 1 GiB device-to-device copy as the calibration kernel")
+    ap.add_argument("--profile-ids", action="store_true",
+                    help="short ID-list search run on the 16M-point config (for rocprofv3)")
     ap.add_argument("--block-waves", type=int, default=0, help="--profile-traffic: waves/WG")
     ap.add_argument("--chunks", type=int, default=0, help="--profile-traffic: leaf chunks")
     ap.add_argument("--donate", type=int, default=0, help="--profile-traffic: donation")
@@ -539,6 +563,9 @@ def main():
         return
     if args.profile_traffic:
         profile_traffic(qcp, args)
+        return
+    if args.profile_ids:
+        profile_ids(qcp, args)
         return
     log("generating workload c2 (scale %.3f)" % args.scale)
     w = qcp.workload("c2", scale=args.scale)
