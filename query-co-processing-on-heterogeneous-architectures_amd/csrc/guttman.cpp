@@ -298,7 +298,8 @@ void build_d(const float* objs, uint64_t n, bool box, uint32_t F, GuttmanTree* o
 
 bool guttman_build(const float* objs, uint64_t n, uint32_t D, bool box, uint32_t F,
                    GuttmanTree* out) {
-  if (F < 2 || (D != 2 && D != 3) || n >= 0xFFFFFFFFull) return false;
+  // a minimum fill of 1 (F < 4) lets single-child chains grow without bound
+  if (F < 4 || (D != 2 && D != 3) || n >= 0xFFFFFFFFull) return false;
   if (n == 0) {
     out->first.clear();
     out->count.clear();

@@ -27,7 +27,7 @@ struct GuttmanTree {
 
 // Inserts objs[i] (n rows of E floats: points [p(D)], boxes [lo(D), hi(D)]) in
 // index order into a tree of node capacity F (minimum fill F/2).  n == 0
-// gives an empty tree.  Returns false for unsupported D or F < 2.
+// gives an empty tree.  Returns false for D other than 2 or 3, or F < 4.
 bool guttman_build(const float* objs, uint64_t n, uint32_t D, bool box, uint32_t F,
                    GuttmanTree* out);
 
