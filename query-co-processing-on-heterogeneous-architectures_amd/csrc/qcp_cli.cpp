@@ -33,7 +33,8 @@
 //             (one tree per index, data from --data); -f rebuilds even when
 //             --index exists
 // Options of this driver:
-//   --data FILE --queries FILE --dims D --fanout F --boxes --center
+//   --data FILE --queries FILE --dims D --fanout F (default: the reference's
+//   degree, 192 for MPHR, 128 otherwise) --boxes --center
 //   --key ref|b16 --mode persistent|plain|hybrid --index FILE (load/save)
 //   --out FILE (per-query count and hash, text) --contain --json
 //   --gen-data FILE --gen-queries FILE --seed S
@@ -62,7 +63,7 @@ struct Opts {
   double selectivity = 1e-4;
   bool force = false;
   std::string data, queries, index_file, out, gen_data, gen_queries;
-  uint32_t dims = 3, fanout = 64;
+  uint32_t dims = 3, fanout = 0;  // 0: the reference's degree of the index type
   bool boxes = false, center = false, b16 = false, contain = false, json = false;
   std::string mode;
   uint64_t seed = 1234;
@@ -231,6 +232,9 @@ int main(int argc, char** argv) {
     fprintf(stderr, "qcp_cli: index type %s has no counterpart here\n", o.index_type.c_str());
     return 2;
   }
+  // the reference's compile-time degrees (src/common/config.h:6-8): MPHR leaves
+  // 192, R-tree / BVH nodes 128
+  if (!o.fanout) o.fanout = layout == QCP_LAYOUT_MPHR ? 192u : 128u;
   if (mode.empty()) mode = "persistent";
   uint32_t smode = QCP_SEARCH_PERSISTENT;
   if (mode == "plain") smode = QCP_SEARCH_PLAIN;

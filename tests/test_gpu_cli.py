@@ -97,3 +97,17 @@ def test_cli_cluster_types(gpu, O, files, u):
     exp = O.build_packed(pts, 3, False, 64, O.KEY_REF).search_dfs(q)
     c, _ = read_out(out)
     assert np.array_equal(c, exp["counts"])
+
+
+def test_cli_reference_degrees(gpu, O, files):
+    """Without --fanout each index type takes the reference's degree: 192 for
+    MPHR, 128 for the R-tree (whose depth-first IDs, and so hashes, depend on it)."""
+    d, data, queries, pts, q = files
+    log = run("-i", "m", "--data", data, "--queries", queries, "--dims", "3")
+    assert "fanout 192" in log
+    out = str(d / "o_r128.txt")
+    log = run("-i", "r", "--data", data, "--queries", queries, "--dims", "3", "--out", out)
+    assert "fanout 128" in log
+    exp = O.build_packed(pts, 3, False, 128, O.KEY_REF, layout="guttman").search_dfs(q)
+    c, h = read_out(out)
+    assert np.array_equal(c, exp["counts"]) and np.array_equal(h, exp["hashes"])
