@@ -457,6 +457,31 @@ def profile_ids(qcp, args):
     print(json.dumps({"profile": "id lists", "runs": runs}), flush=True)
 
 
+def profile_c5(qcp, args):
+    """A short, fixed run on the 134M-point 2-D skewed config (objects 1.07 GB,
+    larger than the 256 MB Infinity Cache), for kernel traces and HBM counters:
+    F=64, the first 50,000 queries, `steps` canonical persistent searches
+    (counts+hash, donation on).  Prints one JSON line with each launch's
+    algorithmic bytes and device time."""
+    w = qcp.workload("c5", scale=args.scale)
+    buf = qcp.DeviceBuffer(w["objs"])
+    ix = qcp.Index.build(buf, 2, fanout=64)
+    q = w["queries"][:50000]
+    launches = []
+    with ix.batch(q) as b:
+        for _ in range(args.steps):
+            st = b.run(donate=1)
+            launches.append({"algorithmic_bytes": algorithmic_bytes(st, len(q), 2, 2),
+                             "search_ms": st["search_ms"], "kernel_ms": st["kernel_ms"],
+                             "total_hits": int(st["total_hits"])})
+            log("c5 profile: %.3f ms search kernel, %.3g algorithmic bytes" %
+                (st["search_ms"], launches[-1]["algorithmic_bytes"]))
+    ix.close()
+    buf.close()
+    print(json.dumps({"profile": "c5 search", "queries": len(q), "launches": launches}),
+          flush=True)
+
+
 def profile_traffic(qcp, args):
     """A short, fixed run for counter collection on the 16M-point config: with
     --sweep, `steps` persistent searches (counts+hash) of every configuration
@@ -539,6 +564,8 @@ def main():
     ap.add_argument("--profile-traffic", action="store_true",
                     help="short run for rocprofv3 --pmc: the search at the given fanout, "
                          "then a 1 GiB device-to-device copy as the calibration kernel")
+    ap.add_argument("--profile-c5", action="store_true",
+                    help="short search run on the 134M-point 2-D config (for rocprofv3)")
     ap.add_argument("--profile-ids", action="store_true",
                     help="short ID-list search run on the 16M-point config (for rocprofv3)")
     ap.add_argument("--block-waves", type=int, default=0, help="--profile-traffic: waves/WG")
@@ -566,6 +593,9 @@ def main():
         return
     if args.profile_ids:
         profile_ids(qcp, args)
+        return
+    if args.profile_c5:
+        profile_c5(qcp, args)
         return
     log("generating workload c2 (scale %.3f)" % args.scale)
     w = qcp.workload("c2", scale=args.scale)
