@@ -66,9 +66,9 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
 }
 
 // ---------------------------------------------------------------------------
-// Radix sort.  Tile = 256 threads x 16 keys; key i of a tile sits at
-// tile_start + it*256 + tid, so each 256-key sub-tile is in index order and
-// LSD passes stay stable.
+// Radix sort.  Tile = 4096 consecutive keys (256 threads x 16); a pass is
+// stable because each tile is ranked in index order and tiles are placed in
+// tile order.
 constexpr int kSortThreads = 256;
 constexpr int kSortKPT = 16;
 constexpr int kSortTile = kSortThreads * kSortKPT;
@@ -379,41 +379,50 @@ hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
   return scan_impl<uint32_t, unsigned long long>(in, n, out, block_sums, max_blocks, s);
 }
 
-// Stable scatter of one 4096-key tile for one 8-bit digit: the tile is first
-// ranked into LDS in (digit, index) order -- wave multisplit (8 ballots) per
-// 256 keys, buckets placed at the tile's own digit offsets -- and then written
-// out in that order, so the keys of one bucket go to consecutive global
-// positions (runs of ~16 keys) instead of one scattered 8-byte store each.
+// Stable scatter of one 4096-key tile for one 8-bit digit.  Each of the 4
+// waves owns 1024 consecutive keys of the tile (16 chunks of 64) and loads them
+// all up front; it ranks each chunk by digit with a wave multisplit (8
+// ballots) against wave-private LDS counters, keeping keys and ranks in
+// registers.  After one barrier the per-digit offset of each wave inside the
+// tile is formed, after another the keys are placed into LDS in (digit, index)
+// order, and then written out bucket by bucket, so the keys of one bucket go
+// to consecutive global positions (runs of ~16 keys) instead of one scattered
+// 8-byte store each.  Three barriers per tile.
 __global__ __launch_bounds__(kSortThreads) void k_scatter(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned shift,
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ offs, uint32_t ntiles) {
   constexpr int W = kSortThreads / kWave;
+  constexpr int CH = kSortTile / kSortThreads;  // chunks of 64 keys per wave
   __shared__ uint64_t sk[kSortTile];
   __shared__ uint32_t sv[kSortTile];
   __shared__ uint32_t gstart[256];  // global position of this tile's bucket
   __shared__ uint32_t lstart[256];  // the bucket's start inside the tile
-  __shared__ uint32_t lrun[256];    // keys of the bucket placed so far
   __shared__ uint32_t scan_sh[kScanThreads];
-  __shared__ uint32_t wcnt[W][256];
+  __shared__ uint32_t wcnt[W][256];  // per wave: keys per digit, then the wave's offsets
   const int tid = threadIdx.x;
   const int w = tid / kWave;
+  const uint32_t lane = lane_id();
   gstart[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
   uint32_t tot_tile;
   lstart[tid] = block_excl_scan<uint32_t>(hist[(uint64_t)tid * ntiles + blockIdx.x], scan_sh,
                                           &tot_tile);
-  lrun[tid] = 0;
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) wcnt[ww][tid] = 0;
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-  for (int it = 0; it < kSortKPT; ++it) {
-    const uint64_t i = base + (uint64_t)it * kSortThreads + tid;
-    const bool valid = i < n;
-    const uint64_t k = valid ? kin[i] : 0;
-    const uint32_t v = valid ? vin[i] : 0;
-    const uint32_t dig = (uint32_t)(k >> shift) & 0xFF;
-    // wave-level multisplit: lanes holding the same digit
+  const uint64_t wbase = (uint64_t)blockIdx.x * kSortTile + (uint64_t)w * CH * kWave;
+  uint64_t k[CH];
+  uint32_t v[CH], rk[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const uint64_t i = wbase + (uint64_t)c * kWave + lane;
+    k[c] = i < n ? kin[i] : 0;
+    v[c] = i < n ? vin[i] : 0;
+  }
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const bool valid = wbase + (uint64_t)c * kWave + lane < n;
+    const uint32_t dig = (uint32_t)(k[c] >> shift) & 0xFF;
     uint64_t peers = ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -421,30 +430,41 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
       peers &= ((dig >> b) & 1u) ? bb : ~bb;
     }
     const uint32_t r = rank_below(peers);
-    if (valid && r == 0) wcnt[w][dig] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = lstart[dig] + lrun[dig] + r;
-      for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][dig];
-      sk[pos] = k;
-      sv[pos] = v;
-    }
-    __syncthreads();
-    uint32_t tot = 0;
+    // every lane reads its digit's count before the group's first lane adds
+    // the group (one wave: the LDS serves its accesses in program order)
+    const uint32_t before = valid ? wcnt[w][dig] : 0u;
+    rk[c] = before + r;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && r == 0) wcnt[w][dig] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  {
+    uint32_t acc = lstart[tid];  // digit tid: wave ww's keys start after earlier waves'
 #pragma unroll
     for (int ww = 0; ww < W; ++ww) {
-      tot += wcnt[ww][tid];
-      wcnt[ww][tid] = 0;
+      const uint32_t c = wcnt[ww][tid];
+      wcnt[ww][tid] = acc;
+      acc += c;
     }
-    lrun[tid] += tot;
-    __syncthreads();
   }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    if (wbase + (uint64_t)c * kWave + lane < n) {
+      const uint32_t pos = wcnt[w][(uint32_t)(k[c] >> shift) & 0xFF] + rk[c];
+      sk[pos] = k[c];
+      sv[pos] = v[c];
+    }
+  }
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
   const uint32_t cnt = (uint32_t)(n - base < (uint64_t)kSortTile ? n - base : kSortTile);
   for (uint32_t p = tid; p < cnt; p += kSortThreads) {
-    const uint64_t k = sk[p];
-    const uint32_t dig = (uint32_t)(k >> shift) & 0xFF;
+    const uint64_t kk = sk[p];
+    const uint32_t dig = (uint32_t)(kk >> shift) & 0xFF;
     const uint32_t pos = gstart[dig] + (p - lstart[dig]);
-    kout[pos] = k;
+    kout[pos] = kk;
     vout[pos] = sv[p];
   }
 }
