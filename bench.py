@@ -75,6 +75,33 @@ def contained_search(b, q, D, E, steps, warmup, kw, canon):
                                       np.array_equal(got["hashes"], canon["hashes"]))}
 
 
+def build_roofline(inf, n, D, keys):
+    """Algorithmic bytes of one build (per object: read it (E), write its key
+    (K = 8), one minimal sort read + write of key and index (2(K+4)), the
+    permutation: read the index, gather the object, write it (4 + 2E); then
+    per node: read its children's boxes (its objects at a leaf) and write its
+    box), over the build time; and the bytes the LSD
+    radix sort actually moves (per pass and key: histogram read 8, scatter read
+    8 + 4 and write 8 + 4) over the sort phase's time."""
+    E = D * 4 * (2 if inf["obj_kind"] == 1 else 1)
+    K = 8
+    box = 2 * D * 4
+    m = int(inf["n_nodes"])
+    # every node writes its box; leaves read their objects, every other node's
+    # box is read once by its parent
+    pack = n * E + max(m - 1, 0) * box + m * box
+    abytes = n * (E + K + 2 * (K + 4) + 4 + 2 * E) + pack
+    ms = inf["build_ms"]
+    key_or = int(np.bitwise_or.reduce(keys)) if n else 0
+    passes = (key_or.bit_length() + 7) // 8
+    moved = passes * n * (8 + 12 + 12)
+    sort_ms = inf["build_phase_ms"][1]
+    return {"bound": "hbm", "achieved": abytes / (ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": abytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "algorithmic_bytes": abytes, "sort_passes": passes, "sort_bytes_moved": moved,
+            "sort_GBps": moved / (sort_ms * 1e-3) / 1e9 if sort_ms > 0 else None}
+
+
 def build_figures(qcp, ix, pts, D, threads, with_cpu, key_mode="ref"):
     """GPU build time of the index's configuration (best of 3 in-place rebuilds
     from device-resident input) and, with_cpu, the oracle's threaded CPU build
@@ -93,6 +120,8 @@ def build_figures(qcp, ix, pts, D, threads, with_cpu, key_mode="ref"):
            "gpu_mkeys_per_s": n / (best["build_ms"] * 1e-3) / 1e6,
            "gpu_phase_ms": dict(zip(("keys", "sort", "gather", "pack"),
                                     best["build_phase_ms"][:4]))}
+    ex = ix.export()
+    out["roofline"] = build_roofline(best, n, D, ex["keys"])
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
@@ -102,7 +131,6 @@ def build_figures(qcp, ix, pts, D, threads, with_cpu, key_mode="ref"):
         k, perm, sobjs, boxes, ms, bad = O.build_mt(
             pts, D, False, F, O.KEY_REF if key_mode == "ref" else O.KEY_B16, nthreads=threads)
         dt = time.perf_counter() - t0
-        ex = ix.export()
         ok = bool(bad == 0 and np.array_equal(k, ex["keys"]) and np.array_equal(perm, ex["perm"])
                   and np.array_equal(boxes.view(np.uint32), ex["boxes"].view(np.uint32)))
         out["cpu"] = {"value": n / dt / 1e6, "unit": "Mkeys/s", "cores": threads, "kind": "port",
@@ -482,6 +510,37 @@ def profile_c5(qcp, args):
           flush=True)
 
 
+def profile_build(qcp, args):
+    """A short, fixed run of the build on the 134M-point 2-D skewed config, for
+    kernel traces: `steps` eager rebuilds (the host reads the keys' bit width
+    and runs that many sort passes), then `steps` replays of the graph-captured
+    build (every key byte's pass is captured; passes above the keys' bit width
+    end at once on the device).  Prints one JSON line with the phase times."""
+    w = qcp.workload("c5", scale=args.scale)
+    buf = qcp.DeviceBuffer(w["objs"])
+    n = len(w["objs"])
+    out = {"profile": "c5 build", "n": n, "eager": [], "graph_ms": []}
+    ix = qcp.Index.build(buf, 2, fanout=64)
+    for _ in range(args.steps):
+        ix.rebuild(buf)
+        inf = ix.info()
+        out["eager"].append({"build_ms": inf["build_ms"],
+                             "phase_ms": dict(zip(("keys", "sort", "gather", "pack"),
+                                                  inf["build_phase_ms"][:4]))})
+        log("c5 eager build %.3f ms" % inf["build_ms"])
+    gix = qcp.Index.build(buf, 2, fanout=64, graph=True)
+    for _ in range(args.steps):
+        gix.rebuild(buf)
+        out["graph_ms"].append(gix.info()["build_ms"])
+        log("c5 graph build %.3f ms" % out["graph_ms"][-1])
+    out["graph_equals_eager"] = bool(
+        np.array_equal(gix.export()["boxes"], ix.export()["boxes"]))
+    gix.close()
+    ix.close()
+    buf.close()
+    print(json.dumps(out), flush=True)
+
+
 def profile_traffic(qcp, args):
     """A short, fixed run for counter collection on the 16M-point config: with
     --sweep, `steps` persistent searches (counts+hash) of every configuration
@@ -566,6 +625,8 @@ def main():
                          "then a 1 GiB device-to-device copy as the calibration kernel")
     ap.add_argument("--profile-c5", action="store_true",
                     help="short search run on the 134M-point 2-D config (for rocprofv3)")
+    ap.add_argument("--profile-build", action="store_true",
+                    help="short build run on the 134M-point 2-D config (for rocprofv3)")
     ap.add_argument("--profile-ids", action="store_true",
                     help="short ID-list search run on the 16M-point config (for rocprofv3)")
     ap.add_argument("--block-waves", type=int, default=0, help="--profile-traffic: waves/WG")
@@ -596,6 +657,9 @@ def main():
         return
     if args.profile_c5:
         profile_c5(qcp, args)
+        return
+    if args.profile_build:
+        profile_build(qcp, args)
         return
     log("generating workload c2 (scale %.3f)" % args.scale)
     w = qcp.workload("c2", scale=args.scale)

@@ -1,7 +1,10 @@
 // build_kernels.hip -- GPU bulk-load pipeline for gfx950:
-//   Hilbert keys -> stable LSD radix sort (8-bit digits, LDS histograms and
-//   wave-level multisplit ranking) -> gather into rank-ordered SoA objects ->
-//   node MBRs (packed levels bottom-up, or explicit BVH / Guttman levels deepest first).
+//   Hilbert keys (LDS transition table) -> stable one-sweep LSD radix sort
+//   (8-bit digits, wave-level multisplit ranking, decoupled look-back) ->
+//   gather into rank-ordered SoA objects -> node MBRs (packed levels
+//   bottom-up, or explicit BVH / Guttman levels deepest first).  The
+//   per-pass histogram + scan + scatter sort (launch_sort_pass) orders query
+//   batches by their centres' keys.
 //
 // Reference steps replaced: Tree::AssignHilbertIndexToBranches
 // (src/tree/tree.cpp:596-639), Sorter::Sort + ReassignHilbertIndexes
@@ -20,13 +23,19 @@ namespace qcp {
 // ---------------------------------------------------------------------------
 // Hilbert keys: one thread per object.  Key source is the lower corner (the
 // reference reads only the first D entries of the branch, src/tree/tree.cpp:600)
-// or the box centre (lo+hi)*0.5f.
+// or the box centre (lo+hi)*0.5f.  Each workgroup first fills an LDS table of
+// the walk's transitions over L levels (hilbert_key.h), then each key takes
+// nb/L lookups instead of nb per-level steps.
 template <int D, bool BOX, bool REF, bool CENTER>
 __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, uint64_t n,
                                               unsigned nb, uint64_t* __restrict__ keys,
                                               uint32_t* __restrict__ idx,
                                               unsigned long long* key_or, uint32_t* bad) {
   constexpr int E = BOX ? 2 * D : D;
+  __shared__ uint16_t tab[HilbertTable<D>::SIZE];
+  for (uint32_t e = threadIdx.x; e < HilbertTable<D>::SIZE; e += blockDim.x)
+    tab[e] = hilbert_table_entry<D>(e);
+  __syncthreads();
   uint64_t acc = 0;
   bool any_bad = false, any_inv = false;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -52,7 +61,7 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
 #pragma unroll
       for (int d = 0; d < D; ++d) any_inv |= v[d] > v[D + d];
     }
-    uint64_t k = hilbert_key<D>(c, nb);
+    uint64_t k = hilbert_key_table<D>(c, nb, tab);
     keys[i] = k;
     idx[i] = (uint32_t)i;
     acc |= k;
@@ -190,18 +199,59 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const TI* __restric
 
 // ---------------------------------------------------------------------------
 // Gather the rank-ordered objects into SoA: obj[e][r] = objs[perm[r]][e].
-template <int E>
+// The reads are random (rank order against input order), so each thread keeps
+// kGatherU of them in flight: it loads kGatherU permutation entries, then the
+// kGatherU objects (8- or 16-byte vector loads where the object size allows),
+// then writes them.  VEC: the input pointer is 16-byte aligned (the launcher
+// checks), so objects of 2, 4 or 6 floats load as float2 / float4.
+constexpr int kGatherU = 4;
+
+template <int E, bool VEC>
+struct ObjVec {  // an object as the widest aligned loads its size allows
+  float v[E];
+  __device__ __forceinline__ void load(const float* __restrict__ o) {
+    if constexpr (VEC && E % 4 == 0) {
+#pragma unroll
+      for (int e = 0; e < E; e += 4) {
+        const float4 t = *reinterpret_cast<const float4*>(o + e);
+        v[e] = t.x; v[e + 1] = t.y; v[e + 2] = t.z; v[e + 3] = t.w;
+      }
+    } else if constexpr (VEC && E % 2 == 0) {
+#pragma unroll
+      for (int e = 0; e < E; e += 2) {
+        const float2 t = *reinterpret_cast<const float2*>(o + e);
+        v[e] = t.x; v[e + 1] = t.y;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = o[e];
+    }
+  }
+};
+
+template <int E, bool VEC>
 __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ objs,
                                                 const uint32_t* __restrict__ perm, uint64_t n,
                                                 float* __restrict__ obj, uint64_t stride) {
-  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-       r += (uint64_t)gridDim.x * blockDim.x) {
-    const float* o = objs + (uint64_t)perm[r] * E;
-    float v[E];
+  const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; r + (kGatherU - 1) * T < n; r += kGatherU * T) {
+    uint32_t p[kGatherU];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = o[e];
+    for (int u = 0; u < kGatherU; ++u) p[u] = perm[r + u * T];
+    ObjVec<E, VEC> o[kGatherU];
 #pragma unroll
-    for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = v[e];
+    for (int u = 0; u < kGatherU; ++u) o[u].load(objs + (uint64_t)p[u] * E);
+#pragma unroll
+    for (int u = 0; u < kGatherU; ++u)
+#pragma unroll
+      for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r + u * T] = o[u].v[e];
+  }
+  for (; r < n; r += T) {
+    ObjVec<E, VEC> o;
+    o.load(objs + (uint64_t)perm[r] * E);
+#pragma unroll
+    for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = o.v[e];
   }
 }
 
@@ -469,6 +519,272 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   }
 }
 
+// ---------------------------------------------------------------------------
+// One-sweep LSD radix sort of the index build.  One kernel reads the keys once
+// and counts the digits of every pass; each pass is then a single kernel: a
+// workgroup takes the next tile in start order (an atomic tile counter),
+// ranks it as k_scatter does, publishes its per-digit counts, and learns how
+// many keys of each digit earlier tiles hold by looking back over their
+// published words (decoupled look-back): a word is 0 (not yet published),
+// kStatusAgg | count (that tile's own count) or kStatusPrefix | count (the
+// count over tiles 0..t inclusive).  Every tile publishes its own count before
+// it waits on any other tile, and a tile's predecessors hold workgroups that
+// started before it, so every wait ends.
+constexpr int kMaxSortPasses = 8;
+constexpr uint64_t kStatusAgg = 1ull << 62;
+constexpr uint64_t kStatusPrefix = 1ull << 63;
+
+// Passes that reorder: those below the byte width of the OR of all keys (all
+// of them when key_or is null).
+__device__ __forceinline__ unsigned real_passes(unsigned passes,
+                                                const unsigned long long* key_or) {
+  if (key_or == nullptr) return passes;
+  const unsigned long long ko = *key_or;
+  const unsigned bits = ko ? 64u - (unsigned)__clzll((long long)ko) : 0u;
+  const unsigned real = (bits + 7) / 8;
+  return real < passes ? real : passes;
+}
+
+// Digit counts of every pass that reorders (ghist[p][256], zeroed by the
+// caller) from one read of the keys; also zeroes the first pass's look-back
+// words.  A wave whose keys share one digit adds them with one atomic (the
+// high digits of clustered keys), otherwise each key adds its own.
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist_all(
+    const uint64_t* __restrict__ keys, uint64_t n, unsigned passes,
+    const unsigned long long* __restrict__ key_or, uint32_t* __restrict__ ghist,
+    uint64_t* __restrict__ status0, uint64_t status_words) {
+  __shared__ uint32_t h[kMaxSortPasses][256];
+  const int tid = threadIdx.x;
+  passes = real_passes(passes, key_or);
+#pragma unroll
+  for (int p = 0; p < kMaxSortPasses; ++p) h[p][tid] = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kSortThreads + tid; i < status_words;
+       i += (uint64_t)gridDim.x * kSortThreads)
+    status0[i] = 0;
+  __syncthreads();
+  const uint32_t nt = (uint32_t)((n + kSortTile - 1) / kSortTile);
+  for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    const uint64_t base = (uint64_t)t * kSortTile;
+    uint64_t k[kSortKPT];
+#pragma unroll
+    for (int it = 0; it < kSortKPT; ++it) {
+      const uint64_t i = base + (uint64_t)it * kSortThreads + tid;
+      k[it] = i < n ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int it = 0; it < kSortKPT; ++it) {
+      const bool valid = base + (uint64_t)it * kSortThreads + tid < n;
+      const uint64_t vmask = ballot(valid);
+      if (vmask == 0) continue;
+      // the low digits of Hilbert keys are spread out: one atomic per key
+      const unsigned hi = passes > 2 ? passes - 2 : 0;
+      if (valid)
+        for (unsigned p = 0; p < hi; ++p) atomicAdd(&h[p][(k[it] >> (8 * p)) & 0xFF], 1u);
+      // the top two may be shared by a whole wave (clustered objects)
+      const int first = __builtin_ctzll(vmask);
+      const uint64_t k0 = ((uint64_t)__builtin_amdgcn_readlane((int)(k[it] >> 32), first) << 32) |
+                          (uint32_t)__builtin_amdgcn_readlane((int)k[it], first);
+      const uint64_t x = k[it] ^ k0;
+      for (unsigned p = hi; p < passes; ++p) {
+        if (ballot(valid && ((x >> (8 * p)) & 0xFF)) == 0) {
+          if ((int)lane_id() == first) atomicAdd(&h[p][(k0 >> (8 * p)) & 0xFF], (uint32_t)__popcll(vmask));
+        } else if (valid) {
+          atomicAdd(&h[p][(k[it] >> (8 * p)) & 0xFF], 1u);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (unsigned p = 0; p < passes; ++p)
+    if (h[p][tid]) atomicAdd(&ghist[p * 256 + tid], h[p][tid]);
+}
+
+// One pass (digit `pass` of `passes`).  status: this pass's look-back words
+// (ntiles x 256, zero at entry); status_next: the next pass's, zeroed here
+// (row blockIdx.x; null for the last pass); tile_ctr[pass] zero at entry.
+// key_or (graph-captured builds, which run a pass for every byte a key can
+// hold): passes at or above the keys' actual byte width end at once, the first
+// of them copying its input to its output when the remaining pass count is odd,
+// so the sorted keys land where the full pass count would put them.
+__global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned pass,
+    unsigned passes, const uint32_t* __restrict__ ghist, uint64_t* status,
+    uint64_t* __restrict__ status_next, uint32_t* tile_ctr,
+    const unsigned long long* __restrict__ key_or) {
+  constexpr int W = kSortThreads / kWave;
+  constexpr int CH = kSortTile / kSortThreads;  // chunks of 64 keys per wave
+  // 52.5 KB in all, so that three workgroups share a CU's 160 KB
+  __shared__ uint64_t sk[kSortTile];
+  __shared__ uint32_t sv[kSortTile];  // its first 256 words double as scan scratch
+  __shared__ uint32_t gstart[256];  // global position of this tile's bucket
+  __shared__ uint16_t lstart[256];  // the bucket's start inside the tile
+  __shared__ uint16_t wcnt[W][256];  // per wave: keys per digit, then the wave's offsets
+  __shared__ uint32_t s_tile;
+  uint32_t* const scan_sh = sv;
+  const int tid = threadIdx.x;
+  const int w = tid / kWave;
+  const uint32_t lane = lane_id();
+  const unsigned shift = 8 * pass;
+  if (key_or != nullptr) {
+    const unsigned real = real_passes(passes, key_or);
+    if (pass >= real) {
+      if (pass == real && ((passes - real) & 1u)) {
+        const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+        for (uint32_t j = tid; j < kSortTile; j += kSortThreads) {
+          const uint64_t i = base + j;
+          if (i < n) {
+            kout[i] = kin[i];
+            vout[i] = vin[i];
+          }
+        }
+      }
+      return;
+    }
+  }
+  if (tid == 0) s_tile = atomicAdd(&tile_ctr[pass], 1u);
+  if (status_next != nullptr) status_next[(uint64_t)blockIdx.x * 256 + tid] = 0;
+  uint32_t tot;
+  const uint32_t gbase = block_excl_scan<uint32_t>(ghist[pass * 256 + tid], scan_sh, &tot);
+#pragma unroll
+  for (int ww = 0; ww < W; ++ww) wcnt[ww][tid] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t wbase = (uint64_t)tile * kSortTile + (uint64_t)w * CH * kWave;
+  uint64_t k[CH];
+  uint32_t v[CH], rk[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const uint64_t i = wbase + (uint64_t)c * kWave + lane;
+    k[c] = i < n ? kin[i] : 0;
+    v[c] = i < n ? vin[i] : 0;
+  }
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const bool valid = wbase + (uint64_t)c * kWave + lane < n;
+    const uint32_t dig = (uint32_t)(k[c] >> shift) & 0xFF;
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bb = ballot((dig >> b) & 1u);
+      peers &= ((dig >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t r = rank_below(peers);
+    // every lane reads its digit's count before the group's first lane adds
+    // the group (one wave: the LDS serves its accesses in program order)
+    const uint32_t before = valid ? (uint32_t)wcnt[w][dig] : 0u;
+    rk[c] = before + r;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && r == 0) wcnt[w][dig] = (uint16_t)(before + (uint32_t)__popcll(peers));
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // digit tid: this tile's count, and each wave's offset among the tile's keys
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int ww = 0; ww < W; ++ww) {
+    const uint32_t c = wcnt[ww][tid];
+    wcnt[ww][tid] = (uint16_t)cnt;
+    cnt += c;
+  }
+  uint64_t* const mine = status + (uint64_t)tile * 256 + tid;
+  __hip_atomic_store(mine, (tile == 0 ? kStatusPrefix : kStatusAgg) | (uint64_t)cnt,
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t tot_tile;
+  const uint32_t ls = block_excl_scan<uint32_t>(cnt, scan_sh, &tot_tile);
+  uint32_t excl = 0;
+  if (tile > 0) {
+    // kLookback predecessors' words per round trip, consumed in order up to
+    // the first inclusive prefix; a word not yet published ends the round
+    // (sleep, then reload from there).  Tiles below 0 read as a zero prefix.
+    constexpr int kLookback = 4;
+    int32_t t = (int32_t)tile - 1;
+    for (;;) {
+      uint64_t wv[kLookback];
+#pragma unroll
+      for (int j = 0; j < kLookback; ++j)
+        wv[j] = t - j >= 0 ? __hip_atomic_load(status + (uint64_t)(t - j) * 256 + tid,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : kStatusPrefix;
+      bool stop = false, done = false;
+      int used = 0;
+#pragma unroll
+      for (int j = 0; j < kLookback; ++j) {
+        if (!stop) {
+          if (!(wv[j] & (kStatusAgg | kStatusPrefix))) {
+            stop = true;
+          } else {
+            excl += (uint32_t)wv[j];
+            ++used;
+            if (wv[j] & kStatusPrefix) stop = done = true;
+          }
+        }
+      }
+      if (done) break;
+      t -= used;
+      if (used == 0) __builtin_amdgcn_s_sleep(1);
+    }
+    __hip_atomic_store(mine, kStatusPrefix | (uint64_t)(excl + cnt), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  gstart[tid] = gbase + excl;
+  lstart[tid] = (uint16_t)ls;
+#pragma unroll
+  for (int ww = 0; ww < W; ++ww) wcnt[ww][tid] = (uint16_t)(wcnt[ww][tid] + ls);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    if (wbase + (uint64_t)c * kWave + lane < n) {
+      const uint32_t pos = wcnt[w][(uint32_t)(k[c] >> shift) & 0xFF] + rk[c];
+      sk[pos] = k[c];
+      sv[pos] = v[c];
+    }
+  }
+  __syncthreads();
+  const uint64_t base = (uint64_t)tile * kSortTile;
+  const uint32_t tcnt = (uint32_t)(n - base < (uint64_t)kSortTile ? n - base : kSortTile);
+  for (uint32_t p = tid; p < tcnt; p += kSortThreads) {
+    const uint64_t kk = sk[p];
+    const uint32_t dig = (uint32_t)(kk >> shift) & 0xFF;
+    const uint32_t pos = gstart[dig] + (p - lstart[dig]);
+    kout[pos] = kk;
+    vout[pos] = sv[p];
+  }
+}
+
+uint64_t sort_status_words(uint64_t n) { return 2ull * sort_tiles(n > 0 ? n : 1) * 256; }
+
+hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2,
+                            uint64_t n, unsigned passes, const unsigned long long* key_or,
+                            uint32_t* ghist, uint64_t* status, hipStream_t s) {
+  if (passes > (unsigned)kMaxSortPasses) return hipErrorInvalidValue;
+  if (n == 0 || passes == 0) return hipSuccess;
+  const uint32_t nt = sort_tiles(n);
+  const uint64_t words = (uint64_t)nt * 256;
+  uint32_t* tile_ctr = ghist + kMaxSortPasses * 256;
+  hipError_t e = launch_fill_u32(ghist, kMaxSortPasses * 256 + kMaxSortPasses, 0u, s);
+  if (e != hipSuccess) return e;
+  const unsigned g = nt < 2048 ? nt : 2048;
+  k_sort_hist_all<<<g, kSortThreads, 0, s>>>(keys, n, passes, key_or, ghist, status, words);
+  uint64_t* kin = keys;
+  uint32_t* vin = vals;
+  uint64_t* kout = keys2;
+  uint32_t* vout = vals2;
+  for (unsigned p = 0; p < passes; ++p) {
+    uint64_t* st = status + (uint64_t)(p & 1u) * words;
+    uint64_t* st_next = p + 1 < passes ? status + (uint64_t)((p + 1) & 1u) * words : nullptr;
+    k_sort_onesweep<<<nt, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, p, passes, ghist, st,
+                                                st_next, tile_ctr, key_or);
+    uint64_t* tk = kin;
+    kin = kout;
+    kout = tk;
+    uint32_t* tv = vin;
+    vin = vout;
+    vout = tv;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
                             uint32_t* vout, uint64_t n, unsigned shift, uint32_t* hist,
                             uint32_t* hist_scan, uint32_t* block_sums, uint32_t max_blocks,
@@ -484,13 +800,18 @@ hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* 
 hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, unsigned E,
                          float* obj, uint64_t stride, hipStream_t s) {
   unsigned g = grid_for(n, 256, 16384);
+  const bool vec = ((uintptr_t)objs & 15u) == 0;
+#define QCP_GATHER(EE)                                                  \
+  if (vec) k_gather<EE, true><<<g, 256, 0, s>>>(objs, perm, n, obj, stride); \
+  else k_gather<EE, false><<<g, 256, 0, s>>>(objs, perm, n, obj, stride)
   switch (E) {
-    case 2: k_gather<2><<<g, 256, 0, s>>>(objs, perm, n, obj, stride); break;
-    case 3: k_gather<3><<<g, 256, 0, s>>>(objs, perm, n, obj, stride); break;
-    case 4: k_gather<4><<<g, 256, 0, s>>>(objs, perm, n, obj, stride); break;
-    case 6: k_gather<6><<<g, 256, 0, s>>>(objs, perm, n, obj, stride); break;
+    case 2: QCP_GATHER(2); break;
+    case 3: QCP_GATHER(3); break;
+    case 4: QCP_GATHER(4); break;
+    case 6: QCP_GATHER(6); break;
     default: return hipErrorInvalidValue;
   }
+#undef QCP_GATHER
   return hipGetLastError();
 }
 

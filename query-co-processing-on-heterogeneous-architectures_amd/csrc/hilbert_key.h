@@ -19,39 +19,126 @@
 
 namespace qcp {
 
+// One level of the walk.  The reference keeps `flip` (1 << the rotation of
+// the level above, 0 at the top) and the digit of the level above; both enter
+// only as flip ^ above ^ t, so the walk's whole state is the pair
+// s = flip ^ above and the rotation.  Takes this level's digit t (bit d =
+// coordinate d), returns the D bits it appends to the index, advances (s, rot).
+template <int D>
+QCP_HD uint32_t hilbert_level(uint32_t& s, unsigned& rot, uint32_t t) {
+  const uint32_t nd_ones = (1u << D) - 1u;
+  const uint32_t x = s ^ t;
+  const uint32_t bits = ((x >> rot) | (x << (D - rot))) & nd_ones;
+  s = (1u << rot) ^ t;
+  // adjust_rotation: rotation += 1 + (position of the lowest set bit of
+  // bits, counted from 1, if that bit lies below the top dimension)
+  uint32_t low = bits & (0u - bits) & (nd_ones >> 1);
+  while (low) {
+    low >>= 1;
+    ++rot;
+  }
+  if (++rot >= (unsigned)D) rot -= D;
+  return bits;
+}
+
+// Final steps: XOR with nthbits >> 1 (nthbits = one 1 per level, at the low
+// bit of each D-bit digit) and the prefix XOR (inverse Gray code).
+template <int D>
+QCP_HD uint64_t hilbert_finish(uint64_t index, unsigned nb) {
+  const uint64_t nthbits = ((1ull << (D * nb)) - 1ull) / ((1ull << D) - 1ull);
+  index ^= nthbits >> 1;
+  for (unsigned s = 1; s < (unsigned)D * nb; s *= 2) index ^= index >> s;
+  return index;
+}
+
 // Hilbert index of c[0..D) with nb bits per coordinate (D*nb <= 63, nb >= 2).
 template <int D>
 QCP_HD uint64_t hilbert_key(const uint32_t* c, unsigned nb) {
-  const uint32_t nd_ones = (1u << D) - 1u;
-  const uint32_t nd1_ones = nd_ones >> 1;
   uint64_t index = 0;
-  unsigned rotation = 0;
-  uint32_t flip = 0;
-  uint32_t above = 0;  // digit of the level above (0 above the top)
-  uint64_t nthbits = 0;
+  uint32_t s = 0;
+  unsigned rot = 0;
   for (int b = (int)nb - 1; b >= 0; --b) {
     uint32_t t = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) t |= ((c[d] >> b) & 1u) << d;
-    uint32_t g = t ^ above;  // coords ^= coords >> nDims
-    above = t;
-    uint32_t x = flip ^ g;
-    uint32_t bits = ((x >> rotation) | (x << (D - rotation))) & nd_ones;
-    index = (index << D) | bits;
-    flip = 1u << rotation;
-    // adjust_rotation: rotation += 1 + (position of the lowest set bit of
-    // bits, counted from 1, if that bit lies below the top dimension)
-    uint32_t low = bits & (0u - bits) & nd1_ones;
-    while (low) {
-      low >>= 1;
-      ++rotation;
-    }
-    if (++rotation >= (unsigned)D) rotation -= D;
-    nthbits = (nthbits << D) | 1u;
+    index = (index << D) | hilbert_level<D>(s, rot, t);
   }
-  index ^= nthbits >> 1;
-  for (unsigned s = 1; s < (unsigned)D * nb; s *= 2) index ^= index >> s;
-  return index;
+  return hilbert_finish<D>(index, nb);
+}
+
+// The same walk, L levels per step, from a transition table: the entry for
+// state (s, rot) and the L digits of L consecutive levels (top level in the
+// high bits) holds the D*L index bits those levels append (low 8 bits) and the
+// state after them (s | rot << D, bits 8..15).  L = 4 for D = 2 (8 states x
+// 256 inputs), L = 2 for D = 3 (24 states x 64 inputs): 2048 and 1536 entries.
+template <int D>
+struct HilbertTable {
+  static constexpr int L = D == 2 ? 4 : 2;
+  static constexpr int IN = D * L;                  // input bits per step
+  static constexpr int STATES = (1 << D) * D;
+  static constexpr uint32_t SIZE = (uint32_t)STATES << IN;
+};
+
+template <int D>
+QCP_HD uint16_t hilbert_table_entry(uint32_t e) {
+  constexpr int L = HilbertTable<D>::L, IN = HilbertTable<D>::IN;
+  const uint32_t in = e & ((1u << IN) - 1u);
+  const uint32_t st = e >> IN;
+  uint32_t s = st & ((1u << D) - 1u);
+  unsigned rot = st >> D;
+  uint32_t out = 0;
+  for (int l = L - 1; l >= 0; --l)
+    out = (out << D) | hilbert_level<D>(s, rot, (in >> (D * l)) & ((1u << D) - 1u));
+  return (uint16_t)(out | ((s | (rot << D)) << 8));
+}
+
+// Bit b of coordinate d moved to bit D*b + d (the digits of every level,
+// concatenated from the top): 31-bit coordinates for D = 2, 21-bit for D = 3.
+QCP_HD uint64_t spread2(uint32_t v) {
+  uint64_t x = v;
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+QCP_HD uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1FFFFFu;
+  x = (x | (x << 32)) & 0x001F00000000FFFFull;
+  x = (x | (x << 16)) & 0x001F0000FF0000FFull;
+  x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+  x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+template <int D>
+QCP_HD uint64_t interleave(const uint32_t* c) {
+  if (D == 2) return spread2(c[0]) | (spread2(c[1]) << 1);
+  return spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+}
+
+// hilbert_key through the table: the top nb % L levels one by one, then L at
+// a time.  Equal to hilbert_key<D>(c, nb) for every input (D in {2, 3}).
+template <int D>
+QCP_HD uint64_t hilbert_key_table(const uint32_t* c, unsigned nb, const uint16_t* tab) {
+  constexpr int L = HilbertTable<D>::L, IN = HilbertTable<D>::IN;
+  const uint64_t m = interleave<D>(c);
+  uint64_t index = 0;
+  uint32_t s = 0;
+  unsigned rot = 0;
+  int b = (int)nb - 1;
+  for (const int top = (int)nb - (int)(nb % L); b >= top; --b)
+    index = (index << D) | hilbert_level<D>(s, rot, (uint32_t)(m >> (D * b)) & ((1u << D) - 1u));
+  for (; b >= 0; b -= L) {
+    const uint32_t in = (uint32_t)(m >> (D * (b - L + 1))) & ((1u << IN) - 1u);
+    const uint32_t e = tab[((s | (rot << D)) << IN) | in];
+    index = (index << IN) | (e & 0xFFu);
+    s = (e >> 8) & ((1u << D) - 1u);
+    rot = e >> (8 + D);
+  }
+  return hilbert_finish<D>(index, nb);
 }
 
 // Quantiser of src/mapper/hilbert_mapper.cpp:22 -- (ll)(1000000*p), one FP32

@@ -83,9 +83,8 @@ static qcp_status set_device(int dev) {
 static void index_free_scratch(qcp_index* ix) {
   dfree(ix->d_keys2);
   dfree(ix->d_perm2);
-  dfree(ix->d_hist);
-  dfree(ix->d_hist_scan);
-  dfree(ix->d_block_sums);
+  dfree(ix->d_sort_ghist);
+  dfree(ix->d_sort_status);
 }
 
 static void index_free(qcp_index* ix) {
@@ -249,12 +248,10 @@ static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** 
   if (launch_fill_f32(ix->d_obj, (uint64_t)ix->E * ix->obj_stride, 0.0f, ix->stream) != hipSuccess)
     return fail(QCP_EHIP);
   // sort scratch
-  const uint32_t nt = sort_tiles(n > 0 ? n : 1);
   if ((s = dalloc(&ix->d_keys2, n)) != QCP_OK) return fail(s);
   if ((s = dalloc(&ix->d_perm2, n)) != QCP_OK) return fail(s);
-  if ((s = dalloc(&ix->d_hist, (uint64_t)nt * 256)) != QCP_OK) return fail(s);
-  if ((s = dalloc(&ix->d_hist_scan, (uint64_t)nt * 256 + 1)) != QCP_OK) return fail(s);
-  if ((s = dalloc(&ix->d_block_sums, kMaxScanBlocks + 1)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_sort_ghist, kSortGhistWords)) != QCP_OK) return fail(s);
+  if ((s = dalloc(&ix->d_sort_status, sort_status_words(n))) != QCP_OK) return fail(s);
   *out = ix;
   return QCP_OK;
 }
@@ -343,7 +340,8 @@ static qcp_status enqueue_boxes(qcp_index* ix) {
 }
 
 // Sort passes of a graph-captured build: every byte a key can hold (an even
-// count for every mode, so the sorted keys land back in d_keys / d_perm).
+// count for every mode, so the sorted keys land back in d_keys / d_perm);
+// passes above the keys' actual width end at once on the device.
 static unsigned graph_sort_passes(const qcp_index* ix) { return (ix->D * ix->nb + 7) / 8; }
 
 // The whole build as one asynchronous sequence (no host synchronisation), for
@@ -356,17 +354,9 @@ static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned pas
   HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
                       ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, ix->d_perm,
                       ix->d_key_or, ix->d_bad, s));
-  uint64_t* kin = ix->d_keys;
-  uint32_t* vin = ix->d_perm;
-  uint64_t* kout = ix->d_keys2;
-  uint32_t* vout = ix->d_perm2;
-  for (unsigned p = 0; p < passes; ++p) {
-    HIP_TRY(launch_sort_pass(kin, vin, kout, vout, n, 8 * p, ix->d_hist, ix->d_hist_scan,
-                             ix->d_block_sums, kMaxScanBlocks, s));
-    std::swap(kin, kout);
-    std::swap(vin, vout);
-  }
-  if (kin != ix->d_keys) return QCP_EINVAL;  // odd pass counts are not captured
+  if (passes & 1u) return QCP_EINVAL;  // odd pass counts are not captured
+  HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
+                           ix->d_key_or, ix->d_sort_ghist, ix->d_sort_status, s));
   HIP_TRY(launch_gather(d_objs, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s));
   return enqueue_boxes(ix);
 }
@@ -440,17 +430,9 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
       bits = (unsigned)kb;
     }
     const unsigned passes = (bits + 7) / 8;
-    uint64_t* kin = ix->d_keys;
-    uint32_t* vin = ix->d_perm;
-    uint64_t* kout = ix->d_keys2;
-    uint32_t* vout = ix->d_perm2;
-    for (unsigned p = 0; p < passes; ++p) {
-      HIP_TRY(launch_sort_pass(kin, vin, kout, vout, n, 8 * p, ix->d_hist, ix->d_hist_scan,
-                               ix->d_block_sums, kMaxScanBlocks, s));
-      std::swap(kin, kout);
-      std::swap(vin, vout);
-    }
-    if (kin != ix->d_keys) {
+    HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
+                             nullptr, ix->d_sort_ghist, ix->d_sort_status, s));
+    if (passes & 1u) {
       std::swap(ix->d_keys, ix->d_keys2);
       std::swap(ix->d_perm, ix->d_perm2);
     }

@@ -34,9 +34,8 @@ struct qcp_index {
   // build scratch
   uint64_t* d_keys2 = nullptr;
   uint32_t* d_perm2 = nullptr;
-  uint32_t* d_hist = nullptr;
-  uint32_t* d_hist_scan = nullptr;
-  uint32_t* d_block_sums = nullptr;
+  uint32_t* d_sort_ghist = nullptr;   // digit counts of every pass + tile counters
+  uint64_t* d_sort_status = nullptr;  // look-back words of two consecutive passes
   unsigned long long* d_key_or = nullptr;
   uint32_t* d_bad = nullptr;
   float* d_in = nullptr;         // device copy of host input

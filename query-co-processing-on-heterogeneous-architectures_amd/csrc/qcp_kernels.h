@@ -25,6 +25,17 @@ hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* 
                             uint32_t* vout, uint64_t n, unsigned shift, uint32_t* hist,
                             uint32_t* hist_scan, uint32_t* block_sums, uint32_t max_blocks,
                             hipStream_t s);
+// Whole LSD radix sort of (keys, vals) over `passes` 8-bit digits, one-sweep
+// form; the result is in keys/vals for an even pass count, keys2/vals2 for an
+// odd one.  key_or: null, or the device word holding the OR of all keys, whose
+// bit width then decides on the device how many passes actually reorder (the
+// rest end at once).  ghist: kMaxSortPasses*256 + kMaxSortPasses words;
+// status: sort_status_words(n) words.
+uint64_t sort_status_words(uint64_t n);
+hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2,
+                            uint64_t n, unsigned passes, const unsigned long long* key_or,
+                            uint32_t* ghist, uint64_t* status, hipStream_t s);
+constexpr uint32_t kSortGhistWords = 8 * 256 + 8;
 hipError_t scan_u32_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* block_sums,
                         uint32_t max_blocks, hipStream_t s);
 hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
