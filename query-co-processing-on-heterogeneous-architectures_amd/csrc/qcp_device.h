@@ -165,9 +165,11 @@ __device__ __forceinline__ bool overlaps(const float* ql, const float* qh,
 template <int D>
 __device__ __forceinline__ bool overlap_test(const float* ql, const float* qh, const float* lo,
                                              const float* hi) {
+  // one running maximum, so that each further dimension is a single v_max3
+  // (fmaxf ignores NaN operands in any grouping, so the order is free)
   float t = fmaxf(ql[0] - hi[0], lo[0] - qh[0]);
 #pragma unroll
-  for (int d = 1; d < D; ++d) t = fmaxf(t, fmaxf(ql[d] - hi[d], lo[d] - qh[d]));
+  for (int d = 1; d < D; ++d) t = fmaxf(fmaxf(t, ql[d] - hi[d]), lo[d] - qh[d]);
   return !(t > 0.0f);
 }
 

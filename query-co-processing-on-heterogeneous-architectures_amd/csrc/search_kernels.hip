@@ -279,8 +279,10 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
       for (int e = 0; e < E; ++e) {
         const GlobalF32* base = reinterpret_cast<const GlobalF32*>(
             uni64(reinterpret_cast<uint64_t>(col[e] + cs)));
+        // a signed lane index lets the chunk offsets fold into the loads'
+        // immediate offset field (one lane-offset register for all chunks)
 #pragma unroll
-        for (int u = 0; u < U; ++u) w[u][e] = base[u * kWave + lane];
+        for (int u = 0; u < U; ++u) w[u][e] = base[u * kWave + (int)lane];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
