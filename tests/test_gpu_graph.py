@@ -34,6 +34,41 @@ def test_graph_build_replays(gpu, O, layout, D, box, F):
     buf.close()
 
 
+def test_graph_build_key_widths(gpu, O):
+    """A captured build runs a sort pass for every byte a key can hold (8 for
+    2-D REF keys); passes above the keys' actual byte width end on the device,
+    the first of them copying when the number of such passes is odd.  One
+    captured graph replayed over inputs whose keys need 0, 2, 5, 6, 7 and 8
+    bytes (points scaled inside the REF quantiser's domain) equals the eager
+    build and the oracle every time."""
+    n, D, F = 50000, 2, 64
+    base = rand_points(n, D, 31)
+    cases = [("zero", np.zeros_like(base)), ("x1e-4", base * np.float32(1e-4)),
+             ("x1", base), ("x16", base * np.float32(16.0)),
+             ("x100", base * np.float32(100.0)), ("x2000", base * np.float32(2000.0)),
+             ("x1 again", base)]
+    widths = []
+    buf = gpu.DeviceBuffer(cases[0][1])
+    ix = gpu.Index.build(buf, D, fanout=F, graph=True)
+    for rep, (name, objs) in enumerate(cases):
+        objs = np.ascontiguousarray(objs, dtype=np.float32)
+        if rep:
+            buf.write(objs)
+            ix.rebuild(buf)
+        got = ix.export()
+        t = O.build_packed(objs, D, False, F, O.KEY_REF)
+        assert not t.domain_error
+        widths.append((int(np.bitwise_or.reduce(t.keys)).bit_length() + 7) // 8)
+        assert_bits_equal(got["keys"], t.keys, "%s keys" % name)
+        assert_bits_equal(got["perm"], t.perm, "%s perm" % name)
+        assert_bits_equal(got["boxes"], t.boxes, "%s boxes" % name)
+        eager = gpu.Index.build(objs, D, fanout=F).export()
+        for k in ("keys", "perm", "sobjs", "boxes"):
+            assert_bits_equal(got[k], eager[k], "%s graph vs eager %s" % (name, k))
+    buf.close()
+    assert widths[:6] == [0, 2, 5, 6, 7, 8], widths
+
+
 @pytest.mark.parametrize("donate", [0, 1])
 def test_graph_search_replays(gpu, O, donate):
     pts = rand_points(200000, 3, 11)
