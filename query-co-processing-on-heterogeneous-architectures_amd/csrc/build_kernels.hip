@@ -26,6 +26,32 @@ namespace qcp {
 // or the box centre (lo+hi)*0.5f.  Each workgroup first fills an LDS table of
 // the walk's transitions over L levels (hilbert_key.h), then each key takes
 // nb/L lookups instead of nb per-level steps.
+template <int D>
+__device__ __forceinline__ void fill_hilbert_table(uint16_t* tab) {
+  for (uint32_t e = threadIdx.x; e < HilbertTable<D>::SIZE; e += blockDim.x)
+    tab[e] = hilbert_table_entry<D>(e);
+  __syncthreads();
+}
+
+// The key of one object v[E] (E = 2D for boxes); `bad` set for a coordinate
+// outside the REF quantiser's domain.
+template <int D, bool BOX, bool REF, bool CENTER>
+__device__ __forceinline__ uint64_t object_key(const float* v, unsigned nb, const uint16_t* tab,
+                                               bool& bad) {
+  uint32_t c[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    float p = v[d];
+    if (CENTER && BOX) p = (v[d] + v[D + d]) * 0.5f;
+    if (REF) {
+      bad |= !quant_ref(p, nb, &c[d]);
+    } else {
+      c[d] = quant_b16(p);
+    }
+  }
+  return hilbert_key_table<D>(c, nb, tab);
+}
+
 template <int D, bool BOX, bool REF, bool CENTER>
 __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, uint64_t n,
                                               unsigned nb, uint64_t* __restrict__ keys,
@@ -33,9 +59,7 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
                                               unsigned long long* key_or, uint32_t* bad) {
   constexpr int E = BOX ? 2 * D : D;
   __shared__ uint16_t tab[HilbertTable<D>::SIZE];
-  for (uint32_t e = threadIdx.x; e < HilbertTable<D>::SIZE; e += blockDim.x)
-    tab[e] = hilbert_table_entry<D>(e);
-  __syncthreads();
+  fill_hilbert_table<D>(tab);
   uint64_t acc = 0;
   bool any_bad = false, any_inv = false;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -44,24 +68,13 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
     float v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = o[e];
-    uint32_t c[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      float p = v[d];
-      if (CENTER && BOX) p = (v[d] + v[D + d]) * 0.5f;
-      if (REF) {
-        any_bad |= !quant_ref(p, nb, &c[d]);
-      } else {
-        c[d] = quant_b16(p);
-      }
-    }
 #pragma unroll
     for (int e = 0; e < E; ++e) any_bad |= (v[e] != v[e]);  // NaN objects are rejected
     if (BOX) {
 #pragma unroll
       for (int d = 0; d < D; ++d) any_inv |= v[d] > v[D + d];
     }
-    uint64_t k = hilbert_key_table<D>(c, nb, tab);
+    const uint64_t k = object_key<D, BOX, REF, CENTER>(v, nb, tab, any_bad);
     keys[i] = k;
     idx[i] = (uint32_t)i;
     acc |= k;
@@ -252,6 +265,64 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ objs,
     o.load(objs + (uint64_t)perm[r] * E);
 #pragma unroll
     for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = o.v[e];
+  }
+}
+
+// The gather after a sort that ended in packed elements (rank r holds
+// ((key >> dropped bits) << pbits) | input index): writes the rank-ordered
+// objects as k_gather does and the permutation.  The sorted keys are not
+// written (the packing dropped their low bytes, and like the reference's
+// build, which replaces them by ranks, nothing after the build needs them);
+// k_keys_sorted recomputes them from the rank-ordered objects on demand.
+template <int E, bool VEC>
+__global__ __launch_bounds__(256) void k_gather_unpack(const float* __restrict__ objs,
+                                                       const uint64_t* __restrict__ packed,
+                                                       uint64_t n, unsigned pbits,
+                                                       float* __restrict__ obj, uint64_t stride,
+                                                       uint32_t* __restrict__ perm_out) {
+  const uint64_t mask = (1ull << pbits) - 1ull;
+  const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; r + (kGatherU - 1) * T < n; r += kGatherU * T) {
+    uint32_t p[kGatherU];
+#pragma unroll
+    for (int u = 0; u < kGatherU; ++u) p[u] = (uint32_t)(packed[r + u * T] & mask);
+    ObjVec<E, VEC> o[kGatherU];
+#pragma unroll
+    for (int u = 0; u < kGatherU; ++u) o[u].load(objs + (uint64_t)p[u] * E);
+#pragma unroll
+    for (int u = 0; u < kGatherU; ++u) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r + u * T] = o[u].v[e];
+      perm_out[r + u * T] = p[u];
+    }
+  }
+  for (; r < n; r += T) {
+    const uint32_t p = (uint32_t)(packed[r] & mask);
+    ObjVec<E, VEC> o;
+    o.load(objs + (uint64_t)p * E);
+#pragma unroll
+    for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = o.v[e];
+    perm_out[r] = p;
+  }
+}
+
+// Sorted keys recomputed from the rank-ordered SoA objects (obj[e][r]) by the
+// same object_key as k_keys.
+template <int D, bool BOX, bool REF, bool CENTER>
+__global__ __launch_bounds__(256) void k_keys_sorted(const float* __restrict__ obj,
+                                                     uint64_t stride, uint64_t n, unsigned nb,
+                                                     uint64_t* __restrict__ keys) {
+  constexpr int E = BOX ? 2 * D : D;
+  __shared__ uint16_t tab[HilbertTable<D>::SIZE];
+  fill_hilbert_table<D>(tab);
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    float v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = obj[(uint64_t)e * stride + r];
+    bool bad = false;
+    keys[r] = object_key<D, BOX, REF, CENTER>(v, nb, tab, bad);
   }
 }
 
@@ -599,19 +670,26 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist_all(
     if (h[p][tid]) atomicAdd(&ghist[p * 256 + tid], h[p][tid]);
 }
 
-// One pass (digit `pass` of `passes`).  status: this pass's look-back words
-// (ntiles x 256, zero at entry); status_next: the next pass's, zeroed here
-// (row blockIdx.x; null for the last pass); tile_ctr[pass] zero at entry.
+// One pass (digit `pass` of `passes`), its digit at bit `shift` of the input
+// element.  status: this pass's look-back words (ntiles x 256, zero at entry);
+// status_next: the next pass's, zeroed here (row blockIdx.x; null for the last
+// pass); tile_ctr[pass] zero at entry.
 // key_or (graph-captured builds, which run a pass for every byte a key can
 // hold): passes at or above the keys' actual byte width end at once, the first
 // of them copying its input to its output when the remaining pass count is odd,
 // so the sorted keys land where the full pass count would put them.
+// Packed elements: once the key bits still to be sorted and the index fit one
+// 64-bit word, a pass writes ((key >> pack_shift) << pbits) | index (OUTP) and
+// later passes move that word alone (INP): 8 bytes per key each way instead of
+// 8 + 4.
+template <bool INP, bool OUTP>
 __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned pass,
-    unsigned passes, const uint32_t* __restrict__ ghist, uint64_t* status,
-    uint64_t* __restrict__ status_next, uint32_t* tile_ctr,
-    const unsigned long long* __restrict__ key_or) {
+    unsigned passes, unsigned shift, unsigned pack_shift, unsigned pbits,
+    const uint32_t* __restrict__ ghist, uint64_t* status, uint64_t* __restrict__ status_next,
+    uint32_t* tile_ctr, const unsigned long long* __restrict__ key_or) {
+  static_assert(OUTP || !INP, "a packed pass writes packed elements");
   constexpr int W = kSortThreads / kWave;
   constexpr int CH = kSortTile / kSortThreads;  // chunks of 64 keys per wave
   // 52.5 KB in all, so that three workgroups share a CU's 160 KB
@@ -625,7 +703,6 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
   const int tid = threadIdx.x;
   const int w = tid / kWave;
   const uint32_t lane = lane_id();
-  const unsigned shift = 8 * pass;
   if (key_or != nullptr) {
     const unsigned real = real_passes(passes, key_or);
     if (pass >= real) {
@@ -635,7 +712,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
           const uint64_t i = base + j;
           if (i < n) {
             kout[i] = kin[i];
-            vout[i] = vin[i];
+            if (!INP) vout[i] = vin[i];
           }
         }
       }
@@ -657,7 +734,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
   for (int c = 0; c < CH; ++c) {
     const uint64_t i = wbase + (uint64_t)c * kWave + lane;
     k[c] = i < n ? kin[i] : 0;
-    v[c] = i < n ? vin[i] : 0;
+    v[c] = (!INP && i < n) ? vin[i] : 0;
   }
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
@@ -737,7 +814,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
     if (wbase + (uint64_t)c * kWave + lane < n) {
       const uint32_t pos = wcnt[w][(uint32_t)(k[c] >> shift) & 0xFF] + rk[c];
       sk[pos] = k[c];
-      sv[pos] = v[c];
+      if (!INP) sv[pos] = v[c];
     }
   }
   __syncthreads();
@@ -747,8 +824,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
     const uint64_t kk = sk[p];
     const uint32_t dig = (uint32_t)(kk >> shift) & 0xFF;
     const uint32_t pos = gstart[dig] + (p - lstart[dig]);
-    kout[pos] = kk;
-    vout[pos] = sv[p];
+    if (INP) {
+      kout[pos] = kk;
+    } else if (OUTP) {
+      kout[pos] = ((kk >> pack_shift) << pbits) | (uint64_t)sv[p];
+    } else {
+      kout[pos] = kk;
+      vout[pos] = sv[p];
+    }
   }
 }
 
@@ -756,8 +839,12 @@ uint64_t sort_status_words(uint64_t n) { return 2ull * sort_tiles(n > 0 ? n : 1)
 
 hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2,
                             uint64_t n, unsigned passes, const unsigned long long* key_or,
-                            uint32_t* ghist, uint64_t* status, hipStream_t s) {
+                            uint32_t* ghist, uint64_t* status, hipStream_t s, int pack_pass,
+                            unsigned pbits) {
   if (passes > (unsigned)kMaxSortPasses) return hipErrorInvalidValue;
+  if (pack_pass >= 0 && (key_or != nullptr || pack_pass + 1 >= (int)passes || pbits == 0 ||
+                         pbits > 32))
+    return hipErrorInvalidValue;
   if (n == 0 || passes == 0) return hipSuccess;
   const uint32_t nt = sort_tiles(n);
   const uint64_t words = (uint64_t)nt * 256;
@@ -773,8 +860,18 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
   for (unsigned p = 0; p < passes; ++p) {
     uint64_t* st = status + (uint64_t)(p & 1u) * words;
     uint64_t* st_next = p + 1 < passes ? status + (uint64_t)((p + 1) & 1u) * words : nullptr;
-    k_sort_onesweep<<<nt, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, p, passes, ghist, st,
-                                                st_next, tile_ctr, key_or);
+    if (pack_pass < 0 || (int)p < pack_pass) {
+      k_sort_onesweep<false, false><<<nt, kSortThreads, 0, s>>>(
+          kin, vin, kout, vout, n, p, passes, 8 * p, 0, 0, ghist, st, st_next, tile_ctr, key_or);
+    } else if ((int)p == pack_pass) {
+      k_sort_onesweep<false, true><<<nt, kSortThreads, 0, s>>>(
+          kin, vin, kout, vout, n, p, passes, 8 * p, 8 * (p + 1), pbits, ghist, st, st_next,
+          tile_ctr, nullptr);
+    } else {
+      k_sort_onesweep<true, true><<<nt, kSortThreads, 0, s>>>(
+          kin, nullptr, kout, nullptr, n, p, passes, pbits + 8 * (p - pack_pass - 1), 0, pbits,
+          ghist, st, st_next, tile_ctr, nullptr);
+    }
     uint64_t* tk = kin;
     kin = kout;
     kout = tk;
@@ -812,6 +909,55 @@ hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, un
     default: return hipErrorInvalidValue;
   }
 #undef QCP_GATHER
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_unpack(const float* objs, const uint64_t* packed, uint64_t n,
+                                unsigned pbits, unsigned E, float* obj, uint64_t stride,
+                                uint32_t* perm_out, hipStream_t s) {
+  if (pbits == 0 || pbits > 32) return hipErrorInvalidValue;
+  unsigned g = grid_for(n, 256, 16384);
+  const bool vec = ((uintptr_t)objs & 15u) == 0;
+#define QCP_GU(EE)                                                                          \
+  if (vec) k_gather_unpack<EE, true><<<g, 256, 0, s>>>(objs, packed, n, pbits, obj, stride,  \
+                                                       perm_out);                           \
+  else k_gather_unpack<EE, false><<<g, 256, 0, s>>>(objs, packed, n, pbits, obj, stride, perm_out)
+  switch (E) {
+    case 2: QCP_GU(2); break;
+    case 3: QCP_GU(3); break;
+    case 4: QCP_GU(4); break;
+    case 6: QCP_GU(6); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef QCP_GU
+  return hipGetLastError();
+}
+
+hipError_t launch_keys_sorted(const float* obj, uint64_t stride, uint64_t n, unsigned D,
+                              bool box, bool ref, bool center, unsigned nb, uint64_t* keys,
+                              hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  unsigned g = grid_for(n, 256, 4096);
+#define QCP_KS(DD, BB, RR, CC) \
+  k_keys_sorted<DD, BB, RR, CC><<<g, 256, 0, s>>>(obj, stride, n, nb, keys)
+  if (D == 2) {
+    if (box) {
+      if (ref) { if (center) QCP_KS(2, true, true, true); else QCP_KS(2, true, true, false); }
+      else { if (center) QCP_KS(2, true, false, true); else QCP_KS(2, true, false, false); }
+    } else {
+      if (ref) QCP_KS(2, false, true, false); else QCP_KS(2, false, false, false);
+    }
+  } else if (D == 3) {
+    if (box) {
+      if (ref) { if (center) QCP_KS(3, true, true, true); else QCP_KS(3, true, true, false); }
+      else { if (center) QCP_KS(3, true, false, true); else QCP_KS(3, true, false, false); }
+    } else {
+      if (ref) QCP_KS(3, false, true, false); else QCP_KS(3, false, false, false);
+    }
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef QCP_KS
   return hipGetLastError();
 }
 

@@ -349,6 +349,7 @@ static unsigned graph_sort_passes(const qcp_index* ix) { return (ix->D * ix->nb 
 static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned passes) {
   hipStream_t s = ix->stream;
   const uint64_t n = ix->n;
+  ix->keys_stale = false;
   HIP_TRY(launch_fill_u32((uint32_t*)ix->d_key_or, 2, 0u, s));
   HIP_TRY(launch_fill_u32(ix->d_bad, 1, 0u, s));
   HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
@@ -411,6 +412,7 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
   uint64_t key_or = 0;
   uint32_t bad = 0;
   if (n > 0) {
+    ix->keys_stale = false;
     HIP_TRY(launch_fill_u32((uint32_t*)ix->d_key_or, 2, 0u, s));
     HIP_TRY(launch_fill_u32(ix->d_bad, 1, 0u, s));
     HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
@@ -430,9 +432,24 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
       bits = (unsigned)kb;
     }
     const unsigned passes = (bits + 7) / 8;
+    // Packed sort elements from the first pass after which the key bits still
+    // to be sorted and the input index fit 64 bits; the gather then recomputes
+    // the keys from the objects.  Not for k-means keys (not a function of the
+    // object alone) or Guttman builds (their host step reads the sorted keys).
+    int pack_pass = -1;
+    unsigned pbits = 1;
+    while (pbits < 32 && (1ull << pbits) < n) ++pbits;
+    if (!guttman && ix->cfg.order != QCP_ORDER_KMEANS_HILBERT) {
+      for (unsigned j = 0; j + 1 < passes; ++j) {
+        if (bits - 8 * (j + 1) + pbits <= 64) {
+          pack_pass = (int)j;
+          break;
+        }
+      }
+    }
     HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
-                             nullptr, ix->d_sort_ghist, ix->d_sort_status, s));
-    if (passes & 1u) {
+                             nullptr, ix->d_sort_ghist, ix->d_sort_status, s, pack_pass, pbits));
+    if (pack_pass < 0 && (passes & 1u)) {
       std::swap(ix->d_keys, ix->d_keys2);
       std::swap(ix->d_perm, ix->d_perm2);
     }
@@ -441,7 +458,14 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
       QCP_TRY(guttman_from_sorted(ix, d_objs, &host_ms));
       HIP_TRY(hipEventRecord(ix->ev[6], s));
     }
-    HIP_TRY(launch_gather(d_objs, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s));
+    if (pack_pass >= 0) {
+      const uint64_t* packed = (passes & 1u) ? ix->d_keys2 : ix->d_keys;
+      HIP_TRY(launch_gather_unpack(d_objs, packed, n, pbits, ix->E, ix->d_obj, ix->obj_stride,
+                                   ix->d_perm, s));
+      ix->keys_stale = true;
+    } else {
+      HIP_TRY(launch_gather(d_objs, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s));
+    }
     HIP_TRY(hipEventRecord(ix->ev[3], s));
     QCP_TRY(enqueue_boxes(ix));
   } else {
@@ -648,6 +672,12 @@ qcp_status qcp_export(const qcp_index* ix, uint64_t* sorted_keys, uint32_t* perm
   QCP_TRY(set_device(ix->device));
   const uint64_t n = ix->n;
   hipStream_t s = ix->stream;
+  if (sorted_keys && n && ix->keys_stale) {
+    HIP_TRY(launch_keys_sorted(ix->d_obj, ix->obj_stride, n, ix->D, ix->box,
+                               ix->cfg.key_mode == QCP_KEY_REF, ix->cfg.key_src == QCP_SRC_CENTER,
+                               ix->nb, ix->d_keys, s));
+    ix->keys_stale = false;
+  }
   if (sorted_keys && n) HIP_TRY(hipMemcpyAsync(sorted_keys, ix->d_keys, 8 * n, hipMemcpyDeviceToHost, s));
   if (perm && n) HIP_TRY(hipMemcpyAsync(perm, ix->d_perm, 4 * n, hipMemcpyDeviceToHost, s));
   std::vector<float> soa;

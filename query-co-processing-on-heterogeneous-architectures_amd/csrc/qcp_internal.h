@@ -46,6 +46,9 @@ struct qcp_index {
   // every object box has lo <= hi in each dimension, so a node box inside a
   // query implies that every object under it overlaps the query
   bool contain_ok = true;
+  // d_keys does not hold the sorted keys (a packed sort leaves them out);
+  // qcp_export recomputes them from the rank-ordered objects
+  mutable bool keys_stale = false;
   // host copies for the co-processing search: explicit topology (kept from the
   // build) and the node boxes [node][lo(D), hi(D)] (mirrored on first use,
   // cleared by every build)

@@ -31,10 +31,16 @@ hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* 
 // bit width then decides on the device how many passes actually reorder (the
 // rest end at once).  ghist: kMaxSortPasses*256 + kMaxSortPasses words;
 // status: sort_status_words(n) words.
+// pack_pass >= 0 (key_or null): pass pack_pass writes packed elements
+// ((key >> 8*(pack_pass+1)) << pbits) | val, which the remaining passes move
+// alone; the result is then packed, in keys (even pass count) or keys2, and
+// the caller unpacks it with launch_gather_unpack.  Needs
+// key bits - 8*(pack_pass+1) + pbits <= 64 and vals < 2^pbits.
 uint64_t sort_status_words(uint64_t n);
 hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2,
                             uint64_t n, unsigned passes, const unsigned long long* key_or,
-                            uint32_t* ghist, uint64_t* status, hipStream_t s);
+                            uint32_t* ghist, uint64_t* status, hipStream_t s, int pack_pass = -1,
+                            unsigned pbits = 0);
 constexpr uint32_t kSortGhistWords = 8 * 256 + 8;
 hipError_t scan_u32_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* block_sums,
                         uint32_t max_blocks, hipStream_t s);
@@ -42,6 +48,15 @@ hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
                         unsigned long long* block_sums, uint32_t max_blocks, hipStream_t s);
 hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, unsigned E,
                          float* obj, uint64_t stride, hipStream_t s);
+// launch_gather plus the unpacking of a packed sort result into perm_out; the
+// sorted keys are left to launch_keys_sorted.
+hipError_t launch_gather_unpack(const float* objs, const uint64_t* packed, uint64_t n,
+                                unsigned pbits, unsigned E, float* obj, uint64_t stride,
+                                uint32_t* perm_out, hipStream_t s);
+// Sorted keys from the rank-ordered SoA objects (the key of each rank's object).
+hipError_t launch_keys_sorted(const float* obj, uint64_t stride, uint64_t n, unsigned D,
+                              bool box, bool ref, bool center, unsigned nb, uint64_t* keys,
+                              hipStream_t s);
 hipError_t launch_leaf_boxes(const float* obj, uint64_t ostride, uint64_t n, unsigned D,
                              bool box, uint32_t F, uint64_t nleaves, float* nbox,
                              uint64_t nstride, uint64_t leaf_off, hipStream_t s);

@@ -99,6 +99,32 @@ def test_points_bvh(gpu, O, D, F):
     check_case(gpu, O, pts, D, False, F, layout="bvh")
 
 
+@pytest.mark.parametrize("D,box,scale,n,pack_pass", [
+    (2, False, 1e-5, 20000, None),   # 8-bit keys: one pass, nothing to pack
+    (2, False, 1.0, 50000, 0),       # 40-bit keys: 32 + 16 bits after the first pass
+    (3, False, 1.0, 50000, 1),       # 60-bit keys: 44 + 16 bits after the second
+    (2, False, 2000.0, 300000, 2),   # 62-bit keys, 19-bit indices: 38 + 19 after the third
+    (2, True, 1.0, 50000, 0),        # boxes keyed by their centres
+    (3, True, 1.0, 50000, 1),
+])
+def test_packed_sort_passes(gpu, O, D, box, scale, n, pack_pass):
+    """The eager build packs (key bits still to sort, input index) into one
+    64-bit word from the first pass after which they fit, and its gather
+    recomputes the keys from the objects; every packing point (and none) gives
+    the oracle's keys, permutation, objects and node boxes."""
+    objs = (rand_boxes(n, D, 41) if box else rand_points(n, D, 41)) * np.float32(scale)
+    objs = np.ascontiguousarray(objs, dtype=np.float32)
+    layout = "rtree" if box else "mphr"
+    t = _oracle_tree(O, objs, D, box, 64, "ref", "center" if box else "lo", layout)
+    bits = int(np.bitwise_or.reduce(t.keys)).bit_length()
+    passes = (bits + 7) // 8
+    pbits = max(1, int(n - 1).bit_length())
+    want = next((j for j in range(passes - 1) if bits - 8 * (j + 1) + pbits <= 64), None)
+    assert want == pack_pass, (bits, pbits, want)
+    check_case(gpu, O, objs, D, box, 64, key_src="center" if box else "lo", layout=layout,
+               nq=100, seed=7)
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 4095, 4096, 4097])
 def test_edge_sizes(gpu, O, n):
     pts = rand_points(n, 2, seed=n)
