@@ -268,42 +268,69 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ objs,
   }
 }
 
-// The gather after a sort that ended in packed elements (rank r holds
-// ((key >> dropped bits) << pbits) | input index): writes the rank-ordered
-// objects as k_gather does and the permutation.  The sorted keys are not
-// written (the packing dropped their low bytes, and like the reference's
-// build, which replaces them by ranks, nothing after the build needs them);
-// k_keys_sorted recomputes them from the rank-ordered objects on demand.
-template <int E, bool VEC>
-__global__ __launch_bounds__(256) void k_gather_unpack(const float* __restrict__ objs,
-                                                       const uint64_t* __restrict__ packed,
-                                                       uint64_t n, unsigned pbits,
-                                                       float* __restrict__ obj, uint64_t stride,
-                                                       uint32_t* __restrict__ perm_out) {
-  const uint64_t mask = (1ull << pbits) - 1ull;
+// The build's gather: one 64-rank chunk per wave and iteration (a
+// wave-uniform loop).  PACKED: the sort ended in packed elements (rank r holds
+// ((key >> dropped bits) << pbits) | input index); the permutation is unpacked
+// into perm_out, and the sorted keys are not written (the packing dropped
+// their low bytes, and like the reference's build, which replaces them by
+// ranks, nothing after the build needs them; k_keys_sorted recomputes them on
+// demand).  FL = 32 or 64 (packed layouts whose fanout is FL): the wave also
+// reduces the boxes of the leaves its chunk covers, with the padding rule of
+// k_leaf_boxes, so the leaf level does not read the objects again.
+template <int D, bool BOX, bool VEC, bool PACKED, int FL>
+__global__ __launch_bounds__(256) void k_gather_objs(
+    const float* __restrict__ objs, const uint32_t* __restrict__ perm,
+    const uint64_t* __restrict__ packed, uint64_t n, unsigned pbits, float* __restrict__ obj,
+    uint64_t stride, uint32_t* __restrict__ perm_out, float* __restrict__ nbox,
+    uint64_t nstride, uint64_t leaf_off) {
+  constexpr int E = BOX ? 2 * D : D;
+  const uint32_t lane = lane_id();
+  const uint64_t mask = PACKED ? (1ull << pbits) - 1ull : 0ull;
   const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
-  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; r + (kGatherU - 1) * T < n; r += kGatherU * T) {
-    uint32_t p[kGatherU];
-#pragma unroll
-    for (int u = 0; u < kGatherU; ++u) p[u] = (uint32_t)(packed[r + u * T] & mask);
-    ObjVec<E, VEC> o[kGatherU];
-#pragma unroll
-    for (int u = 0; u < kGatherU; ++u) o[u].load(objs + (uint64_t)p[u] * E);
-#pragma unroll
-    for (int u = 0; u < kGatherU; ++u) {
-#pragma unroll
-      for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r + u * T] = o[u].v[e];
-      perm_out[r + u * T] = p[u];
-    }
-  }
-  for (; r < n; r += T) {
-    const uint32_t p = (uint32_t)(packed[r] & mask);
+  for (uint64_t wb = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(uint32_t)(kWave - 1));
+       wb < n; wb += T) {
+    const uint64_t r = wb + lane;
+    const bool valid = r < n;
     ObjVec<E, VEC> o;
-    o.load(objs + (uint64_t)p * E);
+    if (valid) {
+      const uint32_t p = PACKED ? (uint32_t)(packed[r] & mask) : perm[r];
+      o.load(objs + (uint64_t)p * E);
 #pragma unroll
-    for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = o.v[e];
-    perm_out[r] = p;
+      for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = o.v[e];
+      if (PACKED) perm_out[r] = p;
+    }
+    if constexpr (FL > 0) {
+      float lo[D], hi[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        lo[d] = valid ? o.v[d] : __builtin_huge_valf();
+        hi[d] = valid ? o.v[BOX ? D + d : d] : -__builtin_huge_valf();
+      }
+#pragma unroll
+      for (int off = FL / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const float l = __shfl_xor(lo[d], off, kWave);
+          const float h = __shfl_xor(hi[d], off, kWave);
+          lo[d] = (lo[d] > l) ? l : lo[d];
+          hi[d] = (hi[d] < h) ? h : hi[d];
+        }
+      }
+      if ((lane & (FL - 1)) == 0 && valid) {
+        const uint64_t leaf = r / FL;  // r is this leaf's first rank
+        const bool pad = n - r < (uint64_t)FL;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          float l = lo[d], h = hi[d];
+          if (pad) {
+            l = (l > 1.0f) ? 1.0f : l;
+            h = (h < 0.0f) ? 0.0f : h;
+          }
+          nbox[(uint64_t)d * nstride + leaf_off + leaf] = l;
+          nbox[(uint64_t)(D + d) * nstride + leaf_off + leaf] = h;
+        }
+      }
+    }
   }
 }
 
@@ -912,24 +939,40 @@ hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, un
   return hipGetLastError();
 }
 
-hipError_t launch_gather_unpack(const float* objs, const uint64_t* packed, uint64_t n,
-                                unsigned pbits, unsigned E, float* obj, uint64_t stride,
-                                uint32_t* perm_out, hipStream_t s) {
-  if (pbits == 0 || pbits > 32) return hipErrorInvalidValue;
+hipError_t launch_gather_objs(const float* objs, const uint32_t* perm, const uint64_t* packed,
+                              uint64_t n, unsigned pbits, unsigned D, bool box, float* obj,
+                              uint64_t stride, uint32_t* perm_out, uint32_t fuse_leaves,
+                              float* nbox, uint64_t nstride, uint64_t leaf_off, hipStream_t s) {
+  const bool pk = packed != nullptr;
+  if (pk ? (pbits == 0 || pbits > 32 || perm_out == nullptr) : perm == nullptr)
+    return hipErrorInvalidValue;
+  if (fuse_leaves != 0 && fuse_leaves != 32 && fuse_leaves != 64) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
   unsigned g = grid_for(n, 256, 16384);
   const bool vec = ((uintptr_t)objs & 15u) == 0;
-#define QCP_GU(EE)                                                                          \
-  if (vec) k_gather_unpack<EE, true><<<g, 256, 0, s>>>(objs, packed, n, pbits, obj, stride,  \
-                                                       perm_out);                           \
-  else k_gather_unpack<EE, false><<<g, 256, 0, s>>>(objs, packed, n, pbits, obj, stride, perm_out)
-  switch (E) {
-    case 2: QCP_GU(2); break;
-    case 3: QCP_GU(3); break;
-    case 4: QCP_GU(4); break;
-    case 6: QCP_GU(6); break;
-    default: return hipErrorInvalidValue;
+#define QCP_GO3(DD, BB, VV, PP, FF)                                                        \
+  k_gather_objs<DD, BB, VV, PP, FF><<<g, 256, 0, s>>>(objs, perm, packed, n, pbits, obj,   \
+                                                      stride, perm_out, nbox, nstride,     \
+                                                      leaf_off)
+#define QCP_GO2(DD, BB, VV, PP)                                      \
+  switch (fuse_leaves) {                                             \
+    case 32: QCP_GO3(DD, BB, VV, PP, 32); break;                     \
+    case 64: QCP_GO3(DD, BB, VV, PP, 64); break;                     \
+    default: QCP_GO3(DD, BB, VV, PP, 0); break;                      \
   }
-#undef QCP_GU
+#define QCP_GO1(DD, BB)                                              \
+  if (vec) { if (pk) { QCP_GO2(DD, BB, true, true) } else { QCP_GO2(DD, BB, true, false) } } \
+  else { if (pk) { QCP_GO2(DD, BB, false, true) } else { QCP_GO2(DD, BB, false, false) } }
+  if (D == 2) {
+    if (box) { QCP_GO1(2, true) } else { QCP_GO1(2, false) }
+  } else if (D == 3) {
+    if (box) { QCP_GO1(3, true) } else { QCP_GO1(3, false) }
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef QCP_GO1
+#undef QCP_GO2
+#undef QCP_GO3
   return hipGetLastError();
 }
 

@@ -318,13 +318,30 @@ static qcp_status guttman_from_sorted(qcp_index* ix, const float* d_objs, double
 
 // Node boxes of the rank-ordered objects in d_obj: bottom-up over the packed
 // levels, or level by level of an explicit tree.
-static qcp_status enqueue_boxes(qcp_index* ix) {
+// Fanout whose leaf boxes the build's gather writes itself (0: none).
+static uint32_t fused_leaf_fanout(const qcp_index* ix) {
+  return (!ix->expl && ix->h >= 1 && (ix->F == 32 || ix->F == 64)) ? ix->F : 0;
+}
+
+// The build's gather (see launch_gather_objs) into the index's arrays.
+static hipError_t enqueue_gather(qcp_index* ix, const float* d_objs, const uint64_t* packed,
+                                 unsigned pbits) {
+  const uint32_t fl = fused_leaf_fanout(ix);
+  return launch_gather_objs(d_objs, packed ? nullptr : ix->d_perm, packed, ix->n, pbits, ix->D,
+                            ix->box, ix->d_obj, ix->obj_stride, packed ? ix->d_perm : nullptr,
+                            fl, ix->d_nbox, ix->node_stride, fl ? ix->level_off[ix->h - 1] : 0,
+                            ix->stream);
+}
+
+// leaves_done: the gather already wrote the leaf level (fused_leaf_fanout).
+static qcp_status enqueue_boxes(qcp_index* ix, bool leaves_done = false) {
   hipStream_t s = ix->stream;
   if (!ix->expl) {
     const int h = ix->h;
-    HIP_TRY(launch_leaf_boxes(ix->d_obj, ix->obj_stride, ix->n, ix->D, ix->box, ix->F,
-                              ix->level_count[h - 1], ix->d_nbox, ix->node_stride,
-                              ix->level_off[h - 1], s));
+    if (!leaves_done)
+      HIP_TRY(launch_leaf_boxes(ix->d_obj, ix->obj_stride, ix->n, ix->D, ix->box, ix->F,
+                                ix->level_count[h - 1], ix->d_nbox, ix->node_stride,
+                                ix->level_off[h - 1], s));
     for (int l = h - 2; l >= 0; --l)
       HIP_TRY(launch_level_boxes(ix->d_nbox, ix->node_stride, ix->D, ix->level_off[l + 1],
                                  ix->level_count[l + 1], ix->F, ix->level_off[l],
@@ -358,8 +375,8 @@ static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned pas
   if (passes & 1u) return QCP_EINVAL;  // odd pass counts are not captured
   HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
                            ix->d_key_or, ix->d_sort_ghist, ix->d_sort_status, s));
-  HIP_TRY(launch_gather(d_objs, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s));
-  return enqueue_boxes(ix);
+  HIP_TRY(enqueue_gather(ix, d_objs, nullptr, 0));
+  return enqueue_boxes(ix, fused_leaf_fanout(ix) != 0);
 }
 
 // Graph-captured build: captured once per input pointer, replayed after.
@@ -460,14 +477,13 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
     }
     if (pack_pass >= 0) {
       const uint64_t* packed = (passes & 1u) ? ix->d_keys2 : ix->d_keys;
-      HIP_TRY(launch_gather_unpack(d_objs, packed, n, pbits, ix->E, ix->d_obj, ix->obj_stride,
-                                   ix->d_perm, s));
+      HIP_TRY(enqueue_gather(ix, d_objs, packed, pbits));
       ix->keys_stale = true;
     } else {
-      HIP_TRY(launch_gather(d_objs, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s));
+      HIP_TRY(enqueue_gather(ix, d_objs, nullptr, 0));
     }
     HIP_TRY(hipEventRecord(ix->ev[3], s));
-    QCP_TRY(enqueue_boxes(ix));
+    QCP_TRY(enqueue_boxes(ix, fused_leaf_fanout(ix) != 0));
   } else {
     if (guttman) QCP_TRY(guttman_adopt(ix, nullptr, nullptr, nullptr, &host_ms));
     HIP_TRY(hipEventRecord(ix->ev[1], s));

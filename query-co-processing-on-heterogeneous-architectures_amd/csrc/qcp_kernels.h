@@ -48,11 +48,15 @@ hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
                         unsigned long long* block_sums, uint32_t max_blocks, hipStream_t s);
 hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, unsigned E,
                          float* obj, uint64_t stride, hipStream_t s);
-// launch_gather plus the unpacking of a packed sort result into perm_out; the
-// sorted keys are left to launch_keys_sorted.
-hipError_t launch_gather_unpack(const float* objs, const uint64_t* packed, uint64_t n,
-                                unsigned pbits, unsigned E, float* obj, uint64_t stride,
-                                uint32_t* perm_out, hipStream_t s);
+// The build's gather: obj[e][r] = objs[perm[r]][e], with the permutation
+// either given (perm) or unpacked from packed sort elements (packed, pbits;
+// written to perm_out; the sorted keys are then left to launch_keys_sorted).
+// fuse_leaves = 32 or 64 (packed layouts of that fanout): also writes the
+// leaf level's boxes (nbox at leaf_off), as launch_leaf_boxes would.
+hipError_t launch_gather_objs(const float* objs, const uint32_t* perm, const uint64_t* packed,
+                              uint64_t n, unsigned pbits, unsigned D, bool box, float* obj,
+                              uint64_t stride, uint32_t* perm_out, uint32_t fuse_leaves,
+                              float* nbox, uint64_t nstride, uint64_t leaf_off, hipStream_t s);
 // Sorted keys from the rank-ordered SoA objects (the key of each rank's object).
 hipError_t launch_keys_sorted(const float* obj, uint64_t stride, uint64_t n, unsigned D,
                               bool box, bool ref, bool center, unsigned nb, uint64_t* keys,
