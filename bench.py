@@ -715,6 +715,21 @@ def main():
         log("contained: %.3e queries/s, %.3f ms kernel, %d runs, leaf entries %d" %
             (contained["queries_per_s"], contained["kernel_ms"], contained["contained_runs"],
              contained["leaf_entries"]))
+    # counts alone, the reference search's own output (MPHR::Search keeps hit
+    # totals, src/tree/mphr.cpp:196-272); reported beside, never as, `value`
+    for _ in range(args.warmup):
+        b.run(hash=False, **kw)
+    cms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cst = b.run(hash=False, **kw)
+        cms.append(cst["search_ms"])
+    cwall = time.perf_counter() - t0
+    counts_only = {"queries_per_s": q * args.steps / cwall, "ms_per_step": cwall * 1e3 / args.steps,
+                   "search_ms": float(np.mean(cms)),
+                   "matches": bool(np.array_equal(b.results()["counts"], got["counts"]))}
+    log("counts only: %.3e queries/s, %.3f ms search kernel" %
+        (counts_only["queries_per_s"], counts_only["search_ms"]))
     # full sorted rank-ID lists (1.7e10 IDs, 68 GB, stay in HBM); the first
     # queries' lists are checked against the oracle
     id_runs = []
@@ -824,6 +839,7 @@ def main():
                      "hbm_GBps": (traffic["bytes_per_launch"] / (savg * 1e-3) / 1e9
                                   if traffic else None)},
         "contained": contained,
+        "counts_only": counts_only,
         "hybrid": hybrid,
         "id_lists": id_runs,
         "plain": plain,
