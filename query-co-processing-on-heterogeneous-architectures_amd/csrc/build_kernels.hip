@@ -211,6 +211,31 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const TI* __restric
 }
 
 // ---------------------------------------------------------------------------
+// The passes of a sort whose plan is made on the device from the OR of all
+// keys (graph-captured builds, which run a pass for every byte a key can
+// hold): `real` passes reorder, and the elements are packed from pass `pack`
+// on (-1: never; pbits = 0 disables packing).
+struct SortPlan {
+  unsigned real;
+  int pack;
+};
+
+__device__ __forceinline__ SortPlan sort_plan(unsigned passes, unsigned pbits,
+                                              const unsigned long long* key_or) {
+  const unsigned long long ko = *key_or;
+  const unsigned bits = ko ? 64u - (unsigned)__clzll((long long)ko) : 0u;
+  SortPlan pl;
+  pl.real = (bits + 7) / 8 < passes ? (bits + 7) / 8 : passes;
+  pl.pack = -1;
+  if (pbits)
+    for (unsigned j = 0; j + 1 < pl.real; ++j)
+      if (bits - 8 * (j + 1) + pbits <= 64) {
+        pl.pack = (int)j;
+        break;
+      }
+  return pl;
+}
+
 // Gather the rank-ordered objects into SoA: obj[e][r] = objs[perm[r]][e].
 // The reads are random (rank order against input order), so each thread keeps
 // kGatherU of them in flight: it loads kGatherU permutation entries, then the
@@ -278,7 +303,7 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ objs,
 // reduces the boxes of the leaves its chunk covers, with the padding rule of
 // k_leaf_boxes, so the leaf level does not read the objects again.
 template <int D, bool BOX, bool VEC, bool PACKED, int FL>
-__global__ __launch_bounds__(256) void k_gather_objs(
+__device__ __forceinline__ void gather_objs_body(
     const float* __restrict__ objs, const uint32_t* __restrict__ perm,
     const uint64_t* __restrict__ packed, uint64_t n, unsigned pbits, float* __restrict__ obj,
     uint64_t stride, uint32_t* __restrict__ perm_out, float* __restrict__ nbox,
@@ -332,6 +357,34 @@ __global__ __launch_bounds__(256) void k_gather_objs(
       }
     }
   }
+}
+
+template <int D, bool BOX, bool VEC, bool PACKED, int FL>
+__global__ __launch_bounds__(256) void k_gather_objs(
+    const float* __restrict__ objs, const uint32_t* __restrict__ perm,
+    const uint64_t* __restrict__ packed, uint64_t n, unsigned pbits, float* __restrict__ obj,
+    uint64_t stride, uint32_t* __restrict__ perm_out, float* __restrict__ nbox,
+    uint64_t nstride, uint64_t leaf_off) {
+  gather_objs_body<D, BOX, VEC, PACKED, FL>(objs, perm, packed, n, pbits, obj, stride, perm_out,
+                                            nbox, nstride, leaf_off);
+}
+
+// The same after a device-planned sort (k_sort_onesweep_planned over
+// `passes`): the plan, made from the OR of all keys, says whether the result
+// is packed (in `packed`) or a permutation (in `perm`).
+template <int D, bool BOX, bool VEC, int FL>
+__global__ __launch_bounds__(256) void k_gather_objs_planned(
+    const float* __restrict__ objs, const uint32_t* __restrict__ perm,
+    const uint64_t* __restrict__ packed, uint64_t n, unsigned pbits, float* __restrict__ obj,
+    uint64_t stride, uint32_t* __restrict__ perm_out, float* __restrict__ nbox,
+    uint64_t nstride, uint64_t leaf_off, unsigned passes,
+    const unsigned long long* __restrict__ key_or) {
+  if (sort_plan(passes, pbits, key_or).pack >= 0)
+    gather_objs_body<D, BOX, VEC, true, FL>(objs, perm, packed, n, pbits, obj, stride, perm_out,
+                                            nbox, nstride, leaf_off);
+  else
+    gather_objs_body<D, BOX, VEC, false, FL>(objs, perm, packed, n, pbits, obj, stride, perm_out,
+                                             nbox, nstride, leaf_off);
 }
 
 // Sorted keys recomputed from the rank-ordered SoA objects (obj[e][r]) by the
@@ -697,63 +750,39 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist_all(
     if (h[p][tid]) atomicAdd(&ghist[p * 256 + tid], h[p][tid]);
 }
 
-// One pass (digit `pass` of `passes`), its digit at bit `shift` of the input
-// element.  status: this pass's look-back words (ntiles x 256, zero at entry);
-// status_next: the next pass's, zeroed here (row blockIdx.x; null for the last
-// pass); tile_ctr[pass] zero at entry.
-// key_or (graph-captured builds, which run a pass for every byte a key can
-// hold): passes at or above the keys' actual byte width end at once, the first
-// of them copying its input to its output when the remaining pass count is odd,
-// so the sorted keys land where the full pass count would put them.
-// Packed elements: once the key bits still to be sorted and the index fit one
-// 64-bit word, a pass writes ((key >> pack_shift) << pbits) | index (OUTP) and
-// later passes move that word alone (INP): 8 bytes per key each way instead of
-// 8 + 4.
+// LDS of one pass: 52.5 KB in all, so that three workgroups share a CU's
+// 160 KB.  sv's first 256 words double as scan scratch.
+struct SortLds {
+  uint64_t sk[kSortTile];
+  uint32_t sv[kSortTile];
+  uint32_t gstart[256];  // global position of this tile's bucket
+  uint16_t lstart[256];  // the bucket's start inside the tile
+  uint16_t wcnt[kSortThreads / kWave][256];  // per wave: keys per digit, then its offsets
+  uint32_t s_tile;
+};
+
+// The work of one pass on one tile (see k_sort_onesweep).
 template <bool INP, bool OUTP>
-__global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
-    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+__device__ __forceinline__ void sort_pass_body(
+    SortLds& L, const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned pass,
-    unsigned passes, unsigned shift, unsigned pack_shift, unsigned pbits,
-    const uint32_t* __restrict__ ghist, uint64_t* status, uint64_t* __restrict__ status_next,
-    uint32_t* tile_ctr, const unsigned long long* __restrict__ key_or) {
+    unsigned shift, unsigned pack_shift, unsigned pbits, const uint32_t* __restrict__ ghist,
+    uint64_t* status, uint64_t* __restrict__ status_next, uint32_t* tile_ctr) {
   static_assert(OUTP || !INP, "a packed pass writes packed elements");
   constexpr int W = kSortThreads / kWave;
   constexpr int CH = kSortTile / kSortThreads;  // chunks of 64 keys per wave
-  // 52.5 KB in all, so that three workgroups share a CU's 160 KB
-  __shared__ uint64_t sk[kSortTile];
-  __shared__ uint32_t sv[kSortTile];  // its first 256 words double as scan scratch
-  __shared__ uint32_t gstart[256];  // global position of this tile's bucket
-  __shared__ uint16_t lstart[256];  // the bucket's start inside the tile
-  __shared__ uint16_t wcnt[W][256];  // per wave: keys per digit, then the wave's offsets
-  __shared__ uint32_t s_tile;
-  uint32_t* const scan_sh = sv;
+  uint32_t* const scan_sh = L.sv;
   const int tid = threadIdx.x;
   const int w = tid / kWave;
   const uint32_t lane = lane_id();
-  if (key_or != nullptr) {
-    const unsigned real = real_passes(passes, key_or);
-    if (pass >= real) {
-      if (pass == real && ((passes - real) & 1u)) {
-        const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-        for (uint32_t j = tid; j < kSortTile; j += kSortThreads) {
-          const uint64_t i = base + j;
-          if (i < n) {
-            kout[i] = kin[i];
-            if (!INP) vout[i] = vin[i];
-          }
-        }
-      }
-      return;
-    }
-  }
-  if (tid == 0) s_tile = atomicAdd(&tile_ctr[pass], 1u);
+  if (tid == 0) L.s_tile = atomicAdd(&tile_ctr[pass], 1u);
   if (status_next != nullptr) status_next[(uint64_t)blockIdx.x * 256 + tid] = 0;
   uint32_t tot;
   const uint32_t gbase = block_excl_scan<uint32_t>(ghist[pass * 256 + tid], scan_sh, &tot);
 #pragma unroll
-  for (int ww = 0; ww < W; ++ww) wcnt[ww][tid] = 0;
+  for (int ww = 0; ww < W; ++ww) L.wcnt[ww][tid] = 0;
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = L.s_tile;
   const uint64_t wbase = (uint64_t)tile * kSortTile + (uint64_t)w * CH * kWave;
   uint64_t k[CH];
   uint32_t v[CH], rk[CH];
@@ -776,10 +805,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
     const uint32_t r = rank_below(peers);
     // every lane reads its digit's count before the group's first lane adds
     // the group (one wave: the LDS serves its accesses in program order)
-    const uint32_t before = valid ? (uint32_t)wcnt[w][dig] : 0u;
+    const uint32_t before = valid ? (uint32_t)L.wcnt[w][dig] : 0u;
     rk[c] = before + r;
     __builtin_amdgcn_wave_barrier();
-    if (valid && r == 0) wcnt[w][dig] = (uint16_t)(before + (uint32_t)__popcll(peers));
+    if (valid && r == 0) L.wcnt[w][dig] = (uint16_t)(before + (uint32_t)__popcll(peers));
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
@@ -787,8 +816,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
   uint32_t cnt = 0;
 #pragma unroll
   for (int ww = 0; ww < W; ++ww) {
-    const uint32_t c = wcnt[ww][tid];
-    wcnt[ww][tid] = (uint16_t)cnt;
+    const uint32_t c = L.wcnt[ww][tid];
+    L.wcnt[ww][tid] = (uint16_t)cnt;
     cnt += c;
   }
   uint64_t* const mine = status + (uint64_t)tile * 256 + tid;
@@ -831,35 +860,91 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
     __hip_atomic_store(mine, kStatusPrefix | (uint64_t)(excl + cnt), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   }
-  gstart[tid] = gbase + excl;
-  lstart[tid] = (uint16_t)ls;
+  L.gstart[tid] = gbase + excl;
+  L.lstart[tid] = (uint16_t)ls;
 #pragma unroll
-  for (int ww = 0; ww < W; ++ww) wcnt[ww][tid] = (uint16_t)(wcnt[ww][tid] + ls);
+  for (int ww = 0; ww < W; ++ww) L.wcnt[ww][tid] = (uint16_t)(L.wcnt[ww][tid] + ls);
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     if (wbase + (uint64_t)c * kWave + lane < n) {
-      const uint32_t pos = wcnt[w][(uint32_t)(k[c] >> shift) & 0xFF] + rk[c];
-      sk[pos] = k[c];
-      if (!INP) sv[pos] = v[c];
+      const uint32_t pos = L.wcnt[w][(uint32_t)(k[c] >> shift) & 0xFF] + rk[c];
+      L.sk[pos] = k[c];
+      if (!INP) L.sv[pos] = v[c];
     }
   }
   __syncthreads();
   const uint64_t base = (uint64_t)tile * kSortTile;
   const uint32_t tcnt = (uint32_t)(n - base < (uint64_t)kSortTile ? n - base : kSortTile);
   for (uint32_t p = tid; p < tcnt; p += kSortThreads) {
-    const uint64_t kk = sk[p];
+    const uint64_t kk = L.sk[p];
     const uint32_t dig = (uint32_t)(kk >> shift) & 0xFF;
-    const uint32_t pos = gstart[dig] + (p - lstart[dig]);
+    const uint32_t pos = L.gstart[dig] + (p - L.lstart[dig]);
     if (INP) {
       kout[pos] = kk;
     } else if (OUTP) {
-      kout[pos] = ((kk >> pack_shift) << pbits) | (uint64_t)sv[p];
+      kout[pos] = ((kk >> pack_shift) << pbits) | (uint64_t)L.sv[p];
     } else {
       kout[pos] = kk;
-      vout[pos] = sv[p];
+      vout[pos] = L.sv[p];
     }
   }
+}
+
+// One pass (digit `pass` of `passes`), its digit at bit `shift` of the input
+// element.  status: this pass's look-back words (ntiles x 256, zero at entry);
+// status_next: the next pass's, zeroed here (row blockIdx.x; null for the last
+// pass); tile_ctr[pass] zero at entry.
+// Packed elements: once the key bits still to be sorted and the index fit one
+// 64-bit word, a pass writes ((key >> pack_shift) << pbits) | index (OUTP) and
+// later passes move that word alone (INP): 8 bytes per key each way instead of
+// 8 + 4.
+template <bool INP, bool OUTP>
+__global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned pass,
+    unsigned shift, unsigned pack_shift, unsigned pbits, const uint32_t* __restrict__ ghist,
+    uint64_t* status, uint64_t* __restrict__ status_next, uint32_t* tile_ctr) {
+  __shared__ SortLds L;
+  sort_pass_body<INP, OUTP>(L, kin, vin, kout, vout, n, pass, shift, pack_shift, pbits, ghist,
+                            status, status_next, tile_ctr);
+}
+
+// One pass of a device-planned sort.  Passes at or above `real` end at once,
+// the first of them copying its input to its output when the remaining pass
+// count is odd, so the result lands where the full pass count would put it
+// (packed when the plan packs).
+__global__ __launch_bounds__(kSortThreads) void k_sort_onesweep_planned(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned pass,
+    unsigned passes, unsigned pbits, const uint32_t* __restrict__ ghist, uint64_t* status,
+    uint64_t* __restrict__ status_next, uint32_t* tile_ctr,
+    const unsigned long long* __restrict__ key_or) {
+  __shared__ SortLds L;
+  const SortPlan pl = sort_plan(passes, pbits, key_or);
+  if (pass >= pl.real) {
+    if (pass == pl.real && ((passes - pl.real) & 1u)) {
+      const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+      for (uint32_t j = threadIdx.x; j < kSortTile; j += kSortThreads) {
+        const uint64_t i = base + j;
+        if (i < n) {
+          kout[i] = kin[i];
+          if (pl.pack < 0) vout[i] = vin[i];
+        }
+      }
+    }
+    return;
+  }
+  if (pl.pack < 0 || (int)pass < pl.pack)
+    sort_pass_body<false, false>(L, kin, vin, kout, vout, n, pass, 8 * pass, 0, 0, ghist, status,
+                                 status_next, tile_ctr);
+  else if ((int)pass == pl.pack)
+    sort_pass_body<false, true>(L, kin, vin, kout, vout, n, pass, 8 * pass, 8 * (pass + 1),
+                                pbits, ghist, status, status_next, tile_ctr);
+  else
+    sort_pass_body<true, true>(L, kin, vin, kout, vout, n, pass,
+                               pbits + 8 * (pass - (unsigned)pl.pack - 1), 0, pbits, ghist,
+                               status, status_next, tile_ctr);
 }
 
 uint64_t sort_status_words(uint64_t n) { return 2ull * sort_tiles(n > 0 ? n : 1) * 256; }
@@ -869,9 +954,9 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
                             uint32_t* ghist, uint64_t* status, hipStream_t s, int pack_pass,
                             unsigned pbits) {
   if (passes > (unsigned)kMaxSortPasses) return hipErrorInvalidValue;
-  if (pack_pass >= 0 && (key_or != nullptr || pack_pass + 1 >= (int)passes || pbits == 0 ||
-                         pbits > 32))
+  if (pack_pass >= 0 && (key_or != nullptr || pack_pass + 1 >= (int)passes || pbits == 0))
     return hipErrorInvalidValue;
+  if (pbits > 32) return hipErrorInvalidValue;
   if (n == 0 || passes == 0) return hipSuccess;
   const uint32_t nt = sort_tiles(n);
   const uint64_t words = (uint64_t)nt * 256;
@@ -887,17 +972,20 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
   for (unsigned p = 0; p < passes; ++p) {
     uint64_t* st = status + (uint64_t)(p & 1u) * words;
     uint64_t* st_next = p + 1 < passes ? status + (uint64_t)((p + 1) & 1u) * words : nullptr;
-    if (pack_pass < 0 || (int)p < pack_pass) {
+    if (key_or != nullptr) {
+      k_sort_onesweep_planned<<<nt, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, p, passes,
+                                                          pbits, ghist, st, st_next, tile_ctr,
+                                                          key_or);
+    } else if (pack_pass < 0 || (int)p < pack_pass) {
       k_sort_onesweep<false, false><<<nt, kSortThreads, 0, s>>>(
-          kin, vin, kout, vout, n, p, passes, 8 * p, 0, 0, ghist, st, st_next, tile_ctr, key_or);
+          kin, vin, kout, vout, n, p, 8 * p, 0, 0, ghist, st, st_next, tile_ctr);
     } else if ((int)p == pack_pass) {
       k_sort_onesweep<false, true><<<nt, kSortThreads, 0, s>>>(
-          kin, vin, kout, vout, n, p, passes, 8 * p, 8 * (p + 1), pbits, ghist, st, st_next,
-          tile_ctr, nullptr);
+          kin, vin, kout, vout, n, p, 8 * p, 8 * (p + 1), pbits, ghist, st, st_next, tile_ctr);
     } else {
       k_sort_onesweep<true, true><<<nt, kSortThreads, 0, s>>>(
-          kin, nullptr, kout, nullptr, n, p, passes, pbits + 8 * (p - pack_pass - 1), 0, pbits,
-          ghist, st, st_next, tile_ctr, nullptr);
+          kin, nullptr, kout, nullptr, n, p, pbits + 8 * (p - pack_pass - 1), 0, pbits, ghist,
+          st, st_next, tile_ctr);
     }
     uint64_t* tk = kin;
     kin = kout;
@@ -973,6 +1061,44 @@ hipError_t launch_gather_objs(const float* objs, const uint32_t* perm, const uin
 #undef QCP_GO1
 #undef QCP_GO2
 #undef QCP_GO3
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_objs_planned(const float* objs, const uint32_t* perm,
+                                      const uint64_t* packed, uint64_t n, unsigned pbits,
+                                      unsigned passes, const unsigned long long* key_or,
+                                      unsigned D, bool box, float* obj, uint64_t stride,
+                                      uint32_t* perm_out, uint32_t fuse_leaves, float* nbox,
+                                      uint64_t nstride, uint64_t leaf_off, hipStream_t s) {
+  if (perm == nullptr || packed == nullptr || perm_out == nullptr || key_or == nullptr ||
+      pbits > 32)
+    return hipErrorInvalidValue;
+  if (fuse_leaves != 0 && fuse_leaves != 32 && fuse_leaves != 64) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  unsigned g = grid_for(n, 256, 16384);
+  const bool vec = ((uintptr_t)objs & 15u) == 0;
+#define QCP_GP3(DD, BB, VV, FF)                                                              \
+  k_gather_objs_planned<DD, BB, VV, FF><<<g, 256, 0, s>>>(objs, perm, packed, n, pbits, obj, \
+                                                          stride, perm_out, nbox, nstride,   \
+                                                          leaf_off, passes, key_or)
+#define QCP_GP2(DD, BB, VV)                          \
+  switch (fuse_leaves) {                             \
+    case 32: QCP_GP3(DD, BB, VV, 32); break;         \
+    case 64: QCP_GP3(DD, BB, VV, 64); break;         \
+    default: QCP_GP3(DD, BB, VV, 0); break;          \
+  }
+#define QCP_GP1(DD, BB) \
+  if (vec) { QCP_GP2(DD, BB, true) } else { QCP_GP2(DD, BB, false) }
+  if (D == 2) {
+    if (box) { QCP_GP1(2, true) } else { QCP_GP1(2, false) }
+  } else if (D == 3) {
+    if (box) { QCP_GP1(3, true) } else { QCP_GP1(3, false) }
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef QCP_GP1
+#undef QCP_GP2
+#undef QCP_GP3
   return hipGetLastError();
 }
 

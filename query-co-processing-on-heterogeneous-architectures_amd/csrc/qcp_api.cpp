@@ -318,6 +318,13 @@ static qcp_status guttman_from_sorted(qcp_index* ix, const float* d_objs, double
 
 // Node boxes of the rank-ordered objects in d_obj: bottom-up over the packed
 // levels, or level by level of an explicit tree.
+// Bit width of the input index in a packed sort element (indices < n).
+static unsigned index_bits(uint64_t n) {
+  unsigned pbits = 1;
+  while (pbits < 32 && (1ull << pbits) < n) ++pbits;
+  return pbits;
+}
+
 // Fanout whose leaf boxes the build's gather writes itself (0: none).
 static uint32_t fused_leaf_fanout(const qcp_index* ix) {
   return (!ix->expl && ix->h >= 1 && (ix->F == 32 || ix->F == 64)) ? ix->F : 0;
@@ -366,17 +373,25 @@ static unsigned graph_sort_passes(const qcp_index* ix) { return (ix->D * ix->nb 
 static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned passes) {
   hipStream_t s = ix->stream;
   const uint64_t n = ix->n;
-  ix->keys_stale = false;
+  // the sort's plan (passes that reorder, packing) is made on the device, so
+  // d_keys may end packed: exports recompute the keys from the objects (the
+  // graph launch sites set keys_stale again for every replay)
+  ix->keys_stale = true;
   HIP_TRY(launch_fill_u32((uint32_t*)ix->d_key_or, 2, 0u, s));
   HIP_TRY(launch_fill_u32(ix->d_bad, 1, 0u, s));
   HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
                       ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, ix->d_perm,
                       ix->d_key_or, ix->d_bad, s));
   if (passes & 1u) return QCP_EINVAL;  // odd pass counts are not captured
+  const unsigned pbits = index_bits(n);
   HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
-                           ix->d_key_or, ix->d_sort_ghist, ix->d_sort_status, s));
-  HIP_TRY(enqueue_gather(ix, d_objs, nullptr, 0));
-  return enqueue_boxes(ix, fused_leaf_fanout(ix) != 0);
+                           ix->d_key_or, ix->d_sort_ghist, ix->d_sort_status, s, -1, pbits));
+  const uint32_t fl = fused_leaf_fanout(ix);
+  HIP_TRY(launch_gather_objs_planned(d_objs, ix->d_perm, ix->d_keys, n, pbits, passes,
+                                     ix->d_key_or, ix->D, ix->box, ix->d_obj, ix->obj_stride,
+                                     ix->d_perm, fl, ix->d_nbox, ix->node_stride,
+                                     fl ? ix->level_off[ix->h - 1] : 0, s));
+  return enqueue_boxes(ix, fl != 0);
 }
 
 // Graph-captured build: captured once per input pointer, replayed after.
@@ -405,6 +420,7 @@ static qcp_status index_run_build_graph(qcp_index* ix, const float* d_objs) {
   }
   HIP_TRY(hipEventRecord(ix->ev[0], s));
   HIP_TRY(hipGraphLaunch(ix->build_exec, s));
+  ix->keys_stale = true;  // each replay may leave packed elements in d_keys
   HIP_TRY(hipEventRecord(ix->ev[4], s));
   uint32_t bad = 0;
   HIP_TRY(hipMemcpyAsync(&bad, ix->d_bad, 4, hipMemcpyDeviceToHost, s));
@@ -454,8 +470,7 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
     // the keys from the objects.  Not for k-means keys (not a function of the
     // object alone) or Guttman builds (their host step reads the sorted keys).
     int pack_pass = -1;
-    unsigned pbits = 1;
-    while (pbits < 32 && (1ull << pbits) < n) ++pbits;
+    const unsigned pbits = index_bits(n);
     if (!guttman && ix->cfg.order != QCP_ORDER_KMEANS_HILBERT) {
       for (unsigned j = 0; j + 1 < passes; ++j) {
         if (bits - 8 * (j + 1) + pbits <= 64) {
@@ -1102,6 +1117,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     }
     HIP_TRY(hipEventRecord(b->ev[0], s));
     HIP_TRY(hipGraphLaunch(b->exec, s));
+    if (b->pipe_objs && ix->n) ix->keys_stale = true;  // the replayed rebuild's sort
     HIP_TRY(hipEventRecord(b->ev[1], s));
   } else {
     HIP_TRY(hipEventRecord(b->ev[0], s));

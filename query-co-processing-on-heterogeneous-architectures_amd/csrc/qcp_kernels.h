@@ -27,15 +27,18 @@ hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* 
                             hipStream_t s);
 // Whole LSD radix sort of (keys, vals) over `passes` 8-bit digits, one-sweep
 // form; the result is in keys/vals for an even pass count, keys2/vals2 for an
-// odd one.  key_or: null, or the device word holding the OR of all keys, whose
-// bit width then decides on the device how many passes actually reorder (the
-// rest end at once).  ghist: kMaxSortPasses*256 + kMaxSortPasses words;
-// status: sort_status_words(n) words.
+// odd one.  ghist: kMaxSortPasses*256 + kMaxSortPasses words; status:
+// sort_status_words(n) words.
 // pack_pass >= 0 (key_or null): pass pack_pass writes packed elements
 // ((key >> 8*(pack_pass+1)) << pbits) | val, which the remaining passes move
 // alone; the result is then packed, in keys (even pass count) or keys2, and
-// the caller unpacks it with launch_gather_unpack.  Needs
+// the caller unpacks it in launch_gather_objs.  Needs
 // key bits - 8*(pack_pass+1) + pbits <= 64 and vals < 2^pbits.
+// key_or non-null (graph-captured builds): the device word holding the OR of
+// all keys decides on the device how many passes reorder (the rest end at
+// once, the result landing where `passes` passes would put it) and, when
+// pbits > 0, from which pass the elements are packed (launch_gather_objs_planned
+// then unpacks).
 uint64_t sort_status_words(uint64_t n);
 hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2,
                             uint64_t n, unsigned passes, const unsigned long long* key_or,
@@ -57,6 +60,14 @@ hipError_t launch_gather_objs(const float* objs, const uint32_t* perm, const uin
                               uint64_t n, unsigned pbits, unsigned D, bool box, float* obj,
                               uint64_t stride, uint32_t* perm_out, uint32_t fuse_leaves,
                               float* nbox, uint64_t nstride, uint64_t leaf_off, hipStream_t s);
+// launch_gather_objs after a device-planned sort (launch_sort_keys with key_or):
+// the result is packed (in packed) or a permutation (in perm) as the plan says.
+hipError_t launch_gather_objs_planned(const float* objs, const uint32_t* perm,
+                                      const uint64_t* packed, uint64_t n, unsigned pbits,
+                                      unsigned passes, const unsigned long long* key_or,
+                                      unsigned D, bool box, float* obj, uint64_t stride,
+                                      uint32_t* perm_out, uint32_t fuse_leaves, float* nbox,
+                                      uint64_t nstride, uint64_t leaf_off, hipStream_t s);
 // Sorted keys from the rank-ordered SoA objects (the key of each rank's object).
 hipError_t launch_keys_sorted(const float* obj, uint64_t stride, uint64_t n, unsigned D,
                               bool box, bool ref, bool center, unsigned nb, uint64_t* keys,
