@@ -322,7 +322,7 @@ __device__ __forceinline__ void gather_objs_body(
       o.load(objs + (uint64_t)p * E);
 #pragma unroll
       for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = o.v[e];
-      if (PACKED) perm_out[r] = p;
+      if (PACKED || perm_out != nullptr) perm_out[r] = p;
     }
     if constexpr (FL > 0) {
       float lo[D], hi[D];
@@ -371,20 +371,25 @@ __global__ __launch_bounds__(256) void k_gather_objs(
 
 // The same after a device-planned sort (k_sort_onesweep_planned over
 // `passes`): the plan, made from the OR of all keys, says whether the result
-// is packed (in `packed`) or a permutation (in `perm`).
+// is packed or a permutation, and where the last reordering pass left it
+// (packed / perm for an even count of such passes, packed2 / perm2 for an odd
+// one).  The permutation always ends in perm_out (= perm).
 template <int D, bool BOX, bool VEC, int FL>
 __global__ __launch_bounds__(256) void k_gather_objs_planned(
-    const float* __restrict__ objs, const uint32_t* __restrict__ perm,
-    const uint64_t* __restrict__ packed, uint64_t n, unsigned pbits, float* __restrict__ obj,
-    uint64_t stride, uint32_t* __restrict__ perm_out, float* __restrict__ nbox,
-    uint64_t nstride, uint64_t leaf_off, unsigned passes,
+    const float* __restrict__ objs, const uint32_t* perm, const uint32_t* perm2,
+    const uint64_t* __restrict__ packed, const uint64_t* __restrict__ packed2, uint64_t n,
+    unsigned pbits, float* __restrict__ obj, uint64_t stride, uint32_t* perm_out,
+    float* __restrict__ nbox, uint64_t nstride, uint64_t leaf_off, unsigned passes,
     const unsigned long long* __restrict__ key_or) {
-  if (sort_plan(passes, pbits, key_or).pack >= 0)
-    gather_objs_body<D, BOX, VEC, true, FL>(objs, perm, packed, n, pbits, obj, stride, perm_out,
-                                            nbox, nstride, leaf_off);
+  const SortPlan pl = sort_plan(passes, pbits, key_or);
+  const bool second = (pl.real & 1u) != 0;
+  if (pl.pack >= 0)
+    gather_objs_body<D, BOX, VEC, true, FL>(objs, nullptr, second ? packed2 : packed, n, pbits,
+                                            obj, stride, perm_out, nbox, nstride, leaf_off);
   else
-    gather_objs_body<D, BOX, VEC, false, FL>(objs, perm, packed, n, pbits, obj, stride, perm_out,
-                                             nbox, nstride, leaf_off);
+    gather_objs_body<D, BOX, VEC, false, FL>(objs, second ? perm2 : perm, nullptr, n, pbits, obj,
+                                             stride, second ? perm_out : nullptr, nbox, nstride,
+                                             leaf_off);
 }
 
 // Sorted keys recomputed from the rank-ordered SoA objects (obj[e][r]) by the
@@ -911,9 +916,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
 }
 
 // One pass of a device-planned sort.  Passes at or above `real` end at once,
-// the first of them copying its input to its output when the remaining pass
-// count is odd, so the result lands where the full pass count would put it
-// (packed when the plan packs).
+// so the result stays where the last reordering pass put it: in the first
+// buffers for an even `real`, the second for an odd one (packed when the plan
+// packs); k_gather_objs_planned reads it there.
 __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep_planned(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned pass,
@@ -922,19 +927,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep_planned(
     const unsigned long long* __restrict__ key_or) {
   __shared__ SortLds L;
   const SortPlan pl = sort_plan(passes, pbits, key_or);
-  if (pass >= pl.real) {
-    if (pass == pl.real && ((passes - pl.real) & 1u)) {
-      const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-      for (uint32_t j = threadIdx.x; j < kSortTile; j += kSortThreads) {
-        const uint64_t i = base + j;
-        if (i < n) {
-          kout[i] = kin[i];
-          if (pl.pack < 0) vout[i] = vin[i];
-        }
-      }
-    }
-    return;
-  }
+  if (pass >= pl.real) return;
   if (pl.pack < 0 || (int)pass < pl.pack)
     sort_pass_body<false, false>(L, kin, vin, kout, vout, n, pass, 8 * pass, 0, 0, ghist, status,
                                  status_next, tile_ctr);
@@ -1064,23 +1057,24 @@ hipError_t launch_gather_objs(const float* objs, const uint32_t* perm, const uin
   return hipGetLastError();
 }
 
-hipError_t launch_gather_objs_planned(const float* objs, const uint32_t* perm,
-                                      const uint64_t* packed, uint64_t n, unsigned pbits,
-                                      unsigned passes, const unsigned long long* key_or,
-                                      unsigned D, bool box, float* obj, uint64_t stride,
-                                      uint32_t* perm_out, uint32_t fuse_leaves, float* nbox,
-                                      uint64_t nstride, uint64_t leaf_off, hipStream_t s) {
-  if (perm == nullptr || packed == nullptr || perm_out == nullptr || key_or == nullptr ||
-      pbits > 32)
+hipError_t launch_gather_objs_planned(const float* objs, uint32_t* perm, const uint32_t* perm2,
+                                      const uint64_t* packed, const uint64_t* packed2,
+                                      uint64_t n, unsigned pbits, unsigned passes,
+                                      const unsigned long long* key_or, unsigned D, bool box,
+                                      float* obj, uint64_t stride, uint32_t fuse_leaves,
+                                      float* nbox, uint64_t nstride, uint64_t leaf_off,
+                                      hipStream_t s) {
+  if (perm == nullptr || perm2 == nullptr || packed == nullptr || packed2 == nullptr ||
+      key_or == nullptr || pbits > 32)
     return hipErrorInvalidValue;
   if (fuse_leaves != 0 && fuse_leaves != 32 && fuse_leaves != 64) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
   unsigned g = grid_for(n, 256, 16384);
   const bool vec = ((uintptr_t)objs & 15u) == 0;
 #define QCP_GP3(DD, BB, VV, FF)                                                              \
-  k_gather_objs_planned<DD, BB, VV, FF><<<g, 256, 0, s>>>(objs, perm, packed, n, pbits, obj, \
-                                                          stride, perm_out, nbox, nstride,   \
-                                                          leaf_off, passes, key_or)
+  k_gather_objs_planned<DD, BB, VV, FF><<<g, 256, 0, s>>>(objs, perm, perm2, packed, packed2, \
+                                                          n, pbits, obj, stride, perm, nbox,   \
+                                                          nstride, leaf_off, passes, key_or)
 #define QCP_GP2(DD, BB, VV)                          \
   switch (fuse_leaves) {                             \
     case 32: QCP_GP3(DD, BB, VV, 32); break;         \

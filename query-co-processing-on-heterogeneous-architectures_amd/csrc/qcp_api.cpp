@@ -387,9 +387,9 @@ static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned pas
   HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
                            ix->d_key_or, ix->d_sort_ghist, ix->d_sort_status, s, -1, pbits));
   const uint32_t fl = fused_leaf_fanout(ix);
-  HIP_TRY(launch_gather_objs_planned(d_objs, ix->d_perm, ix->d_keys, n, pbits, passes,
-                                     ix->d_key_or, ix->D, ix->box, ix->d_obj, ix->obj_stride,
-                                     ix->d_perm, fl, ix->d_nbox, ix->node_stride,
+  HIP_TRY(launch_gather_objs_planned(d_objs, ix->d_perm, ix->d_perm2, ix->d_keys, ix->d_keys2,
+                                     n, pbits, passes, ix->d_key_or, ix->D, ix->box, ix->d_obj,
+                                     ix->obj_stride, fl, ix->d_nbox, ix->node_stride,
                                      fl ? ix->level_off[ix->h - 1] : 0, s));
   return enqueue_boxes(ix, fl != 0);
 }

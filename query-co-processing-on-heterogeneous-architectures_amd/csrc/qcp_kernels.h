@@ -36,9 +36,9 @@ hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* 
 // key bits - 8*(pack_pass+1) + pbits <= 64 and vals < 2^pbits.
 // key_or non-null (graph-captured builds): the device word holding the OR of
 // all keys decides on the device how many passes reorder (the rest end at
-// once, the result landing where `passes` passes would put it) and, when
-// pbits > 0, from which pass the elements are packed (launch_gather_objs_planned
-// then unpacks).
+// once, leaving the result where the last reordering pass put it) and, when
+// pbits > 0, from which pass the elements are packed; launch_gather_objs_planned
+// finds the result and unpacks it.
 uint64_t sort_status_words(uint64_t n);
 hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2,
                             uint64_t n, unsigned passes, const unsigned long long* key_or,
@@ -60,14 +60,17 @@ hipError_t launch_gather_objs(const float* objs, const uint32_t* perm, const uin
                               uint64_t n, unsigned pbits, unsigned D, bool box, float* obj,
                               uint64_t stride, uint32_t* perm_out, uint32_t fuse_leaves,
                               float* nbox, uint64_t nstride, uint64_t leaf_off, hipStream_t s);
-// launch_gather_objs after a device-planned sort (launch_sort_keys with key_or):
-// the result is packed (in packed) or a permutation (in perm) as the plan says.
-hipError_t launch_gather_objs_planned(const float* objs, const uint32_t* perm,
-                                      const uint64_t* packed, uint64_t n, unsigned pbits,
-                                      unsigned passes, const unsigned long long* key_or,
-                                      unsigned D, bool box, float* obj, uint64_t stride,
-                                      uint32_t* perm_out, uint32_t fuse_leaves, float* nbox,
-                                      uint64_t nstride, uint64_t leaf_off, hipStream_t s);
+// launch_gather_objs after a device-planned sort (launch_sort_keys with key_or
+// over keys/packed, keys2/packed2 and vals/perm, vals2/perm2): the plan says
+// whether the result is packed or a permutation and in which buffer; the
+// permutation ends in perm either way.
+hipError_t launch_gather_objs_planned(const float* objs, uint32_t* perm, const uint32_t* perm2,
+                                      const uint64_t* packed, const uint64_t* packed2,
+                                      uint64_t n, unsigned pbits, unsigned passes,
+                                      const unsigned long long* key_or, unsigned D, bool box,
+                                      float* obj, uint64_t stride, uint32_t fuse_leaves,
+                                      float* nbox, uint64_t nstride, uint64_t leaf_off,
+                                      hipStream_t s);
 // Sorted keys from the rank-ordered SoA objects (the key of each rank's object).
 hipError_t launch_keys_sorted(const float* obj, uint64_t stride, uint64_t n, unsigned D,
                               bool box, bool ref, bool center, unsigned nb, uint64_t* keys,
