@@ -36,11 +36,13 @@ def test_graph_build_replays(gpu, O, layout, D, box, F):
 
 def test_graph_build_key_widths(gpu, O):
     """A captured build runs a sort pass for every byte a key can hold (8 for
-    2-D REF keys); passes above the keys' actual byte width end on the device,
-    the first of them copying when the number of such passes is odd.  One
-    captured graph replayed over inputs whose keys need 0, 2, 5, 6, 7 and 8
-    bytes (points scaled inside the REF quantiser's domain) equals the eager
-    build and the oracle every time."""
+    2-D REF keys) and plans on the device: passes above the keys' actual byte
+    width end at once, elements are packed from the first pass after which the
+    remaining key bits and the index fit 64 bits, and the gather reads the
+    result from whichever buffer the last reordering pass wrote (odd and even
+    counts).  One captured graph replayed over inputs whose keys need 0, 2, 5,
+    6, 7 and 8 bytes (points scaled inside the REF quantiser's domain) equals
+    the eager build and the oracle every time."""
     n, D, F = 50000, 2, 64
     base = rand_points(n, D, 31)
     cases = [("zero", np.zeros_like(base)), ("x1e-4", base * np.float32(1e-4)),
