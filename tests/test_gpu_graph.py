@@ -40,12 +40,14 @@ def test_graph_build_key_widths(gpu, O):
     width end at once, elements are packed from the first pass after which the
     remaining key bits and the index fit 64 bits, and the gather reads the
     result from whichever buffer the last reordering pass wrote (odd and even
-    counts).  One captured graph replayed over inputs whose keys need 0, 2, 5,
-    6, 7 and 8 bytes (points scaled inside the REF quantiser's domain) equals
-    the eager build and the oracle every time."""
+    counts).  One captured graph replayed over inputs whose keys need 0, 1, 2,
+    5, 6, 7 and 8 bytes (points scaled inside the REF quantiser's domain; one
+    byte is the unpacked case) equals the eager build and the oracle every
+    time."""
     n, D, F = 50000, 2, 64
     base = rand_points(n, D, 31)
-    cases = [("zero", np.zeros_like(base)), ("x1e-4", base * np.float32(1e-4)),
+    cases = [("zero", np.zeros_like(base)), ("x1e-5", base * np.float32(1e-5)),
+             ("x1e-4", base * np.float32(1e-4)),
              ("x1", base), ("x16", base * np.float32(16.0)),
              ("x100", base * np.float32(100.0)), ("x2000", base * np.float32(2000.0)),
              ("x1 again", base)]
@@ -68,7 +70,7 @@ def test_graph_build_key_widths(gpu, O):
         for k in ("keys", "perm", "sobjs", "boxes"):
             assert_bits_equal(got[k], eager[k], "%s graph vs eager %s" % (name, k))
     buf.close()
-    assert widths[:6] == [0, 2, 5, 6, 7, 8], widths
+    assert widths[:7] == [0, 1, 2, 5, 6, 7, 8], widths
 
 
 @pytest.mark.parametrize("donate", [0, 1])
