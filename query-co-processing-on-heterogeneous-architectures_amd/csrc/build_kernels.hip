@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
     }
     const uint64_t k = object_key<D, BOX, REF, CENTER>(v, nb, tab, any_bad);
     keys[i] = k;
-    idx[i] = (uint32_t)i;
+    if (idx != nullptr) idx[i] = (uint32_t)i;  // callers whose sort reads input indices
     acc |= k;
   }
   // wave OR, one atomic per wave
@@ -318,7 +318,8 @@ __device__ __forceinline__ void gather_objs_body(
     const bool valid = r < n;
     ObjVec<E, VEC> o;
     if (valid) {
-      const uint32_t p = PACKED ? (uint32_t)(packed[r] & mask) : perm[r];
+      const uint32_t p = PACKED ? (uint32_t)(packed[r] & mask)
+                                : (perm != nullptr ? perm[r] : (uint32_t)r);
       o.load(objs + (uint64_t)p * E);
 #pragma unroll
       for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = o.v[e];
@@ -383,7 +384,10 @@ __global__ __launch_bounds__(256) void k_gather_objs_planned(
     const unsigned long long* __restrict__ key_or) {
   const SortPlan pl = sort_plan(passes, pbits, key_or);
   const bool second = (pl.real & 1u) != 0;
-  if (pl.pack >= 0)
+  if (pl.real == 0)  // no pass reordered: the permutation is the identity
+    gather_objs_body<D, BOX, VEC, false, FL>(objs, nullptr, nullptr, n, pbits, obj, stride,
+                                             perm_out, nbox, nstride, leaf_off);
+  else if (pl.pack >= 0)
     gather_objs_body<D, BOX, VEC, true, FL>(objs, nullptr, second ? packed2 : packed, n, pbits,
                                             obj, stride, perm_out, nbox, nstride, leaf_off);
   else
@@ -510,6 +514,11 @@ __global__ __launch_bounds__(256) void k_tree_boxes(const float* __restrict__ ob
 }
 
 // Fill helpers (kernels, so that build and search stay graph-capturable).
+__global__ void k_iota_u32(uint32_t* p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = (uint32_t)i;
+}
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -795,7 +804,8 @@ __device__ __forceinline__ void sort_pass_body(
   for (int c = 0; c < CH; ++c) {
     const uint64_t i = wbase + (uint64_t)c * kWave + lane;
     k[c] = i < n ? kin[i] : 0;
-    v[c] = (!INP && i < n) ? vin[i] : 0;
+    // the first pass has no index array: an element's index is its position
+    v[c] = (!INP && i < n) ? (vin != nullptr ? vin[i] : (uint32_t)i) : 0;
   }
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
@@ -959,7 +969,7 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
   const unsigned g = nt < 2048 ? nt : 2048;
   k_sort_hist_all<<<g, kSortThreads, 0, s>>>(keys, n, passes, key_or, ghist, status, words);
   uint64_t* kin = keys;
-  uint32_t* vin = vals;
+  uint32_t* vin = nullptr;  // the first pass takes positions as the input indices
   uint64_t* kout = keys2;
   uint32_t* vout = vals2;
   for (unsigned p = 0; p < passes; ++p) {
@@ -983,7 +993,7 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
     uint64_t* tk = kin;
     kin = kout;
     kout = tk;
-    uint32_t* tv = vin;
+    uint32_t* tv = p == 0 ? vals : vin;
     vin = vout;
     vout = tv;
   }
@@ -1265,6 +1275,12 @@ hipError_t launch_tree_boxes(const float* obj, uint64_t ostride, const uint32_t*
   } else {
     return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_iota_u32(uint32_t* p, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_iota_u32<<<grid_for(n, 256, 16384), 256, 0, s>>>(p, n);
   return hipGetLastError();
 }
 

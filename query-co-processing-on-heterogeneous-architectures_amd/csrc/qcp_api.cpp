@@ -380,7 +380,7 @@ static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned pas
   HIP_TRY(launch_fill_u32((uint32_t*)ix->d_key_or, 2, 0u, s));
   HIP_TRY(launch_fill_u32(ix->d_bad, 1, 0u, s));
   HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
-                      ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, ix->d_perm,
+                      ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, nullptr,
                       ix->d_key_or, ix->d_bad, s));
   if (passes & 1u) return QCP_EINVAL;  // odd pass counts are not captured
   const unsigned pbits = index_bits(n);
@@ -449,7 +449,7 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
     HIP_TRY(launch_fill_u32((uint32_t*)ix->d_key_or, 2, 0u, s));
     HIP_TRY(launch_fill_u32(ix->d_bad, 1, 0u, s));
     HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
-                        ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, ix->d_perm,
+                        ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, nullptr,
                         ix->d_key_or, ix->d_bad, s));
     HIP_TRY(hipMemcpyAsync(&key_or, ix->d_key_or, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(&bad, ix->d_bad, 4, hipMemcpyDeviceToHost, s));
@@ -481,6 +481,7 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
     }
     HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
                              nullptr, ix->d_sort_ghist, ix->d_sort_status, s, pack_pass, pbits));
+    if (passes == 0) HIP_TRY(launch_iota_u32(ix->d_perm, n, s));  // all keys equal
     if (pack_pass < 0 && (passes & 1u)) {
       std::swap(ix->d_keys, ix->d_keys2);
       std::swap(ix->d_perm, ix->d_perm2);

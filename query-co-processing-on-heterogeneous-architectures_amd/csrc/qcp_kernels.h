@@ -12,6 +12,8 @@ namespace qcp {
 // flag bits of the key kernel's `bad` word
 constexpr uint32_t kBadDomain = 1u;    // NaN, or outside the quantiser's domain
 constexpr uint32_t kBadInverted = 2u;  // a box with lo > hi in some dimension
+// keys[i] of each object and, when idx is non-null, idx[i] = i; the OR of all
+// keys into *key_or and domain / inverted-box flags into *bad.
 hipError_t launch_keys(const float* objs, uint64_t n, unsigned D, bool box, bool ref,
                        bool center, unsigned nb, uint64_t* keys, uint32_t* idx,
                        unsigned long long* key_or, uint32_t* bad, hipStream_t s);
@@ -88,6 +90,7 @@ hipError_t launch_tree_boxes(const float* obj, uint64_t ostride, const uint32_t*
                              uint32_t pad, uint64_t lo, uint64_t hi, float* nbox,
                              uint64_t nstride, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_iota_u32(uint32_t* p, uint64_t n, hipStream_t s);  // p[i] = i
 hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s);
 
 // search
