@@ -2,9 +2,8 @@
 //   Hilbert keys (LDS transition table) -> stable one-sweep LSD radix sort
 //   (8-bit digits, wave-level multisplit ranking, decoupled look-back) ->
 //   gather into rank-ordered SoA objects -> node MBRs (packed levels
-//   bottom-up, or explicit BVH / Guttman levels deepest first).  The
-//   per-pass histogram + scan + scatter sort (launch_sort_pass) orders query
-//   batches by their centres' keys.
+//   bottom-up, or explicit BVH / Guttman levels deepest first).  The same
+//   sort orders query batches by their centres' keys.
 //
 // Reference steps replaced: Tree::AssignHilbertIndexToBranches
 // (src/tree/tree.cpp:596-639), Sorter::Sort + ReassignHilbertIndexes
@@ -95,23 +94,6 @@ constexpr int kSortThreads = 256;
 constexpr int kSortKPT = 16;
 constexpr int kSortTile = kSortThreads * kSortKPT;
 
-__global__ __launch_bounds__(kSortThreads) void k_hist(const uint64_t* __restrict__ keys,
-                                                       uint64_t n, unsigned shift,
-                                                       uint32_t* __restrict__ hist,
-                                                       uint32_t ntiles) {
-  __shared__ uint32_t cnt[256];
-  const int tid = threadIdx.x;
-  cnt[tid] = 0;
-  __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-#pragma unroll
-  for (int it = 0; it < kSortKPT; ++it) {
-    uint64_t i = base + (uint64_t)it * kSortThreads + tid;
-    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 0xFF], 1u);
-  }
-  __syncthreads();
-  hist[(uint64_t)tid * ntiles + blockIdx.x] = cnt[tid];
-}
 
 // ---------------------------------------------------------------------------
 // Device-wide exclusive scan (three phases).  Each block owns `per_block`
@@ -594,101 +576,14 @@ hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
   return scan_impl<uint32_t, unsigned long long>(in, n, out, block_sums, max_blocks, s);
 }
 
-// Stable scatter of one 4096-key tile for one 8-bit digit.  Each of the 4
-// waves owns 1024 consecutive keys of the tile (16 chunks of 64) and loads them
-// all up front; it ranks each chunk by digit with a wave multisplit (8
-// ballots) against wave-private LDS counters, keeping keys and ranks in
-// registers.  After one barrier the per-digit offset of each wave inside the
-// tile is formed, after another the keys are placed into LDS in (digit, index)
-// order, and then written out bucket by bucket, so the keys of one bucket go
-// to consecutive global positions (runs of ~16 keys) instead of one scattered
-// 8-byte store each.  Three barriers per tile.
-__global__ __launch_bounds__(kSortThreads) void k_scatter(
-    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned shift,
-    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ offs, uint32_t ntiles) {
-  constexpr int W = kSortThreads / kWave;
-  constexpr int CH = kSortTile / kSortThreads;  // chunks of 64 keys per wave
-  __shared__ uint64_t sk[kSortTile];
-  __shared__ uint32_t sv[kSortTile];
-  __shared__ uint32_t gstart[256];  // global position of this tile's bucket
-  __shared__ uint32_t lstart[256];  // the bucket's start inside the tile
-  __shared__ uint32_t scan_sh[kScanThreads];
-  __shared__ uint32_t wcnt[W][256];  // per wave: keys per digit, then the wave's offsets
-  const int tid = threadIdx.x;
-  const int w = tid / kWave;
-  const uint32_t lane = lane_id();
-  gstart[tid] = offs[(uint64_t)tid * ntiles + blockIdx.x];
-  uint32_t tot_tile;
-  lstart[tid] = block_excl_scan<uint32_t>(hist[(uint64_t)tid * ntiles + blockIdx.x], scan_sh,
-                                          &tot_tile);
-#pragma unroll
-  for (int ww = 0; ww < W; ++ww) wcnt[ww][tid] = 0;
-  __syncthreads();
-  const uint64_t wbase = (uint64_t)blockIdx.x * kSortTile + (uint64_t)w * CH * kWave;
-  uint64_t k[CH];
-  uint32_t v[CH], rk[CH];
-#pragma unroll
-  for (int c = 0; c < CH; ++c) {
-    const uint64_t i = wbase + (uint64_t)c * kWave + lane;
-    k[c] = i < n ? kin[i] : 0;
-    v[c] = i < n ? vin[i] : 0;
-  }
-#pragma unroll
-  for (int c = 0; c < CH; ++c) {
-    const bool valid = wbase + (uint64_t)c * kWave + lane < n;
-    const uint32_t dig = (uint32_t)(k[c] >> shift) & 0xFF;
-    uint64_t peers = ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t bb = ballot((dig >> b) & 1u);
-      peers &= ((dig >> b) & 1u) ? bb : ~bb;
-    }
-    const uint32_t r = rank_below(peers);
-    // every lane reads its digit's count before the group's first lane adds
-    // the group (one wave: the LDS serves its accesses in program order)
-    const uint32_t before = valid ? wcnt[w][dig] : 0u;
-    rk[c] = before + r;
-    __builtin_amdgcn_wave_barrier();
-    if (valid && r == 0) wcnt[w][dig] = before + (uint32_t)__popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-  }
-  __syncthreads();
-  {
-    uint32_t acc = lstart[tid];  // digit tid: wave ww's keys start after earlier waves'
-#pragma unroll
-    for (int ww = 0; ww < W; ++ww) {
-      const uint32_t c = wcnt[ww][tid];
-      wcnt[ww][tid] = acc;
-      acc += c;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < CH; ++c) {
-    if (wbase + (uint64_t)c * kWave + lane < n) {
-      const uint32_t pos = wcnt[w][(uint32_t)(k[c] >> shift) & 0xFF] + rk[c];
-      sk[pos] = k[c];
-      sv[pos] = v[c];
-    }
-  }
-  __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-  const uint32_t cnt = (uint32_t)(n - base < (uint64_t)kSortTile ? n - base : kSortTile);
-  for (uint32_t p = tid; p < cnt; p += kSortThreads) {
-    const uint64_t kk = sk[p];
-    const uint32_t dig = (uint32_t)(kk >> shift) & 0xFF;
-    const uint32_t pos = gstart[dig] + (p - lstart[dig]);
-    kout[pos] = kk;
-    vout[pos] = sv[p];
-  }
-}
 
 // ---------------------------------------------------------------------------
-// One-sweep LSD radix sort of the index build.  One kernel reads the keys once
-// and counts the digits of every pass; each pass is then a single kernel: a
-// workgroup takes the next tile in start order (an atomic tile counter),
-// ranks it as k_scatter does, publishes its per-digit counts, and learns how
+// One-sweep LSD radix sort (index builds and query ordering).  One kernel reads
+// the keys once and counts the digits of every pass; each pass is then a
+// single kernel: a workgroup takes the next tile in start order (an atomic
+// tile counter), ranks it into LDS in (digit, index) order (a wave multisplit
+// of 8 ballots per key against wave-private counters), writes each digit's
+// keys as one contiguous run, publishes its per-digit counts, and learns how
 // many keys of each digit earlier tiles hold by looking back over their
 // published words (decoupled look-back): a word is 0 (not yet published),
 // kStatusAgg | count (that tile's own count) or kStatusPrefix | count (the
@@ -1000,17 +895,6 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
   return hipGetLastError();
 }
 
-hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
-                            uint32_t* vout, uint64_t n, unsigned shift, uint32_t* hist,
-                            uint32_t* hist_scan, uint32_t* block_sums, uint32_t max_blocks,
-                            hipStream_t s) {
-  uint32_t nt = sort_tiles(n);
-  k_hist<<<nt, kSortThreads, 0, s>>>(kin, n, shift, hist, nt);
-  hipError_t e = scan_u32_u32(hist, (uint64_t)nt * 256, hist_scan, block_sums, max_blocks, s);
-  if (e != hipSuccess) return e;
-  k_scatter<<<nt, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, hist, hist_scan, nt);
-  return hipGetLastError();
-}
 
 hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, unsigned E,
                          float* obj, uint64_t stride, hipStream_t s) {

@@ -782,9 +782,8 @@ void qcp_batch_destroy(qcp_batch* b) {
   dfree(b->d_qperm2);
   dfree(b->d_qkeys);
   dfree(b->d_qkeys2);
-  dfree(b->d_qhist);
-  dfree(b->d_qhist_scan);
-  dfree(b->d_qsums);
+  dfree(b->d_qghist);
+  dfree(b->d_qstatus);
   dfree(b->d_qor);
   dfree(b->d_qbad);
   dfree(b->d_ids);
@@ -816,14 +815,12 @@ static qcp_status batch_alloc(qcp_index* ix, uint64_t q, qcp_batch** out) {
   if ((s = dalloc(&b->d_offsets, q + 1)) != QCP_OK) return fail(s);
   if ((s = dalloc(&b->d_scan_sums, kMaxScanBlocks + 1)) != QCP_OK) return fail(s);
   {
-    const uint32_t nt = sort_tiles(q > 0 ? q : 1);
     if ((s = dalloc(&b->d_qperm, q)) != QCP_OK) return fail(s);
     if ((s = dalloc(&b->d_qperm2, q)) != QCP_OK) return fail(s);
     if ((s = dalloc(&b->d_qkeys, q)) != QCP_OK) return fail(s);
     if ((s = dalloc(&b->d_qkeys2, q)) != QCP_OK) return fail(s);
-    if ((s = dalloc(&b->d_qhist, (uint64_t)nt * 256)) != QCP_OK) return fail(s);
-    if ((s = dalloc(&b->d_qhist_scan, (uint64_t)nt * 256 + 1)) != QCP_OK) return fail(s);
-    if ((s = dalloc(&b->d_qsums, kMaxScanBlocks + 1)) != QCP_OK) return fail(s);
+    if ((s = dalloc(&b->d_qghist, kSortGhistWords)) != QCP_OK) return fail(s);
+    if ((s = dalloc(&b->d_qstatus, sort_status_words(q))) != QCP_OK) return fail(s);
     if ((s = dalloc(&b->d_qor, 1)) != QCP_OK) return fail(s);
     if ((s = dalloc(&b->d_qbad, 1)) != QCP_OK) return fail(s);
   }
@@ -939,31 +936,21 @@ unsigned qcp::persistent_grid(const qcp_index* ix, int out, unsigned block) {
 extern "C" {
 
 // Spatial processing order of a batch: the 16-bit Hilbert key of each query
-// box's centre, radix-sorted (stable) with the query index as payload.  All on
-// the index's stream, no host synchronisation.
+// box's centre, radix-sorted (stable, one-sweep) with the query index as
+// payload; 2D passes, an even count, so the order lands in d_qperm.  All on
+// the index's stream, no host synchronisation (graph-capturable).
 static qcp_status batch_order(qcp_batch* b) {
   qcp_index* ix = b->ix;
   hipStream_t s = ix->stream;
   const uint64_t q = b->q;
   HIP_TRY(launch_fill_u32((uint32_t*)b->d_qor, 2, 0u, s));
   HIP_TRY(launch_fill_u32(b->d_qbad, 1, 0u, s));
-  HIP_TRY(launch_keys(b->d_queries, q, ix->D, true, false, true, 16, b->d_qkeys, b->d_qperm,
+  HIP_TRY(launch_keys(b->d_queries, q, ix->D, true, false, true, 16, b->d_qkeys, nullptr,
                       b->d_qor, b->d_qbad, s));
-  uint64_t* kin = b->d_qkeys;
-  uint32_t* vin = b->d_qperm;
-  uint64_t* kout = b->d_qkeys2;
-  uint32_t* vout = b->d_qperm2;
   const unsigned passes = (16 * ix->D + 7) / 8;
-  for (unsigned p = 0; p < passes; ++p) {
-    HIP_TRY(launch_sort_pass(kin, vin, kout, vout, q, 8 * p, b->d_qhist, b->d_qhist_scan,
-                             b->d_qsums, kMaxScanBlocks, s));
-    std::swap(kin, kout);
-    std::swap(vin, vout);
-  }
-  if (vin != b->d_qperm) {
-    std::swap(b->d_qperm, b->d_qperm2);
-    std::swap(b->d_qkeys, b->d_qkeys2);
-  }
+  if (passes & 1u) return QCP_EINVAL;
+  HIP_TRY(launch_sort_keys(b->d_qkeys, b->d_qperm, b->d_qkeys2, b->d_qperm2, q, passes, nullptr,
+                           b->d_qghist, b->d_qstatus, s));
   return QCP_OK;
 }
 

@@ -70,9 +70,8 @@ struct qcp_batch {
   uint32_t* d_qperm2 = nullptr;
   uint64_t* d_qkeys = nullptr;
   uint64_t* d_qkeys2 = nullptr;
-  uint32_t* d_qhist = nullptr;
-  uint32_t* d_qhist_scan = nullptr;
-  uint32_t* d_qsums = nullptr;
+  uint32_t* d_qghist = nullptr;   // one-sweep sort: digit counts + tile counters
+  uint64_t* d_qstatus = nullptr;  // one-sweep sort: look-back words
   unsigned long long* d_qor = nullptr;
   uint32_t* d_qbad = nullptr;
   uint32_t* d_ids = nullptr;

@@ -23,10 +23,6 @@ hipError_t launch_assign(const float* objs, uint64_t n, unsigned D, bool box, bo
                          hipStream_t s);
 hipError_t launch_compose(uint64_t* keys, const uint32_t* cid, const uint32_t* rank, uint64_t n,
                           unsigned shift_key, unsigned shift_rank, hipStream_t s);
-hipError_t launch_sort_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
-                            uint32_t* vout, uint64_t n, unsigned shift, uint32_t* hist,
-                            uint32_t* hist_scan, uint32_t* block_sums, uint32_t max_blocks,
-                            hipStream_t s);
 // Whole LSD radix sort of (keys, vals) over `passes` 8-bit digits, one-sweep
 // form; the result is in keys/vals for an even pass count, keys2/vals2 for an
 // odd one.  ghist: kMaxSortPasses*256 + kMaxSortPasses words; status:
