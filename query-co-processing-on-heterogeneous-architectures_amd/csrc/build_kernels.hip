@@ -218,16 +218,11 @@ __device__ __forceinline__ SortPlan sort_plan(unsigned passes, unsigned pbits,
   return pl;
 }
 
-// Gather the rank-ordered objects into SoA: obj[e][r] = objs[perm[r]][e].
-// The reads are random (rank order against input order), so each thread keeps
-// kGatherU of them in flight: it loads kGatherU permutation entries, then the
-// kGatherU objects (8- or 16-byte vector loads where the object size allows),
-// then writes them.  VEC: the input pointer is 16-byte aligned (the launcher
-// checks), so objects of 2, 4 or 6 floats load as float2 / float4.
-constexpr int kGatherU = 4;
-
+// An object of E floats loaded as the widest aligned loads its size allows.
+// VEC: the input pointer is 16-byte aligned (the launchers check), so objects
+// of 2, 4 or 6 floats load as float2 / float4.
 template <int E, bool VEC>
-struct ObjVec {  // an object as the widest aligned loads its size allows
+struct ObjVec {
   float v[E];
   __device__ __forceinline__ void load(const float* __restrict__ o) {
     if constexpr (VEC && E % 4 == 0) {
@@ -249,31 +244,6 @@ struct ObjVec {  // an object as the widest aligned loads its size allows
   }
 };
 
-template <int E, bool VEC>
-__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ objs,
-                                                const uint32_t* __restrict__ perm, uint64_t n,
-                                                float* __restrict__ obj, uint64_t stride) {
-  const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
-  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; r + (kGatherU - 1) * T < n; r += kGatherU * T) {
-    uint32_t p[kGatherU];
-#pragma unroll
-    for (int u = 0; u < kGatherU; ++u) p[u] = perm[r + u * T];
-    ObjVec<E, VEC> o[kGatherU];
-#pragma unroll
-    for (int u = 0; u < kGatherU; ++u) o[u].load(objs + (uint64_t)p[u] * E);
-#pragma unroll
-    for (int u = 0; u < kGatherU; ++u)
-#pragma unroll
-      for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r + u * T] = o[u].v[e];
-  }
-  for (; r < n; r += T) {
-    ObjVec<E, VEC> o;
-    o.load(objs + (uint64_t)perm[r] * E);
-#pragma unroll
-    for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = o.v[e];
-  }
-}
 
 // The build's gather: one 64-rank chunk per wave and iteration (a
 // wave-uniform loop).  PACKED: the sort ended in packed elements (rank r holds
@@ -567,10 +537,6 @@ static hipError_t scan_impl(const TI* in, uint64_t n, TO* out, TO* block_sums,
   return hipGetLastError();
 }
 
-hipError_t scan_u32_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* block_sums,
-                        uint32_t max_blocks, hipStream_t s) {
-  return scan_impl<uint32_t, uint32_t>(in, n, out, block_sums, max_blocks, s);
-}
 hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
                         unsigned long long* block_sums, uint32_t max_blocks, hipStream_t s) {
   return scan_impl<uint32_t, unsigned long long>(in, n, out, block_sums, max_blocks, s);
@@ -896,23 +862,6 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
 }
 
 
-hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, unsigned E,
-                         float* obj, uint64_t stride, hipStream_t s) {
-  unsigned g = grid_for(n, 256, 16384);
-  const bool vec = ((uintptr_t)objs & 15u) == 0;
-#define QCP_GATHER(EE)                                                  \
-  if (vec) k_gather<EE, true><<<g, 256, 0, s>>>(objs, perm, n, obj, stride); \
-  else k_gather<EE, false><<<g, 256, 0, s>>>(objs, perm, n, obj, stride)
-  switch (E) {
-    case 2: QCP_GATHER(2); break;
-    case 3: QCP_GATHER(3); break;
-    case 4: QCP_GATHER(4); break;
-    case 6: QCP_GATHER(6); break;
-    default: return hipErrorInvalidValue;
-  }
-#undef QCP_GATHER
-  return hipGetLastError();
-}
 
 hipError_t launch_gather_objs(const float* objs, const uint32_t* perm, const uint64_t* packed,
                               uint64_t n, unsigned pbits, unsigned D, bool box, float* obj,

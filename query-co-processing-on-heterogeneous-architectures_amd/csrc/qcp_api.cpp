@@ -598,9 +598,9 @@ qcp_status qcp_build_ranked(const qcp_build_cfg* cfg, const float* objs, uint64_
       return fail(QCP_EHIP);
     }
     if (hipMemcpyAsync(ix->d_in, objs, (uint64_t)ix->E * 4 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-        launch_gather(ix->d_in, ix->d_perm, n, ix->E, ix->d_obj, ix->obj_stride, s) != hipSuccess)
+        enqueue_gather(ix, ix->d_in, nullptr, 0) != hipSuccess)
       return fail(QCP_EHIP);
-    if ((st = enqueue_boxes(ix)) != QCP_OK) return fail(st);
+    if ((st = enqueue_boxes(ix, fused_leaf_fanout(ix) != 0)) != QCP_OK) return fail(st);
     if (hipStreamSynchronize(s) != hipSuccess) return fail(QCP_EHIP);
     dfree(ix->d_in);
   }

@@ -43,12 +43,8 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
                             uint32_t* ghist, uint64_t* status, hipStream_t s, int pack_pass = -1,
                             unsigned pbits = 0);
 constexpr uint32_t kSortGhistWords = 8 * 256 + 8;
-hipError_t scan_u32_u32(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* block_sums,
-                        uint32_t max_blocks, hipStream_t s);
 hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
                         unsigned long long* block_sums, uint32_t max_blocks, hipStream_t s);
-hipError_t launch_gather(const float* objs, const uint32_t* perm, uint64_t n, unsigned E,
-                         float* obj, uint64_t stride, hipStream_t s);
 // The build's gather: obj[e][r] = objs[perm[r]][e], with the permutation
 // either given (perm) or unpacked from packed sort elements (packed, pbits;
 // written to perm_out; the sorted keys are then left to launch_keys_sorted).
