@@ -13,7 +13,7 @@
  *   - per-query hit counts (the reference's per-block h_hit[],
  *     src/tree/mphr.cpp:244-252) plus, new here, sorted rank-ID lists and an
  *     order-independent ID-set hash: the sum over hit rank IDs of
- *     id*id + 0x9E3779B97F4A7C15*id, mod 2^64.
+ *     id*id + 0x9E3779B97F4A7C15*id + 0xC2B2AE3D27D4EB4F*id*id*id, mod 2^64.
  * Conventions that differ from the reference on purpose: every error is a
  * status code (nothing calls exit()), every handle owns its device memory and
  * HIP stream (no device globals, no singletons), and inputs are caller-owned.

@@ -402,6 +402,13 @@ class Batch:
         self._mode = "persistent"
         return st.as_dict()
 
+    def offsets_range(self, q0, q1):
+        """Offsets (relative to query q0) of the ID lists of queries [q0, q1), without
+        copying the IDs themselves."""
+        off = np.zeros(q1 - q0 + 1, np.uint64)
+        check(lib().qcp_batch_ids(self._h, q0, q1, _p(off), None, 0), "qcp_batch_ids")
+        return off
+
     def ids_range(self, q0, q1):
         """Offsets (relative) and sorted rank-ID lists of queries [q0, q1)."""
         off = np.zeros(q1 - q0 + 1, np.uint64)

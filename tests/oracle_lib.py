@@ -15,6 +15,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "_build", "libqcp_oracle.so")
 
 KEY_REF, KEY_B16 = 0, 1
+HASH_K, HASH_K3 = 0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F
 SRC_LO, SRC_CENTER = 0, 1
 
 _lib = None
@@ -206,6 +207,32 @@ class Tree:
         lib().orc_search_dfs_mt(*self._args(), ptr(q), nq, nthreads, ptr(counts), ptr(hashes))
         return counts, hashes
 
+    def search_ids_mt(self, queries, nthreads):
+        """Counts, hashes, offsets and sorted ID lists of every query, the DFS run
+        over nthreads Python threads (contiguous query ranges; the C oracle runs
+        without the GIL)."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        counts, hashes = self.search_dfs_mt(q, nthreads)
+        offsets = np.zeros(nq + 1, dtype=np.uint64)
+        offsets[1:] = np.cumsum(counts.astype(np.uint64))
+        ids = np.zeros(max(int(offsets[-1]), 1), dtype=np.uint32)
+        parts = 8 * nthreads  # more ranges than threads: heavy queries cluster
+        cuts = [nq * t // parts for t in range(parts + 1)]
+        L = lib()
+
+        def run(t):
+            if cuts[t] < cuts[t + 1]:
+                L.orc_search_dfs(*self._args(), ptr(q), cuts[t], cuts[t + 1], None, None,
+                                 ptr(offsets), ptr(ids), None, None)
+
+        with ThreadPoolExecutor(max_workers=nthreads) as ex:
+            list(ex.map(run, range(parts)))
+        return {"counts": counts, "hashes": hashes, "offsets": offsets,
+                "ids": ids[: int(offsets[-1])]}
+
     def search_mprs_mt(self, queries, nthreads):
         assert self.layout == "packed"
         q = np.ascontiguousarray(queries, dtype=np.float32)
@@ -260,10 +287,11 @@ def search_brute(sobjs, D, is_box, queries, want_ids=False):
 
 
 def id_hash(ids):
-    """numpy restatement of orc_id_hash per ID (uint64 wraparound): id^2 + K*id."""
+    """numpy restatement of orc_id_hash per ID (uint64 wraparound):
+    id^2 + K*id + K3*id^3."""
     x = ids.astype(np.uint64)
     with np.errstate(over="ignore"):
-        return x * x + np.uint64(0x9E3779B97F4A7C15) * x
+        return x * x + np.uint64(HASH_K) * x + np.uint64(HASH_K3) * (x * x * x)
 
 
 def build_mt(objs, D, is_box, F, key_mode, key_src=SRC_LO, nthreads=8):

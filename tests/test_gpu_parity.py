@@ -22,7 +22,7 @@ def _oracle_tree(O, objs, D, is_box, F, key_mode, key_src, layout):
 
 
 def check_case(gpu, O, objs, D, is_box, F, key_mode="ref", key_src="lo", layout="mphr",
-               nq=200, seed=0, plain=True):
+               nq=200, seed=0, plain=True, queries=None):
     tag = "D=%d box=%d F=%d key=%s src=%s layout=%s n=%d" % (D, is_box, F, key_mode, key_src,
                                                              layout, len(objs))
     print("case", tag, flush=True)
@@ -40,7 +40,8 @@ def check_case(gpu, O, objs, D, is_box, F, key_mode="ref", key_src="lo", layout=
     assert_bits_equal(ex["count"], t.count, tag + " node count")
     assert_bits_equal(ex["leaf"], t.leaf, tag + " node leaf flag")
     assert_bits_equal(ex["boxes"], t.boxes[: t.n_nodes], tag + " node MBRs")
-    queries = rand_queries(nq, D, seed + 17, objs if n else None)
+    if queries is None:
+        queries = rand_queries(nq, D, seed + 17, objs if n else None)
     exp = t.search_dfs(queries, want_ids=True)
     got = ix.search(queries, mode="persistent", ids=True)
     assert_results_equal(got, exp, tag + " persistent")
@@ -346,3 +347,59 @@ def test_ids_range_pagination(gpu, O):
             assert_bits_equal(ids, exp["ids"][eo[0]:eo[-1]], "ids %d..%d" % (q0, q1))
         with pytest.raises(gpu.QcpError):
             b.ids_range(10, 201)
+
+
+def test_equal_moment_id_sets(gpu, O):
+    """Two queries whose hit sets {1,5,6} and {2,3,7} agree in count, sum and
+    sum of squares: their ID-set hashes differ (the cube term) and every search
+    form returns the oracle's lists and hashes."""
+    x = np.arange(1, 8, dtype=np.float32) * np.float32(0.1)
+    y = np.array([0.2, 0.6, 0.6, 0.9, 0.2, 0.2, 0.6], np.float32)  # rank r+1 -> y[r]
+    pts = np.ascontiguousarray(np.stack([x, y], 1))
+    q = np.array([[0.0, 0.1, 1.0, 0.3], [0.0, 0.5, 1.0, 0.7]], np.float32)
+    t = O.Tree(pts, 2, False, 4)
+    exp = t.search_dfs(q, want_ids=True)
+    assert exp["ids"].tolist() == [1, 5, 6, 2, 3, 7]
+    assert exp["hashes"][0] != exp["hashes"][1]
+    for layout in ("mphr", "bvh"):
+        ix = gpu.Index.build_ranked(pts, 2, fanout=4, layout=layout)
+        assert_results_equal(ix.search(q, ids=True), exp, "equal moments " + layout)
+        assert_results_equal(ix.search(q, donate=1, contain=True), exp,
+                             "equal moments contained " + layout, ids=False)
+        if layout == "mphr":
+            pl = ix.search(q, mode="plain")
+            assert_bits_equal(pl["hashes"], exp["hashes"], "equal moments plain")
+        ix.close()
+
+
+@pytest.mark.parametrize("layout,box", [("mphr", False), ("bvh", False), ("rtree", True)])
+def test_subnormal_coordinates(gpu, O, layout, box):
+    """Objects and query bounds in the FP32 subnormal range, queries whose
+    bounds miss an object by one subnormal step: the leaf test's subtraction
+    form equals the reference's compares only if denormals are kept."""
+    rng = np.random.default_rng(17)
+    n, D = 30000, 2
+    tiny = np.float32(2.0 ** -149)  # the smallest subnormal
+    lo = rng.integers(0, 1 << 16, size=(n, D)).astype(np.float32) * tiny
+    # a tenth of the objects in the normal range, so the key sort has work
+    normal = rng.random(n) < 0.1
+    lo[normal] = rng.random((int(normal.sum()), D), dtype=np.float32)
+    if box:
+        side = rng.integers(0, 1 << 10, size=(n, D)).astype(np.float32) * tiny
+        objs = np.ascontiguousarray(np.concatenate([lo, lo + side], 1), np.float32)
+    else:
+        objs = np.ascontiguousarray(lo)
+    qs = []
+    pick = rng.integers(0, n, size=64)
+    for i in pick:
+        p = lo[i]
+        qs.append(np.concatenate([p, p]))                   # exactly on an object
+        qs.append(np.concatenate([p + tiny, p + tiny]))     # one step above it
+        qs.append(np.concatenate([p - tiny, p - tiny]))     # one step below it
+    c = rng.integers(0, 1 << 16, size=(64, D)).astype(np.float32) * tiny
+    h = rng.integers(1, 1 << 13, size=(64, 1)).astype(np.float32) * tiny
+    qs.extend(np.concatenate([c - h, c + h], 1))
+    q = np.ascontiguousarray(np.array(qs, np.float32))
+    assert (np.abs(q[q != 0]) < np.float32(2.0 ** -126)).mean() > 0.5
+    check_case(gpu, O, objs, D, box, 64, layout=layout, nq=0, plain=layout != "bvh",
+               queries=q)
