@@ -290,44 +290,75 @@ qcp_status qcp_autotune(const qcp_build_cfg *cfg, const float *d_objs, uint64_t 
  * New (the reference has no serving path; its closest interface is the
  * per-batch MPHR::Search loop, src/tree/mphr.cpp:231-258).  A server keeps the
  * index resident and runs a dispatcher thread: requests are queued as they are
- * submitted, and whenever one of `slots` in-flight batch slots is free and the
- * result-buffer budget allows, the waiting requests (up to max_batch) are
- * admitted as one batch on that slot's HIP stream, so batches that arrive while
- * others are on the GPU overlap on the device.  Outputs: count + ID-set hash. */
+ * submitted, and whenever one of `slots` in-flight batch slots is free the
+ * waiting requests (up to max_batch) are taken as one batch on that slot's HIP
+ * stream, so batches that arrive while others are on the GPU overlap on the
+ * device.  Admission is charged the batch's result bytes against
+ * result_budget (a batch alone is always admitted):
+ *   - count + ID-set hash service: 12 bytes per query, checked before launch;
+ *   - ID-list service (outputs has QCP_OUT_IDS): the batch's count pass runs
+ *     first and sizes its sorted rank-ID lists; the ID pass is admitted when
+ *     4 bytes per ID + 12 per query fit beside the batches in flight.
+ * Results are kept in a table of max_requests entries until they are read:
+ * qcp_server_result / _wait read (and free) a count + hash result;
+ * qcp_server_result_ids reads (and frees) an ID-list result; qcp_server_release
+ * frees one unread.  Submissions are refused (QCP_EBUSY) only while the table
+ * is full of unread results. */
 typedef struct qcp_server qcp_server;
 typedef struct {
   uint32_t slots;              /* in-flight batches (HIP streams), default 4          */
   uint32_t max_batch;          /* queries per batch, default 4096                     */
   uint64_t result_budget;      /* bytes of result buffers in flight, default 64 MiB   */
-  uint32_t max_requests;       /* result-table capacity (tickets), default 1 << 22    */
+  uint32_t max_requests;       /* result-table entries (unread tickets), default 1 << 20 */
   uint32_t grid_blocks;        /* per-batch grid cap (0 = fill the device)            */
   uint32_t contain;            /* as qcp_search_cfg.contain (same results)            */
-  uint32_t reserved;
+  uint32_t outputs;            /* QCP_OUT_* mask: counts + hash (0 means that), or with
+                                  QCP_OUT_IDS also the sorted rank-ID lists          */
 } qcp_serve_cfg;
 typedef struct {
   uint64_t submitted, completed, batches;
   uint64_t deferred_by_budget; /* admissions postponed because the budget was full   */
   uint64_t max_batch_seen, max_inflight_seen;
-  double p50_us, p99_us, mean_us, max_us;  /* submit -> result latency           */
+  double p50_us, p99_us, mean_us, max_us;  /* submit -> result latency (p50/p99 from a
+                                              histogram of 1/128-octave bins)        */
+  uint64_t bytes_peak;         /* most result bytes admitted at once                 */
+  uint64_t ids_served;         /* rank IDs delivered (ID-list service)               */
 } qcp_serve_stats;
 
 qcp_status qcp_server_start(qcp_index *ix, const qcp_serve_cfg *cfg, qcp_server **out);
-/* Thread-safe, non-blocking: queue one query box; *ticket identifies it. */
+/* Thread-safe, non-blocking: queue one query box; *ticket identifies it.
+ * QCP_EBUSY while the result table is full of unread results. */
 qcp_status qcp_server_submit(qcp_server *sv, const float *box, uint64_t *ticket);
-/* QCP_OK with the result once done, QCP_EBUSY while pending. */
+/* QCP_OK with the result once done (the entry is freed, except with ID-list
+ * service, where the IDs are still to be read), QCP_EBUSY while pending,
+ * QCP_EINVAL for a ticket not in the table (never issued or already read). */
 qcp_status qcp_server_result(qcp_server *sv, uint64_t ticket, uint32_t *count, uint64_t *hash,
                              double *latency_us);
-/* Block until the ticket is done. */
+/* Block until the ticket is done, then as qcp_server_result. */
 qcp_status qcp_server_wait(qcp_server *sv, uint64_t ticket, uint32_t *count, uint64_t *hash,
                            double *latency_us);
+/* ID-list service: the sorted rank-ID list of a finished request.  ids == NULL:
+ * only *n_ids; otherwise the list is copied (QCP_EBUSY if cap is too small)
+ * and the entry freed. */
+qcp_status qcp_server_result_ids(qcp_server *sv, uint64_t ticket, uint32_t *ids, uint64_t cap,
+                                 uint64_t *n_ids);
+/* Free a finished request's entry without reading it. */
+qcp_status qcp_server_release(qcp_server *sv, uint64_t ticket);
 qcp_status qcp_server_stats(qcp_server *sv, qcp_serve_stats *out);
 /* Drain outstanding work and stop the dispatcher. */
 qcp_status qcp_server_stop(qcp_server *sv);
 /* Load driver: submits query i at start + arrival_s[i] (arrival_s ascending,
- * seconds) from a native thread, waits for all, and returns per-query results
- * and submit-to-result latencies. */
+ * seconds) from a native thread while another reads the results out in ticket
+ * order, and returns per-query results and submit-to-result latencies. */
 qcp_status qcp_server_replay(qcp_server *sv, const float *boxes, const double *arrival_s,
                              uint64_t n, uint32_t *counts, uint64_t *hashes, double *latency_us);
+/* The same for ID-list service: also offsets[n+1] (prefix of the counts) and
+ * the lists into ids (ids_cap entries; ids == NULL reads them out without
+ * copying; QCP_EBUSY at the end if they did not fit). */
+qcp_status qcp_server_replay_ids(qcp_server *sv, const float *boxes, const double *arrival_s,
+                                 uint64_t n, uint32_t *counts, uint64_t *hashes,
+                                 double *latency_us, uint64_t *offsets, uint32_t *ids,
+                                 uint64_t ids_cap);
 
 /* ---- seeded generator (replaces generator/cuRAND.cpp:17-75, without cuRAND) ----
  * Philox4x32-10 (Salmon et al., SC'11).  Word i of (seed, stream) is output

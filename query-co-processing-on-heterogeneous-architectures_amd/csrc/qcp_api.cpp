@@ -364,8 +364,11 @@ static qcp_status enqueue_boxes(qcp_index* ix, bool leaves_done = false) {
 }
 
 // Sort passes of a graph-captured build: every byte a key can hold (an even
-// count for every mode, so the sorted keys land back in d_keys / d_perm);
-// passes above the keys' actual width end at once on the device.
+// count for every mode).  Passes above the keys' actual width end at once on
+// the device, so the result lies where the last reordering pass left it:
+// d_keys/d_perm after an even number of reordering passes, d_keys2/d_perm2
+// after an odd number, packed or not (launch_gather_objs_planned reads the
+// same plan from the OR of the keys and finds it).
 static unsigned graph_sort_passes(const qcp_index* ix) { return (ix->D * ix->nb + 7) / 8; }
 
 // The whole build as one asynchronous sequence (no host synchronisation), for

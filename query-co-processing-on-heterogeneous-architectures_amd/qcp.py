@@ -79,7 +79,7 @@ class ServeCfg(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("result_budget", ctypes.c_uint64), ("max_requests", ctypes.c_uint32),
                 ("grid_blocks", ctypes.c_uint32), ("contain", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("outputs", ctypes.c_uint32)]
 
 
 class ServeStats(ctypes.Structure):
@@ -87,7 +87,8 @@ class ServeStats(ctypes.Structure):
                 ("batches", ctypes.c_uint64), ("deferred_by_budget", ctypes.c_uint64),
                 ("max_batch_seen", ctypes.c_uint64), ("max_inflight_seen", ctypes.c_uint64),
                 ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double),
-                ("mean_us", ctypes.c_double), ("max_us", ctypes.c_double)]
+                ("mean_us", ctypes.c_double), ("max_us", ctypes.c_double),
+                ("bytes_peak", ctypes.c_uint64), ("ids_served", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -149,6 +150,11 @@ def lib():
         L.qcp_server_stats.argtypes = [P, ctypes.POINTER(ServeStats)]
         L.qcp_server_stop.argtypes = [P]
         L.qcp_server_replay.argtypes = [P, P, P, ctypes.c_uint64, P, P, P]
+        L.qcp_server_replay_ids.argtypes = [P, P, P, ctypes.c_uint64, P, P, P, P, P,
+                                            ctypes.c_uint64]
+        L.qcp_server_result_ids.argtypes = [P, ctypes.c_uint64, P, ctypes.c_uint64,
+                                            ctypes.POINTER(ctypes.c_uint64)]
+        L.qcp_server_release.argtypes = [P, ctypes.c_uint64]
         L.qcp_philox4x32_10.argtypes = [P, P, P]
         L.qcp_philox4x32_10.restype = None
         for fn in ("qcp_gen_words", "qcp_gen_uniform", "qcp_gen_normal"):
@@ -475,13 +481,17 @@ def autotune(objs, dims, sample_queries, fanouts=(32, 64, 128), layout="mphr", k
 
 
 class Server:
-    """Continuous-batching server over a resident index (counts + ID-set hash)."""
+    """Continuous-batching server over a resident index: counts + ID-set hash,
+    or with ids=True also the sorted rank-ID lists (admission then charged the
+    lists' bytes, sized by each batch's count pass)."""
 
     def __init__(self, index, slots=4, max_batch=4096, result_budget=0, max_requests=0,
-                 grid_blocks=0, contain=False):
+                 grid_blocks=0, contain=False, ids=False):
         self.index = index
+        self.ids = ids
+        outputs = OUT_COUNTS | OUT_HASH | (OUT_IDS if ids else 0)
         cfg = ServeCfg(slots, max_batch, result_budget, max_requests, grid_blocks,
-                       1 if contain else 0, 0)
+                       1 if contain else 0, outputs)
         self._h = P()
         check(lib().qcp_server_start(index._h, ctypes.byref(cfg), ctypes.byref(self._h)),
               "qcp_server_start")
@@ -493,6 +503,8 @@ class Server:
         return t.value
 
     def wait(self, ticket):
+        """(count, hash, latency_us); with ID-list service the IDs are still to be
+        read with result_ids (or the ticket released)."""
         c = ctypes.c_uint32()
         h = ctypes.c_uint64()
         lat = ctypes.c_double()
@@ -500,16 +512,41 @@ class Server:
                                     ctypes.byref(lat)), "qcp_server_wait")
         return c.value, h.value, lat.value
 
-    def replay(self, boxes, arrival_s):
+    def result_ids(self, ticket):
+        """The sorted rank-ID list of a finished request (frees its entry)."""
+        n = ctypes.c_uint64()
+        check(lib().qcp_server_result_ids(self._h, ticket, None, 0, ctypes.byref(n)),
+              "qcp_server_result_ids")
+        out = np.zeros(max(n.value, 1), np.uint32)
+        check(lib().qcp_server_result_ids(self._h, ticket, _p(out), n.value, ctypes.byref(n)),
+              "qcp_server_result_ids")
+        return out[: n.value]
+
+    def release(self, ticket):
+        check(lib().qcp_server_release(self._h, ticket), "qcp_server_release")
+
+    def replay(self, boxes, arrival_s, ids_cap=None):
+        """Submit boxes[i] at arrival_s[i] (native threads), read every result out;
+        returns (counts, hashes, latency_us), plus (offsets, ids) with ID-list
+        service (ids_cap = 0: lists served but not copied)."""
         boxes = np.ascontiguousarray(boxes, np.float32)
         arr = np.ascontiguousarray(arrival_s, np.float64)
         n = len(arr)
         counts = np.zeros(max(n, 1), np.uint32)
         hashes = np.zeros(max(n, 1), np.uint64)
         lat = np.zeros(max(n, 1), np.float64)
-        check(lib().qcp_server_replay(self._h, _p(boxes), _p(arr), n, _p(counts), _p(hashes),
-                                      _p(lat)), "qcp_server_replay")
-        return counts[:n], hashes[:n], lat[:n]
+        if not self.ids:
+            check(lib().qcp_server_replay(self._h, _p(boxes), _p(arr), n, _p(counts),
+                                          _p(hashes), _p(lat)), "qcp_server_replay")
+            return counts[:n], hashes[:n], lat[:n]
+        offsets = np.zeros(n + 1, np.uint64)
+        cap = 0 if ids_cap is None else int(ids_cap)
+        idsb = np.zeros(max(cap, 1), np.uint32) if cap else None
+        check(lib().qcp_server_replay_ids(self._h, _p(boxes), _p(arr), n, _p(counts),
+                                          _p(hashes), _p(lat), _p(offsets), _p(idsb), cap),
+              "qcp_server_replay_ids")
+        return (counts[:n], hashes[:n], lat[:n], offsets,
+                idsb[: int(offsets[-1])] if idsb is not None else None)
 
     def stats(self):
         st = ServeStats()

@@ -177,6 +177,16 @@ def test_c4_served_results(gpu, O):
     assert stats["completed"] == len(queries) and stats["batches"] >= 1
     _log(t0, "c4: %d served queries equal, p50 %.3f ms p99 %.3f ms, %d batches" %
          (len(queries), stats["p50_us"] / 1e3, stats["p99_us"] / 1e3, stats["batches"]))
+    # ID-list service: every served list against the oracle's
+    exp = t.search_ids_mt(queries, _threads())
+    with gpu.Server(ix, ids=True, result_budget=16 << 20) as sv:
+        c, h, lat, off, ids = sv.replay(queries, arrivals, ids_cap=int(exp["offsets"][-1]))
+        stats = sv.stats()
+    assert_bits_equal(c, counts, "c4 served ID counts")
+    assert_bits_equal(off, exp["offsets"], "c4 served ID offsets")
+    assert_bits_equal(ids, exp["ids"], "c4 served ID lists")
+    _log(t0, "c4 ID lists: %d IDs equal, p50 %.3f ms p99 %.3f ms, peak %d result bytes" %
+         (len(ids), stats["p50_us"] / 1e3, stats["p99_us"] / 1e3, stats["bytes_peak"]))
     ix.close()
 
 
@@ -202,10 +212,16 @@ def test_c5_timed_build_paths(gpu, O):
                 _check_results(b, counts, hashes, tag + " donation", st)
                 st = b.run(donate=1, contain=True, block_waves=8)
                 _check_results(b, counts, hashes, tag + " donation + containment", st)
-                if graph:
+            if graph:
+                # all lists would be 4 bytes x 1.6e11 hits: the ID search runs on a
+                # batch of sampled queries (the heaviest among them)
+                idx = _sample(counts, k=300, heavy=20, seed=5)
+                sq = np.ascontiguousarray(queries[idx])
+                with ix.batch(sq) as b:
                     b.run(ids=True, donate=1)
-                    idx = _sample(counts, k=300, heavy=20, seed=5)
-                    _check_ids(b, t, queries, counts, idx, tag)
+                    n_ids = _check_ids(b, t, sq, counts[idx], np.arange(len(idx)), tag)
+                _log(t0, "%s: ID lists of %d sampled queries equal (%d IDs)" %
+                     (tag, len(idx), n_ids))
             ix.close()
             _log(t0, tag + " equal")
     finally:
