@@ -36,6 +36,114 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpus():
+    """The host CPUs this process may use, and the thread count the CPU legs use:
+    every CPU of the process's affinity set and cgroup quota, at most 16 (the
+    CPU share a one-GPU job is given on the measurement boxes)."""
+    info = {"os_cpu_count": os.cpu_count()}
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    info["affinity_cpus"] = aff
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpu_quota"] = quota
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    info["cpu_model"] = model
+    threads = aff
+    if quota:
+        threads = min(threads, max(1, int(quota)))
+    info["threads_used"] = max(1, min(threads, 16))
+    return info
+
+
+def measure_traffic(F, kw, steps=2, timeout_s=240):
+    """HBM bytes per launch of the tuned search, measured in this run: two
+    rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch)
+    over `bench.py --profile-traffic` in a child process, each scaled by what
+    the same counter reads for the 1 GiB device copy that run ends with.
+    Returns a dict, or None when a pass fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    cmd_tail = ["--", sys.executable, os.path.join(ROOT, "bench.py"), "--profile-traffic",
+                "--fanout", str(F), "--block-waves", str(kw["block_waves"]),
+                "--chunks", str(kw["chunks"]), "--donate", str(kw["donate"]),
+                "--steps", str(steps)]
+    out = {"commands": []}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="qcp_pmc_")
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc"] + cmd_tail
+        out["commands"].append(" ".join(["rocprofv3", "--pmc", counter, "--output-format", "csv",
+                                         "-d", "<dir>", "-o", "pmc"] + cmd_tail[:1] +
+                                        ["python3", "bench.py"] + cmd_tail[3:]))
+        log("traffic: %s pass" % counter)
+        try:
+            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           timeout=timeout_s, check=False)
+        except (OSError, subprocess.TimeoutExpired) as e:
+            log("traffic: %s pass failed: %s" % (counter, e))
+            shutil.rmtree(d, ignore_errors=True)
+            return None
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        search, copy = [], []
+        for fn in files:
+            with open(fn) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name = row.get("Kernel_Name", "")
+                    v = float(row["Counter_Value"])
+                    if "k_search" in name:
+                        search.append(v)
+                    elif "opy" in name:  # the runtime's copy kernels
+                        copy.append((int(row.get("Dispatch_Id", 0)), v))
+        shutil.rmtree(d, ignore_errors=True)
+        if not search or not copy:
+            log("traffic: %s pass gave no k_search or copy dispatch" % counter)
+            return None
+        # the run's last copy is the 1 GiB device-to-device one
+        factor = (1 << 30) / 1024.0 / max(copy)[1]
+        key = "fetch" if counter == "FETCH_SIZE" else "write"
+        out[key + "_kib_per_launch"] = float(np.mean(search))
+        out[key + "_factor"] = factor
+        out[key + "_bytes_per_launch"] = float(np.mean(search)) * factor * 1024.0
+    out["bytes_per_launch"] = out["fetch_bytes_per_launch"] + out["write_bytes_per_launch"]
+    return out
+
+
+def memory_probes(qcp):
+    """Read bandwidth of the levels a search's data can come from, with the leaf
+    scan's load form: a 2 MiB window (each XCD's L2), 200 MiB (the Infinity
+    Cache; about the 16M-point tree's size) and 4 GiB (HBM)."""
+    out = {}
+    for name, win, rounds in (("l2_2MiB", 2 << 20, 4096), ("infinity_cache_200MiB", 200 << 20, 200),
+                              ("hbm_4GiB", 4 << 30, 1024)):
+        g, ms = qcp.probe_read_gbps(win, rounds)
+        out[name] = g
+        log("probe %s: %.0f GB/s (%.3f ms)" % (name, g, ms))
+    return out
+
+
 def algorithmic_bytes(st, q, D, E):
     """Canonical-traversal bytes of one search launch: per query its box
     (2D*4), every entry of every visited internal node (2D*4 each), every
@@ -198,53 +306,72 @@ def cpu_baseline(qcp, ix, queries, got, budget_s, threads):
                                (mm, threads, mdt), "matches_gpu": mok}}
 
 
-def serving_bench(qcp, rates, seconds, scale, contain_rate=None):
+def serving_bench(qcp, rates, seconds, scale, contain_rate=None, ids=True):
     """The serving config: 33,554,432 uniform 3-D points resident (MPHR, F=64);
     Poisson arrivals at each rate for `seconds`; each query a box of volume
     selectivity log-uniform in [1e-7, 1e-3]; continuous batching over 4 slots.
-    Latency = submit -> result on the host."""
+    Count + hash service at every rate; with `ids`, the same arrivals again with
+    sorted rank-ID lists served to the host (admission charged the lists' bytes
+    against a 64 MiB budget).  Latency = submit -> result on the host; p50/p99
+    from every request's latency."""
     nmax = int(max(rates) * seconds)
     w = qcp.workload("c4", scale=scale, nq=nmax)
     ix = qcp.Index.build(w["objs"], 3, fanout=64)
     queries = w["queries"]
-    ref = ix.search(queries[:2000], donate=0)
+    nref = min(nmax, 5000)
+    with ix.batch(queries[:nref]) as rb:
+        rb.run(ids=True)
+        ref = rb.results(ids=True)
     out = []
-    for r in rates:
-        n = int(r * seconds)
+
+    def one(r, n, served_ids, contain=False):
         arr = qcp.poisson_arrivals(n, float(r), stream=3)
-        with qcp.Server(ix, slots=4, max_batch=4096) as sv:
-            counts, hashes, lat = sv.replay(queries[:n], arr)
+        k = min(n, nref)
+        ok_ids = None
+        if served_ids:
+            # untimed: the first k lists copied out and checked
+            with qcp.Server(ix, slots=4, max_batch=4096, ids=True) as sv:
+                _, _, _, _, idl = sv.replay(queries[:k], arr[:k], ids_cap=int(ref["offsets"][k]))
+            ok_ids = bool(np.array_equal(idl, ref["ids"][: int(ref["offsets"][k])]))
+        with qcp.Server(ix, slots=4, max_batch=4096, contain=contain, ids=served_ids) as sv:
+            if served_ids:
+                # every list served to the host and read out there (not copied on)
+                counts, hashes, lat, _, _ = sv.replay(queries[:n], arr, ids_cap=0)
+            else:
+                counts, hashes, lat = sv.replay(queries[:n], arr)
             st = sv.stats()
+        ok = bool(np.array_equal(counts[:k], ref["counts"][:k]) and
+                  np.array_equal(hashes[:k], ref["hashes"][:k]))
         span = float(np.max(arr + lat * 1e-6))
-        ok = bool(np.array_equal(counts[:2000], ref["counts"][: min(n, 2000)]) and
-                  np.array_equal(hashes[:2000], ref["hashes"][: min(n, 2000)]))
-        out.append({"qps_offered": r, "queries": n, "p50_ms": st["p50_us"] / 1e3,
-                    "p99_ms": st["p99_us"] / 1e3, "max_ms": st["max_us"] / 1e3,
-                    "throughput_qps": n / span, "batches": st["batches"],
-                    "mean_batch": n / max(st["batches"], 1),
-                    "max_inflight": st["max_inflight_seen"], "matches_batch_api": ok})
-        log("serving %6d qps: p50 %.3f ms p99 %.3f ms, %d batches" %
-            (r, st["p50_us"] / 1e3, st["p99_us"] / 1e3, st["batches"]))
+        rec = {"qps_offered": r, "queries": n, "outputs": "counts+hash+ids" if served_ids
+               else "counts+hash", "p50_ms": float(np.percentile(lat, 50)) / 1e3,
+               "p99_ms": float(np.percentile(lat, 99)) / 1e3, "max_ms": float(lat.max()) / 1e3,
+               "throughput_qps": n / span, "batches": st["batches"],
+               "mean_batch": n / max(st["batches"], 1), "max_inflight": st["max_inflight_seen"],
+               "matches_batch_api": ok}
+        if contain:
+            rec["contain"] = True
+        if served_ids:
+            rec.update({"ids_served": int(st["ids_served"]), "id_bytes_per_s":
+                        4.0 * st["ids_served"] / span, "result_bytes_peak": int(st["bytes_peak"]),
+                        "deferred_by_budget": int(st["deferred_by_budget"]),
+                        "ids_match_batch_api": ok_ids})
+        log("serving %6d qps%s%s: p50 %.3f ms p99 %.3f ms, %d batches" %
+            (r, " ids" if served_ids else "", " contain" if contain else "",
+             rec["p50_ms"], rec["p99_ms"], st["batches"]))
+        return rec
+
+    for r in rates:
+        out.append(one(r, int(r * seconds), False))
     if contain_rate:
-        # the same replay with the containment shortcut (reported apart)
-        r = contain_rate
-        n = int(r * seconds)
-        arr = qcp.poisson_arrivals(n, float(r), stream=3)
-        with qcp.Server(ix, slots=4, max_batch=4096, contain=True) as sv:
-            counts, hashes, lat = sv.replay(queries[:n], arr)
-            st = sv.stats()
-        ok = bool(np.array_equal(counts[:2000], ref["counts"][: min(n, 2000)]) and
-                  np.array_equal(hashes[:2000], ref["hashes"][: min(n, 2000)]))
-        out.append({"qps_offered": r, "queries": n, "contain": True,
-                    "p50_ms": st["p50_us"] / 1e3, "p99_ms": st["p99_us"] / 1e3,
-                    "max_ms": st["max_us"] / 1e3, "batches": st["batches"],
-                    "matches_batch_api": ok})
-        log("serving %6d qps, contain: p50 %.3f ms p99 %.3f ms" %
-            (r, st["p50_us"] / 1e3, st["p99_us"] / 1e3))
+        out.append(one(contain_rate, int(contain_rate * seconds), False, contain=True))
+    if ids:
+        for r in rates:
+            out.append(one(r, int(r * seconds), True))
     ix.close()
     return {"workload": "33,554,432 uniform 3-D points, Poisson arrivals, selectivity "
-                        "log-uniform [1e-7,1e-3], counts+hash", "seconds_per_rate": seconds,
-            "rates": out}
+                        "log-uniform [1e-7,1e-3]; counts+hash, then sorted ID lists",
+            "seconds_per_rate": seconds, "rates": out}
 
 
 def oracle_tree_from(ix):
@@ -294,7 +421,7 @@ def timed_search(b, steps, **kw):
 
 def run_config(qcp, name, args):
     """Measurement of one of the other configurations; returns a JSON-able dict."""
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_cpus()["threads_used"]
     out = {"config": name}
     if name == "c1":
         w = qcp.workload("c1")
@@ -595,24 +722,6 @@ def profile_traffic(qcp, args):
                       "copy_rc": rc, "launches": launches}), flush=True)
 
 
-def read_traffic(name, F, kw):
-    """HBM bytes per launch measured for this search configuration (counts +
-    hash), from the committed counter summary, or None if it was not measured."""
-    p = os.path.join(ROOT, "profiles", name)
-    if not os.path.exists(p):
-        return None
-    try:
-        with open(p) as f:
-            t = json.load(f)
-    except Exception:
-        return None
-    key = (F, kw["block_waves"], kw["chunks"], kw["donate"])
-    for e in t.get("configs", []):
-        if (e["fanout"], e["block_waves"], e["chunks"], e["donate"]) == key:
-            return e
-    return None
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -628,7 +737,9 @@ def main():
     ap.add_argument("--no-ids", action="store_true")
     ap.add_argument("--no-contain", action="store_true",
                     help="skip the separately reported containment-shortcut run")
-    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure HBM traffic")
     ap.add_argument("--profile-traffic", action="store_true",
                     help="short run for rocprofv3 --pmc: the search at the given fanout, "
                          "then a 1 GiB device-to-device copy as the calibration kernel")
@@ -730,33 +841,43 @@ def main():
                    "matches": bool(np.array_equal(b.results()["counts"], got["counts"]))}
     log("counts only: %.3e queries/s, %.3f ms search kernel" %
         (counts_only["queries_per_s"], counts_only["search_ms"]))
-    # full sorted rank-ID lists (1.7e10 IDs, 68 GB, stay in HBM); the first
-    # queries' lists are checked against the oracle
+    # full sorted rank-ID lists (1.7e10 IDs, 68 GB, stay in HBM); the lists of
+    # 1,000 sampled queries plus the 50 heaviest are checked against the oracle
     id_runs = []
     if not args.no_ids:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         t_small = oracle_tree_from(ix)
-        exp_small = t_small.search_dfs(queries[:40], want_ids=True)
+        cnt = got["counts"]
+        rng = np.random.default_rng(11)
+        sample = np.unique(np.concatenate([np.argsort(cnt, kind="stable")[::-1][:50],
+                                           rng.choice(q, size=min(q, 1000), replace=False)]))
+        exp_s = t_small.search_ids_mt(queries[sample], host_cpus()["threads_used"])
         for contain in (False, True):
             b.run(ids=True, contain=contain, **kw)
             t0 = time.perf_counter()
             ist = b.run(ids=True, contain=contain, **kw)
             iwall = time.perf_counter() - t0
-            off, idl = b.ids_range(0, 40)
-            ok = bool(np.array_equal(off, exp_small["offsets"]) and
-                      np.array_equal(idl, exp_small["ids"]))
+            ok = True
+            for j, qi in enumerate(sample):
+                _, idl = b.ids_range(int(qi), int(qi) + 1)
+                e0, e1 = int(exp_s["offsets"][j]), int(exp_s["offsets"][j + 1])
+                ok = ok and bool(np.array_equal(idl, exp_s["ids"][e0:e1]))
             id_runs.append({"contain": contain, "queries_per_s": q / iwall, "wall_ms": iwall * 1e3,
                             "device_ms": ist["kernel_ms"], "ids": int(ist["total_hits"]),
-                            "id_bytes": 4 * int(ist["total_hits"]), "first_40_match_oracle": ok})
-            log("IDs%s: %.3e queries/s, %.1f ms wall, %.1f ms device, %d IDs" %
-                (" (contain)" if contain else "", q / iwall, iwall * 1e3, ist["kernel_ms"],
-                 ist["total_hits"]))
+                            "id_bytes": 4 * int(ist["total_hits"]),
+                            "sampled_lists_checked": int(len(sample)),
+                            "sampled_ids_checked": int(exp_s["offsets"][-1]),
+                            "sampled_lists_match_oracle": ok})
+            log("IDs%s: %.3e queries/s, %.1f ms wall, %.1f ms device, %d IDs; %d sampled "
+                "lists %s" % (" (contain)" if contain else "", q / iwall, iwall * 1e3,
+                              ist["kernel_ms"], ist["total_hits"], len(sample),
+                              "equal the oracle's" if ok else "DIFFER"))
         b.run(**kw)  # counts + hashes again for the checks below
     # the CPU-GPU co-processing form: host threads walk the upper levels, the
     # GPU searches the subtrees of the hand-off level (leaves = the reference's split)
     hybrid = []
     if not args.no_hybrid:
-        hthreads = min(os.cpu_count() or 1, 16)
+        hthreads = host_cpus()["threads_used"]
         for level in (0, 1):
             b.run(mode="hybrid", handoff_level=level, cpu_threads=hthreads)
             hst = b.run(mode="hybrid", handoff_level=level, cpu_threads=hthreads)
@@ -770,18 +891,30 @@ def main():
                 (level or "leaves", q / (hst["total_ms"] * 1e-3), hst["total_ms"]))
     plain = None
     if not args.no_plain:
-        pq = queries[: min(q, 20000)]
-        with ix.batch(pq) as pb:
-            pst = pb.run(mode="plain", plain_batch=128)
-            pres = pb.results()
-        plain = {"queries_per_s": len(pq) / (pst["total_ms"] * 1e-3), "sample_queries": len(pq),
-                 "batch": 128, "total_ms": pst["total_ms"], "launches": pst["launches"],
-                 "matches_persistent": bool(np.array_equal(pres["counts"], got["counts"][: len(pq)]))}
-        log("plain: %.3e queries/s over %d queries" % (plain["queries_per_s"], len(pq)))
+        # the reference's scheme on the same 100,000 queries: one launch per
+        # 128 queries, a synchronous copy of the counts after each; one wave per
+        # query, and the reference's 192-thread (3-wave) workgroup
+        forms = []
+        for pw in (1, 3):
+            pst = b.run(mode="plain", plain_batch=128, plain_waves=pw)
+            pres = b.results()
+            forms.append({"waves_per_workgroup": pw, "queries_per_s": q / (pst["total_ms"] * 1e-3),
+                          "total_ms": pst["total_ms"], "launches": pst["launches"],
+                          "matches_persistent": bool(
+                              np.array_equal(pres["counts"], got["counts"]) and
+                              np.array_equal(pres["hashes"], got["hashes"]))})
+            log("plain (%d-wave workgroups): %.3e queries/s over %d queries" %
+                (pw, forms[-1]["queries_per_s"], q))
+        best = max(forms, key=lambda f: f["queries_per_s"])
+        plain = {"queries_per_s": best["queries_per_s"], "queries": q, "batch": 128,
+                 "waves_per_workgroup": best["waves_per_workgroup"], "forms": forms}
+        b.run(**kw)  # counts + hashes of the persistent form again
     cpu = None
-    threads = min(os.cpu_count() or 1, 16)
+    host = host_cpus()
+    threads = host["threads_used"]
     if not args.no_cpu:
         cpu = cpu_baseline(qcp, ix, queries, got, args.cpu_budget, threads)
+        cpu["host"] = host
         log("cpu baseline: %.3e queries/s (%s); MPRS %.3e queries/s" %
             (cpu["value"], cpu["sample"], cpu["mprs"]["value"]))
     build = build_figures(qcp, ix, pts, D, threads, not args.no_cpu)
@@ -808,7 +941,8 @@ def main():
         ix.close()
         serving = serving_bench(qcp, (10000, 50000, 100000), args.serve_seconds, args.scale,
                                 contain_rate=None if args.no_contain else 50000)
-    traffic = read_traffic("traffic_c2_search.json", F, kw)
+    traffic = None if args.no_traffic else measure_traffic(F, kw)
+    probes = memory_probes(qcp)
     kname = "k_search<3,false,%d,false,1,%d> (block %d threads, donation %s)" % (
         (F + 63) // 64, kw["chunks"] or 4, 64 * (kw["block_waves"] or 4),
         "on" if kw["donate"] else "off")
@@ -831,13 +965,26 @@ def main():
                    "n": n, "queries": q, "fanout": F, "autotune_ms": tune,
                    "tree_levels": info["level_count"], "build_ms": info["build_ms"],
                    "total_hits": int(st["total_hits"]), "outputs": "counts+hash"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS,
+        # The tree (201 MB of objects, 1.6 MB of node boxes) is resident in the
+        # Infinity Cache and neighbouring queries share leaves in each XCD's L2,
+        # so the algorithmic bytes are served by the L2 and exceed what HBM
+        # delivers; the roofline is the L2's measured dword-read rate (the leaf
+        # scan's load form), and the HBM side is reported from this run's counters.
+        "roofline": {"bound": "l2", "achieved": achieved, "peak": probes["l2_2MiB"],
+                     "unit": "GB/s", "frac": achieved / probes["l2_2MiB"],
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "peak_source": "measured in this run: dword-load streaming over a 2 MiB "
+                                    "window (each XCD's L2), qcp_probe_read_gbps",
                      "kernel": kname, "kernel_ms": savg, "device_ms_per_step": kavg,
                      "algorithmic_bytes": abytes,
-                     "hbm_GBps": (traffic["bytes_per_launch"] / (savg * 1e-3) / 1e9
-                                  if traffic else None)},
+                     "level_peaks_GBps": probes,
+                     "vs_infinity_cache_frac": achieved / probes["infinity_cache_200MiB"],
+                     "hbm": ({"achieved": traffic["bytes_per_launch"] / (savg * 1e-3) / 1e9,
+                              "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                              "frac": traffic["bytes_per_launch"] / (savg * 1e-3) / 1e9 /
+                              PEAK_HBM_GBS,
+                              "traffic_over_algorithmic": traffic["bytes_per_launch"] / abytes,
+                              "measured": traffic} if traffic else None)},
         "contained": contained,
         "counts_only": counts_only,
         "hybrid": hybrid,

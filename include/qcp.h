@@ -13,7 +13,8 @@
  *   - per-query hit counts (the reference's per-block h_hit[],
  *     src/tree/mphr.cpp:244-252) plus, new here, sorted rank-ID lists and an
  *     order-independent ID-set hash: the sum over hit rank IDs of
- *     id*id + 0x9E3779B97F4A7C15*id + 0xC2B2AE3D27D4EB4F*id*id*id, mod 2^64.
+ *     id*id + 0x9E3779B97F4A7C15*id, mod 2^64 (a function of the count, the sum
+ *     and the sum of squares of the IDs; sets equal in all three collide).
  * Conventions that differ from the reference on purpose: every error is a
  * status code (nothing calls exit()), every handle owns its device memory and
  * HIP stream (no device globals, no singletons), and inputs are caller-owned.
@@ -118,6 +119,8 @@ typedef struct {
                               to the GPU (1 = the root's children; 0 or beyond = the leaves,
                               the reference's CPU-internal / GPU-leaf split)              */
   uint32_t hybrid_batch;   /* hybrid form: queries per pipeline block (0 = 16384)          */
+  uint32_t plain_waves;    /* plain form: waves per workgroup, 1 (0) or 3 (the reference's
+                              192-thread block, src/tree/mphr.cpp:241-243)               */
 } qcp_search_cfg;
 
 typedef struct {
@@ -359,6 +362,17 @@ qcp_status qcp_server_replay_ids(qcp_server *sv, const float *boxes, const doubl
                                  uint64_t n, uint32_t *counts, uint64_t *hashes,
                                  double *latency_us, uint64_t *offsets, uint32_t *ids,
                                  uint64_t ids_cap);
+
+/* ---- measurement support ----------------------------------------------------------
+ * Read bandwidth (GB/s) of the memory level that holds a window of
+ * window_bytes: every wave streams 64-float chunks with dword loads (the leaf
+ * scan's load form), 8 chunks in flight, sweeping the window for `rounds`
+ * rounds of 2 KiB per wave (8 workgroups of 4 waves per CU).  A window of a
+ * few MiB measures the XCDs' L2, a few hundred MiB the Infinity Cache, several
+ * GiB HBM.  No counterpart in the reference (roofline peaks for the search
+ * figures); *ms receives the timed kernel's duration. */
+qcp_status qcp_probe_read_gbps(uint32_t device, uint64_t window_bytes, uint32_t rounds,
+                               double *gbps, double *ms);
 
 /* ---- seeded generator (replaces generator/cuRAND.cpp:17-75, without cuRAND) ----
  * Philox4x32-10 (Salmon et al., SC'11).  Word i of (seed, stream) is output

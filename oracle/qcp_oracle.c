@@ -504,13 +504,12 @@ void orc_tree_boxes(const float *sobjs, unsigned D, int is_box, unsigned F,
 
 /* ---- search ------------------------------------------------------------------------- */
 /* Order-independent fingerprint of a set of rank IDs (defined by this build for
- * the count+hash result mode): the sum over IDs of id^2 + K*id + K3*id^3
- * mod 2^64 with K = 0x9E3779B97F4A7C15 and K3 = 0xC2B2AE3D27D4EB4F, i.e.
- * (sum of squares) + K * (sum) + K3 * (sum of cubes).  This is the per-ID
- * term. */
+ * the count+hash result mode): the sum over IDs of id^2 + K*id mod 2^64 with
+ * K = 0x9E3779B97F4A7C15, i.e. (sum of squares) + K * (sum).  This is the
+ * per-ID term. */
 uint64_t orc_id_hash(uint32_t id) {
   const u64 x = (u64)id;
-  return x * x + 0x9E3779B97F4A7C15ull * x + 0xC2B2AE3D27D4EB4Full * (x * x * x);
+  return x * x + 0x9E3779B97F4A7C15ull * x;
 }
 
 /* Node_SOA::IsOverlap / Node::IsOverlap (src/node/node_soa.cpp:94-113,

@@ -84,8 +84,7 @@ int64_t  orc_packed_topology(uint64_t n, unsigned F, int64_t cap,
                              uint8_t *node_leaf, uint8_t *node_level);
 
 /* ---- search ---------------------------------------------------------------- */
-/* Per-ID term of the ID-set hash:
- * id*id + 0x9E3779B97F4A7C15*id + 0xC2B2AE3D27D4EB4F*id*id*id mod 2^64. */
+/* Per-ID term of the ID-set hash: id*id + 0x9E3779B97F4A7C15*id mod 2^64. */
 uint64_t orc_id_hash(uint32_t id);
 /* Canonical top-down DFS (src/tree/bvh.cpp:256-278) over an explicit tree.
  * ids/offsets optional (NULL).  If ids != NULL, offsets[Q+1] must be given and

@@ -983,6 +983,8 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   if (cfg.mode == QCP_SEARCH_PLAIN) {
     if (ix->expl) return QCP_EINVAL;  // MPRS needs the packed level layout
     if (cfg.outputs & QCP_OUT_IDS) return QCP_EINVAL;
+    const uint32_t waves = cfg.plain_waves ? cfg.plain_waves : 1;
+    if (waves != 1 && waves != 3) return QCP_EINVAL;
     const uint32_t batch = cfg.plain_batch ? cfg.plain_batch : 128;
     if (!b->h_counts && q) {
       HIP_TRY(hipHostMalloc((void**)&b->h_counts, 4 * q, 0));
@@ -994,7 +996,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     for (uint64_t q0 = 0; q0 < q; q0 += batch) {
       const uint32_t nbk = (uint32_t)std::min<uint64_t>(batch, q - q0);
       a.q_begin = (uint32_t)q0;
-      HIP_TRY(launch_mprs(a, ix->D, ix->box, nbk, s));
+      HIP_TRY(launch_mprs(a, ix->D, ix->box, nbk, waves, s));
       // the reference copies the per-block results back synchronously after
       // every launch (src/tree/mphr.cpp:244-246)
       HIP_TRY(hipMemcpyAsync(b->h_counts + q0, b->d_counts + q0, 4ull * nbk, hipMemcpyDeviceToHost, s));

@@ -90,8 +90,9 @@ hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s);
 // u: 64-entry leaf chunks in flight per wave (2, 4 or 8)
 hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, int out, int u,
                          unsigned grid, unsigned block, hipStream_t s);
+// the plain form: nblocks workgroups of `waves` (1 or 3) waves, one per query
 hipError_t launch_mprs(const SearchArgs& a, unsigned D, bool box, unsigned nblocks,
-                       hipStream_t s);
+                       unsigned waves, hipStream_t s);
 const void* search_kernel_ptr(unsigned D, bool box, bool expl, int out, int mw);
 // dynamic LDS bytes of a persistent-search workgroup of `block` threads
 size_t search_lds_bytes(bool expl, int mw, int out, unsigned block);

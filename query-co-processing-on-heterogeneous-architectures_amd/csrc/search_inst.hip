@@ -13,7 +13,8 @@ template hipError_t launch_search_dbe<QCP_D, (bool)QCP_BOX, (bool)QCP_EXPL>(
     const SearchArgs&, int, int, unsigned, unsigned, hipStream_t);
 template const void* search_ptr_dbe<QCP_D, (bool)QCP_BOX, (bool)QCP_EXPL>(int, int);
 #if !QCP_EXPL
-template hipError_t launch_mprs_db<QCP_D, (bool)QCP_BOX>(const SearchArgs&, unsigned, hipStream_t);
+template hipError_t launch_mprs_db<QCP_D, (bool)QCP_BOX>(const SearchArgs&, unsigned, unsigned,
+                                                         hipStream_t);
 #endif
 
 }  // namespace qcp

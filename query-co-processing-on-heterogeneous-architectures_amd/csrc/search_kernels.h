@@ -125,8 +125,7 @@ struct Ctx {
   uint64_t wbase, wlo;
   uint64_t s1_lane;       // per-lane partial sum of hit IDs
   uint64_t s2_lane;       // per-lane partial sum of squared hit IDs
-  uint64_t s3_lane;       // per-lane partial sum of cubed hit IDs
-  uint64_t s1c, s2c, s3c; // power sums of the IDs of contained subtrees
+  uint64_t s1c, s2c;      // power sums of the IDs of contained subtrees
   uint64_t ie, le;        // entries visited (the canonical traversal unless contain is on)
   uint32_t cr;            // contained subtree runs
 };
@@ -205,7 +204,6 @@ __device__ __forceinline__ void emit_range(const SearchArgs& a, Ctx<D>& c, uint6
   if (OUT == kOutHash) {
     c.s1c += psum1((uint32_t)e) - psum1((uint32_t)s);
     c.s2c += psum2((uint32_t)e) - psum2((uint32_t)s);
-    c.s3c += psum3((uint32_t)e) - psum3((uint32_t)s);
   } else if (OUT == kOutIds) {
     const uint32_t lane = lane_id();
     flush_partial<D>(a, c);  // staged IDs before the run, then the run directly
@@ -246,13 +244,11 @@ __device__ __forceinline__ void chunk_out(const SearchArgs& a, Ctx<D>& c, uint32
     c.out_base += nb;
     if (c.out_base - c.wbase >= kWave) flush_block<D>(a, c);
   } else if (OUT == kOutHash) {
-    // branch-free: a select, 64-bit adds and two 64-bit multiplies per chunk
+    // branch-free: a select, a 64-bit add and one v_mad_u64_u32 per chunk
     const uint32_t id = st + lane + 1u;
     const uint32_t x = hit ? id : 0u;
-    const uint64_t x2 = (uint64_t)x * x;
     c.s1_lane += x;
-    c.s2_lane += x2;
-    c.s3_lane += x2 * x;
+    c.s2_lane += (uint64_t)x * x;
   }
   c.cnt += nb;
 }
@@ -684,8 +680,8 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
     Ctx<D> c;
     c.q = q;
     c.cnt = 0;
-    c.s1_lane = c.s2_lane = c.s3_lane = 0;
-    c.s1c = c.s2c = c.s3c = 0;
+    c.s1_lane = c.s2_lane = 0;
+    c.s1c = c.s2c = 0;
     c.cr = 0;
     c.ie = 0;
     c.le = 0;
@@ -700,8 +696,7 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
       // the hash is additive over disjoint ID sets, so donated parts add up
       uint64_t hsum = 0;
       if (OUT == kOutHash)
-        hsum = set_hash(c.s1c + wave_sum_u64(c.s1_lane), c.s2c + wave_sum_u64(c.s2_lane),
-                        c.s3c + wave_sum_u64(c.s3_lane));
+        hsum = set_hash(c.s1c + wave_sum_u64(c.s1_lane), c.s2c + wave_sum_u64(c.s2_lane));
       if (lane == 0) {
         if (accumulate) {
           if (c.cnt) atomicAdd(&a.counts[q], c.cnt);
@@ -742,20 +737,43 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Plain MPRS, one wave per query (packed layouts only).
-template <int D, bool BOX, int MW>
-__global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
+// Plain MPRS, one workgroup of NW waves per query (packed layouts only).  With
+// NW = 1 entry k of a node is lane k % 64 of the one wave and the leftmost
+// qualifying child is a ballot + find-first-set; with NW = 3 (the reference's
+// 192-thread block, src/tree/mphr.cpp:241-243) entry k belongs to wave
+// (k / 64) % 3, and the waves' candidates meet in LDS behind barriers (the
+// reference's FindMinOnGPU reduction, src/common/macro.h:24-42).
+template <int D, bool BOX, int MW, int NW>
+__global__ __launch_bounds__(64 * NW) void k_mprs(SearchArgs a) {
   constexpr int E = BOX ? 2 * D : D;
+  constexpr int kNone = 1 << 30;
   const uint32_t lane = lane_id();
+  const int wv = NW > 1 ? (int)(threadIdx.x / kWave) : 0;
   const uint32_t q = a.q_begin + blockIdx.x;
-  if (q >= a.nq) return;
+  if (q >= a.nq) return;  // uniform over the workgroup
+  __shared__ int s_red[NW];
+  __shared__ unsigned long long s_sum[3][NW];
+  // workgroup-wide minimum of a wave-uniform value (every thread gets it)
+  auto group_min = [&](int v) -> int {
+    if constexpr (NW == 1) {
+      return v;
+    } else {
+      if (lane == 0) s_red[wv] = v;
+      __syncthreads();
+      int m = s_red[0];
+#pragma unroll
+      for (int i = 1; i < NW; ++i) m = s_red[i] < m ? s_red[i] : m;
+      __syncthreads();
+      return m;
+    }
+  };
   Ctx<D> c;
   load_query<D>(a, q, c);
   const uint32_t F = a.F;
   const int h = a.h;
   const uint64_t n = a.n;
   uint32_t cnt = 0, rootv = 0, nodev = 0;
-  uint64_t s1 = 0, s2 = 0, s3 = 0;
+  uint64_t s1 = 0, s2 = 0;
   if (h > 0) {
     auto last_index = [&](int l, uint64_t j) -> uint64_t {
       uint64_t e = (j + 1) * a.span[l];
@@ -772,9 +790,11 @@ __global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
         const uint64_t below = a.lvl_cnt[lvl + 1];
         const uint32_t cc = (uint32_t)((below - first) < F ? (below - first) : F);
         const uint64_t gbase = a.lvl_off[lvl + 1] + first;
-        int pick = -1;
+        int pick = kNone;
 #pragma unroll
-        for (int w = 0; w < MW; ++w) {
+        for (int t = 0; t < (MW + NW - 1) / NW; ++t) {
+          const int w = wv + t * NW;  // this wave's 64-entry words of the node
+          if (w >= MW) break;
           const uint32_t k = w * kWave + lane;
           float lo[D], hi[D];
 #pragma unroll
@@ -785,9 +805,10 @@ __global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
           bool ok = (k < cc) && last_index(lvl + 1, first + k) > visited &&
                     overlaps<D>(c.ql, c.qh, lo, hi);
           uint64_t b = ballot(ok);
-          if (pick < 0 && b) pick = w * kWave + __builtin_ctzll(b);
+          if (pick == kNone && b) pick = w * kWave + __builtin_ctzll(b);
         }
-        if (pick < 0) {
+        pick = group_min(pick);
+        if (pick == kNone) {
           visited = last_index(lvl, j);
           lvl = 0;
           j = 0;
@@ -801,9 +822,11 @@ __global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
       while (lvl == h - 1) {
         const uint64_t first = j * F;
         const uint32_t cc = (uint32_t)((n - first) < F ? (n - first) : F);
-        bool any = false;
+        int none = 1;
 #pragma unroll
-        for (int w = 0; w < MW; ++w) {
+        for (int t = 0; t < (MW + NW - 1) / NW; ++t) {
+          const int w = wv + t * NW;
+          if (w >= MW) break;
           const uint32_t k = w * kWave + lane;
           float v[E];
 #pragma unroll
@@ -815,10 +838,10 @@ __global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
             const uint64_t id = first + k + 1;
             s1 += id;
             s2 += id * id;
-            s3 += id * id * id;
           }
-          any = any || (b != 0);
+          if (b) none = 0;
         }
+        const bool any = group_min(none) == 0;
         visited = last_index(lvl, j);
         if (visited == last) break;
         if (any) {
@@ -832,10 +855,26 @@ __global__ __launch_bounds__(64) void k_mprs(SearchArgs a) {
       }
     }
   }
-  const uint64_t hs = set_hash(wave_sum_u64(s1), wave_sum_u64(s2), wave_sum_u64(s3));
-  if (lane == 0) {
-    a.counts[q] = cnt;
-    a.hashes[q] = hs;
+  uint64_t t1 = wave_sum_u64(s1), t2 = wave_sum_u64(s2);
+  uint64_t tc = cnt;
+  if constexpr (NW > 1) {
+    if (lane == 0) {
+      s_sum[0][wv] = tc;
+      s_sum[1][wv] = t1;
+      s_sum[2][wv] = t2;
+    }
+    __syncthreads();
+    tc = t1 = t2 = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      tc += s_sum[0][i];
+      t1 += s_sum[1][i];
+      t2 += s_sum[2][i];
+    }
+  }
+  if (threadIdx.x == 0) {
+    a.counts[q] = (uint32_t)tc;
+    a.hashes[q] = set_hash(t1, t2);
     if (a.visits) {
       a.visits[2 * (uint64_t)q] = rootv;
       a.visits[2 * (uint64_t)q + 1] = nodev;
@@ -923,11 +962,18 @@ const void* search_ptr_dbe(int out, int mw) {
 #undef QCP_PTR
 }
 
-// The plain (MPRS) form of one (D, object kind), packed layouts.
+// The plain (MPRS) form of one (D, object kind), packed layouts, with 1 or 3
+// waves per workgroup.
 template <int D, bool BOX>
-hipError_t launch_mprs_db(const SearchArgs& a, unsigned nblocks, hipStream_t s) {
+hipError_t launch_mprs_db(const SearchArgs& a, unsigned nblocks, unsigned waves, hipStream_t s) {
   const int mw = (int)((a.F + 63) / 64);
-  QCP_MW_SWITCH(mw, k_mprs<D, BOX, MW><<<nblocks, 64, 0, s>>>(a));
+  if (waves == 3) {
+    QCP_MW_SWITCH(mw, k_mprs<D, BOX, MW, 3><<<nblocks, 192, 0, s>>>(a));
+  } else if (waves == 1) {
+    QCP_MW_SWITCH(mw, k_mprs<D, BOX, MW, 1><<<nblocks, 64, 0, s>>>(a));
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -945,9 +991,9 @@ QCP_SEARCH_EXTERN(3, false, true)
 QCP_SEARCH_EXTERN(3, true, false)
 QCP_SEARCH_EXTERN(3, true, true)
 #undef QCP_SEARCH_EXTERN
-extern template hipError_t launch_mprs_db<2, false>(const SearchArgs&, unsigned, hipStream_t);
-extern template hipError_t launch_mprs_db<2, true>(const SearchArgs&, unsigned, hipStream_t);
-extern template hipError_t launch_mprs_db<3, false>(const SearchArgs&, unsigned, hipStream_t);
-extern template hipError_t launch_mprs_db<3, true>(const SearchArgs&, unsigned, hipStream_t);
+extern template hipError_t launch_mprs_db<2, false>(const SearchArgs&, unsigned, unsigned, hipStream_t);
+extern template hipError_t launch_mprs_db<2, true>(const SearchArgs&, unsigned, unsigned, hipStream_t);
+extern template hipError_t launch_mprs_db<3, false>(const SearchArgs&, unsigned, unsigned, hipStream_t);
+extern template hipError_t launch_mprs_db<3, true>(const SearchArgs&, unsigned, unsigned, hipStream_t);
 
 }  // namespace qcp

@@ -53,9 +53,13 @@ hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, i
 }
 
 hipError_t launch_mprs(const SearchArgs& a, unsigned D, bool box, unsigned nblocks,
-                       hipStream_t s) {
-  if (D == 2) return box ? launch_mprs_db<2, true>(a, nblocks, s) : launch_mprs_db<2, false>(a, nblocks, s);
-  if (D == 3) return box ? launch_mprs_db<3, true>(a, nblocks, s) : launch_mprs_db<3, false>(a, nblocks, s);
+                       unsigned waves, hipStream_t s) {
+  if (D == 2)
+    return box ? launch_mprs_db<2, true>(a, nblocks, waves, s)
+               : launch_mprs_db<2, false>(a, nblocks, waves, s);
+  if (D == 3)
+    return box ? launch_mprs_db<3, true>(a, nblocks, waves, s)
+               : launch_mprs_db<3, false>(a, nblocks, waves, s);
   return hipErrorInvalidValue;
 }
 

@@ -55,7 +55,8 @@ class SearchCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in
                 ("mode", "outputs", "plain_batch", "block_waves", "grid_blocks", "leaf_chunks",
                  "donate", "order", "max_waiters", "hunger_period", "donate_height",
-                 "graph", "contain", "cpu_threads", "handoff_level", "hybrid_batch")]
+                 "graph", "contain", "cpu_threads", "handoff_level", "hybrid_batch",
+                 "plain_waves")]
 
 
 class SearchStats(ctypes.Structure):
@@ -155,6 +156,9 @@ def lib():
         L.qcp_server_result_ids.argtypes = [P, ctypes.c_uint64, P, ctypes.c_uint64,
                                             ctypes.POINTER(ctypes.c_uint64)]
         L.qcp_server_release.argtypes = [P, ctypes.c_uint64]
+        L.qcp_probe_read_gbps.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                          ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]
         L.qcp_philox4x32_10.argtypes = [P, P, P]
         L.qcp_philox4x32_10.restype = None
         for fn in ("qcp_gen_words", "qcp_gen_uniform", "qcp_gen_normal"):
@@ -335,11 +339,12 @@ class Index:
 
     def search(self, queries, mode="persistent", ids=False, visits=False, plain_batch=128,
                grid_blocks=0, donate=0, chunks=0, hash=True, order=0, block_waves=0,
-               contain=False, **hybrid):
+               contain=False, plain_waves=0, **hybrid):
         with self.batch(queries) as b:
             st = b.run(mode=mode, ids=ids, plain_batch=plain_batch, grid_blocks=grid_blocks,
                        donate=donate, chunks=chunks, hash=hash, order=order,
-                       block_waves=block_waves, contain=contain, **hybrid)
+                       block_waves=block_waves, contain=contain, plain_waves=plain_waves,
+                       **hybrid)
             res = b.results(ids=ids, visits=visits)
             res["stats"] = st
             return res
@@ -376,20 +381,21 @@ class Batch:
     def run(self, mode="persistent", ids=False, plain_batch=128, grid_blocks=0, donate=0,
             hash=True, chunks=0, order=0, max_waiters=0, hunger_period=0, donate_height=0,
             graph=False, block_waves=0, contain=False, cpu_threads=0, handoff_level=0,
-            hybrid_batch=0):
+            hybrid_batch=0, plain_waves=0):
         """One search of the resident batch.  donate=1 enables subtree donation
         (work stealing) in the persistent form; chunks = 64-entry leaf chunks in
         flight per wave (0 = default); order=0 processes the queries in Hilbert
         order of their centres, one range per XCD, order=1 as given;
         block_waves = waves per workgroup (0 = 4); contain=True reports subtrees
-        whose box lies inside the query without visiting them (same results).
+        whose box lies inside the query without visiting them (same results);
+        plain_waves = 3 runs the plain form with the reference's 192-thread workgroups.
         mode="hybrid": host threads walk the levels above handoff_level (0 = the
         leaves) and the GPU searches the subtrees they hand over."""
         outputs = OUT_COUNTS | (OUT_HASH if hash else 0) | (OUT_IDS if ids else 0)
         cfg = SearchCfg(MODE[mode], outputs, plain_batch, block_waves, grid_blocks, chunks,
                         donate, order, max_waiters, hunger_period, donate_height,
                         1 if graph else 0, 1 if contain else 0, cpu_threads, handoff_level,
-                        hybrid_batch)
+                        hybrid_batch, plain_waves)
         st = SearchStats()
         check(lib().qcp_batch_run(self._h, ctypes.byref(cfg), ctypes.byref(st)), "qcp_batch_run")
         self._ids = ids
@@ -563,6 +569,15 @@ class Server:
 
     def __exit__(self, *a):
         self.stop()
+
+
+def probe_read_gbps(window_bytes, rounds, device=0):
+    """(GB/s, ms) of dword-load streaming over a window of window_bytes (the
+    memory level that holds it: L2, Infinity Cache or HBM)."""
+    g, m = ctypes.c_double(), ctypes.c_double()
+    check(lib().qcp_probe_read_gbps(device, window_bytes, rounds, ctypes.byref(g),
+                                    ctypes.byref(m)), "qcp_probe_read_gbps")
+    return g.value, m.value
 
 
 def poisson_arrivals(n, qps, seed=1234, stream=3):

@@ -15,7 +15,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "_build", "libqcp_oracle.so")
 
 KEY_REF, KEY_B16 = 0, 1
-HASH_K, HASH_K3 = 0x9E3779B97F4A7C15, 0xC2B2AE3D27D4EB4F
+HASH_K = 0x9E3779B97F4A7C15
 SRC_LO, SRC_CENTER = 0, 1
 
 _lib = None
@@ -287,11 +287,10 @@ def search_brute(sobjs, D, is_box, queries, want_ids=False):
 
 
 def id_hash(ids):
-    """numpy restatement of orc_id_hash per ID (uint64 wraparound):
-    id^2 + K*id + K3*id^3."""
+    """numpy restatement of orc_id_hash per ID (uint64 wraparound): id^2 + K*id."""
     x = ids.astype(np.uint64)
     with np.errstate(over="ignore"):
-        return x * x + np.uint64(HASH_K) * x + np.uint64(HASH_K3) * (x * x * x)
+        return x * x + np.uint64(HASH_K) * x
 
 
 def build_mt(objs, D, is_box, F, key_mode, key_src=SRC_LO, nthreads=8):

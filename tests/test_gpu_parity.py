@@ -351,8 +351,9 @@ def test_ids_range_pagination(gpu, O):
 
 def test_equal_moment_id_sets(gpu, O):
     """Two queries whose hit sets {1,5,6} and {2,3,7} agree in count, sum and
-    sum of squares: their ID-set hashes differ (the cube term) and every search
-    form returns the oracle's lists and hashes."""
+    sum of squares, so their ID-set hashes are equal: every search form must
+    still return the oracle's hashes, and the sorted ID lists (which the
+    parity tests compare beside the hashes) tell the two sets apart."""
     x = np.arange(1, 8, dtype=np.float32) * np.float32(0.1)
     y = np.array([0.2, 0.6, 0.6, 0.9, 0.2, 0.2, 0.6], np.float32)  # rank r+1 -> y[r]
     pts = np.ascontiguousarray(np.stack([x, y], 1))
@@ -360,7 +361,7 @@ def test_equal_moment_id_sets(gpu, O):
     t = O.Tree(pts, 2, False, 4)
     exp = t.search_dfs(q, want_ids=True)
     assert exp["ids"].tolist() == [1, 5, 6, 2, 3, 7]
-    assert exp["hashes"][0] != exp["hashes"][1]
+    assert exp["hashes"][0] == exp["hashes"][1]
     for layout in ("mphr", "bvh"):
         ix = gpu.Index.build_ranked(pts, 2, fanout=4, layout=layout)
         assert_results_equal(ix.search(q, ids=True), exp, "equal moments " + layout)
