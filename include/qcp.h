@@ -363,6 +363,26 @@ qcp_status qcp_server_replay_ids(qcp_server *sv, const float *boxes, const doubl
                                  double *latency_us, uint64_t *offsets, uint32_t *ids,
                                  uint64_t ids_cap);
 
+/* ---- search without allocation --------------------------------------------------
+ * No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The persistent search of qcp_batch_run
+ * (cfg->mode QCP_SEARCH_PERSISTENT; counts, or counts + ID-set hash with
+ * QCP_OUT_HASH; no ID lists) of q query boxes already in device memory of the
+ * index's device, into caller-owned device arrays d_counts[q] and, with the
+ * hash, d_hashes[q]; same configuration rules and results as qcp_batch_run.
+ * Every other buffer (queue control block, donation slots, the query-order
+ * sort) is carved from one fixed array of QCP_NOALLOC_SCRATCH_BYTES per GPU
+ * inside the library, held until the process ends; nothing is allocated.
+ * q above QCP_NOALLOC_MAX_QUERIES is refused (QCP_EINVAL).  Returns when the
+ * results are in place.  Calls that overlap on one GPU share that array and
+ * race: serialise them. */
+#define QCP_NOALLOC_MAX_QUERIES 262144u
+#define QCP_NOALLOC_SCRATCH_BYTES (7u << 20)
+qcp_status qcp_search_noalloc(qcp_index *ix, const float *d_queries, uint64_t q,
+                              const qcp_search_cfg *cfg, uint32_t *d_counts,
+                              uint64_t *d_hashes);
+
 /* ---- measurement support ----------------------------------------------------------
  * Read bandwidth (GB/s) of the memory level that holds a window of
  * window_bytes: every wave streams 64-float chunks with dword loads (the leaf

@@ -156,6 +156,7 @@ def lib():
         L.qcp_server_result_ids.argtypes = [P, ctypes.c_uint64, P, ctypes.c_uint64,
                                             ctypes.POINTER(ctypes.c_uint64)]
         L.qcp_server_release.argtypes = [P, ctypes.c_uint64]
+        L.qcp_search_noalloc.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(SearchCfg), P, P]
         L.qcp_probe_read_gbps.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double)]
@@ -569,6 +570,23 @@ class Server:
 
     def __exit__(self, *a):
         self.stop()
+
+
+NOALLOC_MAX_QUERIES = 262144
+
+
+def search_noalloc(index, d_queries, q, d_counts, d_hashes=None, hash=True, donate=0, chunks=0,
+                   block_waves=0, order=0, contain=False, grid_blocks=0):
+    """The persistent search through the entry point that allocates nothing:
+    d_queries / d_counts / d_hashes are DeviceBuffers the caller made (queries
+    q x 2D floats, counts u32[q], hashes u64[q] when hash)."""
+    outputs = OUT_COUNTS | (OUT_HASH if hash else 0)
+    cfg = SearchCfg(MODE["persistent"], outputs, 0, block_waves, grid_blocks, chunks, donate,
+                    order, 0, 0, 0, 0, 1 if contain else 0, 0, 0, 0, 0)
+    check(lib().qcp_search_noalloc(index._h, d_queries.ptr if q else None, q, ctypes.byref(cfg),
+                                   d_counts.ptr if q else None,
+                                   d_hashes.ptr if d_hashes is not None else None),
+          "qcp_search_noalloc")
 
 
 def probe_read_gbps(window_bytes, rounds, device=0):
