@@ -75,6 +75,11 @@ typedef struct {
                          201-259; not with QCP_BUILD_GRAPH)                        */
   uint32_t clusters;  /* k-means order: clusters (0 = 64, at most 4096)             */
   uint32_t kmeans_iters; /* k-means order: Lloyd iterations at most (0 = 20)         */
+  uint32_t partitions;   /* MPHR data partitions P (0 or 1 = one tree; at most 65536 and
+                            at most n): the ranks split as MPHR::Build does (chunk = n/P,
+                            the first partition also taking n % P), one packed subtree
+                            per partition (src/tree/mphr.cpp:60-108, SetNumberOfPartition
+                            279-285); searched from every partition root               */
 } qcp_build_cfg;
 
 typedef struct {
@@ -87,6 +92,8 @@ typedef struct {
   double build_ms;            /* device time of the last build (HIP events) */
   double build_phase_ms[6];   /* keys, sort, gather, pack, topology (GUTTMAN: host
                                  insertion, wall ms), total */
+  uint32_t partitions;        /* MPHR data partitions (1 = one tree); partition p's root
+                                 is node p (nodes are numbered depth-major)  */
 } qcp_index_info;
 
 typedef struct {
@@ -120,7 +127,9 @@ typedef struct {
                               the reference's CPU-internal / GPU-leaf split)              */
   uint32_t hybrid_batch;   /* hybrid form: queries per pipeline block (0 = 16384)          */
   uint32_t plain_waves;    /* plain form: waves per workgroup, 1 (0) or 3 (the reference's
-                              192-thread block, src/tree/mphr.cpp:241-243)               */
+                              192-thread block, src/tree/mphr.cpp:241-243).  A partitioned
+                              MPHR runs the reference's partition mode: one launch per
+                              query with one workgroup per partition (235-257)           */
 } qcp_search_cfg;
 
 typedef struct {

@@ -18,8 +18,10 @@
  *  - one tree per object, no device globals, so several trees can coexist;
  *  - the degree is a runtime value (SetFanout) with the reference's
  *    compile-time degrees as defaults (src/common/config.h:6-8);
- *  - MPHR partitions other than 1 are refused (Build returns false with
- *    QCP_EINVAL): a partition only splits the rank range one tree covers.
+ *  - MPHR partitions (SetNumberOfPartition) are built and searched as the
+ *    reference does (one subtree per contiguous rank range; the plain search
+ *    launches one workgroup per partition per query), without the
+ *    reference's requirement that the block count equal the partition count.
  */
 #ifndef QCP_TREE_HPP
 #define QCP_TREE_HPP
@@ -240,14 +242,12 @@ class MPHR : public Tree {
   void SetNumberOfCUDABlocks(ui number_of_cuda_blocks) {
     search_cfg_.plain_batch = number_of_cuda_blocks;
   }
-  void SetNumberOfPartition(ui number_of_partition) { partitions_ = number_of_partition; }
-  bool Build(std::shared_ptr<io::DataSet> input_data_set) override {
-    if (partitions_ != 1) return Fail(QCP_EINVAL);
-    return Tree::Build(input_data_set);
+  /* MPHR::SetNumberOfPartition (src/tree/mphr.cpp:279-285): P subtrees over
+   * contiguous rank ranges (0 or 1 = one tree); Build refuses P > N. */
+  void SetNumberOfPartition(ui number_of_partition) {
+    build_cfg_.partitions = number_of_partition;
+    if (number_of_partition > 1) tree_type = TREE_TYPE_MPHR_PARTITION;
   }
-
- private:
-  ui partitions_ = 1;
 };
 
 /* BVH (src/tree/bvh.h): rank-prefix splits, degree 128.  The reference

@@ -532,7 +532,7 @@ typedef struct {
 static void dfs_one(const float *sobjs, unsigned D, int is_box,
                     const uint32_t *node_first, const uint32_t *node_count,
                     const uint8_t *node_leaf, const float *node_boxes,
-                    const float *q, dfs_acc *acc) {
+                    const float *q, uint32_t root, dfs_acc *acc) {
   unsigned const B = 2 * D;
   unsigned const E = is_box ? 2 * D : D;
   unsigned const hi_off = is_box ? D : 0;
@@ -540,8 +540,8 @@ static void dfs_one(const float *sobjs, unsigned D, int is_box,
   int64_t st_node[128];
   uint32_t st_next[128];
   int sp = 0;
-  st_node[0] = 0; st_next[0] = 0;
-  if (node_leaf[0]) acc->le += node_count[0]; else acc->ie += node_count[0];
+  st_node[0] = root; st_next[0] = 0;
+  if (node_leaf[root]) acc->le += node_count[root]; else acc->ie += node_count[root];
   while (sp >= 0) {
     int64_t nd = st_node[sp];
     if (node_leaf[nd]) {
@@ -570,6 +570,29 @@ static void dfs_one(const float *sobjs, unsigned D, int is_box,
   }
 }
 
+/* The canonical traversal from each of the roots 0 .. roots-1 in order (one
+ * root for a single tree; a partitioned MPHR's P partition roots), so a
+ * query's IDs come out ascending when the roots cover ascending rank ranges. */
+void orc_search_dfs_roots(const float *sobjs, unsigned D, int is_box,
+                          int64_t n_nodes, const uint32_t *node_first,
+                          const uint32_t *node_count, const uint8_t *node_leaf,
+                          const float *node_boxes, unsigned roots,
+                          const float *queries, uint64_t q0, uint64_t q1,
+                          uint32_t *counts, uint64_t *hashes,
+                          const uint64_t *offsets, uint32_t *ids,
+                          uint64_t *int_entries, uint64_t *leaf_entries) {
+  for (uint64_t q = q0; q < q1; ++q) {
+    dfs_acc acc = {0, 0, ids ? ids + offsets[q] : NULL, 0, 0};
+    for (unsigned r = 0; r < roots && (int64_t)r < n_nodes; ++r)
+      dfs_one(sobjs, D, is_box, node_first, node_count, node_leaf, node_boxes,
+              queries + q * 2 * D, r, &acc);
+    if (counts) counts[q] = acc.cnt;
+    if (hashes) hashes[q] = acc.hash;
+    if (int_entries) int_entries[q] = acc.ie;
+    if (leaf_entries) leaf_entries[q] = acc.le;
+  }
+}
+
 void orc_search_dfs(const float *sobjs, unsigned D, int is_box,
                     int64_t n_nodes, const uint32_t *node_first,
                     const uint32_t *node_count, const uint8_t *node_leaf,
@@ -578,37 +601,32 @@ void orc_search_dfs(const float *sobjs, unsigned D, int is_box,
                     uint32_t *counts, uint64_t *hashes,
                     const uint64_t *offsets, uint32_t *ids,
                     uint64_t *int_entries, uint64_t *leaf_entries) {
-  for (uint64_t q = q0; q < q1; ++q) {
-    dfs_acc acc = {0, 0, ids ? ids + offsets[q] : NULL, 0, 0};
-    if (n_nodes > 0)
-      dfs_one(sobjs, D, is_box, node_first, node_count, node_leaf, node_boxes,
-              queries + q * 2 * D, &acc);
-    if (counts) counts[q] = acc.cnt;
-    if (hashes) hashes[q] = acc.hash;
-    if (int_entries) int_entries[q] = acc.ie;
-    if (leaf_entries) leaf_entries[q] = acc.le;
-  }
+  orc_search_dfs_roots(sobjs, D, is_box, n_nodes, node_first, node_count, node_leaf,
+                       node_boxes, 1, queries, q0, q1, counts, hashes, offsets, ids,
+                       int_entries, leaf_entries);
 }
 
 typedef struct {
   const float *sobjs; unsigned D; int is_box; int64_t n_nodes;
   const uint32_t *nf, *nc; const uint8_t *nl; const float *nb;
   const float *queries; uint64_t q0, q1; uint32_t *counts; uint64_t *hashes;
+  unsigned roots;
 } mt_arg;
 
 static void *mt_worker(void *p) {
   mt_arg *a = (mt_arg *)p;
-  orc_search_dfs(a->sobjs, a->D, a->is_box, a->n_nodes, a->nf, a->nc, a->nl, a->nb,
-                 a->queries, a->q0, a->q1, a->counts, a->hashes, NULL, NULL, NULL, NULL);
+  orc_search_dfs_roots(a->sobjs, a->D, a->is_box, a->n_nodes, a->nf, a->nc, a->nl, a->nb,
+                       a->roots, a->queries, a->q0, a->q1, a->counts, a->hashes, NULL, NULL,
+                       NULL, NULL);
   return NULL;
 }
 
 /* BVH::Search thread split (src/tree/bvh.cpp:195-220): chunk = Q/T, the first
  * thread also takes the remainder. */
-void orc_search_dfs_mt(const float *sobjs, unsigned D, int is_box,
+void orc_search_dfs_mt_roots(const float *sobjs, unsigned D, int is_box,
                        int64_t n_nodes, const uint32_t *node_first,
                        const uint32_t *node_count, const uint8_t *node_leaf,
-                       const float *node_boxes,
+                       const float *node_boxes, unsigned roots,
                        const float *queries, uint64_t nq, unsigned nthreads,
                        uint32_t *counts, uint64_t *hashes) {
   if (nthreads < 1) nthreads = 1;
@@ -618,7 +636,7 @@ void orc_search_dfs_mt(const float *sobjs, unsigned D, int is_box,
   uint64_t s = 0, e = chunk + nq % nthreads;
   for (unsigned t = 0; t < nthreads; ++t) {
     mt_arg a = {sobjs, D, is_box, n_nodes, node_first, node_count, node_leaf,
-                node_boxes, queries, s, e, counts, hashes};
+                node_boxes, queries, s, e, counts, hashes, roots};
     args[t] = a;
     pthread_create(&th[t], NULL, mt_worker, &args[t]);
     s = e;
@@ -629,18 +647,47 @@ void orc_search_dfs_mt(const float *sobjs, unsigned D, int is_box,
   free(args);
 }
 
+void orc_search_dfs_mt(const float *sobjs, unsigned D, int is_box,
+                       int64_t n_nodes, const uint32_t *node_first,
+                       const uint32_t *node_count, const uint8_t *node_leaf,
+                       const float *node_boxes,
+                       const float *queries, uint64_t nq, unsigned nthreads,
+                       uint32_t *counts, uint64_t *hashes) {
+  orc_search_dfs_mt_roots(sobjs, D, is_box, n_nodes, node_first, node_count, node_leaf,
+                          node_boxes, 1, queries, nq, nthreads, counts, hashes);
+}
+
 /* global_RestartScanning_and_ParentCheck, src/tree/mphr.cpp:303-438, serial,
  * over the packed levels (root first; leaves are the last level; node (l,j)
  * covers ranks j*F^(h-l)+1 .. min(N,(j+1)*F^(h-l)), so its "last index" is that
  * upper bound; a leaf's parent is (l-1, j/F); the next leaf is j+1).
  * The reference's partially zeroed t_hit[] (lines 327-330) is not reproduced:
  * the count is the number of overlapping entries. */
+static void mprs_core(const float *sobjs, uint64_t n, unsigned D, int is_box,
+                      unsigned F, const float *node_boxes,
+                      const float *queries, uint64_t nq,
+                      uint32_t *counts, uint64_t *hashes,
+                      const uint64_t *offsets, uint32_t *ids,
+                      uint32_t *visits, uint64_t id_base);
+
 void orc_search_mprs(const float *sobjs, uint64_t n, unsigned D, int is_box,
                      unsigned F, const float *node_boxes,
                      const float *queries, uint64_t nq,
                      uint32_t *counts, uint64_t *hashes,
                      const uint64_t *offsets, uint32_t *ids,
                      uint32_t *visits) {
+  mprs_core(sobjs, n, D, is_box, F, node_boxes, queries, nq, counts, hashes, offsets, ids,
+            visits, 0);
+}
+
+/* The MPRS walk over one packed tree of n entries whose IDs are id_base + 1 ..
+ * id_base + n. */
+static void mprs_core(const float *sobjs, uint64_t n, unsigned D, int is_box,
+                      unsigned F, const float *node_boxes,
+                      const float *queries, uint64_t nq,
+                      uint32_t *counts, uint64_t *hashes,
+                      const uint64_t *offsets, uint32_t *ids,
+                      uint32_t *visits, uint64_t id_base) {
   uint64_t lc[64];
   int h = orc_level_counts(n, F, lc, 64);
   uint64_t off[65], span[65];
@@ -702,7 +749,7 @@ void orc_search_mprs(const float *sobjs, uint64_t n, unsigned D, int is_box,
         for (uint64_t k = 0; k < c; ++k) {
           const float *o = sobjs + (first + k) * E;
           if (overlap(q, o, o + hi_off, D)) {
-            uint32_t id = (uint32_t)(first + k + 1);
+            uint32_t id = (uint32_t)(id_base + first + k + 1);
             if (out) out[cnt] = id;
             cnt++;
             hash += orc_id_hash(id);
@@ -750,6 +797,118 @@ void orc_search_brute(const float *sobjs, uint64_t n, unsigned D, int is_box,
     if (counts) counts[qi] = cnt;
     if (hashes) hashes[qi] = hash;
   }
+}
+
+/* ---- MPHR data partitions ------------------------------------------------------ */
+/* MPHR::Build's split of the rank-sorted branches (src/tree/mphr.cpp:63-65,
+ * 98-100): chunk = n / P; partition 0 takes ranks [0, chunk + n % P), every
+ * later partition the next chunk ranks. */
+void orc_partition_range(uint64_t n, unsigned P, unsigned p, uint64_t *begin, uint64_t *count) {
+  const uint64_t chunk = n / P, first = chunk + n % P;
+  *begin = p == 0 ? 0 : first + (uint64_t)(p - 1) * chunk;
+  *count = p == 0 ? first : chunk;
+}
+
+/* The P bottom-up subtrees of the partitions (each Tree::Bottom_Up over its
+ * range, src/tree/mphr.cpp:67-101) as one explicit tree in this build's
+ * numbering: depth-major (every partition's depth-0 node, i.e. its root, then
+ * every partition's depth-1 nodes, ...), partitions in order within a depth,
+ * so partition p's root is node p.  Arrays as orc_bvh_topology; a leaf's
+ * first is its first global rank.  Returns the node count, or -1 if cap is
+ * too small or a partition would be empty (P > n). */
+int64_t orc_partitioned_topology(uint64_t n, unsigned F, unsigned P, int64_t cap,
+                                 uint32_t *node_first, uint32_t *node_count,
+                                 uint8_t *node_leaf, uint8_t *node_level) {
+  if (P == 0 || n < P || P > 65536) return -1;
+  uint64_t *lc = (uint64_t *)calloc((size_t)P * 64, sizeof(uint64_t));
+  int *h = (int *)calloc(P, sizeof(int));
+  uint64_t dep_cnt[65] = {0}, dep_off[66] = {0};
+  int H = 0;
+  for (unsigned p = 0; p < P; ++p) {
+    uint64_t b, c;
+    orc_partition_range(n, P, p, &b, &c);
+    h[p] = orc_level_counts(c, F, lc + (size_t)p * 64, 64);
+    if (h[p] > H) H = h[p];
+    for (int l = 0; l < h[p]; ++l) dep_cnt[l] += lc[(size_t)p * 64 + l];
+  }
+  for (int d = 0; d < H; ++d) dep_off[d + 1] = dep_off[d] + dep_cnt[d];
+  if ((int64_t)dep_off[H] > cap) {
+    free(lc);
+    free(h);
+    return -1;
+  }
+  /* start[p][d]: partition p's first node id at depth d */
+  uint64_t *start = (uint64_t *)calloc((size_t)P * 64, sizeof(uint64_t));
+  for (int d = 0; d < H; ++d) {
+    uint64_t pos = dep_off[d];
+    for (unsigned p = 0; p < P; ++p) {
+      start[(size_t)p * 64 + d] = pos;
+      if (d < h[p]) pos += lc[(size_t)p * 64 + d];
+    }
+  }
+  for (unsigned p = 0; p < P; ++p) {
+    uint64_t b, c;
+    orc_partition_range(n, P, p, &b, &c);
+    const uint64_t *cnt = lc + (size_t)p * 64;
+    for (int l = 0; l < h[p]; ++l) {
+      const int leaf = l == h[p] - 1;
+      const uint64_t below = leaf ? c : cnt[l + 1];
+      for (uint64_t j = 0; j < cnt[l]; ++j) {
+        const uint64_t id = start[(size_t)p * 64 + l] + j;
+        const uint64_t f = j * F;
+        node_first[id] = (uint32_t)(leaf ? b + f : start[(size_t)p * 64 + l + 1] + f);
+        node_count[id] = (uint32_t)(below - f < F ? below - f : F);
+        node_leaf[id] = (uint8_t)leaf;
+        node_level[id] = (uint8_t)l;
+      }
+    }
+  }
+  free(start);
+  free(lc);
+  free(h);
+  return (int64_t)dep_off[H];
+}
+
+/* The plain search over a partitioned MPHR: global_RestartScanning_and_
+ * ParentCheck with block p starting at partition p's root (g_root_offset[bid],
+ * src/tree/mphr.cpp:314-333) over that partition's subtree, and each query's
+ * per-block results summed as MPHR::Search sums h_hit[] (src/tree/mphr.cpp:
+ * 235-252).  Each partition's node boxes come from its own bottom-up build
+ * (orc_pack over its rank range); IDs stay global ranks. */
+void orc_search_mprs_part(const float *sobjs, uint64_t n, unsigned D, int is_box, unsigned F,
+                          unsigned P, const float *queries, uint64_t nq, uint32_t *counts,
+                          uint64_t *hashes, uint32_t *visits) {
+  unsigned const E = is_box ? 2 * D : D;
+  uint32_t *pc = (uint32_t *)malloc(sizeof(uint32_t) * (nq ? nq : 1));
+  uint64_t *ph = (uint64_t *)malloc(sizeof(uint64_t) * (nq ? nq : 1));
+  uint32_t *pv = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (nq ? nq : 1));
+  for (uint64_t q = 0; q < nq; ++q) {
+    if (counts) counts[q] = 0;
+    if (hashes) hashes[q] = 0;
+    if (visits) visits[2 * q] = visits[2 * q + 1] = 0;
+  }
+  for (unsigned p = 0; p < P && n >= P; ++p) {
+    uint64_t b, c, lcnt[64];
+    orc_partition_range(n, P, p, &b, &c);
+    const int h = orc_level_counts(c, F, lcnt, 64);
+    uint64_t m = 0;
+    for (int l = 0; l < h; ++l) m += lcnt[l];
+    float *boxes = (float *)malloc(sizeof(float) * 2 * D * (m ? m : 1));
+    orc_pack(sobjs + b * E, c, D, is_box, F, boxes);
+    mprs_core(sobjs + b * E, c, D, is_box, F, boxes, queries, nq, pc, ph, NULL, NULL, pv, b);
+    for (uint64_t q = 0; q < nq; ++q) {
+      if (counts) counts[q] += pc[q];
+      if (hashes) hashes[q] += ph[q];
+      if (visits) {
+        visits[2 * q] += pv[2 * q];
+        visits[2 * q + 1] += pv[2 * q + 1];
+      }
+    }
+    free(boxes);
+  }
+  free(pc);
+  free(ph);
+  free(pv);
 }
 
 /* ---- multi-threaded CPU baselines ------------------------------------------- */

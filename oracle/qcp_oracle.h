@@ -25,6 +25,11 @@
  *                        exact covers, a permutation of the inputs, and
  *                        hand-derived splits and descents at small F.
  *   searches             DFS == MPRS == brute-force closed-interval filter.
+ *   partitions           split sizes against the reference's rule, every
+ *                        partition's subtree and boxes against orc_packed_
+ *                        topology / orc_pack of its range (Python mapping of
+ *                        the depth-major ids), DFS == brute force, partition
+ *                        MPRS == brute force.
  */
 #ifndef QCP_ORACLE_H
 #define QCP_ORACLE_H
@@ -108,6 +113,22 @@ void     orc_search_dfs_mt(const float *sobjs, unsigned D, int is_box,
                            const float *node_boxes,
                            const float *queries, uint64_t nq, unsigned nthreads,
                            uint32_t *counts, uint64_t *hashes);
+/* Both searches from each of the roots 0 .. roots-1 in order (a partitioned
+ * MPHR has P roots; one root otherwise). */
+void     orc_search_dfs_roots(const float *sobjs, unsigned D, int is_box,
+                              int64_t n_nodes, const uint32_t *node_first,
+                              const uint32_t *node_count, const uint8_t *node_leaf,
+                              const float *node_boxes, unsigned roots,
+                              const float *queries, uint64_t q0, uint64_t q1,
+                              uint32_t *counts, uint64_t *hashes,
+                              const uint64_t *offsets, uint32_t *ids,
+                              uint64_t *int_entries, uint64_t *leaf_entries);
+void     orc_search_dfs_mt_roots(const float *sobjs, unsigned D, int is_box,
+                                 int64_t n_nodes, const uint32_t *node_first,
+                                 const uint32_t *node_count, const uint8_t *node_leaf,
+                                 const float *node_boxes, unsigned roots,
+                                 const float *queries, uint64_t nq, unsigned nthreads,
+                                 uint32_t *counts, uint64_t *hashes);
 /* MPRS restart-scanning + parent-check (src/tree/mphr.cpp:303-438) over a
  * packed tree, serial.  visits[2*q] = root visits, visits[2*q+1] = node visits. */
 void     orc_search_mprs(const float *sobjs, uint64_t n, unsigned D, int is_box,
@@ -146,6 +167,21 @@ void     orc_kmeans_keys(const float *objs, uint64_t n, unsigned D, int is_box, 
 int64_t  orc_guttman_build(const float *objs, uint64_t n, unsigned D, int is_box, unsigned F,
                            int64_t cap, uint32_t *node_first, uint32_t *node_count,
                            uint8_t *node_leaf, uint8_t *node_level, uint32_t *order);
+/* ---- MPHR data partitions (src/tree/mphr.cpp:60-108, 235-257, 314-333) ---- */
+/* Partition p's rank range: chunk = n/P, partition 0 takes chunk + n%P. */
+void     orc_partition_range(uint64_t n, unsigned P, unsigned p, uint64_t *begin,
+                             uint64_t *count);
+/* The P packed subtrees as one explicit tree, numbered depth-major (partition
+ * p's root is node p); arrays as orc_bvh_topology.  -1 if cap is too small or
+ * P > n.  Node boxes: orc_tree_boxes with F (the padded rule). */
+int64_t  orc_partitioned_topology(uint64_t n, unsigned F, unsigned P, int64_t cap,
+                                  uint32_t *node_first, uint32_t *node_count,
+                                  uint8_t *node_leaf, uint8_t *node_level);
+/* MPRS over every partition's own subtree, per-query results summed over the
+ * partitions (counts, hashes, visits[2q] root / [2q+1] node visits). */
+void     orc_search_mprs_part(const float *sobjs, uint64_t n, unsigned D, int is_box,
+                              unsigned F, unsigned P, const float *queries, uint64_t nq,
+                              uint32_t *counts, uint64_t *hashes, uint32_t *visits);
 /* Closed-form brute force: every object against every query. */
 void     orc_search_brute(const float *sobjs, uint64_t n, unsigned D, int is_box,
                           const float *queries, uint64_t nq,

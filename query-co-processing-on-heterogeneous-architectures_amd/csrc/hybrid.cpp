@@ -76,14 +76,14 @@ void items_packed(const qcp_index* ix, const float* q, uint32_t qi, int L,
   }
 }
 
-// The same over an explicit (BVH) tree; L counts BFS depth from the root.
-void items_explicit(const qcp_index* ix, const float* q, uint32_t qi, int L,
-                    std::vector<unsigned long long>& out, uint64_t& ie,
-                    std::vector<std::pair<int, uint64_t>>& stack) {
+// Items below one internal root node of an explicit tree.
+void items_from(const qcp_index* ix, const float* q, uint32_t qi, int L, uint32_t root,
+                std::vector<unsigned long long>& out, uint64_t& ie,
+                std::vector<std::pair<int, uint64_t>>& stack) {
   const uint32_t D = ix->D;
   const float* nb = ix->h_nbox.data();
   stack.clear();
-  stack.push_back({0, 0});
+  stack.push_back({0, root});
   while (!stack.empty()) {
     const int dep = stack.back().first;
     const uint64_t nd = stack.back().second;
@@ -99,6 +99,20 @@ void items_explicit(const qcp_index* ix, const float* q, uint32_t qi, int L,
       else
         stack.push_back({dep + 1, child});
     }
+  }
+}
+
+// The same over an explicit tree (BVH, Guttman, or a partitioned MPHR, whose
+// roots are nodes 0 .. parts-1, walked in order); L counts depth from a root.
+void items_explicit(const qcp_index* ix, const float* q, uint32_t qi, int L,
+                    std::vector<unsigned long long>& out, uint64_t& ie,
+                    std::vector<std::pair<int, uint64_t>>& stack) {
+  for (uint32_t r = 0; r < ix->parts; ++r) {
+    if (ix->h_count[r] & kLeafFlag) {
+      out.push_back(((unsigned long long)qi << 32) | r);  // a one-leaf partition
+      continue;
+    }
+    items_from(ix, q, qi, L, r, out, ie, stack);
   }
 }
 
@@ -210,7 +224,8 @@ qcp_status qcp::hybrid_run(qcp_batch* b, const qcp_search_cfg& cfg, qcp_search_s
         for (uint64_t qi = a0; qi < a1; ++qi) {
           const float* qp = b->h_queries.data() + qi * 2 * ix->D;
           if (ix->h == 1 || L <= 0) {
-            tl[t].push_back((unsigned long long)qi << 32);  // the root itself
+            for (uint32_t r = 0; r < ix->parts; ++r)  // the roots themselves
+              tl[t].push_back(((unsigned long long)qi << 32) | r);
           } else if (ix->expl) {
             items_explicit(ix, qp, (uint32_t)qi, L, tl[t], t_ie[t], stack);
           } else {

@@ -7,8 +7,8 @@
 //   u64 n | u64 n_nodes | i32 h | u32 pad | u64 level_off[h+1] |
 //   u64 payload bytes | u64 FNV-1a 64 of the payload | payload
 // payload = sorted keys (8n) | perm (4n) | objects SoA (4En) | node boxes SoA
-//           (8D n_nodes) | [explicit layouts] node first (4 n_nodes) |
-//           node count (4 n_nodes)
+//           (8D n_nodes) | [explicit layouts and partitioned MPHR] node first
+//           (4 n_nodes) | node count (4 n_nodes)
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
@@ -21,7 +21,7 @@
 namespace {
 
 constexpr char kMagic[8] = {'Q', 'C', 'P', 'I', 'D', 'X', '0', '1'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;  // 2: qcp_build_cfg carries the MPHR partition count
 
 uint64_t fnv1a(const uint8_t* p, size_t n, uint64_t h) {
   for (size_t i = 0; i < n; ++i) {
@@ -50,7 +50,8 @@ qcp_status qcp_save(const qcp_index* ix, const char* path) try {
   const uint64_t n = in.n, m = in.n_nodes;
   const uint32_t D = in.dims;
   const uint32_t E = in.obj_kind == QCP_OBJ_BOXES ? 2 * D : D;
-  const bool expl = in.layout == QCP_LAYOUT_BVH || in.layout == QCP_LAYOUT_GUTTMAN;
+  const bool expl =
+      in.layout == QCP_LAYOUT_BVH || in.layout == QCP_LAYOUT_GUTTMAN || in.partitions > 1;
   std::vector<uint64_t> keys(n);
   std::vector<uint32_t> perm(n);
   std::vector<float> objs((size_t)E * n), boxes((size_t)2 * D * m);
@@ -81,6 +82,7 @@ qcp_status qcp_save(const qcp_index* ix, const char* path) try {
   cfg.key_mode = in.key_mode;
   cfg.key_src = in.key_src;
   cfg.obj_kind = in.obj_kind;
+  cfg.partitions = in.partitions;
   int32_t h = (int32_t)in.height;
   std::vector<uint64_t> level_off(h + 1, 0);
   for (int l = 0; l < h; ++l) level_off[l + 1] = level_off[l] + in.level_count[l];
@@ -133,7 +135,9 @@ qcp_status qcp_load(const char* path, uint32_t device, qcp_index** out) try {
   if (fread(&pbytes, 8, 1, fc.f) != 1 || fread(&sum, 8, 1, fc.f) != 1) return QCP_EFORMAT;
   const uint32_t D = cfg.dims;
   const uint32_t E = cfg.obj_kind == QCP_OBJ_BOXES ? 2 * D : D;
-  const bool expl = cfg.layout == QCP_LAYOUT_BVH || cfg.layout == QCP_LAYOUT_GUTTMAN;
+  if (cfg.partitions > 65536) return QCP_EFORMAT;
+  const bool expl =
+      cfg.layout == QCP_LAYOUT_BVH || cfg.layout == QCP_LAYOUT_GUTTMAN || cfg.partitions > 1;
   const uint64_t expect = 8 * n + 4 * n + 4ull * E * n + 8ull * D * m + (expl ? 8 * m : 0);
   if (pbytes != expect) return QCP_EFORMAT;
   std::vector<uint8_t> payload(pbytes);

@@ -20,6 +20,7 @@
 #include "../../include/qcp.h"
 #include "bvh_topology.h"
 #include "guttman.h"
+#include "partition_topology.h"
 #include "qcp_internal.h"
 #include "qcp_kernels.h"
 
@@ -98,6 +99,7 @@ static void index_free(qcp_index* ix) {
   dfree(ix->d_ncount);
   dfree(ix->d_nrlo);
   dfree(ix->d_nrhi);
+  dfree(ix->d_nparent);
   dfree(ix->d_keys);
   dfree(ix->d_perm);
   dfree(ix->d_key_or);
@@ -123,6 +125,8 @@ static qcp_status validate_cfg(const qcp_build_cfg* cfg) {
   if (cfg->key_src == QCP_SRC_CENTER && cfg->obj_kind != QCP_OBJ_BOXES) return QCP_EINVAL;
   if (cfg->order > QCP_ORDER_KMEANS_HILBERT || cfg->clusters > 4096) return QCP_EINVAL;
   if (cfg->order == QCP_ORDER_KMEANS_HILBERT && (cfg->flags & QCP_BUILD_GRAPH)) return QCP_EINVAL;
+  if (cfg->partitions > 1 && (cfg->layout != QCP_LAYOUT_MPHR || cfg->partitions > 65536))
+    return QCP_EINVAL;  // partitions are MPHR's (src/tree/mphr.cpp:60-108)
   return QCP_OK;
 }
 
@@ -154,19 +158,21 @@ static qcp_status attach_topology(qcp_index* ix, std::vector<uint32_t>&& first,
   const uint64_t stride = ((m + kNodeSlack + 63) / 64) * 64;
   qcp_status s;
   if (!ix->d_nbox || stride > ix->node_stride) {
-    for (uint32_t** a : {&ix->d_nfirst, &ix->d_ncount, &ix->d_nrlo, &ix->d_nrhi}) dfree(*a);
+    for (uint32_t** a : {&ix->d_nfirst, &ix->d_ncount, &ix->d_nrlo, &ix->d_nrhi, &ix->d_nparent})
+      dfree(*a);
     dfree(ix->d_nbox);
     ix->node_stride = stride;
     if ((s = dalloc(&ix->d_nbox, (uint64_t)2 * ix->D * stride)) != QCP_OK) return s;
     if (launch_fill_f32(ix->d_nbox, (uint64_t)2 * ix->D * stride, 0.0f, ix->stream) != hipSuccess)
       return QCP_EHIP;
-    for (uint32_t** a : {&ix->d_nfirst, &ix->d_ncount, &ix->d_nrlo, &ix->d_nrhi}) {
+    for (uint32_t** a : {&ix->d_nfirst, &ix->d_ncount, &ix->d_nrlo, &ix->d_nrhi, &ix->d_nparent}) {
       if ((s = dalloc(a, stride)) != QCP_OK) return s;
       if (launch_fill_u32(*a, stride, 0u, ix->stream) != hipSuccess) return QCP_EHIP;
     }
   }
-  std::vector<uint32_t> rlo(m), rhi(m);
+  std::vector<uint32_t> rlo(m), rhi(m), parent(m);
   for (uint64_t i = m; i-- > 0;) {
+    parent[i] = (uint32_t)i;  // a root stays its own parent
     if (count[i] & kLeafBit) {
       rlo[i] = first[i];
       rhi[i] = first[i] + (count[i] & ~kLeafBit);
@@ -175,9 +181,12 @@ static qcp_status attach_topology(qcp_index* ix, std::vector<uint32_t>&& first,
       rhi[i] = rhi[first[i] + count[i] - 1];
     }
   }
-  uint32_t* dst[4] = {ix->d_nfirst, ix->d_ncount, ix->d_nrlo, ix->d_nrhi};
-  const uint32_t* src[4] = {first.data(), count.data(), rlo.data(), rhi.data()};
-  for (int i = 0; i < 4 && m; ++i)
+  for (uint64_t i = 0; i < m; ++i)
+    if (!(count[i] & kLeafBit))
+      for (uint32_t k = 0; k < count[i]; ++k) parent[first[i] + k] = (uint32_t)i;
+  uint32_t* dst[5] = {ix->d_nfirst, ix->d_ncount, ix->d_nrlo, ix->d_nrhi, ix->d_nparent};
+  const uint32_t* src[5] = {first.data(), count.data(), rlo.data(), rhi.data(), parent.data()};
+  for (int i = 0; i < 5 && m; ++i)
     HIP_TRY(hipMemcpyAsync(dst[i], src[i], 4 * m, hipMemcpyHostToDevice, ix->stream));
   HIP_TRY(hipStreamSynchronize(ix->stream));
   ix->n_nodes = m;
@@ -195,6 +204,7 @@ static qcp_status attach_topology(qcp_index* ix, std::vector<uint32_t>&& first,
 static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** out) {
   QCP_TRY(validate_cfg(cfg));
   if (n >= 0xFFFFFE00ull) return QCP_EINVAL;  // ranks (+ one fanout) and node ids are 32-bit
+  if (cfg->partitions > 1 && n > 0 && cfg->partitions > n) return QCP_EINVAL;  // an empty partition
   QCP_TRY(set_device((int)cfg->device));
   qcp_index* ix = new (std::nothrow) qcp_index();
   if (!ix) return QCP_ENOMEM;
@@ -204,7 +214,9 @@ static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** 
   ix->F = cfg->fanout;
   ix->box = cfg->obj_kind == QCP_OBJ_BOXES;
   ix->E = ix->box ? 2 * ix->D : ix->D;
-  ix->expl = cfg->layout == QCP_LAYOUT_BVH || cfg->layout == QCP_LAYOUT_GUTTMAN;
+  // MPHR data partitions (an empty index is one empty tree)
+  ix->parts = (cfg->partitions > 1 && n > 0) ? cfg->partitions : 1u;
+  ix->expl = cfg->layout == QCP_LAYOUT_BVH || cfg->layout == QCP_LAYOUT_GUTTMAN || ix->parts > 1;
   ix->nb = cfg->key_mode == QCP_KEY_REF ? (ix->D > 2 ? 20u : 31u) : 16u;
   ix->n = n;
   auto fail = [&](qcp_status s) {
@@ -230,6 +242,11 @@ static qcp_status index_alloc(const qcp_build_cfg* cfg, uint64_t n, qcp_index** 
     if ((s = dalloc(&ix->d_nbox, (uint64_t)2 * ix->D * ix->node_stride)) != QCP_OK) return fail(s);
     if (launch_fill_f32(ix->d_nbox, (uint64_t)2 * ix->D * ix->node_stride, 0.0f, ix->stream) != hipSuccess)
       return fail(QCP_EHIP);
+  } else if (ix->parts > 1) {
+    PartitionTopology t;
+    if (!partition_topology(n, ix->F, ix->parts, kMaxDepth, &t)) return fail(QCP_EINVAL);
+    if ((s = attach_topology(ix, std::move(t.first), std::move(t.count), t.level_off)) != QCP_OK)
+      return fail(s);
   } else if (cfg->layout == QCP_LAYOUT_BVH) {
     BvhTopology t;
     if (!bvh_topology(n, ix->F, kMaxDepth, &t)) return fail(QCP_EINVAL);
@@ -272,6 +289,7 @@ static void set_info_shape(qcp_index* ix) {
   in.key_bits = ix->nb * ix->D;
   for (int l = 0; l < ix->h && l < 32; ++l) in.level_count[l] = ix->level_count[l];
   in.tree_bytes = (uint64_t)ix->E * 4 * ix->n + (uint64_t)2 * ix->D * 4 * ix->n_nodes;
+  in.partitions = ix->parts;
 }
 
 // Guttman layout: insert the objects on the host in their sorted order
@@ -795,6 +813,11 @@ void qcp_batch_destroy(qcp_batch* b) {
   dfree(b->d_slots);
   if (b->h_counts) hipHostFree(b->h_counts);
   if (b->h_visits) hipHostFree(b->h_visits);
+  dfree(b->d_pcounts);
+  dfree(b->d_phashes);
+  dfree(b->d_pvisits);
+  for (void* h : {(void*)b->h_pcounts, (void*)b->h_phashes, (void*)b->h_pvisits})
+    if (h) hipHostFree(h);
   if (b->exec) hipGraphExecDestroy(b->exec);
   for (auto& e : b->ev)
     if (e) hipEventDestroy(e);
@@ -892,6 +915,8 @@ SearchArgs qcp::index_search_args(const qcp_index* ix, const float* d_queries, u
   a.ncount = ix->d_ncount;
   a.nrlo = ix->d_nrlo;
   a.nrhi = ix->d_nrhi;
+  a.nparent = ix->d_nparent;
+  a.parts = ix->parts;
   a.queries = d_queries;
   a.obj_stride = ix->obj_stride;
   a.node_stride = ix->node_stride;
@@ -957,6 +982,66 @@ static qcp_status batch_order(qcp_batch* b) {
   return QCP_OK;
 }
 
+// The reference's partition mode of MPHR::Search (src/tree/mphr.cpp:235-257):
+// one launch per query with one workgroup per partition, each running MPRS
+// over its partition's subtree from its root (g_root_offset[bid], 314-333);
+// the per-workgroup counts, hashes and visit counters are copied back
+// synchronously after every launch and summed on the host, then stored as the
+// batch's per-query results.
+static qcp_status plain_partitions(qcp_batch* b, uint32_t waves, qcp_search_stats* st) {
+  qcp_index* ix = b->ix;
+  hipStream_t s = ix->stream;
+  const uint64_t q = b->q;
+  const uint32_t P = ix->parts;
+  if (!b->d_pcounts) {
+    QCP_TRY(dalloc(&b->d_pcounts, P));
+    QCP_TRY(dalloc(&b->d_phashes, P));
+    QCP_TRY(dalloc(&b->d_pvisits, 2ull * P));
+    HIP_TRY(hipHostMalloc((void**)&b->h_pcounts, 4ull * P, 0));
+    HIP_TRY(hipHostMalloc((void**)&b->h_phashes, 8ull * P, 0));
+    HIP_TRY(hipHostMalloc((void**)&b->h_pvisits, 8ull * P, 0));
+  }
+  std::vector<uint32_t> counts(q), visits(2 * q);
+  std::vector<unsigned long long> hashes(q);
+  SearchArgs a = make_args(b);
+  a.counts = b->d_pcounts;
+  a.hashes = b->d_phashes;
+  a.visits = b->d_pvisits;
+  HIP_TRY(hipEventRecord(b->ev[0], s));
+  uint64_t total = 0;
+  for (uint64_t qi = 0; qi < q; ++qi) {
+    a.q_begin = (uint32_t)qi;
+    HIP_TRY(launch_mprs_part(a, ix->D, ix->box, P, waves, s));
+    HIP_TRY(hipMemcpyAsync(b->h_pcounts, b->d_pcounts, 4ull * P, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(b->h_phashes, b->d_phashes, 8ull * P, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(b->h_pvisits, b->d_pvisits, 8ull * P, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (uint32_t p = 0; p < P; ++p) {
+      counts[qi] += b->h_pcounts[p];
+      hashes[qi] += b->h_phashes[p];
+      visits[2 * qi] += b->h_pvisits[2 * p];
+      visits[2 * qi + 1] += b->h_pvisits[2 * p + 1];
+    }
+    total += counts[qi];
+    st->launches++;
+  }
+  HIP_TRY(hipEventRecord(b->ev[1], s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0;
+  hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+  if (q) {
+    HIP_TRY(hipMemcpyAsync(b->d_counts, counts.data(), 4 * q, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->d_hashes, hashes.data(), 8 * q, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->d_visits, visits.data(), 8 * q, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  st->kernel_ms = ms;
+  st->total_ms = ms;
+  st->total_hits = total;
+  b->total_hits = total;
+  return QCP_OK;
+}
+
 qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_stats* stats) try {
   if (!b) return QCP_EINVAL;
   qcp_search_cfg cfg;
@@ -981,10 +1066,15 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
   b->last_mode = cfg.mode;
   b->last_outputs = cfg.outputs;
   if (cfg.mode == QCP_SEARCH_PLAIN) {
-    if (ix->expl) return QCP_EINVAL;  // MPRS needs the packed level layout
     if (cfg.outputs & QCP_OUT_IDS) return QCP_EINVAL;
     const uint32_t waves = cfg.plain_waves ? cfg.plain_waves : 1;
     if (waves != 1 && waves != 3) return QCP_EINVAL;
+    if (ix->parts > 1) {
+      QCP_TRY(plain_partitions(b, waves, &st));
+      if (stats) *stats = st;
+      return QCP_OK;
+    }
+    if (ix->expl) return QCP_EINVAL;  // MPRS needs the packed level layout
     const uint32_t batch = cfg.plain_batch ? cfg.plain_batch : 128;
     if (!b->h_counts && q) {
       HIP_TRY(hipHostMalloc((void**)&b->h_counts, 4 * q, 0));
@@ -1304,7 +1394,7 @@ __attribute__((visibility("hidden"))) qcp_status qcp_internal_restore(
     if (st != QCP_OK) return fail(st);
   }
   if (ix->n_nodes != n_nodes) return fail(QCP_EFORMAT);
-  if (ix->cfg.layout == QCP_LAYOUT_BVH && n_nodes) {
+  if ((ix->cfg.layout == QCP_LAYOUT_BVH || ix->parts > 1) && n_nodes) {
     std::vector<uint32_t> f(n_nodes), c(n_nodes);
     if (hipMemcpy(f.data(), ix->d_nfirst, 4 * n_nodes, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(c.data(), ix->d_ncount, 4 * n_nodes, hipMemcpyDeviceToHost) != hipSuccess)

@@ -29,9 +29,10 @@
 //   -u U      cluster type (GetClusterType, 431-446): h|hilbert (default),
 //             k|kmeans (k-means clusters ranked by their centres' Hilbert keys,
 //             --clusters K), o|original (the input order is the rank order)
-//   -t T, -y Y, -f   accepted; partitions and data type have no counterpart
-//             (one tree per index, data from --data); -f rebuilds even when
-//             --index exists
+//   -t T      MPHR data partitions (SetNumberOfPartition): T subtrees over
+//             contiguous rank ranges; MPHR indexes only (-i m)
+//   -y Y, -f  accepted; the data type has no counterpart (data from --data);
+//             -f rebuilds even when --index exists
 // Options of this driver:
 //   --data FILE --queries FILE --dims D --fanout F (default: the reference's
 //   degree, 192 for MPHR, 128 otherwise) --boxes --center
@@ -71,6 +72,7 @@ struct Opts {
   uint32_t ref_degree = 192;
   std::string cluster = "h";
   uint32_t clusters = 0;
+  uint32_t partitions = 1;
 };
 
 uint64_t parse_count(const char* s) {
@@ -154,7 +156,8 @@ int main(int argc, char** argv) {
         case 'r': case 'R': o.repeat = (uint32_t)atoi(optarg); break;
         case 'e': case 'E': o.eval = (uint32_t)atoi(optarg); break;
         case 'u': case 'U': o.cluster = optarg; break;
-        case 't': case 'T': case 'y': case 'Y': break;
+        case 't': case 'T': o.partitions = (uint32_t)atoi(optarg); break;
+        case 'y': case 'Y': break;
         case 'f': case 'F': o.force = true; break;
         case 1000: o.data = optarg; break;
         case 1001: o.queries = optarg; break;
@@ -259,7 +262,8 @@ int main(int argc, char** argv) {
     if ((st = qcp_read_reference_mphr(o.ref_dump.c_str(), D, o.ref_degree, kind, objs.data(), n,
                                       &n)) != QCP_OK)
       return fail("reference dump", st);
-    qcp_build_cfg cfg = {D, o.fanout, layout, QCP_KEY_REF, QCP_SRC_LO, kind, 0, 0};
+    qcp_build_cfg cfg = {D, o.fanout, layout, QCP_KEY_REF, QCP_SRC_LO, kind, 0, 0,
+                         QCP_ORDER_HILBERT, 0, 0, o.partitions};
     if ((st = qcp_build_ranked(&cfg, objs.data(), n, &ix)) != QCP_OK) return fail("build", st);
     if (!o.index_file.empty() && (st = qcp_save(ix, o.index_file.c_str())) != QCP_OK)
       return fail("save", st);
@@ -271,7 +275,7 @@ int main(int argc, char** argv) {
     qcp_build_cfg cfg = {D, o.fanout, layout, o.b16 ? (uint32_t)QCP_KEY_B16 : (uint32_t)QCP_KEY_REF,
                          o.center ? (uint32_t)QCP_SRC_CENTER : (uint32_t)QCP_SRC_LO,
                          o.boxes ? (uint32_t)QCP_OBJ_BOXES : (uint32_t)QCP_OBJ_POINTS, 0, 0,
-                         QCP_ORDER_HILBERT, o.clusters, 0};
+                         QCP_ORDER_HILBERT, o.clusters, 0, o.partitions};
     std::string ct = o.cluster;
     for (auto& ch : ct) ch = (char)tolower(ch);
     const bool original = input_order || ct == "o" || ct == "original";

@@ -36,6 +36,8 @@ struct SearchArgs {
   const uint32_t* ncount;    // explicit layouts: count | kLeafBit for leaves
   const uint32_t* nrlo;      // explicit layouts: first rank under each node
   const uint32_t* nrhi;      // explicit layouts: end rank (exclusive) under each node
+  const uint32_t* nparent;   // explicit layouts: parent node (a root: itself)
+  uint32_t parts;            // roots 0 .. parts-1 (MPHR data partitions; 1 otherwise)
   const float* queries;      // [nq][2D] row-major [lo(D), hi(D)]
   const uint32_t* qperm;     // processing order of the queries (nullptr = as given)
   uint64_t obj_stride;

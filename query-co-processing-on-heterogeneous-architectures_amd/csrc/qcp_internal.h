@@ -29,6 +29,8 @@ struct qcp_index {
   uint32_t* d_ncount = nullptr;
   uint32_t* d_nrlo = nullptr;    // explicit layouts: rank range under each node
   uint32_t* d_nrhi = nullptr;
+  uint32_t* d_nparent = nullptr; // explicit layouts: parent node (roots: themselves)
+  uint32_t parts = 1;            // MPHR data partitions (roots are nodes 0 .. parts-1)
   uint64_t* d_keys = nullptr;    // sorted keys [n]
   uint32_t* d_perm = nullptr;    // [n]
   // build scratch
@@ -84,6 +86,13 @@ struct qcp_batch {
   uint64_t slots_dirty = 0;       // slots written by the last donating run
   uint32_t* h_counts = nullptr;   // pinned, plain-form per-batch copies
   uint32_t* h_visits = nullptr;
+  // plain form over a partitioned index: one workgroup's results per partition
+  uint32_t* d_pcounts = nullptr;
+  unsigned long long* d_phashes = nullptr;
+  uint32_t* d_pvisits = nullptr;
+  uint32_t* h_pcounts = nullptr;  // pinned
+  unsigned long long* h_phashes = nullptr;
+  uint32_t* h_pvisits = nullptr;
   hipEvent_t ev[6] = {};
   hipGraphExec_t exec = nullptr;            // captured search (qcp_search_cfg.graph)
   uint32_t exec_key[12] = {};               // configuration baked into exec

@@ -90,6 +90,11 @@ hipError_t launch_fill_f32(float* p, uint64_t n, float v, hipStream_t s);
 // u: 64-entry leaf chunks in flight per wave (2, 4 or 8)
 hipError_t launch_search(const SearchArgs& a, unsigned D, bool box, bool expl, int out, int u,
                          unsigned grid, unsigned block, hipStream_t s);
+// the plain form over a partitioned MPHR: one workgroup of `waves` waves per
+// partition (parts of them) for query a.q_begin; per-workgroup results in
+// a.counts[p], a.hashes[p], a.visits[2p..2p+1]
+hipError_t launch_mprs_part(const SearchArgs& a, unsigned D, bool box, unsigned parts,
+                            unsigned waves, hipStream_t s);
 // the plain form: nblocks workgroups of `waves` (1 or 3) waves, one per query
 hipError_t launch_mprs(const SearchArgs& a, unsigned D, bool box, unsigned nblocks,
                        unsigned waves, hipStream_t s);

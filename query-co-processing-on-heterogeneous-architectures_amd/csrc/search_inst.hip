@@ -12,7 +12,10 @@ namespace qcp {
 template hipError_t launch_search_dbe<QCP_D, (bool)QCP_BOX, (bool)QCP_EXPL>(
     const SearchArgs&, int, int, unsigned, unsigned, hipStream_t);
 template const void* search_ptr_dbe<QCP_D, (bool)QCP_BOX, (bool)QCP_EXPL>(int, int);
-#if !QCP_EXPL
+#if QCP_EXPL
+template hipError_t launch_mprs_part_db<QCP_D, (bool)QCP_BOX>(const SearchArgs&, unsigned,
+                                                              unsigned, hipStream_t);
+#else
 template hipError_t launch_mprs_db<QCP_D, (bool)QCP_BOX>(const SearchArgs&, unsigned, unsigned,
                                                          hipStream_t);
 #endif

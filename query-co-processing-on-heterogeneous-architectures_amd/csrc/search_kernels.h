@@ -690,7 +690,14 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
     c.wbase = c.out_base & ~(uint64_t)(kWave - 1);
     c.wlo = c.out_base;
     load_query<D>(a, q, c);
-    if (a.h > 0) walk<D, BOX, MW, EXPL, OUT, U>(a, c, node, stk);
+    if (a.h > 0) {
+      if (EXPL && a.parts > 1 && got && !a.items) {
+        // a query over a partitioned MPHR: every partition root, in rank order
+        for (uint32_t r = 0; r < a.parts; ++r) walk<D, BOX, MW, EXPL, OUT, U>(a, c, r, stk);
+      } else {
+        walk<D, BOX, MW, EXPL, OUT, U>(a, c, node, stk);
+      }
+    }
     if (OUT == kOutIds) flush_partial<D>(a, c);
     if (OUT != kOutIds) {
       // the hash is additive over disjoint ID sets, so donated parts add up
@@ -882,6 +889,134 @@ __global__ __launch_bounds__(64 * NW) void k_mprs(SearchArgs a) {
   }
 }
 
+// The plain form over a partitioned MPHR (explicit node arrays): workgroup p
+// runs MPRS over partition p's subtree from its root, node p, for query
+// q_begin, as the reference's block bid does from g_root_offset[bid]
+// (src/tree/mphr.cpp:314-333).  A node's "last index" is the end of its rank
+// range (nrhi), a leaf's parent comes from nparent (the reference's leaf
+// back-pointer) and the next leaf is the next node.  Results per workgroup:
+// counts[p], hashes[p], visits[2p] (root visits), visits[2p+1] (node visits).
+template <int D, bool BOX, int MW, int NW>
+__global__ __launch_bounds__(64 * NW) void k_mprs_part(SearchArgs a) {
+  constexpr int E = BOX ? 2 * D : D;
+  constexpr int kNone = 1 << 30;
+  const uint32_t lane = lane_id();
+  const int wv = NW > 1 ? (int)(threadIdx.x / kWave) : 0;
+  const uint32_t q = a.q_begin;
+  const uint32_t root = blockIdx.x;
+  if (q >= a.nq || root >= a.parts) return;  // uniform over the workgroup
+  __shared__ int s_red[NW];
+  __shared__ unsigned long long s_sum[3][NW];
+  auto group_min = [&](int v) -> int {
+    if constexpr (NW == 1) {
+      return v;
+    } else {
+      if (lane == 0) s_red[wv] = v;
+      __syncthreads();
+      int m = s_red[0];
+#pragma unroll
+      for (int i = 1; i < NW; ++i) m = s_red[i] < m ? s_red[i] : m;
+      __syncthreads();
+      return m;
+    }
+  };
+  Ctx<D> c;
+  load_query<D>(a, q, c);
+  uint32_t cnt = 0, rootv = 1, nodev = 0;
+  uint64_t s1 = 0, s2 = 0;
+  uint32_t visited = a.nrlo[root];  // ranks below are done
+  const uint32_t last = a.nrhi[root];
+  uint32_t nd = root;
+  while (visited < last) {
+    while (!(a.ncount[nd] & kLeafBit)) {
+      const uint32_t cf = a.nfirst[nd];
+      const uint32_t cc = a.ncount[nd];
+      int pick = kNone;
+#pragma unroll
+      for (int t = 0; t < (MW + NW - 1) / NW; ++t) {
+        const int w = wv + t * NW;
+        if (w >= MW) break;
+        const uint32_t k = w * kWave + lane;
+        const uint64_t ch = (uint64_t)cf + k;  // within the node arrays' slack
+        float lo[D], hi[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          lo[d] = a.nbox[(uint64_t)d * a.node_stride + ch];
+          hi[d] = a.nbox[(uint64_t)(D + d) * a.node_stride + ch];
+        }
+        const bool ok = (k < cc) && a.nrhi[ch] > visited && overlaps<D>(c.ql, c.qh, lo, hi);
+        const uint64_t b = ballot(ok);
+        if (pick == kNone && b) pick = w * kWave + __builtin_ctzll(b);
+      }
+      pick = group_min(pick);
+      if (pick == kNone) {
+        visited = a.nrhi[nd];
+        nd = root;
+        ++rootv;
+        break;
+      }
+      nd = cf + (uint32_t)pick;
+      ++nodev;
+    }
+    while (a.ncount[nd] & kLeafBit) {
+      const uint32_t f = a.nfirst[nd];
+      const uint32_t cc = a.ncount[nd] & ~kLeafBit;
+      int none = 1;
+#pragma unroll
+      for (int t = 0; t < (MW + NW - 1) / NW; ++t) {
+        const int w = wv + t * NW;
+        if (w >= MW) break;
+        const uint32_t k = w * kWave + lane;
+        float v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = a.obj[(uint64_t)e * a.obj_stride + f + k];
+        const bool hit = (k < cc) && overlaps<D>(c.ql, c.qh, v, BOX ? v + D : v);
+        const uint64_t b = ballot(hit);
+        cnt += (uint32_t)__popcll(b);
+        if (hit) {
+          const uint64_t id = (uint64_t)f + k + 1;
+          s1 += id;
+          s2 += id * id;
+        }
+        if (b) none = 0;
+      }
+      const bool any = group_min(none) == 0;
+      visited = a.nrhi[nd];
+      if (visited == last) break;
+      if (any) {
+        ++nd;
+        ++nodev;
+      } else {
+        nd = a.nparent[nd];
+        if (nd == root) ++rootv; else ++nodev;
+      }
+    }
+  }
+  uint64_t t1 = wave_sum_u64(s1), t2 = wave_sum_u64(s2);
+  uint64_t tc = cnt;
+  if constexpr (NW > 1) {
+    if (lane == 0) {
+      s_sum[0][wv] = tc;
+      s_sum[1][wv] = t1;
+      s_sum[2][wv] = t2;
+    }
+    __syncthreads();
+    tc = t1 = t2 = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      tc += s_sum[0][i];
+      t1 += s_sum[1][i];
+      t2 += s_sum[2][i];
+    }
+  }
+  if (threadIdx.x == 0) {
+    a.counts[root] = (uint32_t)tc;
+    a.hashes[root] = set_hash(t1, t2);
+    a.visits[2 * root] = rootv;
+    a.visits[2 * root + 1] = nodev;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers, instantiated per (D, object kind, layout family) in their own
 // translation units (search_inst.hip compiled once per combination), so the
@@ -977,6 +1112,22 @@ hipError_t launch_mprs_db(const SearchArgs& a, unsigned nblocks, unsigned waves,
   return hipGetLastError();
 }
 
+// The plain form over a partitioned MPHR: one workgroup of 1 or 3 waves per
+// partition, for query a.q_begin.
+template <int D, bool BOX>
+hipError_t launch_mprs_part_db(const SearchArgs& a, unsigned parts, unsigned waves,
+                               hipStream_t s) {
+  const int mw = (int)((a.F + 63) / 64);
+  if (waves == 3) {
+    QCP_MW_SWITCH(mw, k_mprs_part<D, BOX, MW, 3><<<parts, 192, 0, s>>>(a));
+  } else if (waves == 1) {
+    QCP_MW_SWITCH(mw, k_mprs_part<D, BOX, MW, 1><<<parts, 64, 0, s>>>(a));
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // Every combination is instantiated in exactly one translation unit.
 #define QCP_SEARCH_EXTERN(DD, BB, EE)                                                        \
   extern template hipError_t launch_search_dbe<DD, BB, EE>(const SearchArgs&, int, int,      \
@@ -995,5 +1146,13 @@ extern template hipError_t launch_mprs_db<2, false>(const SearchArgs&, unsigned,
 extern template hipError_t launch_mprs_db<2, true>(const SearchArgs&, unsigned, unsigned, hipStream_t);
 extern template hipError_t launch_mprs_db<3, false>(const SearchArgs&, unsigned, unsigned, hipStream_t);
 extern template hipError_t launch_mprs_db<3, true>(const SearchArgs&, unsigned, unsigned, hipStream_t);
+extern template hipError_t launch_mprs_part_db<2, false>(const SearchArgs&, unsigned, unsigned,
+                                                         hipStream_t);
+extern template hipError_t launch_mprs_part_db<2, true>(const SearchArgs&, unsigned, unsigned,
+                                                        hipStream_t);
+extern template hipError_t launch_mprs_part_db<3, false>(const SearchArgs&, unsigned, unsigned,
+                                                         hipStream_t);
+extern template hipError_t launch_mprs_part_db<3, true>(const SearchArgs&, unsigned, unsigned,
+                                                        hipStream_t);
 
 }  // namespace qcp

@@ -63,6 +63,17 @@ hipError_t launch_mprs(const SearchArgs& a, unsigned D, bool box, unsigned nbloc
   return hipErrorInvalidValue;
 }
 
+hipError_t launch_mprs_part(const SearchArgs& a, unsigned D, bool box, unsigned parts,
+                            unsigned waves, hipStream_t s) {
+  if (D == 2)
+    return box ? launch_mprs_part_db<2, true>(a, parts, waves, s)
+               : launch_mprs_part_db<2, false>(a, parts, waves, s);
+  if (D == 3)
+    return box ? launch_mprs_part_db<3, true>(a, parts, waves, s)
+               : launch_mprs_part_db<3, false>(a, parts, waves, s);
+  return hipErrorInvalidValue;
+}
+
 const void* search_kernel_ptr(unsigned D, bool box, bool expl, int out, int mw) {
   if (D == 2) {
     if (box) return expl ? search_ptr_dbe<2, true, true>(out, mw) : search_ptr_dbe<2, true, false>(out, mw);

@@ -37,7 +37,7 @@ class QcpError(RuntimeError):
 class BuildCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in
                 ("dims", "fanout", "layout", "key_mode", "key_src", "obj_kind", "device", "flags",
-                 "order", "clusters", "kmeans_iters")]
+                 "order", "clusters", "kmeans_iters", "partitions")]
 
 ORDER = {"hilbert": 0, "kmeans": 1}
 
@@ -48,7 +48,8 @@ class IndexInfo(ctypes.Structure):
                 ("obj_kind", ctypes.c_uint32), ("key_mode", ctypes.c_uint32),
                 ("key_src", ctypes.c_uint32), ("key_bits", ctypes.c_uint32),
                 ("level_count", ctypes.c_uint64 * 32), ("tree_bytes", ctypes.c_uint64),
-                ("build_ms", ctypes.c_double), ("build_phase_ms", ctypes.c_double * 6)]
+                ("build_ms", ctypes.c_double), ("build_phase_ms", ctypes.c_double * 6),
+                ("partitions", ctypes.c_uint32)]
 
 
 class SearchCfg(ctypes.Structure):
@@ -240,31 +241,34 @@ class Index:
     @classmethod
     def build(cls, objs, dims, fanout=64, layout="mphr", key_mode="ref", key_src="lo",
               obj_kind="points", device=0, graph=False, order="hilbert", clusters=0,
-              kmeans_iters=0):
+              kmeans_iters=0, partitions=1):
         """Build from host objects (N x D points or N x 2D boxes).  With
         graph=True the device pipeline runs as a captured hipGraph; order="kmeans"
-        ranks k-means clusters by their centres' Hilbert keys first."""
+        ranks k-means clusters by their centres' Hilbert keys first; partitions=P
+        splits an MPHR into P subtrees over contiguous rank ranges."""
         if isinstance(objs, DeviceBuffer):
             return cls.build_device(objs, dims, fanout, layout, key_mode, key_src, obj_kind,
-                                    device, graph)
+                                    device, graph, partitions)
         objs = np.ascontiguousarray(objs, dtype=np.float32)
         E = dims * (2 if obj_kind == "boxes" else 1)
         n = objs.size // E if objs.size else 0
         cfg = BuildCfg(dims, fanout, LAYOUT[layout], KEY[key_mode], SRC[key_src],
                        OBJ[obj_kind], device, 1 if graph else 0, ORDER[order], clusters,
-                       kmeans_iters)
+                       kmeans_iters, partitions)
         h = P()
         check(lib().qcp_build(ctypes.byref(cfg), _p(objs) if n else None, n, ctypes.byref(h)),
               "qcp_build")
         return cls(h, dims, obj_kind)
 
     @classmethod
-    def build_ranked(cls, objs, dims, fanout=64, layout="mphr", obj_kind="points", device=0):
+    def build_ranked(cls, objs, dims, fanout=64, layout="mphr", obj_kind="points", device=0,
+                     partitions=1):
         """Build from objects already in rank order (entry i gets rank ID i+1)."""
         objs = np.ascontiguousarray(objs, dtype=np.float32)
         E = dims * (2 if obj_kind == "boxes" else 1)
         n = objs.size // E if objs.size else 0
-        cfg = BuildCfg(dims, fanout, LAYOUT[layout], 0, 0, OBJ[obj_kind], device, 0)
+        cfg = BuildCfg(dims, fanout, LAYOUT[layout], 0, 0, OBJ[obj_kind], device, 0, 0, 0, 0,
+                       partitions)
         h = P()
         check(lib().qcp_build_ranked(ctypes.byref(cfg), _p(objs) if n else None, n,
                                      ctypes.byref(h)), "qcp_build_ranked")
@@ -272,12 +276,12 @@ class Index:
 
     @classmethod
     def build_device(cls, buf, dims, fanout=64, layout="mphr", key_mode="ref", key_src="lo",
-                     obj_kind="points", device=0, graph=False):
+                     obj_kind="points", device=0, graph=False, partitions=1):
         """Build from objects already resident in HBM (a DeviceBuffer)."""
         E = dims * (2 if obj_kind == "boxes" else 1)
         n = buf.nbytes // (4 * E)
         cfg = BuildCfg(dims, fanout, LAYOUT[layout], KEY[key_mode], SRC[key_src],
-                       OBJ[obj_kind], device, 1 if graph else 0)
+                       OBJ[obj_kind], device, 1 if graph else 0, 0, 0, 0, partitions)
         h = P()
         check(lib().qcp_build_device(ctypes.byref(cfg), buf.ptr, n, ctypes.byref(h)),
               "qcp_build_device")

@@ -85,8 +85,18 @@ int main(int argc, char** argv) {
 
   // the reference's setters
   tree::MPHR m;
-  m.SetNumberOfPartition(4);
-  CHECK(!m.Build(input) && m.GetLastStatus() == QCP_EINVAL, "partitions other than 1");
+  // data partitions (the reference's -t): 7 subtrees, searched in the
+  // persistent form and in the reference's partition mode (plain)
+  m.SetNumberOfPartition(7);
+  CHECK(m.GetTreeType() == tree::TREE_TYPE_MPHR_PARTITION, "partitioned tree type");
+  CHECK(m.Build(input) && m.Search(qs, q, 1) == 1, "MPHR with 7 partitions");
+  CHECK(write_results(out + "/m_part7.txt", m), "write partition results");
+  m.SetSearchMode(QCP_SEARCH_PLAIN);
+  CHECK(m.Search(qs, q, 1) == 1, "MPHR with 7 partitions, plain form");
+  CHECK(write_results(out + "/m_part7_plain.txt", m), "write partition plain results");
+  m.SetSearchMode(QCP_SEARCH_PERSISTENT);
+  m.SetNumberOfPartition(n + 1);
+  CHECK(!m.Build(input) && m.GetLastStatus() == QCP_EINVAL, "more partitions than objects");
   m.SetNumberOfPartition(1);
   m.SetNumberOfCUDABlocks(64);
   m.SetSearchMode(QCP_SEARCH_PLAIN);

@@ -80,6 +80,13 @@ def _declare(L):
     L.orc_kmeans_keys.argtypes = [P, U64, UNS, INT, INT, INT, UNS, UNS, P, P, P]
     L.orc_guttman_build.argtypes = [P, U64, UNS, INT, UNS, I64, P, P, P, P, P]
     L.orc_guttman_build.restype = I64
+    L.orc_partition_range.argtypes = [U64, UNS, UNS, P, P]
+    L.orc_search_dfs_roots.argtypes = [P, UNS, INT, I64, P, P, P, P, UNS, P, U64, U64,
+                                       P, P, P, P, P, P]
+    L.orc_search_dfs_mt_roots.argtypes = [P, UNS, INT, I64, P, P, P, P, UNS, P, U64, UNS, P, P]
+    L.orc_partitioned_topology.argtypes = [U64, UNS, UNS, I64, P, P, P, P]
+    L.orc_partitioned_topology.restype = I64
+    L.orc_search_mprs_part.argtypes = [P, U64, UNS, INT, UNS, UNS, P, U64, P, P, P]
 
 
 def ptr(a):
@@ -145,10 +152,11 @@ class Tree:
     layout "guttman" inserts the rank-sorted objects one by one and then holds
     them in depth-first leaf order (self.order maps position -> rank)."""
 
-    def __init__(self, sobjs, D, is_box, F, layout="packed"):
+    def __init__(self, sobjs, D, is_box, F, layout="packed", parts=1):
         self.sobjs = np.ascontiguousarray(sobjs, dtype=np.float32)
         self.n = self.sobjs.shape[0]
         self.D, self.is_box, self.F, self.layout = D, int(bool(is_box)), F, layout
+        self.parts = parts
         cap = max(16, 2 * self.n + 16)
         nf = np.zeros(cap, dtype=np.uint32)
         nc = np.zeros(cap, dtype=np.uint32)
@@ -162,13 +170,16 @@ class Tree:
             self.sobjs = np.ascontiguousarray(self.sobjs[self.order])
         elif layout == "bvh":
             m = L.orc_bvh_topology(self.n, F, cap, ptr(nf), ptr(nc), ptr(nl), ptr(lv))
+        elif layout == "partitioned":
+            m = L.orc_partitioned_topology(self.n, F, parts, cap, ptr(nf), ptr(nc), ptr(nl),
+                                           ptr(lv))
         else:
             m = L.orc_packed_topology(self.n, F, cap, ptr(nf), ptr(nc), ptr(nl), ptr(lv))
         assert m >= 0
         self.n_nodes = m
         self.first, self.count, self.leaf, self.level = nf[:m].copy(), nc[:m].copy(), nl[:m].copy(), lv[:m].copy()
         self.boxes = np.zeros((max(m, 1), 2 * D), dtype=np.float32)
-        if layout in ("bvh", "guttman"):
+        if layout in ("bvh", "guttman", "partitioned"):
             L.orc_tree_boxes(ptr(self.sobjs), D, self.is_box, 0 if layout == "guttman" else F, m,
                              ptr(self.first), ptr(self.count), ptr(self.leaf), ptr(self.boxes))
         else:
@@ -177,7 +188,8 @@ class Tree:
 
     def _args(self):
         return (ptr(self.sobjs), self.D, self.is_box, self.n_nodes, ptr(self.first),
-                ptr(self.count), ptr(self.leaf), ptr(self.boxes))
+                ptr(self.count), ptr(self.leaf), ptr(self.boxes),
+                getattr(self, "parts", 1) if getattr(self, "layout", "") == "partitioned" else 1)
 
     def search_dfs(self, queries, want_ids=False):
         q = np.ascontiguousarray(queries, dtype=np.float32)
@@ -186,14 +198,14 @@ class Tree:
         hashes = np.zeros(nq, dtype=np.uint64)
         ie = np.zeros(nq, dtype=np.uint64)
         le = np.zeros(nq, dtype=np.uint64)
-        lib().orc_search_dfs(*self._args(), ptr(q), 0, nq, ptr(counts), ptr(hashes),
+        lib().orc_search_dfs_roots(*self._args(), ptr(q), 0, nq, ptr(counts), ptr(hashes),
                              None, None, ptr(ie), ptr(le))
         res = {"counts": counts, "hashes": hashes, "int_entries": ie, "leaf_entries": le}
         if want_ids:
             offsets = np.zeros(nq + 1, dtype=np.uint64)
             offsets[1:] = np.cumsum(counts.astype(np.uint64))
             ids = np.zeros(max(int(offsets[-1]), 1), dtype=np.uint32)
-            lib().orc_search_dfs(*self._args(), ptr(q), 0, nq, None, None,
+            lib().orc_search_dfs_roots(*self._args(), ptr(q), 0, nq, None, None,
                                  ptr(offsets), ptr(ids), None, None)
             res["offsets"] = offsets
             res["ids"] = ids[: int(offsets[-1])]
@@ -204,7 +216,7 @@ class Tree:
         nq = q.shape[0]
         counts = np.zeros(nq, dtype=np.uint32)
         hashes = np.zeros(nq, dtype=np.uint64)
-        lib().orc_search_dfs_mt(*self._args(), ptr(q), nq, nthreads, ptr(counts), ptr(hashes))
+        lib().orc_search_dfs_mt_roots(*self._args(), ptr(q), nq, nthreads, ptr(counts), ptr(hashes))
         return counts, hashes
 
     def search_ids_mt(self, queries, nthreads):
@@ -225,7 +237,7 @@ class Tree:
 
         def run(t):
             if cuts[t] < cuts[t + 1]:
-                L.orc_search_dfs(*self._args(), ptr(q), cuts[t], cuts[t + 1], None, None,
+                L.orc_search_dfs_roots(*self._args(), ptr(q), cuts[t], cuts[t + 1], None, None,
                                  ptr(offsets), ptr(ids), None, None)
 
         with ThreadPoolExecutor(max_workers=nthreads) as ex:
@@ -242,6 +254,19 @@ class Tree:
         lib().orc_search_mprs_mt(ptr(self.sobjs), self.n, self.D, self.is_box, self.F,
                                  ptr(self.boxes), ptr(q), nq, nthreads, ptr(counts), ptr(hashes))
         return counts, hashes
+
+    def search_mprs_part(self, queries):
+        """The plain search over every partition (per-query sums of counts,
+        hashes, root and node visits)."""
+        assert self.layout == "partitioned"
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        counts = np.zeros(max(nq, 1), dtype=np.uint32)
+        hashes = np.zeros(max(nq, 1), dtype=np.uint64)
+        visits = np.zeros(2 * max(nq, 1), dtype=np.uint32)
+        lib().orc_search_mprs_part(ptr(self.sobjs), self.n, self.D, self.is_box, self.F,
+                                   self.parts, ptr(q), nq, ptr(counts), ptr(hashes), ptr(visits))
+        return {"counts": counts[:nq], "hashes": hashes[:nq], "visits": visits[: 2 * nq].reshape(nq, 2)}
 
     def search_mprs(self, queries, want_ids=False):
         assert self.layout == "packed"
@@ -351,13 +376,20 @@ def build_kmeans(objs, D, is_box, F, key_mode, K, iters, init_idx, key_src=SRC_L
     return t
 
 
-def build_packed(objs, D, is_box, F, key_mode, key_src=SRC_LO, layout="packed"):
-    """Full oracle build: keys -> stable sort -> packed, BVH or Guttman tree."""
+def partition_range(n, P, p):
+    b, c = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().orc_partition_range(n, P, p, ctypes.byref(b), ctypes.byref(c))
+    return b.value, c.value
+
+
+def build_packed(objs, D, is_box, F, key_mode, key_src=SRC_LO, layout="packed", parts=1):
+    """Full oracle build: keys -> stable sort -> packed, BVH, Guttman or
+    partitioned MPHR tree."""
     objs = np.ascontiguousarray(objs, dtype=np.float32)
     k, bad = keys(objs, D, is_box, key_mode, key_src)
     perm = sort_stable(k)
     sobjs = objs[perm]
-    t = Tree(sobjs, D, is_box, F, layout=layout)
+    t = Tree(sobjs, D, is_box, F, layout=layout, parts=parts)
     if layout == "guttman":
         perm = perm[t.order]
     t.keys = k[perm]

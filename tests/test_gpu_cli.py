@@ -42,10 +42,11 @@ def files(tmp_path_factory):
     ("r", "guttman", []), ("pr", "packed", []), ("b", "bvh", []),
     ("h", "packed", ["-p", "4", "-c", "400"]), ("hb", "bvh", ["-l", "2"]),
     ("hr", "guttman", ["-l", "1"]), ("rls", "guttman-input", ["-p", "4"]),
-    ("m", "packed", ["--contain"])])
+    ("m", "packed", ["--contain"]), ("m", "packed", ["-t", "9"]),
+    ("m", "packed", ["-t", "9", "--mode", "plain"])])
 def test_cli_search_matches_oracle(gpu, O, files, itype, layout, extra):
     d, data, queries, pts, q = files
-    out = str(d / ("out_%s_%d.txt" % (itype, len(extra))))
+    out = str(d / ("out_%s_%s.txt" % (itype, "_".join(x.strip("-") for x in extra))))
     log = run("-i", itype, "--data", data, "--queries", queries, "--dims", "3", "--fanout", "64",
               "--out", out, "-r", "2", *extra)
     assert "Hit :" in log
@@ -56,6 +57,14 @@ def test_cli_search_matches_oracle(gpu, O, files, itype, layout, extra):
     exp = t.search_dfs(q)
     c, h = read_out(out)
     assert np.array_equal(c, exp["counts"]) and np.array_equal(h, exp["hashes"])
+
+
+def test_cli_partitions_need_mphr(gpu, files):
+    """-t with an index type other than MPHR is an error, not ignored."""
+    d, data, queries, pts, q = files
+    r = subprocess.run([CLI, "-i", "b", "-t", "4", "--data", data, "--queries", queries,
+                        "--dims", "3"], capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "invalid argument" in (r.stdout + r.stderr)
 
 
 def test_cli_index_file_and_autotune(gpu, O, files):

@@ -434,3 +434,57 @@ def test_id_hash_collides_on_equal_moment_sets(O):
         want = (x * x + O.HASH_K * x) % (1 << 64)
         assert O.lib().orc_id_hash(x) == want
         assert int(O.id_hash(np.array([x], np.uint32))[0]) == want
+
+
+@pytest.mark.parametrize("n,F,P", [(1000, 16, 3), (5000, 64, 7), (60000, 64, 208), (300, 32, 300),
+                                   (4097, 64, 2), (129, 128, 1)])
+def test_partitioned_mphr(O, n, F, P):
+    """MPHR data partitions: the reference's split (chunk = n/P, the first
+    partition also takes n%P), and every partition's subtree equal to the
+    packed tree of its rank range built on its own (orc_packed_topology +
+    orc_pack), found by a BFS from root p of the depth-major numbering; then
+    DFS over the whole tree and the per-partition MPRS against brute force."""
+    sizes = [O.partition_range(n, P, p) for p in range(P)]
+    assert sizes[0] == (0, n // P + n % P)
+    assert all(c == n // P for _, c in sizes[1:])
+    assert all(sizes[p + 1][0] == sizes[p][0] + sizes[p][1] for p in range(P - 1))
+    assert sizes[-1][0] + sizes[-1][1] == n
+    objs = rand_points(n, 2, seed=n + P)
+    k, _ = O.keys(objs, 2, False, O.KEY_REF)
+    sobjs = np.ascontiguousarray(objs[O.sort_stable(k)])
+    t = O.Tree(sobjs, 2, False, F, layout="partitioned", parts=P)
+    depth_of = np.zeros(t.n_nodes, np.int64)
+    for p, (b, c) in enumerate(sizes):
+        sub = O.Tree(sobjs[b:b + c], 2, False, F)  # the partition alone, packed
+        bfs, i = [p], 0
+        while i < len(bfs):
+            v = bfs[i]
+            i += 1
+            if not t.leaf[v]:
+                bfs.extend(range(int(t.first[v]), int(t.first[v]) + int(t.count[v])))
+        bfs = np.array(bfs)
+        assert len(bfs) == sub.n_nodes
+        assert np.array_equal(t.count[bfs], sub.count) and np.array_equal(t.leaf[bfs], sub.leaf)
+        leaves = sub.leaf.astype(bool)
+        assert np.array_equal(t.first[bfs][leaves], sub.first[leaves] + b)
+        assert np.array_equal(t.boxes[bfs].view(np.uint32), sub.boxes[: sub.n_nodes].view(np.uint32))
+        depth_of[bfs] = sub.level
+    # depth-major numbering: depths never decrease with the node id, roots are 0..P-1
+    assert (np.diff(depth_of) >= 0).all() and (depth_of[:P] == 0).all()
+    q = rand_queries(120, 2, seed=P, objs=sobjs)
+    brute = O.search_brute(sobjs, 2, False, q, want_ids=True)
+    dfs = t.search_dfs(q, want_ids=True)
+    for key in ("counts", "hashes", "offsets", "ids"):
+        assert np.array_equal(dfs[key], brute[key]), key
+    mp = t.search_mprs_part(q)
+    assert np.array_equal(mp["counts"], brute["counts"])
+    assert np.array_equal(mp["hashes"], brute["hashes"])
+    assert (mp["visits"][:, 0] >= P).all()  # every partition's root is visited at least once
+    if P == 1:
+        ref = O.Tree(sobjs, 2, False, F).search_mprs(q)
+        assert np.array_equal(mp["visits"], ref["visits"])
+
+
+def test_partitioned_refuses_empty_partitions(O):
+    nf = np.zeros(64, np.uint32)
+    assert O.lib().orc_partitioned_topology(5, 4, 6, 64, O.ptr(nf), O.ptr(nf), O.ptr(nf), O.ptr(nf)) == -1
