@@ -56,6 +56,7 @@ enum { QCP_SRC_LO = 0, QCP_SRC_CENTER = 1 };    /* key from lower corner or box 
 enum { QCP_OBJ_POINTS = 0, QCP_OBJ_BOXES = 1 };
 enum { QCP_BUILD_GRAPH = 1 };                   /* qcp_build_cfg.flags                  */
 enum { QCP_ORDER_HILBERT = 0, QCP_ORDER_KMEANS_HILBERT = 1 };
+#define QCP_MAX_CLUSTERS (1u << 20) /* k-means order: largest cluster count */
 
 enum { QCP_SEARCH_PERSISTENT = 0, QCP_SEARCH_PLAIN = 1, QCP_SEARCH_HYBRID = 2 };
 enum { QCP_OUT_COUNTS = 1, QCP_OUT_HASH = 2, QCP_OUT_IDS = 4, QCP_OUT_VISITS = 8 };
@@ -73,8 +74,13 @@ typedef struct {
                          clusters ranked by their centres' keys, then Hilbert order
                          within a cluster (KmeansMapper, src/mapper/kmeans_mapper.cpp:
                          201-259; not with QCP_BUILD_GRAPH)                        */
-  uint32_t clusters;  /* k-means order: clusters (0 = 64, at most 4096)             */
-  uint32_t kmeans_iters; /* k-means order: Lloyd iterations at most (0 = 20)         */
+  uint32_t clusters;  /* k-means order: clusters K (0 = 20000, the reference's
+                         DFN_NumberOfClusters; at most QCP_MAX_CLUSTERS; K > n
+                         means n)                                                  */
+  uint32_t kmeans_iters; /* k-means order: Lloyd iterations at most (0 = 1001: the
+                            reference's loop runs while the changed fraction
+                            (float)changes / (float)n exceeds 1e-5 and loop++ < 1000,
+                            src/mapper/kmeans_mapper.cpp:170)                      */
   uint32_t partitions;   /* MPHR data partitions P (0 or 1 = one tree; at most 65536 and
                             at most n): the ranks split as MPHR::Build does (chunk = n/P,
                             the first partition also taking n % P), one packed subtree
@@ -94,6 +100,8 @@ typedef struct {
                                  insertion, wall ms), total */
   uint32_t partitions;        /* MPHR data partitions (1 = one tree); partition p's root
                                  is node p (nodes are numbered depth-major)  */
+  uint32_t kmeans_iterations; /* Lloyd iterations of a k-means build (0 otherwise, and
+                                 for an index loaded from a file)            */
 } qcp_index_info;
 
 typedef struct {

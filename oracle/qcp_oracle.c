@@ -12,6 +12,7 @@
 
 #include <math.h>
 #include <pthread.h>
+#include <unistd.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -1101,21 +1102,55 @@ void orc_search_mprs_mt(const float *sobjs, uint64_t n, unsigned D, int is_box, 
  * 15-259) with the defined variant of this build: K seeds given as object
  * indexes; each iteration assigns every key point (lower corner, or box centre
  * (lo+hi)*0.5f) to the nearest centre by FP32 squared distance summed over
- * d = 0..D-1, ties to the lower cluster, stops when no assignment changed
- * (after the first), else moves each non-empty centre to the mean of its key
- * points: FP64 sums in index order within chunks of 65536 objects, chunk sums
- * added in chunk order, rounded to FP32.  Clusters are ranked by (Hilbert key
- * of the centre, cluster number) and the sort key is
+ * d = 0..D-1, ties to the lower cluster, counting the objects whose cluster
+ * changed; then moves each non-empty centre to the mean of its key points
+ * (FP64 sums in index order within chunks of 65536 objects, chunk sums added in
+ * chunk order, rounded to FP32); and repeats while the reference's condition
+ * holds (kmeans_mapper.cpp:170, kmeans_macro.h:4-5): delta = (float)changes /
+ * (float)n > 1e-5 and loop++ < max_loop, loop starting at 0 (so at most
+ * max_loop + 1 iterations; the reference's max_loop is 1000).  Clusters are
+ * ranked by (Hilbert key of the centre, cluster number) and the sort key is
  * rank << (key bits - s) | key >> s, s = max(0, key bits + kbits - 64),
- * kbits = ceil(log2 K).  keys[n] receives the sort keys, cid[n] the clusters. */
+ * kbits = ceil(log2 K).  keys[n] receives the sort keys, cid[n] the clusters,
+ * *iterations (optional) the iterations run. */
+typedef struct {
+  const float *pts, *cen;
+  uint64_t i0, i1;
+  unsigned D, K;
+  uint32_t *cid;
+  uint64_t changes;
+} km_arg;
+
+static void *km_assign(void *p) {
+  km_arg *a = (km_arg *)p;
+  const unsigned D = a->D;
+  for (uint64_t i = a->i0; i < a->i1; ++i) {
+    float best = 0.0f;
+    uint32_t bk = 0;
+    for (unsigned k = 0; k < a->K; ++k) {
+      float dd = 0.0f;
+      for (unsigned d = 0; d < D; ++d) {
+        float const t = a->pts[i * D + d] - a->cen[k * D + d];
+        dd = dd + t * t;
+      }
+      if (k == 0 || dd < best) { best = dd; bk = k; }
+    }
+    if (a->cid[i] != bk) { a->cid[i] = bk; a->changes++; }
+  }
+  return NULL;
+}
+
 void orc_kmeans_keys(const float *objs, uint64_t n, unsigned D, int is_box, int key_mode,
-                     int key_src, unsigned K, unsigned iters, const uint32_t *init_idx,
-                     uint64_t *keys, uint32_t *cid) {
+                     int key_src, unsigned K, unsigned max_loop, const uint32_t *init_idx,
+                     uint64_t *keys, uint32_t *cid, unsigned *iterations) {
   unsigned const E = is_box ? 2 * D : D;
   unsigned const nb = key_mode == ORC_KEY_REF ? orc_ref_bits(D) : 16u;
   const uint64_t CH = 65536;
   float *pts = (float *)malloc(sizeof(float) * D * (n ? n : 1));
   float *cen = (float *)malloc(sizeof(float) * D * (K ? K : 1));
+  double *part = (double *)malloc(sizeof(double) * D * (K ? K : 1));
+  double *sum = (double *)malloc(sizeof(double) * D * (K ? K : 1));
+  uint64_t *total = (uint64_t *)malloc(sizeof(uint64_t) * (K ? K : 1));
   orc_keys(objs, n, D, is_box, key_mode, key_src, keys);
   for (uint64_t i = 0; i < n; ++i)
     for (unsigned d = 0; d < D; ++d) {
@@ -1126,40 +1161,56 @@ void orc_kmeans_keys(const float *objs, uint64_t n, unsigned D, int is_box, int 
   for (unsigned k = 0; k < K; ++k)
     for (unsigned d = 0; d < D; ++d) cen[k * D + d] = pts[(uint64_t)init_idx[k] * D + d];
   for (uint64_t i = 0; i < n; ++i) cid[i] = 0xFFFFFFFFu;
-  for (unsigned it = 0; it < iters; ++it) {
+  long nt = sysconf(_SC_NPROCESSORS_ONLN);
+  if (nt < 1) nt = 1;
+  if (nt > 64) nt = 64;
+  unsigned loop = 0, its = 0;
+  float delta;
+  do {
+    /* assignment: static object ranges over threads (each object's result
+       does not depend on the split) */
+    pthread_t th[64];
+    km_arg ka[64];
     uint64_t changes = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-      float best = 0.0f;
-      uint32_t bk = 0;
-      for (unsigned k = 0; k < K; ++k) {
-        float dd = 0.0f;
-        for (unsigned d = 0; d < D; ++d) {
-          float const t = pts[i * D + d] - cen[k * D + d];
-          dd = dd + t * t;
-        }
-        if (k == 0 || dd < best) { best = dd; bk = k; }
-      }
-      if (cid[i] != bk) { cid[i] = bk; ++changes; }
+    for (long t = 0; t < nt; ++t) {
+      km_arg x = {pts, cen, n * (uint64_t)t / (uint64_t)nt, n * (uint64_t)(t + 1) / (uint64_t)nt,
+                  D, K, cid, 0};
+      ka[t] = x;
+      pthread_create(&th[t], NULL, km_assign, &ka[t]);
     }
-    if (it > 0 && changes == 0) break;
-    uint64_t nch = (n + CH - 1) / CH;
+    for (long t = 0; t < nt; ++t) {
+      pthread_join(th[t], NULL);
+      changes += ka[t].changes;
+    }
+    ++its;
+    /* update: chunk partial sums in index order, added in chunk order */
     for (unsigned k = 0; k < K; ++k) {
-      double sum[3] = {0, 0, 0};
-      uint64_t total = 0;
-      for (uint64_t c = 0; c < nch; ++c) {
-        double part[3] = {0, 0, 0};
-        uint64_t e = (c + 1) * CH < n ? (c + 1) * CH : n;
-        for (uint64_t i = c * CH; i < e; ++i) {
-          if (cid[i] != k) continue;
-          ++total;
-          for (unsigned d = 0; d < D; ++d) part[d] += (double)pts[i * D + d];
-        }
-        for (unsigned d = 0; d < D; ++d) sum[d] += part[d];
-      }
-      if (total)
-        for (unsigned d = 0; d < D; ++d) cen[k * D + d] = (float)(sum[d] / (double)total);
+      total[k] = 0;
+      for (unsigned d = 0; d < D; ++d) sum[k * D + d] = 0.0;
     }
-  }
+    for (uint64_t c0 = 0; c0 < n; c0 += CH) {
+      const uint64_t e = c0 + CH < n ? c0 + CH : n;
+      for (unsigned k = 0; k < K; ++k)
+        for (unsigned d = 0; d < D; ++d) part[k * D + d] = 0.0;
+      for (uint64_t i = c0; i < e; ++i) {
+        const uint32_t k = cid[i];
+        total[k]++;
+        for (unsigned d = 0; d < D; ++d) part[k * D + d] += (double)pts[i * D + d];
+      }
+      for (unsigned k = 0; k < K; ++k)
+        for (unsigned d = 0; d < D; ++d) sum[k * D + d] += part[k * D + d];
+    }
+    for (unsigned k = 0; k < K; ++k)
+      if (total[k])
+        for (unsigned d = 0; d < D; ++d)
+          cen[k * D + d] = (float)(sum[k * D + d] / (double)total[k]);
+    delta = (float)changes;
+    delta /= (float)n;
+  } while ((double)delta > 0.00001 && loop++ < max_loop);
+  if (iterations) *iterations = its;
+  free(part);
+  free(sum);
+  free(total);
   /* cluster ranks */
   uint64_t *ck = (uint64_t *)malloc(sizeof(uint64_t) * (K ? K : 1));
   uint32_t *ord = (uint32_t *)malloc(sizeof(uint32_t) * (K ? K : 1));

@@ -152,11 +152,14 @@ void     orc_search_mprs_mt(const float *sobjs, uint64_t n, unsigned D, int is_b
                             unsigned nthreads, uint32_t *counts, uint64_t *hashes);
 /* k-means + Hilbert sort keys (this build's defined variant of the reference's
  * KmeansMapper, src/mapper/kmeans_mapper.cpp:15-259): K clusters seeded at
- * init_idx[K], at most iters Lloyd iterations; keys[n] = composite sort keys,
- * cid[n] = final clusters.  Follow with orc_sort_stable. */
+ * init_idx[K]; Lloyd iterations (assignment, then centre update) repeated
+ * while (float)changes / (float)n > 1e-5 and loop++ < max_loop (the
+ * reference's rule, kmeans_mapper.cpp:170, with its max_loop 1000); keys[n] =
+ * composite sort keys, cid[n] = final clusters, *iterations (optional) = the
+ * iterations run.  Follow with orc_sort_stable. */
 void     orc_kmeans_keys(const float *objs, uint64_t n, unsigned D, int is_box, int key_mode,
-                         int key_src, unsigned K, unsigned iters, const uint32_t *init_idx,
-                         uint64_t *keys, uint32_t *cid);
+                         int key_src, unsigned K, unsigned max_loop, const uint32_t *init_idx,
+                         uint64_t *keys, uint32_t *cid, unsigned *iterations);
 /* Guttman R-tree (src/tree/rtree.h; Tree::RTree_Top_Down and SetNodeIndex,
  * src/tree/tree.cpp:126-185,244-256): inserts objs[i] in index order (node
  * capacity F, minimum fill F/2, quadratic split, spherical-volume metric in

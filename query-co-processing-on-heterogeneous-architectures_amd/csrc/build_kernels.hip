@@ -971,41 +971,96 @@ hipError_t launch_keys_sorted(const float* obj, uint64_t stride, uint64_t n, uns
 // k-means + Hilbert order (the assignment step of KmeansMapper::cuda_kmeans,
 // find_nearest_cluster, src/mapper/kmeans_mapper.cpp:100-126,280-370): each
 // object's key point goes to the nearest centre by FP32 squared distance
-// summed over d = 0..D-1 (no contraction), ties to the lower cluster.
-// Centres sit in LDS; `changes` counts objects whose cluster changed.
-template <int D, bool BOX, bool CENTER>
+// summed over d = 0..D-1 (no contraction), ties to the lower cluster.  Any K:
+// the centres pass through LDS in tiles of kAssignTile (centre k's distance is
+// computed exactly as in a single sequential scan, and the tiles are visited
+// in cluster order, so the strict `<` keeps the first minimum); every thread
+// holds R objects in registers so that one broadcast LDS read of a centre
+// serves R distance evaluations.  `changes` counts objects whose cluster
+// changed.
+constexpr uint32_t kAssignTile = 2048;
+
+template <int D>
+struct CentreVec;
+template <>
+struct CentreVec<2> {
+  using T = float2;
+};
+template <>
+struct CentreVec<3> {
+  using T = float4;
+};
+
+template <int D, bool BOX, bool CENTER, int R>
 __global__ __launch_bounds__(256) void k_assign(const float* __restrict__ objs, uint64_t n,
                                                 const float* __restrict__ centers, uint32_t K,
                                                 uint32_t* __restrict__ cid, uint32_t* changes) {
   constexpr int E = BOX ? 2 * D : D;
-  extern __shared__ float qcp_centres[];
-  for (uint32_t t = threadIdx.x; t < K * D; t += blockDim.x) qcp_centres[t] = centers[t];
-  __syncthreads();
-  uint32_t changed = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    float p[D];
+  using V = typename CentreVec<D>::T;
+  __shared__ V tile[kAssignTile];
+  const uint64_t base = (uint64_t)blockIdx.x * (256 * R) + threadIdx.x;
+  float p[R][D];
+  float best[R];
+  uint32_t bk[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint64_t i = base + (uint64_t)r * 256;
+    const uint64_t j = i < n ? i : n - 1;  // tail lanes evaluate a valid object, store nothing
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      p[d] = objs[i * E + d];
-      if (CENTER && BOX) p[d] = (p[d] + objs[i * E + D + d]) * 0.5f;
+      p[r][d] = objs[j * E + d];
+      if (CENTER && BOX) p[r][d] = (p[r][d] + objs[j * E + D + d]) * 0.5f;
     }
-    float best = 0.0f;
-    uint32_t bk = 0;
-    for (uint32_t k = 0; k < K; ++k) {
-      float dd = 0.0f;
+    // cluster 0 starts the scan (the sequential rule's `k == 0` case)
+    float dd = 0.0f;
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const float t = p[d] - qcp_centres[k * D + d];
-        dd = dd + t * t;
+    for (int d = 0; d < D; ++d) {
+      const float t = p[r][d] - centers[d];
+      dd = dd + t * t;
+    }
+    best[r] = dd;
+    bk[r] = 0;
+  }
+  for (uint32_t k0 = 0; k0 < K; k0 += kAssignTile) {
+    const uint32_t m = K - k0 < kAssignTile ? K - k0 : kAssignTile;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < m; t += 256) {
+      V c;
+      c.x = centers[(uint64_t)(k0 + t) * D];
+      c.y = centers[(uint64_t)(k0 + t) * D + 1];
+      if constexpr (D == 3) {
+        c.z = centers[(uint64_t)(k0 + t) * D + 2];
+        c.w = 0.0f;
       }
-      if (k == 0 || dd < best) {
-        best = dd;
-        bk = k;
+      tile[t] = c;
+    }
+    __syncthreads();
+    for (uint32_t t = k0 == 0 ? 1u : 0u; t < m; ++t) {
+      const V c = tile[t];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float t0 = p[r][0] - c.x;
+        const float t1 = p[r][1] - c.y;
+        float dd = 0.0f;
+        dd = dd + t0 * t0;
+        dd = dd + t1 * t1;
+        if constexpr (D == 3) {
+          const float t2 = p[r][2] - c.z;
+          dd = dd + t2 * t2;
+        }
+        if (dd < best[r]) {
+          best[r] = dd;
+          bk[r] = k0 + t;
+        }
       }
     }
-    if (cid[i] != bk) {
-      cid[i] = bk;
+  }
+  uint32_t changed = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint64_t i = base + (uint64_t)r * 256;
+    if (i < n && cid[i] != bk[r]) {
+      cid[i] = bk[r];
       ++changed;
     }
   }
@@ -1028,10 +1083,20 @@ __global__ void k_compose(uint64_t* keys, const uint32_t* __restrict__ cid,
 hipError_t launch_assign(const float* objs, uint64_t n, unsigned D, bool box, bool center,
                          const float* centers, uint32_t K, uint32_t* cid, uint32_t* changes,
                          hipStream_t s) {
-  const unsigned g = grid_for(n, 256, 8192);
-  const size_t lds = (size_t)K * D * sizeof(float);
-#define QCP_ASSIGN(DD, BB, CC) \
-  k_assign<DD, BB, CC><<<g, 256, lds, s>>>(objs, n, centers, K, cid, changes)
+  if (n == 0 || K == 0) return hipSuccess;
+  // four objects a thread once that still gives every CU two workgroups
+  const bool wide = n >= 4ull * 256 * 512;
+  const unsigned R = wide ? 4u : 1u;
+  const uint64_t g64 = (n + 256ull * R - 1) / (256ull * R);
+  if (g64 > 0x7FFFFFFFull) return hipErrorInvalidValue;
+  const unsigned g = (unsigned)g64;
+#define QCP_ASSIGN(DD, BB, CC)                                                   \
+  do {                                                                         \
+    if (wide)                                                                  \
+      k_assign<DD, BB, CC, 4><<<g, 256, 0, s>>>(objs, n, centers, K, cid, changes); \
+    else                                                                       \
+      k_assign<DD, BB, CC, 1><<<g, 256, 0, s>>>(objs, n, centers, K, cid, changes); \
+  } while (0)
   if (D == 2) {
     if (box) { if (center) QCP_ASSIGN(2, true, true); else QCP_ASSIGN(2, true, false); }
     else QCP_ASSIGN(2, false, false);

@@ -9,10 +9,14 @@
 //  - seeds: K distinct objects, index (w * N) >> 32 for the Philox words w of
 //    (seed 1234, stream 5) in order, repeats skipped;
 //  - each iteration assigns every key point to its nearest centre (GPU,
-//    k_assign) and, unless no assignment changed, moves each non-empty
-//    cluster's centre to the mean of its key points: FP64 sums in index order
-//    within chunks of 65536 objects, the chunk sums added in chunk order, then
-//    rounded to FP32 (so any number of host threads gives the same centres);
+//    k_assign) and then moves each non-empty cluster's centre to the mean of
+//    its key points: FP64 sums in index order within chunks of 65536 objects,
+//    the chunk sums added in chunk order, then rounded to FP32 (so any number
+//    of host threads gives the same centres);
+//  - the loop is the reference's (kmeans_mapper.cpp:170, kmeans_macro.h:4-5):
+//    delta = (float)changes / (float)n, repeat while delta > 1e-5 and
+//    loop++ < max_loop, loop from 0, so at most max_loop + 1 iterations; the
+//    reference's max_loop is 1000 and its K 20000, the defaults here;
 //  - clusters are ranked by (Hilbert key of the centre under the index's
 //    quantiser, cluster number);
 //  - an object's sort key is rank << (key bits - s) | key >> s with
@@ -24,6 +28,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -38,6 +46,9 @@
 namespace {
 
 constexpr uint64_t kChunk = 1 << 16;
+constexpr uint32_t kDefaultClusters = 20000;  // DFN_NumberOfClusters, kmeans_macro.h
+constexpr uint32_t kDefaultIters = 1001;      // loop++ < 1000: kMeansLoopIteration + 1
+constexpr double kDeltaMin = 0.00001;         // DFN_KmeansThreashhold
 constexpr uint64_t kSeed = 1234;
 constexpr uint32_t kSeedStream = 5;
 
@@ -50,53 +61,134 @@ struct DevBuf {
   bool alloc(uint64_t n) { return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)) == hipSuccess; }
 };
 
+// A fixed set of host threads that run fn(i) for i in [0, count) together with
+// the calling thread; run() returns when every index is done.
+class Pool {
+ public:
+  explicit Pool(unsigned threads) {
+    for (unsigned t = 1; t < threads; ++t) th_.emplace_back([this] { loop(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  void run(uint64_t count, const std::function<void(uint64_t)>& fn) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      fn_ = &fn;
+      count_ = count;
+      next_.store(0);
+      busy_ = th_.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> l(mu_);
+    done_.wait(l, [this] { return busy_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void work() {
+    for (;;) {
+      const uint64_t i = next_.fetch_add(1);
+      if (i >= count_) return;
+      (*fn_)(i);
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> l(mu_);
+        cv_.wait(l, [&] { return gen_ != seen; });
+        seen = gen_;
+        if (stop_) return;
+      }
+      work();
+      std::lock_guard<std::mutex> g(mu_);
+      if (--busy_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(uint64_t)>* fn_ = nullptr;
+  uint64_t count_ = 0, gen_ = 0;
+  size_t busy_ = 0;
+  bool stop_ = false;
+  std::atomic<uint64_t> next_{0};
+};
+
+// Centre update with the chunk rule above.  Chunks are summed in windows of
+// G at a time (partials G x K x D, bounded to about 256 MiB); within a window
+// each chunk's partials are formed in parallel, then added to the running sums
+// in chunk order, clusters in parallel.  Only clusters present in a chunk are
+// added (adding a chunk's +0.0 partial would leave the sum unchanged).
+void update_centres(Pool& pool, unsigned threads, const float* pts, uint64_t n, uint32_t D,
+                    const uint32_t* cid, uint32_t K, float* centers) {
+  const uint64_t nchunks = (n + kChunk - 1) / kChunk;
+  const uint64_t per_chunk = (uint64_t)K * (8ull * D + 4);
+  const uint64_t G = std::max<uint64_t>(
+      1, std::min<uint64_t>({nchunks, 2ull * threads, (256ull << 20) / per_chunk}));
+  std::vector<double> part(G * K * D, 0.0), sum((size_t)K * D, 0.0);
+  std::vector<uint32_t> cnt(G * K, 0);
+  std::vector<uint64_t> total(K, 0);
+  std::vector<std::vector<uint32_t>> touched(G);
+  const uint64_t kblk = 1024;
+  for (uint64_t w0 = 0; w0 < nchunks; w0 += G) {
+    const uint64_t w = std::min(G, nchunks - w0);
+    pool.run(w, [&](uint64_t g) {
+      double* ps = part.data() + g * K * D;
+      uint32_t* cs = cnt.data() + g * K;
+      std::vector<uint32_t>& tl = touched[g];
+      const uint64_t b = (w0 + g) * kChunk, e = std::min(n, b + kChunk);
+      for (uint64_t i = b; i < e; ++i) {
+        const uint32_t k = cid[i];
+        if (cs[k]++ == 0) tl.push_back(k);
+        for (uint32_t d = 0; d < D; ++d) ps[k * D + d] += (double)pts[i * D + d];
+      }
+    });
+    pool.run((K + kblk - 1) / kblk, [&](uint64_t kb) {
+      const uint64_t k1 = std::min<uint64_t>(K, (kb + 1) * kblk);
+      for (uint64_t g = 0; g < w; ++g)
+        for (uint64_t k = kb * kblk; k < k1; ++k)
+          if (cnt[g * K + k]) {
+            total[k] += cnt[g * K + k];
+            for (uint32_t d = 0; d < D; ++d) sum[k * D + d] += part[(g * K + k) * D + d];
+          }
+    });
+    pool.run(w, [&](uint64_t g) {
+      for (const uint32_t k : touched[g]) {
+        cnt[g * K + k] = 0;
+        for (uint32_t d = 0; d < D; ++d) part[(g * K + k) * D + d] = 0.0;
+      }
+      touched[g].clear();
+    });
+  }
+  for (uint32_t k = 0; k < K; ++k)
+    if (total[k])
+      for (uint32_t d = 0; d < D; ++d) centers[k * D + d] = (float)(sum[k * D + d] / (double)total[k]);
+}
+
 }  // namespace
 
 uint32_t qcp::kmeans_seed_index(uint64_t word, uint64_t n) {
   return (uint32_t)((word * n) >> 32);
 }
 
-void qcp::kmeans_update(const float* pts, uint64_t n, uint32_t D, const uint32_t* cid,
-                        uint32_t K, float* centers, unsigned threads) {
-  const uint64_t nchunks = (n + kChunk - 1) / kChunk;
-  std::vector<double> part(nchunks * K * D, 0.0);
-  std::vector<uint64_t> cnt(nchunks * K, 0);
-  auto work = [&](uint64_t c0, uint64_t c1) {
-    for (uint64_t c = c0; c < c1; ++c) {
-      double* ps = part.data() + c * K * D;
-      uint64_t* cs = cnt.data() + c * K;
-      const uint64_t e = std::min(n, (c + 1) * kChunk);
-      for (uint64_t i = c * kChunk; i < e; ++i) {
-        const uint32_t k = cid[i];
-        cs[k] += 1;
-        for (uint32_t d = 0; d < D; ++d) ps[k * D + d] += (double)pts[i * D + d];
-      }
-    }
-  };
-  threads = std::max(1u, std::min<unsigned>(threads, (unsigned)std::max<uint64_t>(1, nchunks)));
-  std::vector<std::thread> th;
-  for (unsigned t = 0; t < threads; ++t)
-    th.emplace_back(work, nchunks * t / threads, nchunks * (t + 1) / threads);
-  for (auto& x : th) x.join();
-  for (uint32_t k = 0; k < K; ++k) {
-    uint64_t total = 0;
-    double sum[3] = {0.0, 0.0, 0.0};
-    for (uint64_t c = 0; c < nchunks; ++c) {
-      total += cnt[c * K + k];
-      for (uint32_t d = 0; d < D; ++d) sum[d] += part[(c * K + k) * D + d];
-    }
-    if (total)
-      for (uint32_t d = 0; d < D; ++d) centers[k * D + d] = (float)(sum[d] / (double)total);
-  }
-}
-
 qcp_status qcp::kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bits) {
   const uint64_t n = ix->n;
   const uint32_t D = ix->D, E = ix->E;
   hipStream_t s = ix->stream;
-  uint32_t K = ix->cfg.clusters ? ix->cfg.clusters : 64;
+  uint32_t K = ix->cfg.clusters ? ix->cfg.clusters : kDefaultClusters;
   if ((uint64_t)K > n) K = (uint32_t)n;
-  const uint32_t iters = ix->cfg.kmeans_iters ? ix->cfg.kmeans_iters : 20;
+  const uint32_t max_iters = ix->cfg.kmeans_iters ? ix->cfg.kmeans_iters : kDefaultIters;
   const bool center = ix->box && ix->cfg.key_src == QCP_SRC_CENTER;
   // key points on the host (lower corner or centre, the Hilbert key's source)
   std::vector<float> objs((size_t)n * E), pts((size_t)n * D);
@@ -135,7 +227,11 @@ qcp_status qcp::kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bi
   std::vector<uint32_t> cid(n);
   unsigned hw = std::thread::hardware_concurrency();
   hw = std::max(1u, std::min(hw, 16u));
-  for (uint32_t it = 0; it < iters; ++it) {
+  Pool pool(hw);
+  const uint32_t max_loop = max_iters - 1;
+  uint32_t loop = 0, its = 0;
+  float delta = 0.0f;
+  do {
     uint32_t changes = 0;
     HIP_TRY(hipMemcpyAsync(d_c.p, centers.data(), 4ull * K * D, hipMemcpyHostToDevice, s));
     HIP_TRY(launch_fill_u32(d_changes.p, 1, 0u, s));
@@ -143,9 +239,12 @@ qcp_status qcp::kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bi
     HIP_TRY(hipMemcpyAsync(&changes, d_changes.p, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(cid.data(), d_cid.p, 4ull * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    if (it > 0 && changes == 0) break;
-    kmeans_update(pts.data(), n, D, cid.data(), K, centers.data(), hw);
-  }
+    ++its;
+    update_centres(pool, hw, pts.data(), n, D, cid.data(), K, centers.data());
+    delta = (float)changes;
+    delta /= (float)n;
+  } while ((double)delta > kDeltaMin && loop++ < max_loop);
+  ix->kmeans_its = its;
   // cluster ranks by (centre key, cluster number)
   std::vector<std::pair<uint64_t, uint32_t>> ck(K);
   for (uint32_t k = 0; k < K; ++k) {

@@ -123,7 +123,7 @@ static qcp_status validate_cfg(const qcp_build_cfg* cfg) {
   if (cfg->key_mode > QCP_KEY_B16 || cfg->key_src > QCP_SRC_CENTER) return QCP_EINVAL;
   if (cfg->obj_kind > QCP_OBJ_BOXES) return QCP_EINVAL;
   if (cfg->key_src == QCP_SRC_CENTER && cfg->obj_kind != QCP_OBJ_BOXES) return QCP_EINVAL;
-  if (cfg->order > QCP_ORDER_KMEANS_HILBERT || cfg->clusters > 4096) return QCP_EINVAL;
+  if (cfg->order > QCP_ORDER_KMEANS_HILBERT || cfg->clusters > QCP_MAX_CLUSTERS) return QCP_EINVAL;
   if (cfg->order == QCP_ORDER_KMEANS_HILBERT && (cfg->flags & QCP_BUILD_GRAPH)) return QCP_EINVAL;
   if (cfg->partitions > 1 && (cfg->layout != QCP_LAYOUT_MPHR || cfg->partitions > 65536))
     return QCP_EINVAL;  // partitions are MPHR's (src/tree/mphr.cpp:60-108)
@@ -290,6 +290,7 @@ static void set_info_shape(qcp_index* ix) {
   for (int l = 0; l < ix->h && l < 32; ++l) in.level_count[l] = ix->level_count[l];
   in.tree_bytes = (uint64_t)ix->E * 4 * ix->n + (uint64_t)2 * ix->D * 4 * ix->n_nodes;
   in.partitions = ix->parts;
+  in.kmeans_iterations = ix->kmeans_its;
 }
 
 // Guttman layout: insert the objects on the host in their sorted order

@@ -28,7 +28,8 @@
 //   -s S      selectivity of --gen-queries (volume fraction)
 //   -u U      cluster type (GetClusterType, 431-446): h|hilbert (default),
 //             k|kmeans (k-means clusters ranked by their centres' Hilbert keys,
-//             --clusters K), o|original (the input order is the rank order)
+//             --clusters K, default 20000; --kmeans-iters I, default 1001),
+//             o|original (the input order is the rank order)
 //   -t T      MPHR data partitions (SetNumberOfPartition): T subtrees over
 //             contiguous rank ranges; MPHR indexes only (-i m)
 //   -y Y, -f  accepted; the data type has no counterpart (data from --data);
@@ -71,7 +72,7 @@ struct Opts {
   std::string ref_dump;
   uint32_t ref_degree = 192;
   std::string cluster = "h";
-  uint32_t clusters = 0;
+  uint32_t clusters = 0, kmeans_iters = 0;
   uint32_t partitions = 1;
 };
 
@@ -115,10 +116,11 @@ int fail(const char* what, qcp_status s) {
 
 void usage() {
   fprintf(stderr,
-          "usage: qcp_cli -i m|r|pr|b|h|hr|hb|rls -d N -q Q [-b B -p P -l L -c C -r R -e E -s S]\n"
-          "               --data FILE --queries FILE [--dims D --fanout F --boxes --center\n"
-          "               --key ref|b16 --mode persistent|plain|hybrid --index FILE\n"
-          "               --out FILE --contain --json]\n"
+          "usage: qcp_cli -i m|r|pr|b|h|hr|hb|rls -d N -q Q [-b B -p P -l L -c C -r R -e E -s S\n"
+          "               -u h|k|o -t T] --data FILE --queries FILE [--dims D --fanout F\n"
+          "               --boxes --center --key ref|b16 --mode persistent|plain|hybrid\n"
+          "               --index FILE --out FILE --contain --json --clusters K\n"
+          "               --kmeans-iters I]\n"
           "       qcp_cli --gen-data FILE -d N [--dims D --seed S]\n"
           "       qcp_cli --gen-queries FILE -q Q -s SELECTIVITY [--dims D --seed S]\n");
 }
@@ -138,6 +140,7 @@ int main(int argc, char** argv) {
       {"seed", required_argument, nullptr, 1014},     {"help", no_argument, nullptr, 'h'},
       {"ref-dump", required_argument, nullptr, 1015}, {"ref-degree", required_argument, nullptr, 1016},
       {"clusters", required_argument, nullptr, 1017},
+      {"kmeans-iters", required_argument, nullptr, 1018},
       {nullptr, 0, nullptr, 0}};
   // the reference's option string: every letter in both cases
   static const char* shortopts = "c:C:i:I:d:D:q:Q:b:B:p:P:s:S:l:L:r:R:e:E:t:T:y:Y:u:U:fFh";
@@ -177,6 +180,7 @@ int main(int argc, char** argv) {
         case 1015: o.ref_dump = optarg; break;
         case 1016: o.ref_degree = (uint32_t)atoi(optarg); break;
         case 1017: o.clusters = (uint32_t)atoi(optarg); break;
+        case 1018: o.kmeans_iters = (uint32_t)atoi(optarg); break;
         default: usage(); return 2;
       }
     }
@@ -275,7 +279,7 @@ int main(int argc, char** argv) {
     qcp_build_cfg cfg = {D, o.fanout, layout, o.b16 ? (uint32_t)QCP_KEY_B16 : (uint32_t)QCP_KEY_REF,
                          o.center ? (uint32_t)QCP_SRC_CENTER : (uint32_t)QCP_SRC_LO,
                          o.boxes ? (uint32_t)QCP_OBJ_BOXES : (uint32_t)QCP_OBJ_POINTS, 0, 0,
-                         QCP_ORDER_HILBERT, o.clusters, 0, o.partitions};
+                         QCP_ORDER_HILBERT, o.clusters, o.kmeans_iters, o.partitions};
     std::string ct = o.cluster;
     for (auto& ch : ct) ch = (char)tolower(ch);
     const bool original = input_order || ct == "o" || ct == "original";

@@ -31,6 +31,7 @@ struct qcp_index {
   uint32_t* d_nrhi = nullptr;
   uint32_t* d_nparent = nullptr; // explicit layouts: parent node (roots: themselves)
   uint32_t parts = 1;            // MPHR data partitions (roots are nodes 0 .. parts-1)
+  uint32_t kmeans_its = 0;       // Lloyd iterations of the last k-means build
   uint64_t* d_keys = nullptr;    // sorted keys [n]
   uint32_t* d_perm = nullptr;    // [n]
   // build scratch
@@ -112,10 +113,9 @@ SearchArgs index_search_args(const qcp_index* ix, const float* d_queries, uint64
 // Workgroups that fill the device with the persistent kernel of `out` mode.
 unsigned persistent_grid(const qcp_index* ix, int out, unsigned block);
 // k-means + Hilbert order (QCP_ORDER_KMEANS_HILBERT), in kmeans.cpp: replaces
-// the object keys in ix->d_keys by composite keys; *key_bits = significant bits
+// the object keys in ix->d_keys by composite keys; *key_bits = significant bits;
+// ix->kmeans_its = iterations run
 qcp_status kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bits);
-void kmeans_update(const float* pts, uint64_t n, uint32_t D, const uint32_t* cid, uint32_t K,
-                   float* centers, unsigned threads);
 uint32_t kmeans_seed_index(uint64_t word, uint64_t n);
 // the co-processing search (QCP_SEARCH_HYBRID), in hybrid.cpp
 qcp_status hybrid_run(qcp_batch* b, const qcp_search_cfg& cfg, qcp_search_stats* stats);

@@ -49,7 +49,7 @@ class IndexInfo(ctypes.Structure):
                 ("key_src", ctypes.c_uint32), ("key_bits", ctypes.c_uint32),
                 ("level_count", ctypes.c_uint64 * 32), ("tree_bytes", ctypes.c_uint64),
                 ("build_ms", ctypes.c_double), ("build_phase_ms", ctypes.c_double * 6),
-                ("partitions", ctypes.c_uint32)]
+                ("partitions", ctypes.c_uint32), ("kmeans_iterations", ctypes.c_uint32)]
 
 
 class SearchCfg(ctypes.Structure):

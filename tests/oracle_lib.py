@@ -77,7 +77,7 @@ def _declare(L):
     L.orc_build_mt.argtypes = [P, U64, UNS, INT, INT, INT, UNS, UNS, P, P, P, P, P]
     L.orc_build_mt.restype = INT
     L.orc_search_mprs_mt.argtypes = [P, U64, UNS, INT, UNS, P, P, U64, UNS, P, P]
-    L.orc_kmeans_keys.argtypes = [P, U64, UNS, INT, INT, INT, UNS, UNS, P, P, P]
+    L.orc_kmeans_keys.argtypes = [P, U64, UNS, INT, INT, INT, UNS, UNS, P, P, P, P]
     L.orc_guttman_build.argtypes = [P, U64, UNS, INT, UNS, I64, P, P, P, P, P]
     L.orc_guttman_build.restype = I64
     L.orc_partition_range.argtypes = [U64, UNS, UNS, P, P]
@@ -350,22 +350,29 @@ def kmeans_seeds(words, n, K):
     raise ValueError("not enough words")
 
 
-def kmeans_keys(objs, D, is_box, key_mode, K, iters, init_idx, key_src=SRC_LO):
+def kmeans_keys(objs, D, is_box, key_mode, K, max_loop, init_idx, key_src=SRC_LO,
+                want_iterations=False):
+    """Composite sort keys and clusters of the k-means order (max_loop: the
+    reference's loop limit, 1000 there)."""
     objs = np.ascontiguousarray(objs, dtype=np.float32)
     n = objs.shape[0]
     keys = np.zeros(max(n, 1), np.uint64)
     cid = np.zeros(max(n, 1), np.uint32)
+    its = ctypes.c_uint()
     init = np.ascontiguousarray(init_idx, np.uint32)
-    lib().orc_kmeans_keys(ptr(objs), n, D, int(is_box), key_mode, key_src, K, iters, ptr(init),
-                          ptr(keys), ptr(cid))
+    lib().orc_kmeans_keys(ptr(objs), n, D, int(is_box), key_mode, key_src, K, max_loop, ptr(init),
+                          ptr(keys), ptr(cid), ctypes.byref(its))
+    if want_iterations:
+        return keys[:n], cid[:n], its.value
     return keys[:n], cid[:n]
 
 
-def build_kmeans(objs, D, is_box, F, key_mode, K, iters, init_idx, key_src=SRC_LO,
+def build_kmeans(objs, D, is_box, F, key_mode, K, max_loop, init_idx, key_src=SRC_LO,
                  layout="packed"):
-    """Oracle build in the k-means + Hilbert order."""
+    """Oracle build in the k-means + Hilbert order (max_loop as in kmeans_keys)."""
     objs = np.ascontiguousarray(objs, dtype=np.float32)
-    k, cid = kmeans_keys(objs, D, is_box, key_mode, K, iters, init_idx, key_src)
+    k, cid, its = kmeans_keys(objs, D, is_box, key_mode, K, max_loop, init_idx, key_src,
+                              want_iterations=True)
     perm = sort_stable(k)
     t = Tree(objs[perm], D, is_box, F, layout=layout)
     if layout == "guttman":
@@ -373,6 +380,7 @@ def build_kmeans(objs, D, is_box, F, key_mode, K, iters, init_idx, key_src=SRC_L
     t.keys = k[perm]
     t.perm = perm
     t.cid = cid
+    t.iterations = its
     return t
 
 

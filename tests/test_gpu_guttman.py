@@ -123,7 +123,7 @@ def test_guttman_kmeans_order(gpu, O):
     """Guttman insertion in the k-means + Hilbert order."""
     pts = rand_points(30000, 2, 60)
     init = O.kmeans_seeds(gpu.gen_words(4096, 1234, 5), len(pts), 16)
-    t = O.build_kmeans(pts, 2, False, 64, O.KEY_REF, 16, 20, init, layout="guttman")
+    t = O.build_kmeans(pts, 2, False, 64, O.KEY_REF, 16, 1000, init, layout="guttman")
     ix = gpu.Index.build(pts, 2, fanout=64, layout="guttman", order="kmeans", clusters=16)
     _compare_tree(ix.export(), t, "k-means guttman")
     q = rand_queries(300, 2, 61, pts)

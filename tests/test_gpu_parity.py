@@ -297,40 +297,68 @@ def test_kmeans_small_and_boxes(gpu, O):
     pts = rand_points(5, 2, 3)
     ix = gpu.Index.build(pts, 2, fanout=16, order="kmeans", clusters=64)
     init = O.kmeans_seeds(gpu.gen_words(4096, 1234, 5), 5, 5)
-    t = O.build_kmeans(pts, 2, False, 16, O.KEY_REF, 5, 20, init)
+    t = O.build_kmeans(pts, 2, False, 16, O.KEY_REF, 5, 1000, init)
     assert_bits_equal(ix.export()["keys"], t.keys, "k-means keys, K = N")
+    assert ix.info()["kmeans_iterations"] == t.iterations
     boxes = rand_boxes(30000, 2, 5, mean_side=0.01)
     init = O.kmeans_seeds(gpu.gen_words(4096, 1234, 5), len(boxes), 12)
-    t = O.build_kmeans(boxes, 2, True, 64, O.KEY_REF, 12, 20, init, key_src=O.SRC_CENTER)
+    t = O.build_kmeans(boxes, 2, True, 64, O.KEY_REF, 12, 1000, init, key_src=O.SRC_CENTER)
     ix = gpu.Index.build(boxes, 2, fanout=64, layout="rtree", obj_kind="boxes", key_src="center",
                          order="kmeans", clusters=12)
     ex = ix.export()
+    assert ex["info"]["kmeans_iterations"] == t.iterations
     assert_bits_equal(ex["keys"], t.keys, "k-means box keys")
     assert_bits_equal(ex["perm"], t.perm, "k-means box permutation")
     q = rand_queries(200, 2, 6, boxes)
     assert_results_equal(ix.search(q, ids=True), t.search_dfs(q, want_ids=True), "k-means boxes")
 
 
-@pytest.mark.parametrize("D,K,layout,key", [(2, 16, "mphr", "ref"), (3, 64, "mphr", "b16"),
-                                            (3, 7, "bvh", "ref")])
-def test_kmeans_hilbert_order(gpu, O, D, K, layout, key):
+@pytest.mark.parametrize("D,K,layout,key,iters", [(2, 16, "mphr", "ref", 0), (3, 64, "mphr", "b16", 0),
+                                                  (3, 7, "bvh", "ref", 0), (2, 300, "mphr", "ref", 4)])
+def test_kmeans_hilbert_order(gpu, O, D, K, layout, key, iters):
     """k-means + Hilbert order: GPU assignment with host centre updates gives
-    the oracle's clusters, composite keys, ranks, tree and search results."""
+    the oracle's clusters, composite keys, ranks, tree and search results, with
+    the reference's stop rule (iters 0: at most 1001 iterations) or a cap."""
     pts = rand_points(70000, D, 80 + K)
     n = len(pts)
     init = O.kmeans_seeds(gpu.gen_words(4096, 1234, 5), n, K)
     km = O.KEY_REF if key == "ref" else O.KEY_B16
-    t = O.build_kmeans(pts, D, False, 64, km, K, 20, init,
+    t = O.build_kmeans(pts, D, False, 64, km, K, (iters or 1001) - 1, init,
                        layout="bvh" if layout == "bvh" else "packed")
     ix = gpu.Index.build(pts, D, fanout=64, layout=layout, key_mode=key, order="kmeans",
-                         clusters=K)
+                         clusters=K, kmeans_iters=iters)
     ex = ix.export()
+    print("k-means D=%d K=%d: %d iterations" % (D, K, t.iterations), flush=True)
+    assert ex["info"]["kmeans_iterations"] == t.iterations
+    if iters:
+        assert t.iterations == iters
     assert_bits_equal(ex["keys"], t.keys, "k-means keys")
     assert_bits_equal(ex["perm"], t.perm, "k-means permutation")
     assert_bits_equal(ex["boxes"], t.boxes[: t.n_nodes], "k-means node boxes")
     q = rand_queries(300, D, 81, pts)
     exp = t.search_dfs(q, want_ids=True)
     assert_results_equal(ix.search(q, ids=True), exp, "k-means order search")
+
+
+@pytest.mark.parametrize("n,D,K,iters", [(60000, 3, 0, 3), (600000, 2, 40, 0),
+                                         (600000, 3, 5000, 2)])
+def test_kmeans_reference_scale(gpu, O, n, D, K, iters):
+    """The reference's cluster count (K 0 = 20000: ten LDS tiles of centres,
+    the last one partial) and the four-objects-a-thread assignment of large
+    inputs, against the oracle's clusters and composite keys."""
+    pts = rand_points(n, D, 7 + D)
+    Kx = min(K or 20000, n)
+    init = O.kmeans_seeds(gpu.gen_words(1 << 16, 1234, 5), n, Kx)
+    print("k-means n=%d D=%d K=%d iters=%d" % (n, D, Kx, iters), flush=True)
+    keys, cid, its = O.kmeans_keys(pts, D, False, O.KEY_REF, Kx, (iters or 1001) - 1, init,
+                                   want_iterations=True)
+    ix = gpu.Index.build(pts, D, fanout=64, order="kmeans", clusters=K, kmeans_iters=iters)
+    ex = ix.export()
+    assert ex["info"]["kmeans_iterations"] == its
+    perm = O.sort_stable(keys)
+    assert_bits_equal(ex["keys"], keys[perm], "k-means keys")
+    assert_bits_equal(ex["perm"], perm, "k-means permutation")
+    ix.close()
 
 
 def test_ids_range_pagination(gpu, O):

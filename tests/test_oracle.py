@@ -362,14 +362,17 @@ def test_threaded_mprs_equals_serial(O):
     assert np.array_equal(c, ref["counts"]) and np.array_equal(h, ref["hashes"])
 
 
-def _kmeans_numpy(pts, K, iters, init):
-    """Independent numpy restatement of the k-means rule (FP32 distances summed
+def _kmeans_numpy(pts, K, max_loop, init):
+    """Independent numpy restatement of the k-means rule: FP32 distances summed
     over d in order, first minimum wins; FP64 means summed sequentially within
-    chunks of 65536, chunks in order)."""
+    chunks of 65536, chunks in order, after every assignment; repeat while
+    float32(changes) / float32(n) > 1e-5 and loop++ < max_loop (the reference's
+    condition, src/mapper/kmeans_mapper.cpp:170)."""
     n, D = pts.shape
     cen = pts[init].copy()
     cid = np.full(n, 0xFFFFFFFF, np.uint64)
-    for it in range(iters):
+    loop = its = 0
+    while True:
         dd = np.zeros((n, K), np.float32)
         for d in range(D):
             t = pts[:, d:d + 1] - cen[None, :, d]
@@ -377,8 +380,7 @@ def _kmeans_numpy(pts, K, iters, init):
         new = np.argmin(dd, axis=1).astype(np.uint64)
         changes = int((new != cid).sum())
         cid = new
-        if it > 0 and changes == 0:
-            break
+        its += 1
         for k in range(K):
             tot, s = 0, np.zeros(D)
             for c0 in range(0, n, 65536):
@@ -388,15 +390,27 @@ def _kmeans_numpy(pts, K, iters, init):
                     s = s + np.cumsum(sel, axis=0)[-1]
             if tot:
                 cen[k] = (s / tot).astype(np.float32)
-    return cid.astype(np.uint32), cen
+        delta = np.float32(changes) / np.float32(n)
+        if not (float(delta) > 1e-5):
+            break
+        if not loop < max_loop:
+            break
+        loop += 1
+    return cid.astype(np.uint32), cen, its
 
 
-@pytest.mark.parametrize("D,K", [(2, 5), (3, 8)])
-def test_kmeans_order_against_numpy(O, D, K):
-    pts = rand_points(3000, D, 60 + D)
-    init = np.arange(0, 3000, 3000 // K, dtype=np.uint32)[:K]
-    keys, cid = O.kmeans_keys(pts, D, False, O.KEY_REF, K, 12, init)
-    cid_np, cen = _kmeans_numpy(pts, K, 12, init)
+@pytest.mark.parametrize("D,K,max_loop", [(2, 5, 1000), (3, 8, 1000), (2, 16, 2), (3, 7, 0)])
+def test_kmeans_order_against_numpy(O, D, K, max_loop):
+    n = 3000 if K < 16 else 70000
+    pts = rand_points(n, D, 60 + D)
+    init = np.arange(0, n, n // K, dtype=np.uint32)[:K]
+    keys, cid, its = O.kmeans_keys(pts, D, False, O.KEY_REF, K, max_loop, init,
+                                   want_iterations=True)
+    cid_np, cen, its_np = _kmeans_numpy(pts, K, max_loop, init)
+    assert its == its_np
+    assert its <= max_loop + 1
+    if max_loop == 1000:
+        assert its < 1000  # converged by the delta rule, not the cap
     assert np.array_equal(cid, cid_np)
     # composite key: cluster rank (by centre key) above the object's key
     obj_keys, _ = O.keys(pts, D, False, O.KEY_REF)
