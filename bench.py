@@ -20,6 +20,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "query-co-processing-on-heterogeneous-architectures_amd")
 PEAK_HBM_GBS = 8000.0
+# vector ALU issue: 256 CUs x 4 SIMDs x 32 FP32 lanes x 2.4 GHz, one operation
+# per lane and clock (half the 157.3 TFLOP/s FP32 vector peak, which counts an
+# FMA as two)
+PEAK_VALU_TOPS = 78.64
 
 
 def load_qcp():
@@ -587,6 +591,78 @@ def run_config(qcp, name, args):
                                 "sigma 0.002; 20%% uniform); 500,000 data-centred queries, "
                                 "sides U(1e-4,1e-2); counts+hash", **res})
         return out
+    if name == "kmeans":
+        # the reference's k-means order (KmeansMapper) at its own scale: K = 20,000
+        # clusters and its stop rule, on the 1,048,576-point 2-D config; then the
+        # 16,777,216-point 3-D config for three iterations
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+
+        w = qcp.workload("c1", scale=args.scale)
+        pts = w["objs"]
+        n = len(pts)
+        K = min(20000, n)
+        log("kmeans: n=%d K=%d, the reference's stop rule" % (n, K))
+        ix = qcp.Index.build(pts, 2, fanout=64, order="kmeans")
+        inf = ix.info()
+        ix.close()
+        its = inf["kmeans_iterations"]
+        a_ms, u_ms = inf["kmeans_ms"]
+        it_ms = a_ms / max(its, 1)
+        # VALU instructions per (object, centre) pair in the 2-D assignment loop:
+        # 2 sub, 2 mul, 1 add, 1 compare, 2 selects (its gfx950 code)
+        ops = 8.0 * n * K / (it_ms * 1e-3) / 1e12
+        res = {"workload": "1,048,576 uniform 2-D points, K=20,000, changed fraction > 1e-5 "
+                           "and at most 1001 iterations; then 16,777,216 3-D points, 3 "
+                           "iterations",
+               "n": n, "clusters": K, "iterations": its, "build_ms": inf["build_ms"],
+               "assign_ms_total": a_ms, "update_ms_total": u_ms,
+               "assign_ms_per_iteration": it_ms,
+               "update_ms_per_iteration": u_ms / max(its, 1),
+               "pairs_per_s": n * K / (it_ms * 1e-3),
+               "roofline": {"bound": "valu", "achieved": ops, "peak": PEAK_VALU_TOPS,
+                            "unit": "T lane-ops/s", "frac": ops / PEAK_VALU_TOPS}}
+        log("kmeans 2-D: %d iterations, build %.1f ms, assignment %.3f ms/iteration, update "
+            "%.3f ms/iteration" % (its, inf["build_ms"], it_ms, u_ms / max(its, 1)))
+        # parity at full size: two iterations against the oracle, which is also
+        # the CPU baseline (threaded assignment, the same centre update)
+        ix = qcp.Index.build(pts, 2, fanout=64, order="kmeans", kmeans_iters=2)
+        ex = ix.export()
+        ix.close()
+        init = O.kmeans_seeds(qcp.gen_words(1 << 16, 1234, 5), n, K)
+        t0 = time.perf_counter()
+        keys, cid, oits = O.kmeans_keys(pts, 2, False, O.KEY_REF, K, 1, init,
+                                        want_iterations=True)
+        dt = time.perf_counter() - t0
+        perm = O.sort_stable(keys)
+        res["two_iterations_match_oracle"] = bool(
+            oits == 2 and ex["info"]["kmeans_iterations"] == 2 and
+            np.array_equal(ex["keys"], keys[perm]) and np.array_equal(ex["perm"], perm))
+        cores = min(16, os.cpu_count() or 1)
+        res["cpu_baseline"] = {"value": dt * 1e3 / 2, "unit": "ms per iteration",
+                               "cores": cores, "kind": "port",
+                               "sample": "2 iterations of the oracle's k-means (threaded "
+                                         "assignment, centre update) on all %d points" % n}
+        log("kmeans oracle: %.1f ms per iteration on %d threads; GPU keys %s" %
+            (dt * 1e3 / 2, cores, "match" if res["two_iterations_match_oracle"] else "DIFFER"))
+        w3 = qcp.workload("c2", scale=args.scale)
+        p3 = w3["objs"]
+        ix = qcp.Index.build(p3, 3, fanout=64, order="kmeans", kmeans_iters=3)
+        i3 = ix.info()
+        ix.close()
+        m3 = i3["kmeans_ms"][0] / max(i3["kmeans_iterations"], 1)
+        K3 = min(20000, len(p3))
+        ops3 = 11.0 * len(p3) * K3 / (m3 * 1e-3) / 1e12
+        res["three_d"] = {"n": len(p3), "clusters": K3, "iterations": i3["kmeans_iterations"],
+                          "assign_ms_per_iteration": m3,
+                          "update_ms_per_iteration": i3["kmeans_ms"][1] /
+                          max(i3["kmeans_iterations"], 1),
+                          "roofline": {"bound": "valu", "achieved": ops3, "peak": PEAK_VALU_TOPS,
+                                       "unit": "T lane-ops/s", "frac": ops3 / PEAK_VALU_TOPS}}
+        log("kmeans 3-D: %.3f ms/iteration assignment (%.2f of VALU peak)" %
+            (m3, ops3 / PEAK_VALU_TOPS))
+        out.update(res)
+        return out
     if name == "c4":
         out.update(serving_bench(qcp, (10000, 50000, 100000), args.serve_seconds, args.scale,
                                  contain_rate=50000))
@@ -756,7 +832,7 @@ def main():
                     help="--profile-traffic: every configuration the autotuner can pick")
     ap.add_argument("--config", default="c2",
                     help="c2 (default: the headline line); c1, c3, c4 (serving, "
-                         "--serve-seconds per rate) or c5 print their own line")
+                         "--serve-seconds per rate), c5 or kmeans print their own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the box has one); --gpus %d is not "

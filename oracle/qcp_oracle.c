@@ -1163,14 +1163,14 @@ void orc_kmeans_keys(const float *objs, uint64_t n, unsigned D, int is_box, int 
   for (uint64_t i = 0; i < n; ++i) cid[i] = 0xFFFFFFFFu;
   long nt = sysconf(_SC_NPROCESSORS_ONLN);
   if (nt < 1) nt = 1;
-  if (nt > 64) nt = 64;
+  if (nt > 16) nt = 16;
   unsigned loop = 0, its = 0;
   float delta;
   do {
     /* assignment: static object ranges over threads (each object's result
        does not depend on the split) */
-    pthread_t th[64];
-    km_arg ka[64];
+    pthread_t th[16];
+    km_arg ka[16];
     uint64_t changes = 0;
     for (long t = 0; t < nt; ++t) {
       km_arg x = {pts, cen, n * (uint64_t)t / (uint64_t)nt, n * (uint64_t)(t + 1) / (uint64_t)nt,

@@ -978,7 +978,7 @@ hipError_t launch_keys_sorted(const float* obj, uint64_t stride, uint64_t n, uns
 // holds R objects in registers so that one broadcast LDS read of a centre
 // serves R distance evaluations.  `changes` counts objects whose cluster
 // changed.
-constexpr uint32_t kAssignTile = 2048;
+constexpr uint32_t kAssignTile = 1024;
 
 template <int D>
 struct CentreVec;
@@ -1035,6 +1035,7 @@ __global__ __launch_bounds__(256) void k_assign(const float* __restrict__ objs, 
       tile[t] = c;
     }
     __syncthreads();
+#pragma unroll 4
     for (uint32_t t = k0 == 0 ? 1u : 0u; t < m; ++t) {
       const V c = tile[t];
 #pragma unroll
@@ -1084,17 +1085,19 @@ hipError_t launch_assign(const float* objs, uint64_t n, unsigned D, bool box, bo
                          const float* centers, uint32_t K, uint32_t* cid, uint32_t* changes,
                          hipStream_t s) {
   if (n == 0 || K == 0) return hipSuccess;
-  // four objects a thread once that still gives every CU two workgroups
-  const bool wide = n >= 4ull * 256 * 512;
-  const unsigned R = wide ? 4u : 1u;
+  // R objects a thread, the largest of 4, 2, 1 that still leaves 2048
+  // workgroups (eight a CU)
+  const unsigned R = n >= 256ull * 4 * 2048 ? 4u : (n >= 256ull * 2 * 2048 ? 2u : 1u);
   const uint64_t g64 = (n + 256ull * R - 1) / (256ull * R);
   if (g64 > 0x7FFFFFFFull) return hipErrorInvalidValue;
   const unsigned g = (unsigned)g64;
-#define QCP_ASSIGN(DD, BB, CC)                                                   \
-  do {                                                                         \
-    if (wide)                                                                  \
+#define QCP_ASSIGN(DD, BB, CC)                                                        \
+  do {                                                                              \
+    if (R == 4)                                                                     \
       k_assign<DD, BB, CC, 4><<<g, 256, 0, s>>>(objs, n, centers, K, cid, changes); \
-    else                                                                       \
+    else if (R == 2)                                                                \
+      k_assign<DD, BB, CC, 2><<<g, 256, 0, s>>>(objs, n, centers, K, cid, changes); \
+    else                                                                            \
       k_assign<DD, BB, CC, 1><<<g, 256, 0, s>>>(objs, n, centers, K, cid, changes); \
   } while (0)
   if (D == 2) {

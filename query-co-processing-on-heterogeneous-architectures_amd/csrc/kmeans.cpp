@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <functional>
 #include <mutex>
@@ -59,6 +60,17 @@ struct DevBuf {
     if (p) hipFree(p);
   }
   bool alloc(uint64_t n) { return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)) == hipSuccess; }
+};
+
+struct Events {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  bool create() {
+    return hipEventCreate(&e[0]) == hipSuccess && hipEventCreate(&e[1]) == hipSuccess;
+  }
+  ~Events() {
+    for (auto x : e)
+      if (x) hipEventDestroy(x);
+  }
 };
 
 // A fixed set of host threads that run fn(i) for i in [0, count) together with
@@ -231,20 +243,32 @@ qcp_status qcp::kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bi
   const uint32_t max_loop = max_iters - 1;
   uint32_t loop = 0, its = 0;
   float delta = 0.0f;
+  double assign_ms = 0.0, update_ms = 0.0;
+  Events ev;
+  if (!ev.create()) return QCP_EHIP;
   do {
     uint32_t changes = 0;
     HIP_TRY(hipMemcpyAsync(d_c.p, centers.data(), 4ull * K * D, hipMemcpyHostToDevice, s));
     HIP_TRY(launch_fill_u32(d_changes.p, 1, 0u, s));
+    HIP_TRY(hipEventRecord(ev.e[0], s));
     HIP_TRY(launch_assign(d_objs, n, D, ix->box, center, d_c.p, K, d_cid.p, d_changes.p, s));
+    HIP_TRY(hipEventRecord(ev.e[1], s));
     HIP_TRY(hipMemcpyAsync(&changes, d_changes.p, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(cid.data(), d_cid.p, 4ull * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+    assign_ms += ms;
     ++its;
+    const auto t0 = std::chrono::steady_clock::now();
     update_centres(pool, hw, pts.data(), n, D, cid.data(), K, centers.data());
+    update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     delta = (float)changes;
     delta /= (float)n;
   } while ((double)delta > kDeltaMin && loop++ < max_loop);
   ix->kmeans_its = its;
+  ix->kmeans_ms[0] = assign_ms;
+  ix->kmeans_ms[1] = update_ms;
   // cluster ranks by (centre key, cluster number)
   std::vector<std::pair<uint64_t, uint32_t>> ck(K);
   for (uint32_t k = 0; k < K; ++k) {

@@ -291,6 +291,8 @@ static void set_info_shape(qcp_index* ix) {
   in.tree_bytes = (uint64_t)ix->E * 4 * ix->n + (uint64_t)2 * ix->D * 4 * ix->n_nodes;
   in.partitions = ix->parts;
   in.kmeans_iterations = ix->kmeans_its;
+  in.kmeans_ms[0] = ix->kmeans_ms[0];
+  in.kmeans_ms[1] = ix->kmeans_ms[1];
 }
 
 // Guttman layout: insert the objects on the host in their sorted order

@@ -49,7 +49,8 @@ class IndexInfo(ctypes.Structure):
                 ("key_src", ctypes.c_uint32), ("key_bits", ctypes.c_uint32),
                 ("level_count", ctypes.c_uint64 * 32), ("tree_bytes", ctypes.c_uint64),
                 ("build_ms", ctypes.c_double), ("build_phase_ms", ctypes.c_double * 6),
-                ("partitions", ctypes.c_uint32), ("kmeans_iterations", ctypes.c_uint32)]
+                ("partitions", ctypes.c_uint32), ("kmeans_iterations", ctypes.c_uint32),
+                ("kmeans_ms", ctypes.c_double * 2)]
 
 
 class SearchCfg(ctypes.Structure):
@@ -318,6 +319,7 @@ class Index:
         d = {k: getattr(inf, k) for k, _ in IndexInfo._fields_}
         d["level_count"] = [int(x) for x in inf.level_count[: inf.height]]
         d["build_phase_ms"] = list(inf.build_phase_ms)
+        d["kmeans_ms"] = list(inf.kmeans_ms)
         return d
 
     def export(self):

@@ -340,12 +340,12 @@ def test_kmeans_hilbert_order(gpu, O, D, K, layout, key, iters):
     assert_results_equal(ix.search(q, ids=True), exp, "k-means order search")
 
 
-@pytest.mark.parametrize("n,D,K,iters", [(60000, 3, 0, 3), (600000, 2, 40, 0),
-                                         (600000, 3, 5000, 2)])
+@pytest.mark.parametrize("n,D,K,iters", [(60000, 3, 0, 3), (1100000, 2, 40, 0),
+                                         (2200000, 3, 3000, 2)])
 def test_kmeans_reference_scale(gpu, O, n, D, K, iters):
-    """The reference's cluster count (K 0 = 20000: ten LDS tiles of centres,
-    the last one partial) and the four-objects-a-thread assignment of large
-    inputs, against the oracle's clusters and composite keys."""
+    """The reference's cluster count (K 0 = 20000: twenty LDS tiles of centres,
+    the last one partial) and the two- and four-objects-a-thread assignments of
+    large inputs, against the oracle's clusters and composite keys."""
     pts = rand_points(n, D, 7 + D)
     Kx = min(K or 20000, n)
     init = O.kmeans_seeds(gpu.gen_words(1 << 16, 1234, 5), n, Kx)
