@@ -51,32 +51,71 @@ __device__ __forceinline__ uint64_t object_key(const float* v, unsigned nb, cons
   return hilbert_key_table<D>(c, nb, tab);
 }
 
+// Point c of one object v[E]; `bad` set for a coordinate outside the REF
+// quantiser's domain.
+template <int D, bool BOX, bool REF, bool CENTER>
+__device__ __forceinline__ void object_point(const float* v, unsigned nb, uint32_t (&c)[D],
+                                             bool& bad) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    float p = v[d];
+    if (CENTER && BOX) p = (v[d] + v[D + d]) * 0.5f;
+    if (REF) {
+      bad |= !quant_ref(p, nb, &c[d]);
+    } else {
+      c[d] = quant_b16(p);
+    }
+  }
+}
+
+// Keys of kKeysPerThread objects a thread (objects base + r*256 + tid of each
+// 256 x kKeysPerThread block of the grid-stride loop), their walks interleaved.
+constexpr int kKeysPerThread = 4;
+
 template <int D, bool BOX, bool REF, bool CENTER>
 __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, uint64_t n,
                                               unsigned nb, uint64_t* __restrict__ keys,
                                               uint32_t* __restrict__ idx,
                                               unsigned long long* key_or, uint32_t* bad) {
   constexpr int E = BOX ? 2 * D : D;
+  constexpr int R = kKeysPerThread;
   __shared__ uint16_t tab[HilbertTable<D>::SIZE];
   fill_hilbert_table<D>(tab);
   uint64_t acc = 0;
   bool any_bad = false, any_inv = false;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const float* o = objs + i * E;
-    float v[E];
+  for (uint64_t base = (uint64_t)blockIdx.x * (256 * R); base < n;
+       base += (uint64_t)gridDim.x * (256 * R)) {
+    uint32_t c[R][D];
+    float v[R][E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = o[e];
+    for (int r = 0; r < R; ++r) {
+      const uint64_t i = base + (uint64_t)r * 256 + threadIdx.x;
+      const uint64_t j = i < n ? i : n - 1;  // tail lanes read a valid object, store nothing
 #pragma unroll
-    for (int e = 0; e < E; ++e) any_bad |= (v[e] != v[e]);  // NaN objects are rejected
-    if (BOX) {
-#pragma unroll
-      for (int d = 0; d < D; ++d) any_inv |= v[d] > v[D + d];
+      for (int e = 0; e < E; ++e) v[r][e] = objs[j * E + e];
     }
-    const uint64_t k = object_key<D, BOX, REF, CENTER>(v, nb, tab, any_bad);
-    keys[i] = k;
-    if (idx != nullptr) idx[i] = (uint32_t)i;  // callers whose sort reads input indices
-    acc |= k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      // a tail lane repeats object n - 1, which a valid lane also checks
+#pragma unroll
+      for (int e = 0; e < E; ++e) any_bad |= (v[r][e] != v[r][e]);  // NaN objects are rejected
+      if (BOX) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) any_inv |= v[r][d] > v[r][D + d];
+      }
+      object_point<D, BOX, REF, CENTER>(v[r], nb, c[r], any_bad);
+    }
+    uint64_t k[R];
+    hilbert_keys_table<D, R>(c, nb, tab, k);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t i = base + (uint64_t)r * 256 + threadIdx.x;
+      if (i < n) {
+        keys[i] = k[r];
+        if (idx != nullptr) idx[i] = (uint32_t)i;  // callers whose sort reads input indices
+        acc |= k[r];
+      }
+    }
   }
   // wave OR, one atomic per wave
 #pragma unroll
@@ -87,12 +126,16 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
 }
 
 // ---------------------------------------------------------------------------
-// Radix sort.  Tile = 4096 consecutive keys (256 threads x 16); a pass is
+// Radix sort.  Tile = 8192 consecutive keys (512 threads x 16); a pass is
 // stable because each tile is ranked in index order and tiles are placed in
-// tile order.
-constexpr int kSortThreads = 256;
+// tile order.  Threads 0..255 own one digit each in the per-tile steps.
+constexpr int kSortThreads = 512;
 constexpr int kSortKPT = 16;
 constexpr int kSortTile = kSortThreads * kSortKPT;
+// the digit-count kernel's workgroup and work unit
+constexpr int kHistThreads = 256;
+constexpr int kHistKPT = 16;
+constexpr int kHistTile = kHistThreads * kHistKPT;
 
 
 // ---------------------------------------------------------------------------
@@ -494,7 +537,7 @@ static inline unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 655
 hipError_t launch_keys(const float* objs, uint64_t n, unsigned D, bool box, bool ref,
                        bool center, unsigned nb, uint64_t* keys, uint32_t* idx,
                        unsigned long long* key_or, uint32_t* bad, hipStream_t s) {
-  unsigned g = grid_for(n, 256, 8192);
+  unsigned g = grid_for((n + kKeysPerThread - 1) / kKeysPerThread, 256, 8192);
 #define QCP_KEYS(DD, BB, RR, CC) \
   k_keys<DD, BB, RR, CC><<<g, 256, 0, s>>>(objs, n, nb, keys, idx, key_or, bad)
   if (D == 2) {
@@ -575,7 +618,7 @@ __device__ __forceinline__ unsigned real_passes(unsigned passes,
 // caller) from one read of the keys; also zeroes the first pass's look-back
 // words.  A wave whose keys share one digit adds them with one atomic (the
 // high digits of clustered keys), otherwise each key adds its own.
-__global__ __launch_bounds__(kSortThreads) void k_sort_hist_all(
+__global__ __launch_bounds__(kHistThreads) void k_sort_hist_all(
     const uint64_t* __restrict__ keys, uint64_t n, unsigned passes,
     const unsigned long long* __restrict__ key_or, uint32_t* __restrict__ ghist,
     uint64_t* __restrict__ status0, uint64_t status_words) {
@@ -584,22 +627,22 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist_all(
   passes = real_passes(passes, key_or);
 #pragma unroll
   for (int p = 0; p < kMaxSortPasses; ++p) h[p][tid] = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * kSortThreads + tid; i < status_words;
-       i += (uint64_t)gridDim.x * kSortThreads)
+  for (uint64_t i = (uint64_t)blockIdx.x * kHistThreads + tid; i < status_words;
+       i += (uint64_t)gridDim.x * kHistThreads)
     status0[i] = 0;
   __syncthreads();
-  const uint32_t nt = (uint32_t)((n + kSortTile - 1) / kSortTile);
+  const uint32_t nt = (uint32_t)((n + kHistTile - 1) / kHistTile);
   for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
-    const uint64_t base = (uint64_t)t * kSortTile;
-    uint64_t k[kSortKPT];
+    const uint64_t base = (uint64_t)t * kHistTile;
+    uint64_t k[kHistKPT];
 #pragma unroll
-    for (int it = 0; it < kSortKPT; ++it) {
-      const uint64_t i = base + (uint64_t)it * kSortThreads + tid;
+    for (int it = 0; it < kHistKPT; ++it) {
+      const uint64_t i = base + (uint64_t)it * kHistThreads + tid;
       k[it] = i < n ? keys[i] : 0;
     }
 #pragma unroll
-    for (int it = 0; it < kSortKPT; ++it) {
-      const bool valid = base + (uint64_t)it * kSortThreads + tid < n;
+    for (int it = 0; it < kHistKPT; ++it) {
+      const bool valid = base + (uint64_t)it * kHistThreads + tid < n;
       const uint64_t vmask = ballot(valid);
       if (vmask == 0) continue;
       // the low digits of Hilbert keys are spread out: one atomic per key
@@ -625,16 +668,49 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist_all(
     if (h[p][tid]) atomicAdd(&ghist[p * 256 + tid], h[p][tid]);
 }
 
-// LDS of one pass: 52.5 KB in all, so that three workgroups share a CU's
-// 160 KB.  sv's first 256 words double as scan scratch.
+// LDS of one pass: 77.5 KB, so that two workgroups share a CU's 160 KB (and
+// at most 128 VGPRs a lane, four waves a SIMD, for the same reason).  The
+// tile's keys (or packed words) and then, for passes that move a separate
+// index array, its indices pass through `sk` / `sv`; dg keeps each slot's
+// digit.
 struct SortLds {
-  uint64_t sk[kSortTile];
-  uint32_t sv[kSortTile];
-  uint32_t gstart[256];  // global position of this tile's bucket
-  uint16_t lstart[256];  // the bucket's start inside the tile
-  uint16_t wcnt[kSortThreads / kWave][256];  // per wave: keys per digit, then its offsets
+  union {
+    uint64_t sk[kSortTile];
+    uint32_t sv[kSortTile];
+  };
+  uint8_t dg[kSortTile];
+  uint32_t wsum[kSortThreads / kWave];            // wave totals of sort_scan
+  uint32_t gstart[256];                           // global position of this tile's bucket
+  uint16_t lstart[256];                           // the bucket's start inside the tile
+  uint16_t wcnt[kSortThreads / kWave][256];       // per wave: keys per digit, then its offsets
   uint32_t s_tile;
 };
+
+// Exclusive scan of one value per thread over the sort workgroup (wave
+// prefix by shuffles, then the wave totals); *total = the sum.
+__device__ __forceinline__ uint32_t sort_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+  constexpr int W = kSortThreads / kWave;
+  const int w = threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t t = __shfl_up(x, off, kWave);
+    if ((int)lane >= off) x += t;
+  }
+  if (lane == kWave - 1) wsum[w] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int ww = 0; ww < W; ++ww) {
+    const uint32_t t = wsum[ww];
+    if (ww < w) pre += t;
+    tot += t;
+  }
+  *total = tot;
+  __syncthreads();
+  return pre + x - v;
+}
 
 // The work of one pass on one tile (see k_sort_onesweep).
 template <bool INP, bool OUTP>
@@ -645,17 +721,19 @@ __device__ __forceinline__ void sort_pass_body(
     uint64_t* status, uint64_t* __restrict__ status_next, uint32_t* tile_ctr) {
   static_assert(OUTP || !INP, "a packed pass writes packed elements");
   constexpr int W = kSortThreads / kWave;
-  constexpr int CH = kSortTile / kSortThreads;  // chunks of 64 keys per wave
-  uint32_t* const scan_sh = L.sv;
+  constexpr int CH = kSortKPT;  // chunks of 64 keys per wave
   const int tid = threadIdx.x;
+  const bool owner = tid < 256;  // owns digit tid
   const int w = tid / kWave;
   const uint32_t lane = lane_id();
   if (tid == 0) L.s_tile = atomicAdd(&tile_ctr[pass], 1u);
-  if (status_next != nullptr) status_next[(uint64_t)blockIdx.x * 256 + tid] = 0;
+  if (status_next != nullptr && owner) status_next[(uint64_t)blockIdx.x * 256 + tid] = 0;
   uint32_t tot;
-  const uint32_t gbase = block_excl_scan<uint32_t>(ghist[pass * 256 + tid], scan_sh, &tot);
+  const uint32_t gbase = sort_scan(owner ? ghist[pass * 256 + tid] : 0u, L.wsum, &tot);
+  if (owner) {
 #pragma unroll
-  for (int ww = 0; ww < W; ++ww) L.wcnt[ww][tid] = 0;
+    for (int ww = 0; ww < W; ++ww) L.wcnt[ww][tid] = 0;
+  }
   __syncthreads();
   const uint32_t tile = L.s_tile;
   const uint64_t wbase = (uint64_t)tile * kSortTile + (uint64_t)w * CH * kWave;
@@ -690,19 +768,22 @@ __device__ __forceinline__ void sort_pass_body(
   __syncthreads();
   // digit tid: this tile's count, and each wave's offset among the tile's keys
   uint32_t cnt = 0;
+  if (owner) {
 #pragma unroll
-  for (int ww = 0; ww < W; ++ww) {
-    const uint32_t c = L.wcnt[ww][tid];
-    L.wcnt[ww][tid] = (uint16_t)cnt;
-    cnt += c;
+    for (int ww = 0; ww < W; ++ww) {
+      const uint32_t c = L.wcnt[ww][tid];
+      L.wcnt[ww][tid] = (uint16_t)cnt;
+      cnt += c;
+    }
   }
   uint64_t* const mine = status + (uint64_t)tile * 256 + tid;
-  __hip_atomic_store(mine, (tile == 0 ? kStatusPrefix : kStatusAgg) | (uint64_t)cnt,
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (owner)
+    __hip_atomic_store(mine, (tile == 0 ? kStatusPrefix : kStatusAgg) | (uint64_t)cnt,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint32_t tot_tile;
-  const uint32_t ls = block_excl_scan<uint32_t>(cnt, scan_sh, &tot_tile);
+  const uint32_t ls = sort_scan(cnt, L.wsum, &tot_tile);
   uint32_t excl = 0;
-  if (tile > 0) {
+  if (owner && tile > 0) {
     // kLookback predecessors' words per round trip, consumed in order up to
     // the first inclusive prefix; a word not yet published ends the round
     // (sleep, then reload from there).  Tiles below 0 read as a zero prefix.
@@ -736,33 +817,42 @@ __device__ __forceinline__ void sort_pass_body(
     __hip_atomic_store(mine, kStatusPrefix | (uint64_t)(excl + cnt), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   }
-  L.gstart[tid] = gbase + excl;
-  L.lstart[tid] = (uint16_t)ls;
+  if (owner) {
+    L.gstart[tid] = gbase + excl;
+    L.lstart[tid] = (uint16_t)ls;
 #pragma unroll
-  for (int ww = 0; ww < W; ++ww) L.wcnt[ww][tid] = (uint16_t)(L.wcnt[ww][tid] + ls);
+    for (int ww = 0; ww < W; ++ww) L.wcnt[ww][tid] = (uint16_t)(L.wcnt[ww][tid] + ls);
+  }
   __syncthreads();
+  // the tile in (digit, index) order: keys (the packing pass packs them with
+  // their indices first) and their digits
+  uint32_t pos[CH];
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
+    const uint32_t dig = (uint32_t)(k[c] >> shift) & 0xFF;
+    pos[c] = L.wcnt[w][dig] + rk[c];
     if (wbase + (uint64_t)c * kWave + lane < n) {
-      const uint32_t pos = L.wcnt[w][(uint32_t)(k[c] >> shift) & 0xFF] + rk[c];
-      L.sk[pos] = k[c];
-      if (!INP) L.sv[pos] = v[c];
+      L.sk[pos[c]] = (OUTP && !INP) ? (((k[c] >> pack_shift) << pbits) | (uint64_t)v[c]) : k[c];
+      L.dg[pos[c]] = (uint8_t)dig;
     }
   }
   __syncthreads();
   const uint64_t base = (uint64_t)tile * kSortTile;
   const uint32_t tcnt = (uint32_t)(n - base < (uint64_t)kSortTile ? n - base : kSortTile);
   for (uint32_t p = tid; p < tcnt; p += kSortThreads) {
-    const uint64_t kk = L.sk[p];
-    const uint32_t dig = (uint32_t)(kk >> shift) & 0xFF;
-    const uint32_t pos = L.gstart[dig] + (p - L.lstart[dig]);
-    if (INP) {
-      kout[pos] = kk;
-    } else if (OUTP) {
-      kout[pos] = ((kk >> pack_shift) << pbits) | (uint64_t)L.sv[p];
-    } else {
-      kout[pos] = kk;
-      vout[pos] = L.sv[p];
+    const uint32_t dig = L.dg[p];
+    kout[L.gstart[dig] + (p - L.lstart[dig])] = L.sk[p];
+  }
+  if (!OUTP) {
+    // the indices, through the same LDS after every key has left it
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (wbase + (uint64_t)c * kWave + lane < n) L.sv[pos[c]] = v[c];
+    __syncthreads();
+    for (uint32_t p = tid; p < tcnt; p += kSortThreads) {
+      const uint32_t dig = L.dg[p];
+      vout[L.gstart[dig] + (p - L.lstart[dig])] = L.sv[p];
     }
   }
 }
@@ -776,7 +866,7 @@ __device__ __forceinline__ void sort_pass_body(
 // later passes move that word alone (INP): 8 bytes per key each way instead of
 // 8 + 4.
 template <bool INP, bool OUTP>
-__global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
+__global__ __launch_bounds__(kSortThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_sort_onesweep(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned pass,
     unsigned shift, unsigned pack_shift, unsigned pbits, const uint32_t* __restrict__ ghist,
@@ -790,7 +880,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_onesweep(
 // so the result stays where the last reordering pass put it: in the first
 // buffers for an even `real`, the second for an odd one (packed when the plan
 // packs); k_gather_objs_planned reads it there.
-__global__ __launch_bounds__(kSortThreads) void k_sort_onesweep_planned(
+__global__ __launch_bounds__(kSortThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_sort_onesweep_planned(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, unsigned pass,
     unsigned passes, unsigned pbits, const uint32_t* __restrict__ ghist, uint64_t* status,
@@ -827,8 +917,9 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
   uint32_t* tile_ctr = ghist + kMaxSortPasses * 256;
   hipError_t e = launch_fill_u32(ghist, kMaxSortPasses * 256 + kMaxSortPasses, 0u, s);
   if (e != hipSuccess) return e;
-  const unsigned g = nt < 2048 ? nt : 2048;
-  k_sort_hist_all<<<g, kSortThreads, 0, s>>>(keys, n, passes, key_or, ghist, status, words);
+  const uint32_t ht = (uint32_t)((n + kHistTile - 1) / kHistTile);
+  const unsigned g = ht < 2048 ? ht : 2048;
+  k_sort_hist_all<<<g, kHistThreads, 0, s>>>(keys, n, passes, key_or, ghist, status, words);
   uint64_t* kin = keys;
   uint32_t* vin = nullptr;  // the first pass takes positions as the input indices
   uint64_t* kout = keys2;

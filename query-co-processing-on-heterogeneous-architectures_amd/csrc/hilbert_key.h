@@ -141,6 +141,46 @@ QCP_HD uint64_t hilbert_key_table(const uint32_t* c, unsigned nb, const uint16_t
   return hilbert_finish<D>(index, nb);
 }
 
+// R keys at once: the same walk for R points side by side, so that the R
+// chains of dependent table lookups overlap (each step issues R independent
+// LDS reads).  out[r] equals hilbert_key_table<D>(c[r], nb, tab).
+template <int D, int R>
+QCP_HD void hilbert_keys_table(const uint32_t (&c)[R][D], unsigned nb, const uint16_t* tab,
+                               uint64_t (&out)[R]) {
+  constexpr int L = HilbertTable<D>::L, IN = HilbertTable<D>::IN;
+  uint64_t m[R];
+  uint32_t st[R];  // s | rot << D
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    m[r] = interleave<D>(c[r]);
+    out[r] = 0;
+    st[r] = 0;
+  }
+  int b = (int)nb - 1;
+  const int top = (int)nb - (int)(nb % L);
+  for (; b >= top; --b) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      uint32_t s = st[r] & ((1u << D) - 1u);
+      unsigned rot = st[r] >> D;
+      out[r] = (out[r] << D) |
+               hilbert_level<D>(s, rot, (uint32_t)(m[r] >> (D * b)) & ((1u << D) - 1u));
+      st[r] = s | (rot << D);
+    }
+  }
+  for (; b >= 0; b -= L) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t in = (uint32_t)(m[r] >> (D * (b - L + 1))) & ((1u << IN) - 1u);
+      const uint32_t e = tab[(st[r] << IN) | in];
+      out[r] = (out[r] << IN) | (e & 0xFFu);
+      st[r] = e >> 8;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[r] = hilbert_finish<D>(out[r], nb);
+}
+
 // Quantiser of src/mapper/hilbert_mapper.cpp:22 -- (ll)(1000000*p), one FP32
 // multiply then truncation.  Returns false outside [0, 2^nb) or for NaN.
 QCP_HD bool quant_ref(float p, unsigned nb, uint32_t* out) {
