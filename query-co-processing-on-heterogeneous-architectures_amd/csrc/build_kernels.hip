@@ -32,30 +32,18 @@ __device__ __forceinline__ void fill_hilbert_table(uint16_t* tab) {
   __syncthreads();
 }
 
-// The key of one object v[E] (E = 2D for boxes); `bad` set for a coordinate
-// outside the REF quantiser's domain.
-template <int D, bool BOX, bool REF, bool CENTER>
-__device__ __forceinline__ uint64_t object_key(const float* v, unsigned nb, const uint16_t* tab,
-                                               bool& bad) {
-  uint32_t c[D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    float p = v[d];
-    if (CENTER && BOX) p = (v[d] + v[D + d]) * 0.5f;
-    if (REF) {
-      bad |= !quant_ref(p, nb, &c[d]);
-    } else {
-      c[d] = quant_b16(p);
-    }
-  }
-  return hilbert_key_table<D>(c, nb, tab);
+// Bits a coordinate of the keys: the reference's 31 (2-D) / 20 (3-D) for REF
+// keys, 16 for b16 keys.
+template <int D, bool REF>
+constexpr unsigned key_nb() {
+  return REF ? (D == 2 ? 31u : 20u) : 16u;
 }
 
 // Point c of one object v[E]; `bad` set for a coordinate outside the REF
 // quantiser's domain.
 template <int D, bool BOX, bool REF, bool CENTER>
-__device__ __forceinline__ void object_point(const float* v, unsigned nb, uint32_t (&c)[D],
-                                             bool& bad) {
+__device__ __forceinline__ void object_point(const float* v, uint32_t (&c)[D], bool& bad) {
+  constexpr unsigned nb = key_nb<D, REF>();
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     float p = v[d];
@@ -74,7 +62,7 @@ constexpr int kKeysPerThread = 4;
 
 template <int D, bool BOX, bool REF, bool CENTER>
 __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, uint64_t n,
-                                              unsigned nb, uint64_t* __restrict__ keys,
+                                              uint64_t* __restrict__ keys,
                                               uint32_t* __restrict__ idx,
                                               unsigned long long* key_or, uint32_t* bad) {
   constexpr int E = BOX ? 2 * D : D;
@@ -103,10 +91,10 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
 #pragma unroll
         for (int d = 0; d < D; ++d) any_inv |= v[r][d] > v[r][D + d];
       }
-      object_point<D, BOX, REF, CENTER>(v[r], nb, c[r], any_bad);
+      object_point<D, BOX, REF, CENTER>(v[r], c[r], any_bad);
     }
     uint64_t k[R];
-    hilbert_keys_table<D, R>(c, nb, tab, k);
+    hilbert_keys_nb<D, key_nb<D, REF>(), R>(c, tab, k);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint64_t i = base + (uint64_t)r * 256 + threadIdx.x;
@@ -392,21 +380,35 @@ __global__ __launch_bounds__(256) void k_gather_objs_planned(
 }
 
 // Sorted keys recomputed from the rank-ordered SoA objects (obj[e][r]) by the
-// same object_key as k_keys.
+// same walk as k_keys, four a thread.
 template <int D, bool BOX, bool REF, bool CENTER>
 __global__ __launch_bounds__(256) void k_keys_sorted(const float* __restrict__ obj,
-                                                     uint64_t stride, uint64_t n, unsigned nb,
+                                                     uint64_t stride, uint64_t n,
                                                      uint64_t* __restrict__ keys) {
   constexpr int E = BOX ? 2 * D : D;
+  constexpr int R = kKeysPerThread;
   __shared__ uint16_t tab[HilbertTable<D>::SIZE];
   fill_hilbert_table<D>(tab);
-  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-       r += (uint64_t)gridDim.x * blockDim.x) {
-    float v[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = obj[(uint64_t)e * stride + r];
+  for (uint64_t base = (uint64_t)blockIdx.x * (256 * R); base < n;
+       base += (uint64_t)gridDim.x * (256 * R)) {
+    uint32_t c[R][D];
     bool bad = false;
-    keys[r] = object_key<D, BOX, REF, CENTER>(v, nb, tab, bad);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t i = base + (uint64_t)r * 256 + threadIdx.x;
+      const uint64_t j = i < n ? i : n - 1;
+      float v[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = obj[(uint64_t)e * stride + j];
+      object_point<D, BOX, REF, CENTER>(v, c[r], bad);
+    }
+    uint64_t k[R];
+    hilbert_keys_nb<D, key_nb<D, REF>(), R>(c, tab, k);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t i = base + (uint64_t)r * 256 + threadIdx.x;
+      if (i < n) keys[i] = k[r];
+    }
   }
 }
 
@@ -537,9 +539,10 @@ static inline unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 655
 hipError_t launch_keys(const float* objs, uint64_t n, unsigned D, bool box, bool ref,
                        bool center, unsigned nb, uint64_t* keys, uint32_t* idx,
                        unsigned long long* key_or, uint32_t* bad, hipStream_t s) {
+  if (nb != (ref ? (D == 2 ? 31u : 20u) : 16u)) return hipErrorInvalidValue;
   unsigned g = grid_for((n + kKeysPerThread - 1) / kKeysPerThread, 256, 8192);
 #define QCP_KEYS(DD, BB, RR, CC) \
-  k_keys<DD, BB, RR, CC><<<g, 256, 0, s>>>(objs, n, nb, keys, idx, key_or, bad)
+  k_keys<DD, BB, RR, CC><<<g, 256, 0, s>>>(objs, n, keys, idx, key_or, bad)
   if (D == 2) {
     if (box) {
       if (ref) { if (center) QCP_KEYS(2, true, true, true); else QCP_KEYS(2, true, true, false); }
@@ -1034,9 +1037,10 @@ hipError_t launch_keys_sorted(const float* obj, uint64_t stride, uint64_t n, uns
                               bool box, bool ref, bool center, unsigned nb, uint64_t* keys,
                               hipStream_t s) {
   if (n == 0) return hipSuccess;
-  unsigned g = grid_for(n, 256, 4096);
+  if (nb != (ref ? (D == 2 ? 31u : 20u) : 16u)) return hipErrorInvalidValue;
+  unsigned g = grid_for((n + kKeysPerThread - 1) / kKeysPerThread, 256, 4096);
 #define QCP_KS(DD, BB, RR, CC) \
-  k_keys_sorted<DD, BB, RR, CC><<<g, 256, 0, s>>>(obj, stride, n, nb, keys)
+  k_keys_sorted<DD, BB, RR, CC><<<g, 256, 0, s>>>(obj, stride, n, keys)
   if (D == 2) {
     if (box) {
       if (ref) { if (center) QCP_KS(2, true, true, true); else QCP_KS(2, true, true, false); }

@@ -141,13 +141,14 @@ QCP_HD uint64_t hilbert_key_table(const uint32_t* c, unsigned nb, const uint16_t
   return hilbert_finish<D>(index, nb);
 }
 
-// R keys at once: the same walk for R points side by side, so that the R
-// chains of dependent table lookups overlap (each step issues R independent
-// LDS reads).  out[r] equals hilbert_key_table<D>(c[r], nb, tab).
-template <int D, int R>
-QCP_HD void hilbert_keys_table(const uint32_t (&c)[R][D], unsigned nb, const uint16_t* tab,
-                               uint64_t (&out)[R]) {
+// R keys of NB bits a coordinate at once: the same walk for R points side by
+// side, so that the R chains of dependent table lookups overlap (each step
+// issues R independent LDS reads), with every shift a constant.  out[r]
+// equals hilbert_key_table<D>(c[r], NB, tab).
+template <int D, int NB, int R>
+QCP_HD void hilbert_keys_nb(const uint32_t (&c)[R][D], const uint16_t* tab, uint64_t (&out)[R]) {
   constexpr int L = HilbertTable<D>::L, IN = HilbertTable<D>::IN;
+  constexpr int TOP = NB - NB % L;
   uint64_t m[R];
   uint32_t st[R];  // s | rot << D
 #pragma unroll
@@ -156,9 +157,8 @@ QCP_HD void hilbert_keys_table(const uint32_t (&c)[R][D], unsigned nb, const uin
     out[r] = 0;
     st[r] = 0;
   }
-  int b = (int)nb - 1;
-  const int top = (int)nb - (int)(nb % L);
-  for (; b >= top; --b) {
+#pragma unroll
+  for (int b = NB - 1; b >= TOP; --b) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       uint32_t s = st[r] & ((1u << D) - 1u);
@@ -168,7 +168,8 @@ QCP_HD void hilbert_keys_table(const uint32_t (&c)[R][D], unsigned nb, const uin
       st[r] = s | (rot << D);
     }
   }
-  for (; b >= 0; b -= L) {
+#pragma unroll
+  for (int b = TOP - 1; b >= 0; b -= L) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint32_t in = (uint32_t)(m[r] >> (D * (b - L + 1))) & ((1u << IN) - 1u);
@@ -178,7 +179,7 @@ QCP_HD void hilbert_keys_table(const uint32_t (&c)[R][D], unsigned nb, const uin
     }
   }
 #pragma unroll
-  for (int r = 0; r < R; ++r) out[r] = hilbert_finish<D>(out[r], nb);
+  for (int r = 0; r < R; ++r) out[r] = hilbert_finish<D>(out[r], NB);
 }
 
 // Quantiser of src/mapper/hilbert_mapper.cpp:22 -- (ll)(1000000*p), one FP32

@@ -384,8 +384,9 @@ constexpr int kXcds = 8;             // workgroup i runs on XCD i % 8
 // Donate up to `want` of the unvisited children of one frame to the device
 // queue, taking the highest-numbered ones (the donor keeps the next siblings):
 // each becomes an item (query, child node).  Children that do not fit in the
-// slot array stay in `m`.  pending is raised before the slots are written, and
-// the donor's own item keeps pending above zero meanwhile.
+// slot array stay in `m`.  pending is raised (and the raise performed) before
+// the slots are written, and the donor's own item keeps pending above zero
+// meanwhile.
 template <int MW, bool EXPL>
 __device__ __forceinline__ void donate(const SearchArgs& a, uint32_t q, uint32_t node, int level,
                                        Mask<MW>& m, uint32_t want) {
@@ -404,6 +405,12 @@ __device__ __forceinline__ void donate(const SearchArgs& a, uint32_t q, uint32_t
     atomicAdd(&a.ctl->pending, ok);
     atomicAdd(&a.ctl->donated, (unsigned long long)ok);
   }
+  // The increment must be performed before any of the items can be seen: a
+  // taker that finishes one decrements `pending`, and if that decrement came
+  // first while the donor's own item was the last one outstanding, pending
+  // would reach zero, `done` would end the search and donated items would be
+  // lost.  The release fence waits for lane 0's atomics before the slot stores.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   uint32_t cbase;
   if (EXPL) {
     cbase = a.nfirst[node];
