@@ -965,6 +965,23 @@ def main():
                                                       np.array_equal(hres["hashes"], got["hashes"]))})
             log("hybrid (hand-off %s): %.3e queries/s, %.1f ms" %
                 (level or "leaves", q / (hst["total_ms"] * 1e-3), hst["total_ms"]))
+        # the tuned split: the library's co-processing tuner on the sample, then
+        # the whole batch with its choice
+        hkw, hrows = qcp.autotune_hybrid(ix, queries[: min(q, 20000)], cpu_threads=hthreads)
+        b.run(**hkw)
+        hst = b.run(**hkw)
+        hres = b.results()
+        hybrid.append({"handoff_level": ("tuned: %s" % (hkw["handoff_level"] or "leaves")),
+                       "hybrid_batch": hkw["hybrid_batch"],
+                       "queries_per_s": q / (hst["total_ms"] * 1e-3),
+                       "total_ms": hst["total_ms"], "device_ms": hst["kernel_ms"],
+                       "launches": hst["launches"], "cpu_threads": hthreads,
+                       "tuner_entries": len(hrows),
+                       "matches_persistent": bool(np.array_equal(hres["counts"], got["counts"]) and
+                                                  np.array_equal(hres["hashes"], got["hashes"]))})
+        log("hybrid (tuned: hand-off %s, block %d): %.3e queries/s, %.1f ms" %
+            (hkw["handoff_level"] or "leaves", hkw["hybrid_batch"], q / (hst["total_ms"] * 1e-3),
+             hst["total_ms"]))
     plain = None
     if not args.no_plain:
         # the reference's scheme on the same 100,000 queries: one launch per

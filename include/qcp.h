@@ -309,6 +309,31 @@ qcp_status qcp_autotune(const qcp_build_cfg *cfg, const float *d_objs, uint64_t 
                         qcp_search_cfg *best, qcp_tune_entry *table, uint32_t table_cap,
                         uint32_t *table_len);
 
+/* Co-processing policy tuner.  The reference hand-tunes its CPU-GPU split:
+ * the scan level (-l), the pipeline chunk (-c) and its chunk heuristic
+ * (src/tree/hybrid.cpp:833-876, src/evaluator/evaluator.cpp:174-178).  This
+ * times the hybrid form (QCP_SEARCH_HYBRID) of q device-resident sample
+ * queries over ix for every hand-off level and every pipeline block in
+ * {4096, 16384, 65536} (blocks larger than the sample are skipped).  The
+ * levels are 1 .. height-2 and the leaves (0) for packed layouts, and depths
+ * 1 .. min(height-2, 6) and the leaves for explicit ones.  The other fields
+ * come from base (null = defaults: all host cores, counts + hash).  best
+ * receives base with the (handoff_level, hybrid_batch) of the lowest wall
+ * time, host traversal included; table (optional) receives every
+ * measurement. */
+typedef struct {
+  uint32_t handoff_level;  /* 0 = the leaves (the reference's split)            */
+  uint32_t hybrid_batch;   /* queries per pipeline block                        */
+  double total_ms;         /* best of two timed runs after a warm-up, wall      */
+  double device_ms;        /* device time of that run                           */
+  uint64_t launches;       /* search launches of that run                       */
+} qcp_hybrid_tune_entry;
+
+qcp_status qcp_autotune_hybrid(qcp_index *ix, const float *d_queries, uint64_t q,
+                               const qcp_search_cfg *base, qcp_search_cfg *best,
+                               qcp_hybrid_tune_entry *table, uint32_t table_cap,
+                               uint32_t *table_len);
+
 /* ---- serving: continuous batching over a resident index -----------------------
  * New (the reference has no serving path; its closest interface is the
  * per-batch MPHR::Search loop, src/tree/mphr.cpp:231-258).  A server keeps the

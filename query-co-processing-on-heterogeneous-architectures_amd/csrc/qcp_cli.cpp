@@ -24,7 +24,8 @@
 //   -l L      co-processing hand-off level (scan_level)
 //   -c C      co-processing pipeline block, in queries (chunk_size)
 //   -r R      repeats
-//   -e E      evaluation mode: nonzero runs the autotuner (the reference's sweep)
+//   -e E      evaluation mode: nonzero runs the autotuner (the reference's sweep);
+//             with the hybrid form and no -l / -c, also the co-processing tuner
 //   -s S      selectivity of --gen-queries (volume fraction)
 //   -u U      cluster type (GetClusterType, 431-446): h|hilbert (default),
 //             k|kmeans (k-means clusters ranked by their centres' Hilbert keys,
@@ -251,6 +252,9 @@ int main(int argc, char** argv) {
 
   // ---- data and index ------------------------------------------------------
   const uint32_t E = o.boxes ? 2 * D : D;
+  qcp_search_cfg tuned;  // the autotuner's choice (-e)
+  memset(&tuned, 0, sizeof(tuned));
+  bool have_tuned = false;
   qcp_index* ix = nullptr;
   qcp_status st;
   struct stat sb;
@@ -285,10 +289,9 @@ int main(int argc, char** argv) {
     const bool original = input_order || ct == "o" || ct == "original";
     if (ct == "k" || ct == "kmeans") cfg.order = QCP_ORDER_KMEANS_HILBERT;
     else if (!original && ct != "h" && ct != "hilbert") { usage(); return 2; }
-    qcp_search_cfg tuned;
-    memset(&tuned, 0, sizeof(tuned));
     if (o.eval) {
       // the reference's parameter sweep: the autotuner on the first queries
+      have_tuned = true;
       std::vector<float> qs;
       const uint64_t nq = std::min<uint64_t>(o.q ? o.q : file_floats(o.queries) / (2 * D), 20000);
       if (!read_floats(o.queries, nq * 2 * D, qs)) return fail("read queries", QCP_EIO);
@@ -305,13 +308,35 @@ int main(int argc, char** argv) {
                         QCP_OUT_COUNTS | QCP_OUT_HASH, &ix, &tuned, table.data(),
                         (uint32_t)table.size(), &tl);
       qcp_device_free(d_objs);
-      qcp_device_free(d_q);
-      if (st != QCP_OK) return fail("autotune", st);
+      if (st != QCP_OK) {
+        qcp_device_free(d_q);
+        return fail("autotune", st);
+      }
       qcp_index_info inf;
       qcp_index_info_get(ix, &inf);
       printf("autotune: fanout %u, %u waves per workgroup, %u leaf chunks, donation %s "
              "(%u configurations)\n", inf.fanout, tuned.block_waves, tuned.leaf_chunks,
              tuned.donate ? "on" : "off", tl);
+      if (smode == QCP_SEARCH_HYBRID && !o.scan_level && !o.chunk) {
+        // the co-processing split (-l, -c not given): the tuner picks both
+        qcp_search_cfg base = tuned;
+        base.cpu_threads = o.threads;
+        base.contain = o.contain ? 1 : 0;
+        std::vector<qcp_hybrid_tune_entry> ht(256);
+        uint32_t hl = 0;
+        st = qcp_autotune_hybrid(ix, (const float*)d_q, nq, &base, &tuned, ht.data(),
+                                 (uint32_t)ht.size(), &hl);
+        if (st != QCP_OK) {
+          qcp_device_free(d_q);
+          return fail("hybrid autotune", st);
+        }
+        o.scan_level = tuned.handoff_level;
+        o.chunk = tuned.hybrid_batch;
+        printf("hybrid autotune: hand-off level %u%s, %u queries a block (%u configurations)\n",
+               tuned.handoff_level, tuned.handoff_level ? "" : " (the leaves)",
+               tuned.hybrid_batch, hl);
+      }
+      qcp_device_free(d_q);
     } else if ((st = original ? qcp_build_ranked(&cfg, objs.data(), o.n, &ix)
                               : qcp_build(&cfg, objs.data(), o.n, &ix)) != QCP_OK) {
       return fail("build", st);
@@ -344,6 +369,11 @@ int main(int argc, char** argv) {
   sc.cpu_threads = o.threads;
   sc.handoff_level = o.scan_level;
   sc.hybrid_batch = o.chunk;
+  if (have_tuned) {
+    sc.block_waves = tuned.block_waves;
+    sc.leaf_chunks = tuned.leaf_chunks;
+    sc.donate = tuned.donate;
+  }
   qcp_search_stats ss;
   double best_ms = 1e30;
   for (uint32_t r = 0; r < std::max(1u, o.repeat); ++r) {

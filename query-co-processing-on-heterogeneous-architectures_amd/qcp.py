@@ -78,6 +78,12 @@ class TuneEntry(ctypes.Structure):
                 ("kernel_ms", ctypes.c_double)]
 
 
+class HybridTuneEntry(ctypes.Structure):
+    _fields_ = [("handoff_level", ctypes.c_uint32), ("hybrid_batch", ctypes.c_uint32),
+                ("total_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
+                ("launches", ctypes.c_uint64)]
+
+
 class ServeCfg(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("result_budget", ctypes.c_uint64), ("max_requests", ctypes.c_uint32),
@@ -137,6 +143,10 @@ def lib():
         L.qcp_batch_total_hits.restype = ctypes.c_uint64
         L.qcp_search.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(SearchCfg), P, P, P, P,
                                  ctypes.c_uint64, ctypes.POINTER(SearchStats)]
+        L.qcp_autotune_hybrid.argtypes = [P, P, ctypes.c_uint64, ctypes.POINTER(SearchCfg),
+                                          ctypes.POINTER(SearchCfg),
+                                          ctypes.POINTER(HybridTuneEntry), ctypes.c_uint32,
+                                          ctypes.POINTER(ctypes.c_uint32)]
         L.qcp_autotune.argtypes = [ctypes.POINTER(BuildCfg), P, ctypes.c_uint64, P,
                                    ctypes.c_uint64, P, ctypes.c_uint32, ctypes.c_uint32,
                                    ctypes.POINTER(P), ctypes.POINTER(SearchCfg),
@@ -491,6 +501,40 @@ def autotune(objs, dims, sample_queries, fanouts=(32, 64, 128), layout="mphr", k
     rows = [{"fanout": e.fanout, "block_waves": e.block_waves, "leaf_chunks": e.leaf_chunks,
              "donate": e.donate, "kernel_ms": e.kernel_ms} for e in table[: tlen.value]]
     return ix, kw, rows
+
+
+def autotune_hybrid(index, sample_queries, cpu_threads=0, hash=True, block_waves=0, chunks=0,
+                    donate=0, contain=False, device=0):
+    """Run the library's co-processing tuner on `index` (on `device`): returns
+    (dict of Batch.run keyword arguments for the fastest hybrid split, list of
+    all measurements)."""
+    qs = np.ascontiguousarray(sample_queries, np.float32).reshape(-1, 2 * index.D)
+    qbuf = DeviceBuffer(qs, device)
+    base = SearchCfg()
+    base.mode = 2
+    base.outputs = OUT_COUNTS | (OUT_HASH if hash else 0)
+    base.block_waves = block_waves
+    base.leaf_chunks = chunks
+    base.donate = donate
+    base.contain = 1 if contain else 0
+    base.cpu_threads = cpu_threads
+    best = SearchCfg()
+    cap = 256
+    table = (HybridTuneEntry * cap)()
+    tlen = ctypes.c_uint32()
+    try:
+        check(lib().qcp_autotune_hybrid(index._h, qbuf.ptr, len(qs), ctypes.byref(base),
+                                        ctypes.byref(best), table, cap, ctypes.byref(tlen)),
+              "qcp_autotune_hybrid")
+    finally:
+        qbuf.close()
+    kw = {"mode": "hybrid", "handoff_level": best.handoff_level, "hybrid_batch": best.hybrid_batch,
+          "cpu_threads": cpu_threads, "block_waves": block_waves, "chunks": chunks,
+          "donate": donate, "contain": contain}
+    rows = [{"handoff_level": e.handoff_level, "hybrid_batch": e.hybrid_batch,
+             "total_ms": e.total_ms, "device_ms": e.device_ms, "launches": e.launches}
+            for e in table[: tlen.value]]
+    return kw, rows
 
 
 class Server:

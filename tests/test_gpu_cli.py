@@ -82,6 +82,19 @@ def test_cli_index_file_and_autotune(gpu, O, files):
         assert np.array_equal(c, exp["counts"]) and np.array_equal(h, exp["hashes"])
 
 
+def test_cli_hybrid_autotune(gpu, O, files):
+    """-i h -e 1: the autotuner, then the co-processing tuner picks the hand-off
+    level and pipeline block (no -l / -c given); results equal the oracle's."""
+    d, data, queries, pts, q = files
+    out = str(d / "oh.txt")
+    log = run("-i", "h", "--data", data, "--queries", queries, "--dims", "3", "-e", "1",
+              "-p", "4", "--out", out)
+    assert "autotune:" in log and "hybrid autotune: hand-off level" in log, log
+    exp = O.build_packed(pts, 3, False, 64, O.KEY_REF).search_dfs(q)
+    c, h = read_out(out)
+    assert np.array_equal(c, exp["counts"]) and np.array_equal(h, exp["hashes"])
+
+
 def test_cli_reference_dump(gpu, O, files):
     """--ref-dump indexes a reference MPHR dump's objects in its rank order."""
     import refdump

@@ -68,3 +68,26 @@ def test_hybrid_rejects_ids_and_handles_empty(gpu):
         ix.search(rand_queries(10, 2, 2), mode="hybrid", ids=True)
     r = ix.search(np.zeros((0, 4), np.float32), mode="hybrid")
     assert len(r["counts"]) == 0
+
+
+@pytest.mark.parametrize("layout", ["mphr", "bvh"])
+def test_hybrid_tuner(gpu, O, layout):
+    """The co-processing tuner times every hand-off level (the leaves as 0)
+    and pipeline block that fits the sample; its choice is one of the timed
+    entries, the fastest by wall time, and searches with it equal the oracle."""
+    D, F = 3, 32
+    pts = rand_points(80000, D, 12)
+    q = rand_queries(6000, D, 13, pts)
+    ix = gpu.Index.build(pts, D, fanout=F, layout=layout)
+    kw, rows = gpu.autotune_hybrid(ix, q, cpu_threads=4)
+    h = ix.info()["height"]
+    levels = sorted({r["handoff_level"] for r in rows})
+    want = list(range(1, h - 1)) if layout == "mphr" else list(range(1, min(h - 2, 6) + 1))
+    assert levels == sorted([0] + want), (levels, h)
+    assert {r["hybrid_batch"] for r in rows} == {4096, 16384}  # 65536 exceeds the sample
+    best = min(rows, key=lambda r: r["total_ms"])
+    assert (kw["handoff_level"], kw["hybrid_batch"]) == (best["handoff_level"], best["hybrid_batch"])
+    exp = _oracle(O, pts, D, False, F, layout).search_dfs(q)
+    got = ix.search(q, **kw)
+    assert_bits_equal(got["counts"], exp["counts"], "tuned hybrid counts")
+    assert_bits_equal(got["hashes"], exp["hashes"], "tuned hybrid hashes")
