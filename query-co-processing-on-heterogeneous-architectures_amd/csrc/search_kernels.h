@@ -114,7 +114,8 @@ template <int D>
 struct Ctx {
   float ql[D], qh[D];
   uint32_t q;
-  uint32_t cnt;           // hits of this item
+  uint32_t cnt;           // hits of this item (wave-uniform part)
+  uint32_t cnt_lane;      // per-lane hits of the 4-entries-a-lane leaf path
   uint64_t out_base;      // ID write pass: next output slot
   // ID write pass: IDs are staged in a per-wave LDS window of 2 x 64 slots
   // holding output slots [wbase, wbase + 128), wbase a multiple of 64; this
@@ -136,6 +137,8 @@ enum { kOutCount = 0, kOutHash = 1, kOutIds = 2 };
 // FP32 in the global address space (a pointer rebuilt from an integer is
 // otherwise generic, which would turn its loads into flat loads)
 typedef __attribute__((address_space(1))) float GlobalF32;
+typedef float F32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) F32x4 GlobalF32x4;
 
 // Evaluate the children [cbase, cbase+cc) (node ids) of an internal node
 // against the query; for explicit layouts also their leaf flags.
@@ -272,6 +275,45 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
   for (int e = 0; e < E; ++e) col[e] = a.obj + (uint64_t)e * a.obj_stride;
   uint32_t cs = 0, ce = 0;
   while (true) {
+    if constexpr (OUT != kOutIds && U >= 4) {
+      // counts and hashes: whole groups of 256 entries, four consecutive
+      // entries a lane from one 16-byte load per column, U / 4 groups in
+      // flight (a quarter of the chunk path's load instructions for the same
+      // bytes; the order of the hits does not matter for a count or a sum).
+      // Hits are counted per lane and summed when the item ends.  The run
+      // must start on a 16-byte boundary.
+      constexpr int G = U / 4;
+      if ((cs & 3u) == 0) {
+        while (ce - cs >= (uint32_t)(G * 256)) {
+          F32x4 w[G][E];
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const GlobalF32x4* base = reinterpret_cast<const GlobalF32x4*>(
+                uni64(reinterpret_cast<uint64_t>(col[e] + cs)));
+#pragma unroll
+            for (int g = 0; g < G; ++g) w[g][e] = base[g * kWave + (int)lane];
+          }
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              float v[E];
+#pragma unroll
+              for (int e = 0; e < E; ++e) v[e] = w[g][e][j];
+              const bool hit = overlap_test<D>(c.ql, c.qh, v, BOX ? v + D : v);
+              c.cnt_lane += hit ? 1u : 0u;
+              if (OUT == kOutHash) {
+                const uint32_t id = cs + (uint32_t)(g * 256) + 4u * lane + (uint32_t)j + 1u;
+                const uint32_t x = hit ? id : 0u;
+                c.s1_lane += x;
+                c.s2_lane += (uint64_t)x * x;
+              }
+            }
+          }
+          cs += (uint32_t)(G * 256);
+        }
+      }
+    }
     // fast path: whole groups of U chunks inside the current run, with no
     // per-chunk bounds or refill logic and one base pointer per column per
     // group (the chunks are immediate offsets from it)
@@ -687,6 +729,7 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
     Ctx<D> c;
     c.q = q;
     c.cnt = 0;
+    c.cnt_lane = 0;
     c.s1_lane = c.s2_lane = 0;
     c.s1c = c.s2c = 0;
     c.cr = 0;
@@ -706,6 +749,7 @@ __global__ __launch_bounds__(512) void k_search(SearchArgs a) {
       }
     }
     if (OUT == kOutIds) flush_partial<D>(a, c);
+    if (OUT != kOutIds) c.cnt += (uint32_t)wave_sum_u64(c.cnt_lane);
     if (OUT != kOutIds) {
       // the hash is additive over disjoint ID sets, so donated parts add up
       uint64_t hsum = 0;
