@@ -16,6 +16,13 @@
 #else
 #define QCP_HD inline
 #endif
+// loop unrolling hint for the device compiler (host compilers other than
+// clang do not know the pragma)
+#if defined(__clang__)
+#define QCP_UNROLL _Pragma("unroll")
+#else
+#define QCP_UNROLL
+#endif
 
 namespace qcp {
 
@@ -59,7 +66,7 @@ QCP_HD uint64_t hilbert_key(const uint32_t* c, unsigned nb) {
   unsigned rot = 0;
   for (int b = (int)nb - 1; b >= 0; --b) {
     uint32_t t = 0;
-#pragma unroll
+QCP_UNROLL
     for (int d = 0; d < D; ++d) t |= ((c[d] >> b) & 1u) << d;
     index = (index << D) | hilbert_level<D>(s, rot, t);
   }
@@ -151,15 +158,15 @@ QCP_HD void hilbert_keys_nb(const uint32_t (&c)[R][D], const uint16_t* tab, uint
   constexpr int TOP = NB - NB % L;
   uint64_t m[R];
   uint32_t st[R];  // s | rot << D
-#pragma unroll
+QCP_UNROLL
   for (int r = 0; r < R; ++r) {
     m[r] = interleave<D>(c[r]);
     out[r] = 0;
     st[r] = 0;
   }
-#pragma unroll
+QCP_UNROLL
   for (int b = NB - 1; b >= TOP; --b) {
-#pragma unroll
+QCP_UNROLL
     for (int r = 0; r < R; ++r) {
       uint32_t s = st[r] & ((1u << D) - 1u);
       unsigned rot = st[r] >> D;
@@ -168,9 +175,9 @@ QCP_HD void hilbert_keys_nb(const uint32_t (&c)[R][D], const uint16_t* tab, uint
       st[r] = s | (rot << D);
     }
   }
-#pragma unroll
+QCP_UNROLL
   for (int b = TOP - 1; b >= 0; b -= L) {
-#pragma unroll
+QCP_UNROLL
     for (int r = 0; r < R; ++r) {
       const uint32_t in = (uint32_t)(m[r] >> (D * (b - L + 1))) & ((1u << IN) - 1u);
       const uint32_t e = tab[(st[r] << IN) | in];
@@ -178,7 +185,7 @@ QCP_HD void hilbert_keys_nb(const uint32_t (&c)[R][D], const uint16_t* tab, uint
       st[r] = e >> 8;
     }
   }
-#pragma unroll
+QCP_UNROLL
   for (int r = 0; r < R; ++r) out[r] = hilbert_finish<D>(out[r], NB);
 }
 
