@@ -62,6 +62,14 @@ struct DevBuf {
   bool alloc(uint64_t n) { return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)) == hipSuccess; }
 };
 
+struct PinnedU32 {
+  uint32_t* p = nullptr;
+  ~PinnedU32() {
+    if (p) hipHostFree(p);
+  }
+  bool alloc(uint64_t n) { return hipHostMalloc((void**)&p, 4 * (n ? n : 1), 0) == hipSuccess; }
+};
+
 struct Events {
   hipEvent_t e[2] = {nullptr, nullptr};
   bool create() {
@@ -141,52 +149,75 @@ class Pool {
 // G at a time (partials G x K x D, bounded to about 256 MiB); within a window
 // each chunk's partials are formed in parallel, then added to the running sums
 // in chunk order, clusters in parallel.  Only clusters present in a chunk are
-// added (adding a chunk's +0.0 partial would leave the sum unchanged).
-void update_centres(Pool& pool, unsigned threads, const float* pts, uint64_t n, uint32_t D,
-                    const uint32_t* cid, uint32_t K, float* centers) {
-  const uint64_t nchunks = (n + kChunk - 1) / kChunk;
-  const uint64_t per_chunk = (uint64_t)K * (8ull * D + 4);
-  const uint64_t G = std::max<uint64_t>(
-      1, std::min<uint64_t>({nchunks, 2ull * threads, (256ull << 20) / per_chunk}));
-  std::vector<double> part(G * K * D, 0.0), sum((size_t)K * D, 0.0);
-  std::vector<uint32_t> cnt(G * K, 0);
-  std::vector<uint64_t> total(K, 0);
-  std::vector<std::vector<uint32_t>> touched(G);
-  const uint64_t kblk = 1024;
-  for (uint64_t w0 = 0; w0 < nchunks; w0 += G) {
-    const uint64_t w = std::min(G, nchunks - w0);
-    pool.run(w, [&](uint64_t g) {
-      double* ps = part.data() + g * K * D;
-      uint32_t* cs = cnt.data() + g * K;
-      std::vector<uint32_t>& tl = touched[g];
-      const uint64_t b = (w0 + g) * kChunk, e = std::min(n, b + kChunk);
-      for (uint64_t i = b; i < e; ++i) {
-        const uint32_t k = cid[i];
-        if (cs[k]++ == 0) tl.push_back(k);
-        for (uint32_t d = 0; d < D; ++d) ps[k * D + d] += (double)pts[i * D + d];
-      }
-    });
-    pool.run((K + kblk - 1) / kblk, [&](uint64_t kb) {
-      const uint64_t k1 = std::min<uint64_t>(K, (kb + 1) * kblk);
-      for (uint64_t g = 0; g < w; ++g)
-        for (uint64_t k = kb * kblk; k < k1; ++k)
-          if (cnt[g * K + k]) {
-            total[k] += cnt[g * K + k];
-            for (uint32_t d = 0; d < D; ++d) sum[k * D + d] += part[(g * K + k) * D + d];
-          }
-    });
-    pool.run(w, [&](uint64_t g) {
-      for (const uint32_t k : touched[g]) {
-        cnt[g * K + k] = 0;
-        for (uint32_t d = 0; d < D; ++d) part[(g * K + k) * D + d] = 0.0;
-      }
-      touched[g].clear();
-    });
+// added (adding a chunk's +0.0 partial would leave the sum unchanged), and only
+// their partials are cleared again, so the buffers are allocated and zeroed
+// once for all iterations.
+class CentreUpdate {
+ public:
+  CentreUpdate(unsigned threads, uint64_t n, uint32_t D, uint32_t K) : n_(n), D_(D), K_(K) {
+    nchunks_ = (n + kChunk - 1) / kChunk;
+    const uint64_t per_chunk = (uint64_t)K * (8ull * D + 4);
+    G_ = std::max<uint64_t>(
+        1, std::min<uint64_t>({nchunks_, 2ull * threads, (256ull << 20) / per_chunk}));
+    part_.assign(G_ * K * D, 0.0);
+    cnt_.assign(G_ * K, 0);
+    sum_.resize((size_t)K * D);
+    total_.resize(K);
+    // the clusters present in each chunk of the window: at most min(K, chunk)
+    // each, reserved here so that the workers allocate nothing
+    touched_.resize(G_);
+    for (auto& t : touched_) t.reserve(std::min<uint64_t>(K, kChunk));
   }
-  for (uint32_t k = 0; k < K; ++k)
-    if (total[k])
-      for (uint32_t d = 0; d < D; ++d) centers[k * D + d] = (float)(sum[k * D + d] / (double)total[k]);
-}
+
+  void run(Pool& pool, const float* pts, const uint32_t* cid, float* centers) {
+    const uint64_t K = K_, D = D_, n = n_;
+    std::fill(sum_.begin(), sum_.end(), 0.0);
+    std::fill(total_.begin(), total_.end(), 0);
+    const uint64_t kblk = 1024;
+    for (uint64_t w0 = 0; w0 < nchunks_; w0 += G_) {
+      const uint64_t w = std::min(G_, nchunks_ - w0);
+      pool.run(w, [&](uint64_t g) {
+        double* ps = part_.data() + g * K * D;
+        uint32_t* cs = cnt_.data() + g * K;
+        std::vector<uint32_t>& tl = touched_[g];
+        const uint64_t b = (w0 + g) * kChunk, e = std::min(n, b + kChunk);
+        for (uint64_t i = b; i < e; ++i) {
+          const uint32_t k = cid[i];
+          if (cs[k]++ == 0) tl.push_back(k);
+          for (uint64_t d = 0; d < D; ++d) ps[k * D + d] += (double)pts[i * D + d];
+        }
+      });
+      pool.run((K + kblk - 1) / kblk, [&](uint64_t kb) {
+        const uint64_t k1 = std::min<uint64_t>(K, (kb + 1) * kblk);
+        for (uint64_t g = 0; g < w; ++g)
+          for (uint64_t k = kb * kblk; k < k1; ++k)
+            if (cnt_[g * K + k]) {
+              total_[k] += cnt_[g * K + k];
+              for (uint64_t d = 0; d < D; ++d) sum_[k * D + d] += part_[(g * K + k) * D + d];
+            }
+      });
+      pool.run(w, [&](uint64_t g) {
+        for (const uint32_t k : touched_[g]) {
+          cnt_[g * K + k] = 0;
+          for (uint64_t d = 0; d < D; ++d) part_[(g * K + k) * D + d] = 0.0;
+        }
+        touched_[g].clear();
+      });
+    }
+    for (uint64_t k = 0; k < K; ++k)
+      if (total_[k])
+        for (uint64_t d = 0; d < D; ++d)
+          centers[k * D + d] = (float)(sum_[k * D + d] / (double)total_[k]);
+  }
+
+ private:
+  uint64_t n_, nchunks_, G_;
+  uint32_t D_, K_;
+  std::vector<double> part_, sum_;
+  std::vector<uint32_t> cnt_;
+  std::vector<uint64_t> total_;
+  std::vector<std::vector<uint32_t>> touched_;
+};
 
 }  // namespace
 
@@ -236,10 +267,13 @@ qcp_status qcp::kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bi
   if (!d_c.alloc((uint64_t)K * D) || !d_cid.alloc(n) || !d_changes.alloc(1) || !d_rank.alloc(K))
     return QCP_ENOMEM;
   HIP_TRY(launch_fill_u32(d_cid.p, n, 0xFFFFFFFFu, s));
-  std::vector<uint32_t> cid(n);
+  // the clusters come back every iteration: pinned host memory
+  PinnedU32 cid;
+  if (!cid.alloc(n)) return QCP_ENOMEM;
   unsigned hw = std::thread::hardware_concurrency();
   hw = std::max(1u, std::min(hw, 16u));
   Pool pool(hw);
+  CentreUpdate update(hw, n, D, K);
   const uint32_t max_loop = max_iters - 1;
   uint32_t loop = 0, its = 0;
   float delta = 0.0f;
@@ -254,14 +288,14 @@ qcp_status qcp::kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bi
     HIP_TRY(launch_assign(d_objs, n, D, ix->box, center, d_c.p, K, d_cid.p, d_changes.p, s));
     HIP_TRY(hipEventRecord(ev.e[1], s));
     HIP_TRY(hipMemcpyAsync(&changes, d_changes.p, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(cid.data(), d_cid.p, 4ull * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(cid.p, d_cid.p, 4ull * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
     assign_ms += ms;
     ++its;
     const auto t0 = std::chrono::steady_clock::now();
-    update_centres(pool, hw, pts.data(), n, D, cid.data(), K, centers.data());
+    update.run(pool, pts.data(), cid.p, centers.data());
     update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     delta = (float)changes;
     delta /= (float)n;
