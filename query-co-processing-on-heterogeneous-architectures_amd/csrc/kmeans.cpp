@@ -156,7 +156,7 @@ class CentreUpdate {
  public:
   CentreUpdate(unsigned threads, uint64_t n, uint32_t D, uint32_t K) : n_(n), D_(D), K_(K) {
     nchunks_ = (n + kChunk - 1) / kChunk;
-    const uint64_t per_chunk = (uint64_t)K * (8ull * D + 4);
+    const uint64_t per_chunk = std::max<uint64_t>(1, (uint64_t)K * (8ull * D + 4));
     G_ = std::max<uint64_t>(
         1, std::min<uint64_t>({nchunks_, 2ull * threads, (256ull << 20) / per_chunk}));
     part_.assign(G_ * K * D, 0.0);
