@@ -86,17 +86,14 @@ struct Events {
 class Pool {
  public:
   explicit Pool(unsigned threads) {
-    for (unsigned t = 1; t < threads; ++t) th_.emplace_back([this] { loop(); });
-  }
-  ~Pool() {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      stop_ = true;
-      ++gen_;
+    try {
+      for (unsigned t = 1; t < threads; ++t) th_.emplace_back([this] { loop(); });
+    } catch (...) {
+      stop();  // a thread that could not start: end the ones that did
+      throw;
     }
-    cv_.notify_all();
-    for (auto& t : th_) t.join();
   }
+  ~Pool() { stop(); }
   void run(uint64_t count, const std::function<void(uint64_t)>& fn) {
     {
       std::lock_guard<std::mutex> g(mu_);
@@ -120,6 +117,16 @@ class Pool {
       if (i >= count_) return;
       (*fn_)(i);
     }
+  }
+  void stop() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (auto& t : th_)
+      if (t.joinable()) t.join();
   }
   void loop() {
     uint64_t seen = 0;
