@@ -20,15 +20,26 @@
 namespace qcp {
 
 // ---------------------------------------------------------------------------
-// Hilbert keys: one thread per object.  Key source is the lower corner (the
-// reference reads only the first D entries of the branch, src/tree/tree.cpp:600)
-// or the box centre (lo+hi)*0.5f.  Each workgroup first fills an LDS table of
-// the walk's transitions over L levels (hilbert_key.h), then each key takes
-// nb/L lookups instead of nb per-level steps.
-template <int D>
-__device__ __forceinline__ void fill_hilbert_table(uint16_t* tab) {
+// Hilbert keys.  Key source is the lower corner (the reference reads only the
+// first D entries of the branch, src/tree/tree.cpp:600) or the box centre
+// (lo+hi)*0.5f.  Each workgroup first fills LDS tables of the walk's
+// transitions over L levels, indexed by each coordinate's L bits, and over the
+// top nb % L levels from the initial state (hilbert_key.h); a key then takes
+// one start lookup and nb/L table lookups, with no bit interleaving.
+template <int D, int NB>
+struct KeyTables {
+  static constexpr int TOP = NB % HilbertTable<D>::L;
+  uint16_t tab[HilbertTable<D>::SIZE];
+  uint16_t start[1u << (D * (TOP ? TOP : 1))];
+};
+
+template <int D, int NB>
+__device__ __forceinline__ void fill_key_tables(KeyTables<D, NB>& t) {
   for (uint32_t e = threadIdx.x; e < HilbertTable<D>::SIZE; e += blockDim.x)
-    tab[e] = hilbert_table_entry<D>(e);
+    t.tab[e] = hilbert_table_entry_cm<D>(e);
+  if (KeyTables<D, NB>::TOP > 0)
+    for (uint32_t e = threadIdx.x; e < (1u << (D * KeyTables<D, NB>::TOP)); e += blockDim.x)
+      t.start[e] = hilbert_start_entry<D>(KeyTables<D, NB>::TOP, e);
   __syncthreads();
 }
 
@@ -67,8 +78,9 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
                                               unsigned long long* key_or, uint32_t* bad) {
   constexpr int E = BOX ? 2 * D : D;
   constexpr int R = kKeysPerThread;
-  __shared__ uint16_t tab[HilbertTable<D>::SIZE];
-  fill_hilbert_table<D>(tab);
+  constexpr int NB = key_nb<D, REF>();
+  __shared__ KeyTables<D, NB> kt;
+  fill_key_tables<D, NB>(kt);
   uint64_t acc = 0;
   bool any_bad = false, any_inv = false;
   for (uint64_t base = (uint64_t)blockIdx.x * (256 * R); base < n;
@@ -94,7 +106,7 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
       object_point<D, BOX, REF, CENTER>(v[r], c[r], any_bad);
     }
     uint64_t k[R];
-    hilbert_keys_nb<D, key_nb<D, REF>(), R>(c, tab, k);
+    hilbert_keys_cm<D, NB, R>(c, kt.tab, kt.start, k);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint64_t i = base + (uint64_t)r * 256 + threadIdx.x;
@@ -387,8 +399,9 @@ __global__ __launch_bounds__(256) void k_keys_sorted(const float* __restrict__ o
                                                      uint64_t* __restrict__ keys) {
   constexpr int E = BOX ? 2 * D : D;
   constexpr int R = kKeysPerThread;
-  __shared__ uint16_t tab[HilbertTable<D>::SIZE];
-  fill_hilbert_table<D>(tab);
+  constexpr int NB = key_nb<D, REF>();
+  __shared__ KeyTables<D, NB> kt;
+  fill_key_tables<D, NB>(kt);
   for (uint64_t base = (uint64_t)blockIdx.x * (256 * R); base < n;
        base += (uint64_t)gridDim.x * (256 * R)) {
     uint32_t c[R][D];
@@ -403,7 +416,7 @@ __global__ __launch_bounds__(256) void k_keys_sorted(const float* __restrict__ o
       object_point<D, BOX, REF, CENTER>(v, c[r], bad);
     }
     uint64_t k[R];
-    hilbert_keys_nb<D, key_nb<D, REF>(), R>(c, tab, k);
+    hilbert_keys_cm<D, NB, R>(c, kt.tab, kt.start, k);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint64_t i = base + (uint64_t)r * 256 + threadIdx.x;

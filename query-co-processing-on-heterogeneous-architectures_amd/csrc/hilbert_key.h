@@ -148,44 +148,75 @@ QCP_HD uint64_t hilbert_key_table(const uint32_t* c, unsigned nb, const uint16_t
   return hilbert_finish<D>(index, nb);
 }
 
-// R keys of NB bits a coordinate at once: the same walk for R points side by
-// side, so that the R chains of dependent table lookups overlap (each step
-// issues R independent LDS reads), with every shift a constant.  out[r]
-// equals hilbert_key_table<D>(c[r], NB, tab).
-template <int D, int NB, int R>
-QCP_HD void hilbert_keys_nb(const uint32_t (&c)[R][D], const uint16_t* tab, uint64_t (&out)[R]) {
+// The same transition table indexed coordinate-major: entry (s | rot << D)
+// << IN | g_{D-1} << (D-1)L | ... | g_0, where g_d holds coordinate d's L bits
+// of the L levels (top level in the high bit), so a key never needs the
+// coordinates' bits interleaved.  The start table covers the top r = nb % L
+// levels from the walk's initial state: entry g_{D-1} << (D-1)r | ... | g_0
+// (r bits each) holds the D*r index bits (low 8) and the state after them.
+template <int D>
+QCP_HD uint16_t hilbert_table_entry_cm(uint32_t e) {
   constexpr int L = HilbertTable<D>::L, IN = HilbertTable<D>::IN;
-  constexpr int TOP = NB - NB % L;
-  uint64_t m[R];
+  const uint32_t st = e >> IN;
+  uint32_t in = 0;  // level-major digits, top level first
+  for (int l = L - 1; l >= 0; --l) {
+    uint32_t t = 0;
+    for (int d = 0; d < D; ++d) t |= ((e >> (d * L + l)) & 1u) << d;
+    in = (in << D) | t;
+  }
+  return hilbert_table_entry<D>((st << IN) | in);
+}
+
+template <int D>
+QCP_HD uint16_t hilbert_start_entry(int r, uint32_t e) {
+  uint32_t s = 0;
+  unsigned rot = 0;
+  uint32_t out = 0;
+  for (int l = r - 1; l >= 0; --l) {
+    uint32_t t = 0;
+    for (int d = 0; d < D; ++d) t |= ((e >> (d * r + l)) & 1u) << d;
+    out = (out << D) | hilbert_level<D>(s, rot, t);
+  }
+  return (uint16_t)(out | ((s | (rot << D)) << 8));
+}
+
+// R keys of NB bits a coordinate at once, from the coordinate-major table
+// `tab` and the start table `start` (entries for the top NB % L levels; not
+// read when NB is a multiple of L): the R walks run side by side, so that the
+// R chains of dependent table lookups overlap, and every shift and field is
+// a constant.  out[r] equals hilbert_key<D>(c[r], NB).
+template <int D, int NB, int R>
+QCP_HD void hilbert_keys_cm(const uint32_t (&c)[R][D], const uint16_t* tab, const uint16_t* start,
+                            uint64_t (&out)[R]) {
+  constexpr int L = HilbertTable<D>::L, IN = HilbertTable<D>::IN;
+  constexpr int TOP = NB % L;
   uint32_t st[R];  // s | rot << D
-QCP_UNROLL
+  QCP_UNROLL
   for (int r = 0; r < R; ++r) {
-    m[r] = interleave<D>(c[r]);
     out[r] = 0;
     st[r] = 0;
-  }
-QCP_UNROLL
-  for (int b = NB - 1; b >= TOP; --b) {
-QCP_UNROLL
-    for (int r = 0; r < R; ++r) {
-      uint32_t s = st[r] & ((1u << D) - 1u);
-      unsigned rot = st[r] >> D;
-      out[r] = (out[r] << D) |
-               hilbert_level<D>(s, rot, (uint32_t)(m[r] >> (D * b)) & ((1u << D) - 1u));
-      st[r] = s | (rot << D);
+    if (TOP > 0) {
+      uint32_t g = 0;
+      QCP_UNROLL
+      for (int d = 0; d < D; ++d) g |= ((c[r][d] >> (NB - TOP)) & ((1u << TOP) - 1u)) << (d * TOP);
+      const uint32_t e = start[g];
+      out[r] = e & 0xFFu;
+      st[r] = e >> 8;
     }
   }
-QCP_UNROLL
-  for (int b = TOP - 1; b >= 0; b -= L) {
-QCP_UNROLL
+  QCP_UNROLL
+  for (int b = NB - TOP - L; b >= 0; b -= L) {
+    QCP_UNROLL
     for (int r = 0; r < R; ++r) {
-      const uint32_t in = (uint32_t)(m[r] >> (D * (b - L + 1))) & ((1u << IN) - 1u);
-      const uint32_t e = tab[(st[r] << IN) | in];
+      uint32_t g = st[r] << IN;
+      QCP_UNROLL
+      for (int d = 0; d < D; ++d) g |= ((c[r][d] >> b) & ((1u << L) - 1u)) << (d * L);
+      const uint32_t e = tab[g];
       out[r] = (out[r] << IN) | (e & 0xFFu);
       st[r] = e >> 8;
     }
   }
-QCP_UNROLL
+  QCP_UNROLL
   for (int r = 0; r < R; ++r) out[r] = hilbert_finish<D>(out[r], NB);
 }
 
@@ -197,12 +228,9 @@ QCP_HD bool quant_ref(float p, unsigned nb, uint32_t* out) {
     *out = 0;
     return false;
   }
-  long long v = (long long)x;
-  if (v >= (1ll << nb)) {
-    *out = 0;
-    return false;
-  }
-  *out = (uint32_t)v;
+  // 0 <= x < 2^nb <= 2^31: the 32-bit conversion truncates exactly as the
+  // reference's (ll) cast does
+  *out = (uint32_t)x;
   return true;
 }
 

@@ -1,9 +1,12 @@
-// CPU check of the library's table-driven Hilbert key (csrc/hilbert_key.h,
-// hilbert_key_table, the form the GPU key kernel runs) against the oracle's
-// restatement of HilbertMapper::MappingIntoSingle (orc_hilbert_c2i) and the
-// library's per-level walk (hilbert_key): every point of small grids, then
-// random and edge coordinates at every bit width (D = 2: nb 2..31; D = 3:
-// nb 2..21).  Also checks the interleave against bit-by-bit placement.
+// CPU check of the library's table-driven Hilbert keys (csrc/hilbert_key.h)
+// against the oracle's restatement of HilbertMapper::MappingIntoSingle
+// (orc_hilbert_c2i) and the library's per-level walk (hilbert_key): the
+// level-major table form (hilbert_key_table) on every point of small grids,
+// then random and edge coordinates at every bit width (D = 2: nb 2..31;
+// D = 3: nb 2..21); the interleave against bit-by-bit placement; and the form
+// the GPU key kernels run (hilbert_keys_cm: coordinate-major table, start
+// table for the top nb % L levels, four keys side by side) at the key widths
+// they use and at widths that leave every remainder of nb % L.
 // Prints "ok <cases>" and exits 0, or names the first mismatch and exits 1.
 #include <stdint.h>
 #include <stdio.h>
@@ -87,9 +90,65 @@ static bool run(unsigned max_nb, unsigned exhaustive_nb) {
   return true;
 }
 
+template <int D>
+static std::vector<uint16_t> make_table_cm() {
+  std::vector<uint16_t> t(qcp::HilbertTable<D>::SIZE);
+  for (uint32_t e = 0; e < t.size(); ++e) t[e] = qcp::hilbert_table_entry_cm<D>(e);
+  return t;
+}
+
+// hilbert_keys_cm<D, NB, 4> on random, edge and (for small NB) every point.
+template <int D, int NB>
+static bool run_cm() {
+  constexpr int L = qcp::HilbertTable<D>::L;
+  constexpr int TOP = NB % L;
+  const std::vector<uint16_t> tab = make_table_cm<D>();
+  std::vector<uint16_t> start(1u << (D * (TOP ? TOP : 1)));
+  for (uint32_t e = 0; e < start.size(); ++e) start[e] = qcp::hilbert_start_entry<D>(TOP, e);
+  const uint32_t lim = (uint32_t)((1ull << NB) - 1);
+  std::mt19937_64 rng(99 + D * 100 + NB);
+  std::vector<uint32_t> pts;  // D coordinates per point
+  const uint32_t edge[] = {0u, 1u, 2u, lim >> 1, (lim >> 1) + 1u, lim - 1u, lim, 999999u & lim,
+                           524288u & lim};
+  for (uint32_t a : edge)
+    for (uint32_t b : edge)
+      for (uint32_t z : edge) {
+        const uint32_t c[3] = {a, b, z};
+        for (int d = 0; d < D; ++d) pts.push_back(c[d]);
+      }
+  for (int i = 0; i < 40000; ++i)
+    for (int d = 0; d < D; ++d) pts.push_back((uint32_t)(rng() & lim) & (i & 1 ? 0xFFFFFu : ~0u));
+  const size_t n = pts.size() / D;
+  for (size_t i = 0; i < n; i += 4) {
+    uint32_t c[4][D];
+    for (int r = 0; r < 4; ++r)
+      for (int d = 0; d < D; ++d) c[r][d] = pts[((i + r) % n) * D + d];
+    uint64_t got[4];
+    qcp::hilbert_keys_cm<D, NB, 4>(c, tab.data(), start.data(), got);
+    for (int r = 0; r < 4; ++r) {
+      uint64_t c64[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d) c64[d] = c[r][d];
+      const uint64_t want = orc_hilbert_c2i(D, NB, c64);
+      ++cases;
+      if (got[r] != want) {
+        printf("mismatch (coordinate-major) D=%d nb=%d c=(%u,%u,%u): %llx want %llx\n", D, NB,
+               c[r][0], c[r][1], D > 2 ? c[r][D - 1] : 0u, (unsigned long long)got[r],
+               (unsigned long long)want);
+        return false;
+      }
+    }
+  }
+  return true;
+}
+
 int main() {
   if (!run<2>(31, 6)) return 1;
   if (!run<3>(21, 4)) return 1;
+  // the GPU kernels' widths (REF 31 / 20, b16 16) and every remainder of nb % L
+  if (!run_cm<2, 31>() || !run_cm<2, 16>() || !run_cm<2, 29>() || !run_cm<2, 30>() ||
+      !run_cm<2, 5>() || !run_cm<3, 20>() || !run_cm<3, 16>() || !run_cm<3, 21>() ||
+      !run_cm<3, 3>())
+    return 1;
   printf("ok %llu\n", (unsigned long long)cases);
   return 0;
 }
