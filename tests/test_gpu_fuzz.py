@@ -1,7 +1,9 @@
 """Randomised parity sweep: seeded random trees (dimension, object kind, layout,
-fanout, key mode, size) and random search configurations (workgroup size, leaf
-chunks, donation, containment, query order, mode), each compared with the CPU
-oracle's DFS results (counts, ID-set hashes, and sorted ID lists where asked)."""
+fanout, key mode, size, MPHR partitions) and random search configurations
+(workgroup size, leaf chunks, donation, containment, query order, mode), each
+compared with the CPU oracle's DFS results (counts, ID-set hashes, and sorted ID
+lists where asked).  Fanouts that are not multiples of 4 start leaf runs off
+the 16-byte boundary the 4-entries-a-lane leaf path needs."""
 import numpy as np
 import pytest
 
@@ -16,7 +18,7 @@ def _case(rng):
     D = int(rng.choice([2, 3]))
     layout = str(rng.choice(["mphr", "rtree", "bvh"]))
     is_box = bool(rng.random() < 0.4)
-    F = int(rng.choice([16, 24, 32, 64, 100, 128, 200, 256])) if layout != "bvh" else \
+    F = int(rng.choice([16, 24, 30, 32, 64, 66, 100, 128, 200, 256])) if layout != "bvh" else \
         int(rng.choice([16, 32, 64, 128]))
     key = "b16" if rng.random() < 0.25 else "ref"
     n = int(rng.choice([1, 2, 17, 300, 4097, 20000, 60000]))
@@ -39,14 +41,16 @@ def test_random_configuration(gpu, O, case):
         c = rng.random((nq, D), dtype=np.float32)
         s = rng.uniform(0.05, 0.8, (nq, 1)).astype(np.float32)
         q = np.concatenate([c - s / 2, c + s / 2], 1).astype(np.float32)
-    tag = "case %d: D=%d %s box=%d F=%d key=%s src=%s n=%d nq=%d" % (
-        case, D, layout, is_box, F, key, src, n, nq)
+    P = int(rng.choice([2, 5])) if layout == "mphr" and n >= 5 and rng.random() < 0.5 else 1
+    tag = "case %d: D=%d %s box=%d F=%d key=%s src=%s n=%d nq=%d P=%d" % (
+        case, D, layout, is_box, F, key, src, n, nq, P)
     print(tag, flush=True)
     ix = gpu.Index.build(objs, D, fanout=F, layout=layout, key_mode=key, key_src=src,
-                         obj_kind="boxes" if is_box else "points")
+                         obj_kind="boxes" if is_box else "points", partitions=P)
     t = O.build_packed(objs, D, is_box, F, O.KEY_B16 if key == "b16" else O.KEY_REF,
                        O.SRC_CENTER if src == "center" else O.SRC_LO,
-                       layout="bvh" if layout == "bvh" else "packed")
+                       layout="bvh" if layout == "bvh" else ("partitioned" if P > 1 else "packed"),
+                       parts=P)
     exp = t.search_dfs(q, want_ids=True)
     for rep in range(3):
         kw = dict(block_waves=int(rng.choice([1, 2, 4, 8])), chunks=int(rng.choice([2, 4, 8])),
