@@ -1221,6 +1221,210 @@ hipError_t launch_assign(const float* objs, uint64_t n, unsigned D, bool box, bo
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// k-means centre update on the device, in the exact order of the defined rule
+// (FP64 sums of each cluster's key points in index order within a chunk of
+// kKmChunk objects, chunk sums added in chunk order).  One workgroup per
+// chunk walks it in sub-blocks of kKmSub objects, in order: a sub-block's keys
+// (cluster << 11 | position) are bitonic-sorted in LDS and its key points are
+// staged in LDS, so every cluster's members come out in index order, and the
+// thread at the start of a cluster's run adds them one by one onto the chunk's
+// running sums (read, add each member, write back: the same sequence of FP64
+// additions as a serial loop).  The barrier between sub-blocks orders one
+// sub-block's sums before the next.
+constexpr uint32_t kKmSub = 2048;  // 11-bit positions: keys stay below the sentinel
+
+template <int D, bool BOX, bool CENTER>
+__device__ __forceinline__ float km_point(const float* __restrict__ objs, uint64_t o, int d) {
+  constexpr int E = BOX ? 2 * D : D;
+  float p = objs[o * E + d];
+  if (CENTER && BOX) p = (p + objs[o * E + D + d]) * 0.5f;
+  return p;
+}
+
+constexpr uint32_t kKmThreads = 512;
+
+template <int D, bool BOX, bool CENTER>
+__global__ __launch_bounds__(kKmThreads) void k_km_partial(
+    const float* __restrict__ objs, uint64_t n, const uint32_t* __restrict__ cid, uint32_t K,
+    uint64_t c0, double* __restrict__ part, uint32_t* __restrict__ cnt) {
+  constexpr uint32_t T = kKmThreads;
+  constexpr int SLOTS = kKmSub / T;  // sorted positions a thread owns: tid + T j
+  __shared__ uint32_t sk[kKmSub];
+  __shared__ float kp[D][kKmSub];  // the sub-block's key points
+  const uint32_t tid = threadIdx.x;
+  const uint64_t base = (c0 + blockIdx.x) * kKmChunk;
+  double* pc = part + (uint64_t)blockIdx.x * K * D;
+  uint32_t* nc = cnt + (uint64_t)blockIdx.x * K;
+  for (uint32_t sb = 0; sb < kKmChunk / kKmSub; ++sb) {
+    const uint64_t b0 = base + (uint64_t)sb * kKmSub;
+    if (b0 >= n) break;  // uniform
+    const uint32_t m = n - b0 < kKmSub ? (uint32_t)(n - b0) : kKmSub;
+    for (uint32_t t = tid; t < kKmSub; t += T) {
+      sk[t] = t < m ? ((cid[b0 + t] << 11) | t) : 0xFFFFFFFFu;
+      if (t < m) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) kp[d][t] = km_point<D, BOX, CENTER>(objs, b0 + t, d);
+      }
+    }
+    __syncthreads();
+    // bitonic sort: a thread's compare-exchanges of one stage are issued
+    // together (independent pairs), then the stage's barrier
+    for (uint32_t k = 2; k <= kKmSub; k <<= 1)
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        uint32_t a[SLOTS], b[SLOTS];
+#pragma unroll
+        for (int r = 0; r < SLOTS; ++r) {
+          const uint32_t i = tid + T * r, x = i ^ j;
+          a[r] = sk[i];
+          b[r] = x > i ? sk[x] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < SLOTS; ++r) {
+          const uint32_t i = tid + T * r, x = i ^ j;
+          if (x > i && ((a[r] > b[r]) == ((i & k) == 0))) {
+            sk[i] = b[r];
+            sk[x] = a[r];
+          }
+        }
+        __syncthreads();
+      }
+    // Keys are unique (positions differ), so this is the (cluster, position)
+    // order.  The thread owning a run's first slot adds the run; a thread's
+    // runs read their running sums together, then add, then write back.
+    uint32_t kk[SLOTS];
+    bool start[SLOTS];
+    double acc[SLOTS][D];
+    uint32_t len[SLOTS];
+#pragma unroll
+    for (int j = 0; j < SLOTS; ++j) {
+      const uint32_t p = tid + T * j;
+      kk[j] = sk[p] >> 11;
+      start[j] = p < m && (p == 0 || (sk[p - 1] >> 11) != kk[j]);
+      len[j] = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < SLOTS; ++j)
+      if (start[j]) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[j][d] = pc[(uint64_t)kk[j] * D + d];
+      }
+#pragma unroll
+    for (int j = 0; j < SLOTS; ++j)
+      if (start[j]) {
+        for (uint32_t q = tid + T * j; q < m && (sk[q] >> 11) == kk[j]; ++q) {
+          const uint32_t o = sk[q] & (kKmSub - 1);
+#pragma unroll
+          for (int d = 0; d < D; ++d) acc[j][d] += (double)kp[d][o];
+          ++len[j];
+        }
+      }
+#pragma unroll
+    for (int j = 0; j < SLOTS; ++j)
+      if (start[j]) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) pc[(uint64_t)kk[j] * D + d] = acc[j][d];
+        nc[kk[j]] += len[j];
+      }
+    __syncthreads();
+  }
+}
+
+// Cluster k: the chunks' partials added in chunk order (a chunk without
+// members of k adds nothing, as in the serial rule).
+template <int D>
+__global__ void k_km_combine(const double* __restrict__ part, const uint32_t* __restrict__ cnt,
+                             uint64_t nc, uint32_t K, double* __restrict__ sum,
+                             uint64_t* __restrict__ total) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  double s[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) s[d] = sum[k * D + d];
+  uint64_t t = total[k];
+  for (uint64_t c = 0; c < nc; ++c) {
+    const uint32_t m = cnt[c * K + k];
+    if (m) {
+      t += m;
+#pragma unroll
+      for (int d = 0; d < D; ++d) s[d] += part[(c * K + k) * D + d];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) sum[k * D + d] = s[d];
+  total[k] = t;
+}
+
+template <int D>
+__global__ void k_km_finish(const double* __restrict__ sum, const uint64_t* __restrict__ total,
+                            uint32_t K, float* __restrict__ centers) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K || total[k] == 0) return;
+#pragma unroll
+  for (int d = 0; d < D; ++d) centers[k * D + d] = (float)(sum[k * D + d] / (double)total[k]);
+}
+
+template <int D, bool BOX, bool CENTER>
+__global__ void k_km_seed(const float* __restrict__ objs, const uint32_t* __restrict__ idx,
+                          uint32_t K, float* __restrict__ centers) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+#pragma unroll
+  for (int d = 0; d < D; ++d) centers[k * D + d] = km_point<D, BOX, CENTER>(objs, idx[k], d);
+}
+
+#define QCP_KM_DISPATCH(CALL)                                                    \
+  do {                                                                         \
+    if (D == 2) {                                                              \
+      if (box) { if (center) CALL(2, true, true); else CALL(2, true, false); }  \
+      else CALL(2, false, false);                                              \
+    } else if (D == 3) {                                                       \
+      if (box) { if (center) CALL(3, true, true); else CALL(3, true, false); }  \
+      else CALL(3, false, false);                                              \
+    } else {                                                                   \
+      return hipErrorInvalidValue;                                             \
+    }                                                                          \
+  } while (0)
+
+hipError_t launch_km_seed(const float* objs, unsigned D, bool box, bool center,
+                          const uint32_t* idx, uint32_t K, float* centers, hipStream_t s) {
+  if (K == 0) return hipSuccess;
+  const unsigned g = (K + 255) / 256;
+#define QCP_KM_SEED(DD, BB, CC) k_km_seed<DD, BB, CC><<<g, 256, 0, s>>>(objs, idx, K, centers)
+  QCP_KM_DISPATCH(QCP_KM_SEED);
+#undef QCP_KM_SEED
+  return hipGetLastError();
+}
+
+hipError_t launch_km_update(const float* objs, uint64_t n, unsigned D, bool box, bool center,
+                            const uint32_t* cid, uint32_t K, uint64_t c0, uint64_t nc,
+                            double* part, uint32_t* cnt, double* sum, uint64_t* total,
+                            hipStream_t s) {
+  if (nc == 0 || K == 0) return hipSuccess;
+  if (nc > 0x7FFFFFFFull) return hipErrorInvalidValue;
+#define QCP_KM_PART(DD, BB, CC) \
+  k_km_partial<DD, BB, CC><<<(unsigned)nc, kKmThreads, 0, s>>>(objs, n, cid, K, c0, part, cnt)
+  QCP_KM_DISPATCH(QCP_KM_PART);
+#undef QCP_KM_PART
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const unsigned g = (K + 255) / 256;
+  if (D == 2) k_km_combine<2><<<g, 256, 0, s>>>(part, cnt, nc, K, sum, total);
+  else k_km_combine<3><<<g, 256, 0, s>>>(part, cnt, nc, K, sum, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_km_finish(const double* sum, const uint64_t* total, uint32_t K, unsigned D,
+                            float* centers, hipStream_t s) {
+  if (K == 0) return hipSuccess;
+  const unsigned g = (K + 255) / 256;
+  if (D == 2) k_km_finish<2><<<g, 256, 0, s>>>(sum, total, K, centers);
+  else if (D == 3) k_km_finish<3><<<g, 256, 0, s>>>(sum, total, K, centers);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+#undef QCP_KM_DISPATCH
+
 hipError_t launch_compose(uint64_t* keys, const uint32_t* cid, const uint32_t* rank, uint64_t n,
                           unsigned shift_key, unsigned shift_rank, hipStream_t s) {
   k_compose<<<grid_for(n, 256, 8192), 256, 0, s>>>(keys, cid, rank, n, shift_key, shift_rank);

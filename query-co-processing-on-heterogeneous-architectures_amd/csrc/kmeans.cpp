@@ -1,18 +1,20 @@
 // kmeans.cpp -- k-means + Hilbert ordering of the objects (QCP_ORDER_KMEANS_HILBERT).
 //
 // The reference's KmeansMapper (src/mapper/kmeans_mapper.cpp:15-259) clusters
-// the objects with Lloyd's algorithm (GPU assignment, host centre update),
+// the objects with Lloyd's algorithm (GPU assignment, host centre update; here
+// both run on the GPU),
 // gives each cluster centre a Hilbert key, ranks the clusters by it, and then
 // orders objects by (cluster rank, own Hilbert key).  Its composite key
 // cluster_key * N + object_key overflows 64 bits and it skips the last object
 // (kmeans_mapper.cpp:250-253); this build defines the order as follows:
 //  - seeds: K distinct objects, index (w * N) >> 32 for the Philox words w of
 //    (seed 1234, stream 5) in order, repeats skipped;
-//  - each iteration assigns every key point to its nearest centre (GPU,
-//    k_assign) and then moves each non-empty cluster's centre to the mean of
-//    its key points: FP64 sums in index order within chunks of 65536 objects,
-//    the chunk sums added in chunk order, then rounded to FP32 (so any number
-//    of host threads gives the same centres);
+//  - each iteration assigns every key point to its nearest centre (k_assign)
+//    and then moves each non-empty cluster's centre to the mean of its key
+//    points: FP64 sums in index order within chunks of 65536 objects, the chunk
+//    sums added in chunk order, then rounded to FP32 (k_km_partial /
+//    k_km_combine / k_km_finish keep exactly that order, so the result does not
+//    depend on how the work is spread);
 //  - the loop is the reference's (kmeans_mapper.cpp:170, kmeans_macro.h:4-5):
 //    delta = (float)changes / (float)n, repeat while delta > 1e-5 and
 //    loop++ < max_loop, loop from 0, so at most max_loop + 1 iterations; the
@@ -28,16 +30,11 @@
 #include <string.h>
 
 #include <algorithm>
-#include <atomic>
-#include <chrono>
-#include <condition_variable>
-#include <functional>
-#include <mutex>
-#include <thread>
 #include <vector>
 
 #include "hilbert_key.h"
 #include "qcp_internal.h"
+#include "qcp_kernels.h"
 
 #define HIP_TRY(expr)                          \
   do {                                         \
@@ -46,7 +43,6 @@
 
 namespace {
 
-constexpr uint64_t kChunk = 1 << 16;
 constexpr uint32_t kDefaultClusters = 20000;  // DFN_NumberOfClusters, kmeans_macro.h
 constexpr uint32_t kDefaultIters = 1001;      // loop++ < 1000: kMeansLoopIteration + 1
 constexpr double kDeltaMin = 0.00001;         // DFN_KmeansThreashhold
@@ -62,168 +58,17 @@ struct DevBuf {
   bool alloc(uint64_t n) { return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)) == hipSuccess; }
 };
 
-struct PinnedU32 {
-  uint32_t* p = nullptr;
-  ~PinnedU32() {
-    if (p) hipHostFree(p);
-  }
-  bool alloc(uint64_t n) { return hipHostMalloc((void**)&p, 4 * (n ? n : 1), 0) == hipSuccess; }
-};
-
 struct Events {
-  hipEvent_t e[2] = {nullptr, nullptr};
+  hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
   bool create() {
-    return hipEventCreate(&e[0]) == hipSuccess && hipEventCreate(&e[1]) == hipSuccess;
+    for (auto& x : e)
+      if (hipEventCreate(&x) != hipSuccess) return false;
+    return true;
   }
   ~Events() {
     for (auto x : e)
       if (x) hipEventDestroy(x);
   }
-};
-
-// A fixed set of host threads that run fn(i) for i in [0, count) together with
-// the calling thread; run() returns when every index is done.
-class Pool {
- public:
-  explicit Pool(unsigned threads) {
-    try {
-      for (unsigned t = 1; t < threads; ++t) th_.emplace_back([this] { loop(); });
-    } catch (...) {
-      stop();  // a thread that could not start: end the ones that did
-      throw;
-    }
-  }
-  ~Pool() { stop(); }
-  void run(uint64_t count, const std::function<void(uint64_t)>& fn) {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      fn_ = &fn;
-      count_ = count;
-      next_.store(0);
-      busy_ = th_.size();
-      ++gen_;
-    }
-    cv_.notify_all();
-    work();
-    std::unique_lock<std::mutex> l(mu_);
-    done_.wait(l, [this] { return busy_ == 0; });
-    fn_ = nullptr;
-  }
-
- private:
-  void work() {
-    for (;;) {
-      const uint64_t i = next_.fetch_add(1);
-      if (i >= count_) return;
-      (*fn_)(i);
-    }
-  }
-  void stop() {
-    {
-      std::lock_guard<std::mutex> g(mu_);
-      stop_ = true;
-      ++gen_;
-    }
-    cv_.notify_all();
-    for (auto& t : th_)
-      if (t.joinable()) t.join();
-  }
-  void loop() {
-    uint64_t seen = 0;
-    for (;;) {
-      {
-        std::unique_lock<std::mutex> l(mu_);
-        cv_.wait(l, [&] { return gen_ != seen; });
-        seen = gen_;
-        if (stop_) return;
-      }
-      work();
-      std::lock_guard<std::mutex> g(mu_);
-      if (--busy_ == 0) done_.notify_all();
-    }
-  }
-  std::vector<std::thread> th_;
-  std::mutex mu_;
-  std::condition_variable cv_, done_;
-  const std::function<void(uint64_t)>* fn_ = nullptr;
-  uint64_t count_ = 0, gen_ = 0;
-  size_t busy_ = 0;
-  bool stop_ = false;
-  std::atomic<uint64_t> next_{0};
-};
-
-// Centre update with the chunk rule above.  Chunks are summed in windows of
-// G at a time (partials G x K x D, bounded to about 256 MiB); within a window
-// each chunk's partials are formed in parallel, then added to the running sums
-// in chunk order, clusters in parallel.  Only clusters present in a chunk are
-// added (adding a chunk's +0.0 partial would leave the sum unchanged), and only
-// their partials are cleared again, so the buffers are allocated and zeroed
-// once for all iterations.
-class CentreUpdate {
- public:
-  CentreUpdate(unsigned threads, uint64_t n, uint32_t D, uint32_t K) : n_(n), D_(D), K_(K) {
-    nchunks_ = (n + kChunk - 1) / kChunk;
-    const uint64_t per_chunk = std::max<uint64_t>(1, (uint64_t)K * (8ull * D + 4));
-    G_ = std::max<uint64_t>(
-        1, std::min<uint64_t>({nchunks_, 2ull * threads, (256ull << 20) / per_chunk}));
-    part_.assign(G_ * K * D, 0.0);
-    cnt_.assign(G_ * K, 0);
-    sum_.resize((size_t)K * D);
-    total_.resize(K);
-    // the clusters present in each chunk of the window: at most min(K, chunk)
-    // each, reserved here so that the workers allocate nothing
-    touched_.resize(G_);
-    for (auto& t : touched_) t.reserve(std::min<uint64_t>(K, kChunk));
-  }
-
-  void run(Pool& pool, const float* pts, const uint32_t* cid, float* centers) {
-    const uint64_t K = K_, D = D_, n = n_;
-    std::fill(sum_.begin(), sum_.end(), 0.0);
-    std::fill(total_.begin(), total_.end(), 0);
-    const uint64_t kblk = 1024;
-    for (uint64_t w0 = 0; w0 < nchunks_; w0 += G_) {
-      const uint64_t w = std::min(G_, nchunks_ - w0);
-      pool.run(w, [&](uint64_t g) {
-        double* ps = part_.data() + g * K * D;
-        uint32_t* cs = cnt_.data() + g * K;
-        std::vector<uint32_t>& tl = touched_[g];
-        const uint64_t b = (w0 + g) * kChunk, e = std::min(n, b + kChunk);
-        for (uint64_t i = b; i < e; ++i) {
-          const uint32_t k = cid[i];
-          if (cs[k]++ == 0) tl.push_back(k);
-          for (uint64_t d = 0; d < D; ++d) ps[k * D + d] += (double)pts[i * D + d];
-        }
-      });
-      pool.run((K + kblk - 1) / kblk, [&](uint64_t kb) {
-        const uint64_t k1 = std::min<uint64_t>(K, (kb + 1) * kblk);
-        for (uint64_t g = 0; g < w; ++g)
-          for (uint64_t k = kb * kblk; k < k1; ++k)
-            if (cnt_[g * K + k]) {
-              total_[k] += cnt_[g * K + k];
-              for (uint64_t d = 0; d < D; ++d) sum_[k * D + d] += part_[(g * K + k) * D + d];
-            }
-      });
-      pool.run(w, [&](uint64_t g) {
-        for (const uint32_t k : touched_[g]) {
-          cnt_[g * K + k] = 0;
-          for (uint64_t d = 0; d < D; ++d) part_[(g * K + k) * D + d] = 0.0;
-        }
-        touched_[g].clear();
-      });
-    }
-    for (uint64_t k = 0; k < K; ++k)
-      if (total_[k])
-        for (uint64_t d = 0; d < D; ++d)
-          centers[k * D + d] = (float)(sum_[k * D + d] / (double)total_[k]);
-  }
-
- private:
-  uint64_t n_, nchunks_, G_;
-  uint32_t D_, K_;
-  std::vector<double> part_, sum_;
-  std::vector<uint32_t> cnt_;
-  std::vector<uint64_t> total_;
-  std::vector<std::vector<uint32_t>> touched_;
 };
 
 }  // namespace
@@ -234,24 +79,14 @@ uint32_t qcp::kmeans_seed_index(uint64_t word, uint64_t n) {
 
 qcp_status qcp::kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bits) {
   const uint64_t n = ix->n;
-  const uint32_t D = ix->D, E = ix->E;
+  const uint32_t D = ix->D;
   hipStream_t s = ix->stream;
   uint32_t K = ix->cfg.clusters ? ix->cfg.clusters : kDefaultClusters;
   if ((uint64_t)K > n) K = (uint32_t)n;
   const uint32_t max_iters = ix->cfg.kmeans_iters ? ix->cfg.kmeans_iters : kDefaultIters;
   const bool center = ix->box && ix->cfg.key_src == QCP_SRC_CENTER;
-  // key points on the host (lower corner or centre, the Hilbert key's source)
-  std::vector<float> objs((size_t)n * E), pts((size_t)n * D);
-  HIP_TRY(hipMemcpyAsync(objs.data(), d_objs, 4ull * n * E, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  for (uint64_t i = 0; i < n; ++i)
-    for (uint32_t d = 0; d < D; ++d) {
-      float p = objs[i * E + d];
-      if (center) p = (p + objs[i * E + D + d]) * 0.5f;
-      pts[i * D + d] = p;
-    }
-  // seeds
-  std::vector<float> centers((size_t)K * D);
+  // seeds: K distinct object indexes from the Philox words
+  std::vector<uint32_t> seeds(K);
   {
     std::vector<uint8_t> taken(n, 0);
     std::vector<uint32_t> words;
@@ -265,22 +100,25 @@ qcp_status qcp::kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bi
       const uint32_t idx = kmeans_seed_index(words[next++], n);
       if (taken[idx]) continue;
       taken[idx] = 1;
-      for (uint32_t d = 0; d < D; ++d) centers[k * D + d] = pts[(uint64_t)idx * D + d];
-      ++k;
+      seeds[k++] = idx;
     }
   }
+  // the centre update works on windows of G chunks: partials G x K x D
+  // (bounded to about 1 GiB), running sums K x D
+  const uint64_t nchunks = (n + kKmChunk - 1) / kKmChunk;
+  const uint64_t per_chunk = std::max<uint64_t>(1, (uint64_t)K * (8ull * D + 4));
+  const uint64_t G = std::max<uint64_t>(1, std::min<uint64_t>(nchunks, (1ull << 30) / per_chunk));
   DevBuf<float> d_c;
-  DevBuf<uint32_t> d_cid, d_changes, d_rank;
-  if (!d_c.alloc((uint64_t)K * D) || !d_cid.alloc(n) || !d_changes.alloc(1) || !d_rank.alloc(K))
+  DevBuf<uint32_t> d_cid, d_changes, d_rank, d_seeds, d_cnt;
+  DevBuf<double> d_part, d_sum;
+  DevBuf<uint64_t> d_total;
+  if (!d_c.alloc((uint64_t)K * D) || !d_cid.alloc(n) || !d_changes.alloc(1) || !d_rank.alloc(K) ||
+      !d_seeds.alloc(K) || !d_part.alloc(G * K * D) || !d_cnt.alloc(G * K) ||
+      !d_sum.alloc((uint64_t)K * D) || !d_total.alloc(K))
     return QCP_ENOMEM;
+  HIP_TRY(hipMemcpyAsync(d_seeds.p, seeds.data(), 4ull * K, hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_km_seed(d_objs, D, ix->box, center, d_seeds.p, K, d_c.p, s));
   HIP_TRY(launch_fill_u32(d_cid.p, n, 0xFFFFFFFFu, s));
-  // the clusters come back every iteration: pinned host memory
-  PinnedU32 cid;
-  if (!cid.alloc(n)) return QCP_ENOMEM;
-  unsigned hw = std::thread::hardware_concurrency();
-  hw = std::max(1u, std::min(hw, 16u));
-  Pool pool(hw);
-  CentreUpdate update(hw, n, D, K);
   const uint32_t max_loop = max_iters - 1;
   uint32_t loop = 0, its = 0;
   float delta = 0.0f;
@@ -289,27 +127,39 @@ qcp_status qcp::kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bi
   if (!ev.create()) return QCP_EHIP;
   do {
     uint32_t changes = 0;
-    HIP_TRY(hipMemcpyAsync(d_c.p, centers.data(), 4ull * K * D, hipMemcpyHostToDevice, s));
     HIP_TRY(launch_fill_u32(d_changes.p, 1, 0u, s));
     HIP_TRY(hipEventRecord(ev.e[0], s));
     HIP_TRY(launch_assign(d_objs, n, D, ix->box, center, d_c.p, K, d_cid.p, d_changes.p, s));
     HIP_TRY(hipEventRecord(ev.e[1], s));
     HIP_TRY(hipMemcpyAsync(&changes, d_changes.p, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(cid.p, d_cid.p, 4ull * n, hipMemcpyDeviceToHost, s));
+    // centre update, windows of chunks in chunk order
+    HIP_TRY(launch_fill_u32((uint32_t*)d_sum.p, 2ull * K * D, 0u, s));
+    HIP_TRY(launch_fill_u32((uint32_t*)d_total.p, 2ull * K, 0u, s));
+    for (uint64_t c0 = 0; c0 < nchunks; c0 += G) {
+      const uint64_t w = std::min(G, nchunks - c0);
+      HIP_TRY(launch_fill_u32((uint32_t*)d_part.p, 2ull * w * K * D, 0u, s));
+      HIP_TRY(launch_fill_u32(d_cnt.p, w * K, 0u, s));
+      HIP_TRY(launch_km_update(d_objs, n, D, ix->box, center, d_cid.p, K, c0, w, d_part.p,
+                               d_cnt.p, d_sum.p, d_total.p, s));
+    }
+    HIP_TRY(launch_km_finish(d_sum.p, d_total.p, K, D, d_c.p, s));
+    HIP_TRY(hipEventRecord(ev.e[2], s));
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
     assign_ms += ms;
+    HIP_TRY(hipEventElapsedTime(&ms, ev.e[1], ev.e[2]));
+    update_ms += ms;
     ++its;
-    const auto t0 = std::chrono::steady_clock::now();
-    update.run(pool, pts.data(), cid.p, centers.data());
-    update_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     delta = (float)changes;
     delta /= (float)n;
   } while ((double)delta > kDeltaMin && loop++ < max_loop);
   ix->kmeans_its = its;
   ix->kmeans_ms[0] = assign_ms;
   ix->kmeans_ms[1] = update_ms;
+  std::vector<float> centers((size_t)K * D);
+  HIP_TRY(hipMemcpyAsync(centers.data(), d_c.p, 4ull * K * D, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   // cluster ranks by (centre key, cluster number)
   std::vector<std::pair<uint64_t, uint32_t>> ck(K);
   for (uint32_t k = 0; k < K; ++k) {

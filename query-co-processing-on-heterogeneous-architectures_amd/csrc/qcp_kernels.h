@@ -23,6 +23,21 @@ hipError_t launch_assign(const float* objs, uint64_t n, unsigned D, bool box, bo
                          hipStream_t s);
 hipError_t launch_compose(uint64_t* keys, const uint32_t* cid, const uint32_t* rank, uint64_t n,
                           unsigned shift_key, unsigned shift_rank, hipStream_t s);
+// k-means on the device.  launch_km_seed: centers[k] = key point of object
+// idx[k].  launch_km_update: the centre update of chunks [c0, c0 + nc) of
+// kKmChunk objects (FP64 sums in index order within a chunk, into part
+// [nc][K][D] / cnt [nc][K], zero at entry), then their partials added in chunk
+// order into sum [K][D] / total [K].  launch_km_finish: centres = sum / total
+// for every cluster with members.
+constexpr uint64_t kKmChunk = 65536;
+hipError_t launch_km_seed(const float* objs, unsigned D, bool box, bool center,
+                          const uint32_t* idx, uint32_t K, float* centers, hipStream_t s);
+hipError_t launch_km_update(const float* objs, uint64_t n, unsigned D, bool box, bool center,
+                            const uint32_t* cid, uint32_t K, uint64_t c0, uint64_t nc,
+                            double* part, uint32_t* cnt, double* sum, uint64_t* total,
+                            hipStream_t s);
+hipError_t launch_km_finish(const double* sum, const uint64_t* total, uint32_t K, unsigned D,
+                            float* centers, hipStream_t s);
 // Whole LSD radix sort of (keys, vals) over `passes` 8-bit digits, one-sweep
 // form; the result is in keys/vals for an even pass count, keys2/vals2 for an
 // odd one.  ghist: kMaxSortPasses*256 + kMaxSortPasses words; status:
