@@ -102,9 +102,9 @@ typedef struct {
                                  is node p (nodes are numbered depth-major)  */
   uint32_t kmeans_iterations; /* Lloyd iterations of a k-means build (0 otherwise, and
                                  for an index loaded from a file)            */
-  double kmeans_ms[2];        /* k-means build: assignment kernels (HIP events,
-                                 summed over iterations), host centre updates
-                                 (wall) */
+  double kmeans_ms[2];        /* k-means build: assignment kernels and centre-update
+                                 kernels (HIP events, each summed over the
+                                 iterations) */
 } qcp_index_info;
 
 typedef struct {

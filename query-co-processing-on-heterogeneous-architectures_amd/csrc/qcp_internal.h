@@ -32,7 +32,7 @@ struct qcp_index {
   uint32_t* d_nparent = nullptr; // explicit layouts: parent node (roots: themselves)
   uint32_t parts = 1;            // MPHR data partitions (roots are nodes 0 .. parts-1)
   uint32_t kmeans_its = 0;       // Lloyd iterations of the last k-means build
-  double kmeans_ms[2] = {0, 0};  // its assignment (device) and update (host) time
+  double kmeans_ms[2] = {0, 0};  // its assignment and centre-update device time
   uint64_t* d_keys = nullptr;    // sorted keys [n]
   uint32_t* d_perm = nullptr;    // [n]
   // build scratch
