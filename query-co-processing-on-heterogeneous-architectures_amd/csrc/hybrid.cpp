@@ -9,8 +9,9 @@
 // above `handoff_level` on a host mirror of the node boxes and emit one work
 // item (query, node) per overlapping node at that level; a persistent search
 // launch walks those subtrees (by default the leaves themselves, as in the
-// reference).  Queries are processed in blocks, double-buffered: the host
-// traverses block b+1 while the GPU searches block b.  Every overlapping leaf
+// reference).  Queries are taken in the batch's spatial order (Hilbert order
+// of their centres, as the persistent form processes them) and in blocks,
+// double-buffered: the host traverses block b+1 while the GPU searches block b.  Every overlapping leaf
 // is searched exactly once, so counts, hashes and the traversal statistics
 // equal the canonical traversal's.
 #include <hip/hip_runtime.h>
@@ -202,6 +203,10 @@ qcp_status qcp::hybrid_run(qcp_batch* b, const qcp_search_cfg& cfg, qcp_search_s
   } ctl_free{h_ctl};
   std::vector<std::vector<unsigned long long>> tl(nt);
   std::vector<uint64_t> t_ie(nt, 0);
+  // queries in the batch's spatial order: a block's items then cover
+  // neighbouring queries, whose subtrees share leaves in each XCD's L2
+  std::vector<uint32_t> ord;
+  QCP_TRY(batch_order_host(b, cfg.order == 0, ord));
 
   HIP_TRY(launch_fill_u32(b->d_counts, q, 0u, s));
   HIP_TRY(launch_fill_u32((uint32_t*)b->d_hashes, 2 * q, 0u, s));
@@ -221,7 +226,8 @@ qcp_status qcp::hybrid_run(qcp_batch* b, const qcp_search_cfg& cfg, qcp_search_s
       if (a0 >= a1 || ix->h == 0) continue;
       th.emplace_back([&, t, a0, a1] {
         std::vector<std::pair<int, uint64_t>> stack;
-        for (uint64_t qi = a0; qi < a1; ++qi) {
+        for (uint64_t j = a0; j < a1; ++j) {
+          const uint64_t qi = ord[j];
           const float* qp = b->h_queries.data() + qi * 2 * ix->D;
           if (ix->h == 1 || L <= 0) {
             for (uint32_t r = 0; r < ix->parts; ++r)  // the roots themselves

@@ -985,6 +985,28 @@ static qcp_status batch_order(qcp_batch* b) {
   return QCP_OK;
 }
 
+}  // extern "C"
+
+// The batch's spatial order on the host (the co-processing form walks its
+// queries in it, so that the work items of neighbouring queries reach the GPU
+// together): ord[j] = the j-th query, the identity for small batches or
+// order = 1.
+qcp_status qcp::batch_order_host(qcp_batch* b, bool spatial, std::vector<uint32_t>& ord) {
+  const uint64_t q = b->q;
+  ord.resize(q);
+  if (!spatial || q < kOrderMinQueries) {
+    for (uint64_t i = 0; i < q; ++i) ord[i] = (uint32_t)i;
+    return QCP_OK;
+  }
+  QCP_TRY(batch_order(b));
+  hipStream_t s = b->ix->stream;
+  HIP_TRY(hipMemcpyAsync(ord.data(), b->d_qperm, 4ull * q, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return QCP_OK;
+}
+
+extern "C" {
+
 // The reference's partition mode of MPHR::Search (src/tree/mphr.cpp:235-257):
 // one launch per query with one workgroup per partition, each running MPRS
 // over its partition's subtree from its root (g_root_offset[bid], 314-333);

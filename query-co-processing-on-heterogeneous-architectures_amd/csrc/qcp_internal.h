@@ -118,6 +118,9 @@ unsigned persistent_grid(const qcp_index* ix, int out, unsigned block);
 // ix->kmeans_its = iterations run
 qcp_status kmeans_keys(qcp_index* ix, const float* d_objs, uint64_t* key_bits);
 uint32_t kmeans_seed_index(uint64_t word, uint64_t n);
+// the batch's query order on the host: the Hilbert order of the query centres
+// (spatial, batches of 8192 or more) or the given order
+qcp_status batch_order_host(qcp_batch* b, bool spatial, std::vector<uint32_t>& ord);
 // the co-processing search (QCP_SEARCH_HYBRID), in hybrid.cpp
 qcp_status hybrid_run(qcp_batch* b, const qcp_search_cfg& cfg, qcp_search_stats* stats);
 }  // namespace qcp
