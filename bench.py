@@ -187,16 +187,42 @@ def contained_search(b, q, D, E, steps, warmup, kw, canon):
                                       np.array_equal(got["hashes"], canon["hashes"]))}
 
 
+def sort_bytes_per_key(n, E, bits, form, keep_perm):
+    """Bytes a key the build's LSD radix sort moves: the digit counts of all
+    passes read the keys once (8); then per pass, index form: the key (pass 0,
+    the index is the position) or key + index (12) in, key + index out (12),
+    or 8 in / out once the library packs the key bits still to sort with the
+    index into one word; object form: the key (pass 0) or the remainder (8
+    bytes while more than 32 bits remain, else 4) and the payload (E object
+    words + the index word, 4 bytes each; pass 0 reads the E object words) in,
+    the remainder and the payload out (the last pass the payload alone)."""
+    passes = (bits + 7) // 8
+    if not passes:
+        return 0, passes, None
+    pbits = max(1, int(n - 1).bit_length())
+    pack = next((j for j in range(passes - 1) if bits - 8 * (j + 1) + pbits <= 64), None)
+    per_key = 8
+    W4 = 4 * E + (4 if keep_perm else 0)
+    for p in range(passes):
+        if form == 1:
+            last = p + 1 == passes
+            per_key += (8 + 4 * E) if p == 0 else (8 if bits - 8 * p > 32 else 4) + W4
+            per_key += W4 if last else (8 if bits - 8 * (p + 1) > 32 else 4) + W4
+        else:
+            pin = pack is not None and p > pack
+            pout = pack is not None and p >= pack
+            per_key += (8 if p == 0 or pin else 12) + (8 if pout else 12)
+    return per_key, passes, pack
+
+
 def build_roofline(inf, n, D, keys):
     """Algorithmic bytes of one build (per object: read it (E), write its key
     (K = 8), one minimal sort read + write of key and index (2(K+4)), the
     permutation: read the index, gather the object, write it (4 + 2E); then
     per node: read its children's boxes (its objects at a leaf) and write its
     box), over the build time; and the bytes the LSD radix sort actually
-    moves over the sort phase's time: the digit counts of all passes read the
-    keys once (8 per key); a pass reads and writes key and index (12 + 12) until
-    the library packs the key bits still to sort with the index into one word
-    (the pass that packs writes 8, later passes read and write 8 + 8)."""
+    moves (sort_bytes_per_key, for the form the build used) over the sort
+    phase's time."""
     E = D * 4 * (2 if inf["obj_kind"] == 1 else 1)
     K = 8
     box = 2 * D * 4
@@ -208,17 +234,15 @@ def build_roofline(inf, n, D, keys):
     ms = inf["build_ms"]
     key_or = int(np.bitwise_or.reduce(keys)) if n else 0
     bits = key_or.bit_length()
-    passes = (bits + 7) // 8
-    pbits = max(1, int(n - 1).bit_length())
-    pack = next((j for j in range(passes - 1) if bits - 8 * (j + 1) + pbits <= 64), None)
-    per_key = 8 if passes else 0
-    for p in range(passes):
-        per_key += 24 if pack is None or p < pack else (20 if p == pack else 16)
+    per_key, passes, pack_pass = sort_bytes_per_key(n, E // 4, bits, inf["sort_form"],
+                                                    bool(inf["has_perm"]))
     moved = n * per_key
     sort_ms = inf["build_phase_ms"][1]
     return {"bound": "hbm", "achieved": abytes / (ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": abytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-            "algorithmic_bytes": abytes, "sort_passes": passes, "sort_pack_pass": pack,
+            "algorithmic_bytes": abytes,
+            "sort_form": "objects" if inf["sort_form"] else "index + gather",
+            "sort_passes": passes, "sort_pack_pass": pack_pass if not inf["sort_form"] else None,
             "sort_bytes_moved": moved,
             "sort_GBps": moved / (sort_ms * 1e-3) / 1e9 if sort_ms > 0 else None}
 
@@ -252,8 +276,10 @@ def build_figures(qcp, ix, pts, D, threads, with_cpu, key_mode="ref"):
         k, perm, sobjs, boxes, ms, bad = O.build_mt(
             pts, D, False, F, O.KEY_REF if key_mode == "ref" else O.KEY_B16, nthreads=threads)
         dt = time.perf_counter() - t0
-        ok = bool(bad == 0 and np.array_equal(k, ex["keys"]) and np.array_equal(perm, ex["perm"])
-                  and np.array_equal(boxes.view(np.uint32), ex["boxes"].view(np.uint32)))
+        ok = bool(bad == 0 and np.array_equal(k, ex["keys"]) and
+                  (ex["perm"] is None or np.array_equal(perm, ex["perm"])) and
+                  np.array_equal(sobjs.view(np.uint32), ex["sobjs"].view(np.uint32)) and
+                  np.array_equal(boxes.view(np.uint32), ex["boxes"].view(np.uint32)))
         out["cpu"] = {"value": n / dt / 1e6, "unit": "Mkeys/s", "cores": threads, "kind": "port",
                       "sample": "all %d points, oracle build over %d threads, %.1f s" %
                                 (n, threads, dt),
@@ -533,29 +559,62 @@ def run_config(qcp, name, args):
         w = qcp.workload("c5", scale=args.scale)
         pts, q = w["objs"], w["queries"]
         buf = qcp.DeviceBuffer(pts)
-        log("c5 build")
-        ix = qcp.Index.build(buf, 2, fanout=64)
-        eager = []
-        for _ in range(3):
-            ix.rebuild(buf)
-            eager.append(ix.info())
-        gix = qcp.Index.build(buf, 2, fanout=64, graph=True)
-        graph_ms = []
-        for _ in range(3):
-            gix.rebuild(buf)
-            graph_ms.append(gix.info()["build_ms"])
-        assert np.array_equal(gix.export()["boxes"], ix.export()["boxes"])
-        gix.close()
         n = len(pts)
-        best = min(eager, key=lambda i: i["build_ms"])
-        res = {"n": n, "build_ms_eager": best["build_ms"],
-               "build_phase_ms": dict(zip(("keys", "sort", "gather", "pack"),
-                                          best["build_phase_ms"][:4])),
-               "build_mkeys_per_s": n / (best["build_ms"] * 1e-3) / 1e6,
-               "build_ms_graph": min(graph_ms),
-               "build_mkeys_per_s_graph": n / (min(graph_ms) * 1e-3) / 1e6}
-        log("c5 build: %.1f ms eager (%.0f Mkeys/s), %.1f ms graph" %
-            (best["build_ms"], res["build_mkeys_per_s"], min(graph_ms)))
+        log("c5 build")
+        # the library's default form (here the object form, with the
+        # permutation), then the index form and the object form without the
+        # permutation (the reference's build keeps none); each best of three
+        # eager rebuilds and three graph replays, all forms equal
+        forms = {}
+        ref_ex = None
+        for label, sort, keep in (("default", None, True), ("index", "index", True),
+                                  ("objects_no_perm", "objects", False)):
+            fix = qcp.Index.build(buf, 2, fanout=64, sort=sort, keep_perm=keep)
+            eager = []
+            for _ in range(3):
+                fix.rebuild(buf)
+                eager.append(fix.info())
+            gix = qcp.Index.build(buf, 2, fanout=64, graph=True, sort=sort, keep_perm=keep)
+            graph_ms = []
+            for _ in range(3):
+                gix.rebuild(buf)
+                graph_ms.append(gix.info()["build_ms"])
+            ex = fix.export()
+            if ref_ex is None:
+                ref_ex = ex
+            same = bool(np.array_equal(gix.export()["boxes"].view(np.uint32),
+                                       ex["boxes"].view(np.uint32)) and
+                        np.array_equal(ex["boxes"].view(np.uint32),
+                                       ref_ex["boxes"].view(np.uint32)) and
+                        np.array_equal(ex["sobjs"].view(np.uint32),
+                                       ref_ex["sobjs"].view(np.uint32)))
+            gix.close()
+            best = min(eager, key=lambda i: i["build_ms"])
+            forms[label] = {
+                "sort_form": "objects" if best["sort_form"] else "index + gather",
+                "has_perm": bool(best["has_perm"]),
+                "build_ms_eager": best["build_ms"],
+                "build_phase_ms": dict(zip(("keys", "sort", "gather", "pack"),
+                                           best["build_phase_ms"][:4])),
+                "build_mkeys_per_s": n / (best["build_ms"] * 1e-3) / 1e6,
+                "build_ms_graph": min(graph_ms),
+                "build_mkeys_per_s_graph": n / (min(graph_ms) * 1e-3) / 1e6,
+                "roofline": build_roofline(best, n, 2, ex["keys"]),
+                "equals_default_and_graph": same}
+            log("c5 build %s (%s): %.2f ms eager, %.2f ms graph" %
+                (label, forms[label]["sort_form"], best["build_ms"], min(graph_ms)))
+            if label == "default":
+                ix = fix
+            else:
+                fix.close()
+            del ex
+        del ref_ex
+        d = forms["default"]
+        res = {"n": n, "build_ms_eager": d["build_ms_eager"],
+               "build_phase_ms": d["build_phase_ms"],
+               "build_mkeys_per_s": d["build_mkeys_per_s"],
+               "build_ms_graph": d["build_ms_graph"],
+               "build_mkeys_per_s_graph": d["build_mkeys_per_s_graph"], "build_forms": forms}
         with ix.batch(q) as b:
             for donate in (1, 0):
                 st, wall, kms = timed_search(b, 1, donate=donate)
@@ -724,31 +783,58 @@ def profile_c5(qcp, args):
 
 def profile_build(qcp, args):
     """A short, fixed run of the build on the 134M-point 2-D skewed config, for
-    kernel traces: `steps` eager rebuilds (the host reads the keys' bit width
-    and runs that many sort passes), then `steps` replays of the graph-captured
-    build (every key byte's pass is captured; passes above the keys' bit width
-    end at once on the device).  Prints one JSON line with the phase times."""
-    w = qcp.workload("c5", scale=args.scale)
+    kernel traces, in each key-sort form: (key, input index) pairs and a
+    gather; the objects carried through the passes with their input indices;
+    and carried without them (the reference's build keeps none).  Per form:
+    `steps` eager rebuilds (the host reads the keys' bit width and runs that
+    many sort passes), then `steps` replays of the graph-captured build (every
+    key byte's pass is captured; passes above the keys' bit width end at once
+    on the device).  Every form's rank-ordered objects and node boxes must equal
+    the first's (and its permutation, where kept).  --build-config c2 / c3 /
+    c4 runs the same on the 16M-point 3-D (F = 128), 67M-box 2-D (R-tree,
+    centre keys, F = 128) or 33.5M-point 3-D (F = 64) configs.  Prints one
+    JSON line with the phase times."""
+    cfgname = args.build_config
+    w = qcp.workload(cfgname, scale=args.scale)
+    D = w["dims"]
+    F = 128 if cfgname in ("c2", "c3") else 64
+    kw = {"fanout": F, "layout": w.get("layout", "mphr"), "obj_kind": w["obj_kind"],
+          "key_src": w.get("key_src", "lo")}
     buf = qcp.DeviceBuffer(w["objs"])
     n = len(w["objs"])
-    out = {"profile": "c5 build", "n": n, "eager": [], "graph_ms": []}
-    ix = qcp.Index.build(buf, 2, fanout=64)
-    for _ in range(args.steps):
-        ix.rebuild(buf)
-        inf = ix.info()
-        out["eager"].append({"build_ms": inf["build_ms"],
-                             "phase_ms": dict(zip(("keys", "sort", "gather", "pack"),
-                                                  inf["build_phase_ms"][:4]))})
-        log("c5 eager build %.3f ms" % inf["build_ms"])
-    gix = qcp.Index.build(buf, 2, fanout=64, graph=True)
-    for _ in range(args.steps):
-        gix.rebuild(buf)
-        out["graph_ms"].append(gix.info()["build_ms"])
-        log("c5 graph build %.3f ms" % out["graph_ms"][-1])
-    out["graph_equals_eager"] = bool(
-        np.array_equal(gix.export()["boxes"], ix.export()["boxes"]))
-    gix.close()
-    ix.close()
+    out = {"profile": "%s build" % cfgname, "n": n, "build": kw, "forms": {}}
+    ref = None
+    for sort, keep in (("index", True), ("objects", True), ("objects", False)):
+        name = sort + ("" if keep else " (no permutation)")
+        f = {"eager": [], "graph_ms": []}
+        ix = qcp.Index.build(buf, D, sort=sort, keep_perm=keep, **kw)
+        for _ in range(args.steps):
+            ix.rebuild(buf)
+            inf = ix.info()
+            f["eager"].append({"build_ms": inf["build_ms"],
+                               "phase_ms": dict(zip(("keys", "sort", "gather", "pack"),
+                                                    inf["build_phase_ms"][:4]))})
+            log("%s %s eager build %.3f ms" % (cfgname, name, inf["build_ms"]))
+        gix = qcp.Index.build(buf, D, graph=True, sort=sort, keep_perm=keep, **kw)
+        for _ in range(args.steps):
+            gix.rebuild(buf)
+            f["graph_ms"].append(gix.info()["build_ms"])
+            log("%s %s graph build %.3f ms" % (cfgname, name, f["graph_ms"][-1]))
+        ex, gx = ix.export(), gix.export()
+        f["graph_equals_eager"] = bool(
+            np.array_equal(gx["boxes"].view(np.uint32), ex["boxes"].view(np.uint32)) and
+            np.array_equal(gx["sobjs"].view(np.uint32), ex["sobjs"].view(np.uint32)))
+        if ref is None:
+            ref = ex
+        else:
+            f["equals_index_form"] = bool(
+                np.array_equal(ex["boxes"].view(np.uint32), ref["boxes"].view(np.uint32)) and
+                np.array_equal(ex["sobjs"].view(np.uint32), ref["sobjs"].view(np.uint32)) and
+                (ex["perm"] is None or np.array_equal(ex["perm"], ref["perm"])))
+        out["forms"][name] = f
+        del ex, gx
+        gix.close()
+        ix.close()
     buf.close()
     print(json.dumps(out), flush=True)
 
@@ -821,6 +907,8 @@ def main():
                          "then a 1 GiB device-to-device copy as the calibration kernel")
     ap.add_argument("--profile-c5", action="store_true",
                     help="short search run on the 134M-point 2-D config (for rocprofv3)")
+    ap.add_argument("--build-config", default="c5", choices=["c2", "c3", "c4", "c5"],
+                    help="--profile-build: the workload whose build is traced")
     ap.add_argument("--profile-build", action="store_true",
                     help="short build run on the 134M-point 2-D config (for rocprofv3)")
     ap.add_argument("--profile-ids", action="store_true",

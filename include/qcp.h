@@ -54,7 +54,19 @@ enum { QCP_LAYOUT_MPHR = 0, QCP_LAYOUT_RTREE = 1, QCP_LAYOUT_BVH = 2, QCP_LAYOUT
 enum { QCP_KEY_REF = 0, QCP_KEY_B16 = 1 };      /* (ll)(1e6f*p) @ 31|20 bits, or 16 bits */
 enum { QCP_SRC_LO = 0, QCP_SRC_CENTER = 1 };    /* key from lower corner or box centre  */
 enum { QCP_OBJ_POINTS = 0, QCP_OBJ_BOXES = 1 };
-enum { QCP_BUILD_GRAPH = 1 };                   /* qcp_build_cfg.flags                  */
+/* qcp_build_cfg.flags.  QCP_BUILD_GRAPH: run the build as a captured hipGraph.
+ * QCP_BUILD_NO_PERM: the index need not keep the input index of each rank (the
+ * reference keeps none: it overwrites each branch index by its rank,
+ * src/sort/thrust_sorter.cpp:39-47); qcp_index_info.has_perm tells whether it did.
+ * The key sort either sorts (key, input index) and then gathers every object
+ * through the permutation (QCP_BUILD_SORT_INDEX), or moves the objects
+ * themselves (and the input indices unless QCP_BUILD_NO_PERM) through every
+ * pass as the keys' payload, so that no object is read at a random place
+ * (QCP_BUILD_SORT_OBJECTS; not for GUTTMAN layouts or the k-means order).
+ * Neither flag: the object form once the objects take 64 MiB or more (for
+ * layouts and orders it serves).  Both forms give the same index. */
+enum { QCP_BUILD_GRAPH = 1, QCP_BUILD_NO_PERM = 2, QCP_BUILD_SORT_INDEX = 4,
+       QCP_BUILD_SORT_OBJECTS = 8 };
 enum { QCP_ORDER_HILBERT = 0, QCP_ORDER_KMEANS_HILBERT = 1 };
 #define QCP_MAX_CLUSTERS (1u << 20) /* k-means order: largest cluster count */
 
@@ -105,6 +117,9 @@ typedef struct {
   double kmeans_ms[2];        /* k-means build: assignment kernels and centre-update
                                  kernels (HIP events, each summed over the
                                  iterations) */
+  uint32_t sort_form;         /* the last build's key sort: 0 = (key, input index) and a
+                                 gather, 1 = objects carried through the passes */
+  uint32_t has_perm;          /* 1: qcp_export returns the input index of each rank */
 } qcp_index_info;
 
 typedef struct {
@@ -202,7 +217,8 @@ qcp_status qcp_index_info_get(const qcp_index *ix, qcp_index_info *out);
 
 /* Copy the built tree back to the host for inspection and parity checks:
  *   sorted_keys[n]   Hilbert keys in rank order          (may be NULL)
- *   perm[n]          input index of each rank            (may be NULL)
+ *   perm[n]          input index of each rank            (may be NULL; must be NULL,
+ *                    or QCP_EINVAL, when qcp_index_info.has_perm is 0)
  *   sobjs[n*E]       objects in rank order, row-major     (may be NULL)
  *   node_boxes[n_nodes*2D] node MBRs, BFS order, [lo(D),hi(D)] (may be NULL)
  *   node_first/node_count[n_nodes], node_leaf[n_nodes]   (may be NULL)

@@ -13,6 +13,8 @@
 // (src/tree/tree.cpp:1025-1133) and Node::GetMBB (src/node/node.cpp:19-70).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hilbert_key.h"
 #include "qcp_device.h"
 #include "qcp_kernels.h"
@@ -288,6 +290,48 @@ struct ObjVec {
 };
 
 
+// Boxes of the packed leaves of fanout FL (16, 32 or 64) that a wave's
+// 64-rank chunk covers: lane l holds rank r = chunk start + l (object v,
+// `valid` when r < n); each FL-lane group reduces its objects by shuffles, and
+// its first lane writes the leaf's box with the padding rule of k_leaf_boxes.
+template <int D, bool BOX, int FL>
+__device__ __forceinline__ void leaf_box_reduce(const float (&v)[BOX ? 2 * D : D], bool valid,
+                                                uint64_t r, uint64_t n, float* __restrict__ nbox,
+                                                uint64_t nstride, uint64_t leaf_off) {
+  static_assert(FL == 16 || FL == 32 || FL == 64, "leaves of one wave's chunk");
+  const uint32_t lane = lane_id();
+  float lo[D], hi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    lo[d] = valid ? v[d] : __builtin_huge_valf();
+    hi[d] = valid ? v[BOX ? D + d : d] : -__builtin_huge_valf();
+  }
+#pragma unroll
+  for (int off = FL / 2; off > 0; off >>= 1) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float l = __shfl_xor(lo[d], off, kWave);
+      const float h = __shfl_xor(hi[d], off, kWave);
+      lo[d] = (lo[d] > l) ? l : lo[d];
+      hi[d] = (hi[d] < h) ? h : hi[d];
+    }
+  }
+  if ((lane & (FL - 1)) == 0 && valid) {
+    const uint64_t leaf = r / FL;  // r is this leaf's first rank
+    const bool pad = n - r < (uint64_t)FL;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      float l = lo[d], h = hi[d];
+      if (pad) {
+        l = (l > 1.0f) ? 1.0f : l;
+        h = (h < 0.0f) ? 0.0f : h;
+      }
+      nbox[(uint64_t)d * nstride + leaf_off + leaf] = l;
+      nbox[(uint64_t)(D + d) * nstride + leaf_off + leaf] = h;
+    }
+  }
+}
+
 // The build's gather: one 64-rank chunk per wave and iteration (a
 // wave-uniform loop).  PACKED: the sort ended in packed elements (rank r holds
 // ((key >> dropped bits) << pbits) | input index); the permutation is unpacked
@@ -320,38 +364,7 @@ __device__ __forceinline__ void gather_objs_body(
       for (int e = 0; e < E; ++e) obj[(uint64_t)e * stride + r] = o.v[e];
       if (PACKED || perm_out != nullptr) perm_out[r] = p;
     }
-    if constexpr (FL > 0) {
-      float lo[D], hi[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        lo[d] = valid ? o.v[d] : __builtin_huge_valf();
-        hi[d] = valid ? o.v[BOX ? D + d : d] : -__builtin_huge_valf();
-      }
-#pragma unroll
-      for (int off = FL / 2; off > 0; off >>= 1) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const float l = __shfl_xor(lo[d], off, kWave);
-          const float h = __shfl_xor(hi[d], off, kWave);
-          lo[d] = (lo[d] > l) ? l : lo[d];
-          hi[d] = (hi[d] < h) ? h : hi[d];
-        }
-      }
-      if ((lane & (FL - 1)) == 0 && valid) {
-        const uint64_t leaf = r / FL;  // r is this leaf's first rank
-        const bool pad = n - r < (uint64_t)FL;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          float l = lo[d], h = hi[d];
-          if (pad) {
-            l = (l > 1.0f) ? 1.0f : l;
-            h = (h < 0.0f) ? 0.0f : h;
-          }
-          nbox[(uint64_t)d * nstride + leaf_off + leaf] = l;
-          nbox[(uint64_t)(D + d) * nstride + leaf_off + leaf] = h;
-        }
-      }
-    }
+    if constexpr (FL > 0) leaf_box_reduce<D, BOX, FL>(o.v, valid, r, n, nbox, nstride, leaf_off);
   }
 }
 
@@ -485,6 +498,123 @@ __global__ __launch_bounds__(256) void k_leaf_boxes(const float* __restrict__ ob
   const uint64_t first = wave * F;
   const uint32_t cnt = (uint32_t)((n - first) < F ? (n - first) : F);
   wave_box<D, MW>(obj, ostride, BOX ? D : 0, first, cnt, F, nbox, nstride, leaf_off + wave);
+}
+
+// Packed leaves of fanout F (16 to 256, a power of two) from the
+// rank-ordered objects, streamed: lane l of a wave takes the F/16 consecutive
+// ranks from wave start + l*F/16 (one or a few vector loads a coordinate, min
+// and max in rank order), so that one leaf is one 16-lane row, reduced by
+// four DPP row rotations without LDS traffic.  Row g's first lane writes leaf
+// (wave start / F) + g with the padding rule of k_leaf_boxes.  The order of
+// the min/max pairings differs from the one-wave-a-leaf kernel, which only
+// matters for a -0.0f / +0.0f tie (either may stay; no other value depends on
+// the order).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+// row_ror:8, 4, 2, 1 (DPP controls 0x128, 0x124, 0x122, 0x121): every lane
+// of a 16-lane row ends with the row's minimum / maximum
+__device__ __forceinline__ float row_min16(float x) {
+  float y = dpp_f32<0x128>(x);
+  x = (x > y) ? y : x;
+  y = dpp_f32<0x124>(x);
+  x = (x > y) ? y : x;
+  y = dpp_f32<0x122>(x);
+  x = (x > y) ? y : x;
+  y = dpp_f32<0x121>(x);
+  return (x > y) ? y : x;
+}
+__device__ __forceinline__ float row_max16(float x) {
+  float y = dpp_f32<0x128>(x);
+  x = (x < y) ? y : x;
+  y = dpp_f32<0x124>(x);
+  x = (x < y) ? y : x;
+  y = dpp_f32<0x122>(x);
+  x = (x < y) ? y : x;
+  y = dpp_f32<0x121>(x);
+  return (x < y) ? y : x;
+}
+
+// K consecutive floats from p[r0] (K-aligned), those below rank n
+template <int K>
+__device__ __forceinline__ void load_run(const float* __restrict__ p, uint64_t r0, uint64_t n,
+                                         float (&v)[K], bool (&ok)[K]) {
+  if (r0 + K <= n) {
+    if constexpr (K % 4 == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k += 4) {
+        const float4 t = *reinterpret_cast<const float4*>(p + r0 + k);
+        v[k] = t.x; v[k + 1] = t.y; v[k + 2] = t.z; v[k + 3] = t.w;
+      }
+    } else if constexpr (K == 2) {
+      const float2 t = *reinterpret_cast<const float2*>(p + r0);
+      v[0] = t.x; v[1] = t.y;
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = p[r0 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) ok[k] = true;
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      ok[k] = r0 + k < n;
+      v[k] = ok[k] ? p[r0 + k] : 0.0f;
+    }
+  }
+}
+
+template <int D, bool BOX, int F>
+__global__ __launch_bounds__(256) void k_leaf_boxes_row(const float* __restrict__ obj,
+                                                        uint64_t ostride, uint64_t n,
+                                                        float* __restrict__ nbox, uint64_t nstride,
+                                                        uint64_t leaf_off) {
+  static_assert(F >= 16 && F <= 256 && (F & (F - 1)) == 0, "a leaf is a 16-lane row");
+  constexpr int K = F / 16;  // ranks a lane
+  const uint32_t lane = lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+  for (uint64_t wb = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * 4 * F;
+       wb < n; wb += waves * 4 * F) {
+    const uint64_t r0 = wb + (uint64_t)lane * K;
+    float lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      float v[K];
+      bool ok[K];
+      load_run<K>(obj + (uint64_t)d * ostride, r0, n, v, ok);
+      float l = __builtin_huge_valf(), h = -__builtin_huge_valf();
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (ok[k]) {
+          l = (l > v[k]) ? v[k] : l;
+          if (!BOX) h = (h < v[k]) ? v[k] : h;
+        }
+      }
+      if (BOX) {
+        load_run<K>(obj + (uint64_t)(D + d) * ostride, r0, n, v, ok);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (ok[k]) h = (h < v[k]) ? v[k] : h;
+      }
+      lo[d] = row_min16(l);
+      hi[d] = row_max16(h);
+    }
+    if ((lane & 15u) == 0 && r0 < n) {
+      const uint64_t leaf = r0 / F;  // r0 is this leaf's first rank
+      const bool pad = n - r0 < (uint64_t)F;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        float l = lo[d], h = hi[d];
+        if (pad) {
+          l = (l > 1.0f) ? 1.0f : l;
+          h = (h < 0.0f) ? 0.0f : h;
+        }
+        nbox[(uint64_t)d * nstride + leaf_off + leaf] = l;
+        nbox[(uint64_t)(D + d) * nstride + leaf_off + leaf] = h;
+      }
+    }
+  }
 }
 
 // Packed internal level: parent j boxes children [jF, min(child_cnt, jF+F)).
@@ -728,6 +858,129 @@ __device__ __forceinline__ uint32_t sort_scan(uint32_t v, uint32_t* wsum, uint32
   return pre + x - v;
 }
 
+// Per-tile steps of the object-carrying passes (the same steps as
+// sort_pass_body, which keeps its own copy for its register allocation).
+// tile_begin: take the next tile in start order, zero this workgroup's row of
+// the next pass's look-back words, clear the wave counters; returns, to
+// threads 0..255, the global start of their digit's bucket.
+__device__ __forceinline__ uint32_t tile_begin(SortLds& L, unsigned pass,
+                                               const uint32_t* __restrict__ ghist,
+                                               uint64_t* __restrict__ status_next,
+                                               uint32_t* tile_ctr) {
+  constexpr int W = kSortThreads / kWave;
+  const int tid = threadIdx.x;
+  const bool owner = tid < 256;  // owns digit tid
+  if (tid == 0) L.s_tile = atomicAdd(&tile_ctr[pass], 1u);
+  if (status_next != nullptr && owner) status_next[(uint64_t)blockIdx.x * 256 + tid] = 0;
+  uint32_t tot;
+  const uint32_t gbase = sort_scan(owner ? ghist[pass * 256 + tid] : 0u, L.wsum, &tot);
+  if (owner) {
+#pragma unroll
+    for (int ww = 0; ww < W; ++ww) L.wcnt[ww][tid] = 0;
+  }
+  __syncthreads();
+  return gbase;
+}
+
+// tile_rank: rank of each of this wave's keys among the wave's earlier keys of
+// the same digit (rk[c] for chunk c of 64 keys at wbase + 64c; digit(c) is
+// this lane's digit in chunk c), counting each digit's keys into L.wcnt[wave].
+template <class DigitFn>
+__device__ __forceinline__ void tile_rank(SortLds& L, uint64_t wbase, uint64_t n, DigitFn digit,
+                                          uint32_t (&rk)[kSortKPT]) {
+  const int w = threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int c = 0; c < kSortKPT; ++c) {
+    const bool valid = wbase + (uint64_t)c * kWave + lane < n;
+    const uint32_t dig = digit(c);
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bb = ballot((dig >> b) & 1u);
+      peers &= ((dig >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t r = rank_below(peers);
+    // every lane reads its digit's count before the group's first lane adds
+    // the group (one wave: the LDS serves its accesses in program order)
+    const uint32_t before = valid ? (uint32_t)L.wcnt[w][dig] : 0u;
+    rk[c] = before + r;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && r == 0) L.wcnt[w][dig] = (uint16_t)(before + (uint32_t)__popcll(peers));
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+}
+
+// tile_place: the tile's count of each digit, published and then looked back
+// over earlier tiles (decoupled look-back); leaves the bucket's global start
+// in L.gstart, its start inside the tile in L.lstart, and in L.wcnt[wave] each
+// wave's first slot of each digit inside the tile.
+__device__ __forceinline__ void tile_place(SortLds& L, uint32_t tile, uint32_t gbase,
+                                           uint64_t* status) {
+  constexpr int W = kSortThreads / kWave;
+  const int tid = threadIdx.x;
+  const bool owner = tid < 256;
+  // digit tid: this tile's count, and each wave's offset among the tile's keys
+  uint32_t cnt = 0;
+  if (owner) {
+#pragma unroll
+    for (int ww = 0; ww < W; ++ww) {
+      const uint32_t c = L.wcnt[ww][tid];
+      L.wcnt[ww][tid] = (uint16_t)cnt;
+      cnt += c;
+    }
+  }
+  uint64_t* const mine = status + (uint64_t)tile * 256 + tid;
+  if (owner)
+    __hip_atomic_store(mine, (tile == 0 ? kStatusPrefix : kStatusAgg) | (uint64_t)cnt,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t tot_tile;
+  const uint32_t ls = sort_scan(cnt, L.wsum, &tot_tile);
+  uint32_t excl = 0;
+  if (owner && tile > 0) {
+    // kLookback predecessors' words per round trip, consumed in order up to
+    // the first inclusive prefix; a word not yet published ends the round
+    // (sleep, then reload from there).  Tiles below 0 read as a zero prefix.
+    constexpr int kLookback = 4;
+    int32_t t = (int32_t)tile - 1;
+    for (;;) {
+      uint64_t wv[kLookback];
+#pragma unroll
+      for (int j = 0; j < kLookback; ++j)
+        wv[j] = t - j >= 0 ? __hip_atomic_load(status + (uint64_t)(t - j) * 256 + tid,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : kStatusPrefix;
+      bool stop = false, done = false;
+      int used = 0;
+#pragma unroll
+      for (int j = 0; j < kLookback; ++j) {
+        if (!stop) {
+          if (!(wv[j] & (kStatusAgg | kStatusPrefix))) {
+            stop = true;
+          } else {
+            excl += (uint32_t)wv[j];
+            ++used;
+            if (wv[j] & kStatusPrefix) stop = done = true;
+          }
+        }
+      }
+      if (done) break;
+      t -= used;
+      if (used == 0) __builtin_amdgcn_s_sleep(1);
+    }
+    __hip_atomic_store(mine, kStatusPrefix | (uint64_t)(excl + cnt), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (owner) {
+    L.gstart[tid] = gbase + excl;
+    L.lstart[tid] = (uint16_t)ls;
+#pragma unroll
+    for (int ww = 0; ww < W; ++ww) L.wcnt[ww][tid] = (uint16_t)(L.wcnt[ww][tid] + ls);
+  }
+  __syncthreads();
+}
+
 // The work of one pass on one tile (see k_sort_onesweep).
 template <bool INP, bool OUTP>
 __device__ __forceinline__ void sort_pass_body(
@@ -917,6 +1170,214 @@ __global__ __launch_bounds__(kSortThreads) __attribute__((amdgpu_waves_per_eu(4)
                                status, status_next, tile_ctr);
 }
 
+// ---------------------------------------------------------------------------
+// Object-carrying sort (CarryArgs in qcp_kernels.h).  Elements are the key
+// bits still to be sorted plus W payload words: the object's E floats as raw
+// words, then its input index when the index keeps its permutation.  A pass
+// ranks by the low byte of the key remainder, then writes the remainder shifted
+// down by 8 bits and the payload to their places, one array a round through
+// the tile's LDS, 8 bytes an element where it can: payload words go as pairs
+// (2i, 2i+1), and an odd last word rides in the high half of the remainder's
+// 8-byte slot once at most 32 key bits remain (while more remain, the
+// remainder takes the whole slot and the odd word a 4-byte array of its own;
+// the last pass writes no remainder).  The last pass writes the objects
+// straight into the index's rank-ordered SoA array (and the input indices
+// into the permutation), so nothing reads an object at a random place: the
+// index form's gather reads each object through the permutation, a whole
+// 128-byte line for 8 to 24 bytes.
+// Scratch set j: the remainder slots in krem[j] (8 bytes, or 4 when they hold
+// a remainder alone), the pairs as 8-byte arrays at pay[j] + 2i*cs and the odd
+// word as a 4-byte array at pay[j] + 2*(W/2)*cs.
+// SRC = 0: pass 0, which reads the keys and the caller's row-major objects;
+// SRC = 1: a later pass whose remainders in exceed 32 bits (8-byte slots, the
+// odd word apart); SRC = 2: at most 32 bits (4-byte slots, or 8-byte slots
+// holding the odd word).  Pass p reads set (p - 1) & 1 and writes set p & 1.
+template <int E, bool IDX, int SRC>
+__device__ __forceinline__ void carry_pass_body(SortLds& L, const CarryArgs& a, unsigned pass,
+                                                unsigned bits, bool last) {
+  constexpr int W = E + (IDX ? 1 : 0);
+  constexpr int NP = W / 2;        // payload pairs
+  constexpr bool ODD = (W & 1) != 0;
+  constexpr int CH = kSortKPT;
+  // the remainder slot read: 8 bytes for pass 0 and SRC 1, and for SRC 2 when
+  // it carries the odd word
+  constexpr bool KIN64 = SRC != 2 || ODD;
+  using KT = typename std::conditional<KIN64, uint64_t, uint32_t>::type;
+  const unsigned kout_words = last ? 0u : (bits - 8 * (pass + 1) > 32 ? 2u : 1u);
+  const bool fuse_out = ODD && kout_words == 1;  // odd word in the slot's high half
+  const uint64_t n = a.n;
+  const int tid = threadIdx.x;
+  const int w = tid / kWave;
+  const uint32_t lane = lane_id();
+  uint64_t* st = a.status + (uint64_t)(pass & 1u) * a.words;
+  uint64_t* st_next = last ? nullptr : a.status + (uint64_t)((pass + 1) & 1u) * a.words;
+  const uint32_t gbase = tile_begin(L, pass, a.ghist, st_next, a.ghist + kMaxSortPasses * 256);
+  const uint32_t tile = L.s_tile;
+  // the wave's first element (wave-uniform, so that loads take a scalar base
+  // and a 32-bit lane offset)
+  const uint64_t wbase = uni64((uint64_t)tile * kSortTile + (uint64_t)w * CH * kWave);
+  const unsigned src = (pass - 1u) & 1u, dstset = pass & 1u;
+  const KT* __restrict__ kin = (SRC == 0 ? reinterpret_cast<const KT*>(a.keys)
+                                         : reinterpret_cast<const KT*>(a.krem[src])) + wbase;
+  const float* __restrict__ oin = a.objs + wbase * E;  // pass 0
+  // payload word wd of this lane's element in chunk c, pass 0
+  auto word0 = [&](int wd, int c) -> uint32_t {
+    const uint32_t j = (uint32_t)c * kWave + lane;
+    if (wd < E) return __float_as_uint(oin[j * E + wd]);
+    return (uint32_t)(wbase + j);  // the input index is the position
+  };
+  // payload pair i of this lane's element in chunk c
+  auto pair = [&](int i, int c) -> uint64_t {
+    const uint32_t j = (uint32_t)c * kWave + lane;
+    if (SRC == 0) return ((uint64_t)word0(2 * i + 1, c) << 32) | word0(2 * i, c);
+    return (reinterpret_cast<const uint64_t*>(a.pay[src] + (uint64_t)2 * i * a.cs) + wbase)[j];
+  };
+  KT k[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const uint32_t j = (uint32_t)c * kWave + lane;
+    k[c] = wbase + j < n ? kin[j] : (KT)0;
+  }
+  // the key remainder and the odd word of chunk c
+  auto krem = [&](int c) -> uint64_t {
+    return (SRC == 2 && ODD) ? (uint64_t)(uint32_t)k[c] : (uint64_t)k[c];
+  };
+  auto odd = [&](int c) -> uint32_t {
+    if (SRC == 0) return word0(W - 1, c);
+    if (SRC == 2) return (uint32_t)((uint64_t)k[c] >> 32);
+    return (a.pay[src] + (uint64_t)2 * NP * a.cs + wbase)[(uint32_t)c * kWave + lane];
+  };
+  auto digit = [&](int c) { return (uint32_t)krem(c) & 0xFFu; };
+  uint32_t rk[CH];
+  tile_rank(L, wbase, n, digit, rk);
+  tile_place(L, tile, gbase, st);
+  // slots in (digit, index) order: the digits, and the remainders (with the
+  // odd word when it rides along)
+  uint32_t pos[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const uint32_t dig = digit(c);
+    pos[c] = L.wcnt[w][dig] + rk[c];
+    if (wbase + (uint64_t)c * kWave + lane < n) {
+      L.dg[pos[c]] = (uint8_t)dig;
+      if (kout_words == 2)
+        L.sk[pos[c]] = krem(c) >> 8;
+      else if (fuse_out)
+        L.sk[pos[c]] = (krem(c) >> 8) | ((uint64_t)odd(c) << 32);
+      else if (kout_words == 1)
+        L.sv[pos[c]] = (uint32_t)(krem(c) >> 8);
+    }
+  }
+  __syncthreads();
+  const uint64_t base = (uint64_t)tile * kSortTile;
+  const uint32_t tcnt = (uint32_t)(n - base < (uint64_t)kSortTile ? n - base : kSortTile);
+  // destination of slot p
+  auto dst = [&](uint32_t p) {
+    const uint32_t dig = L.dg[p];
+    return L.gstart[dig] + (p - L.lstart[dig]);
+  };
+  if (kout_words != 0) {
+    for (uint32_t p = tid; p < tcnt; p += kSortThreads) {
+      if (kout_words == 2 || fuse_out)
+        a.krem[dstset][dst(p)] = L.sk[p];
+      else
+        reinterpret_cast<uint32_t*>(a.krem[dstset])[dst(p)] = L.sv[p];
+    }
+    __syncthreads();
+  }
+  // the last pass writes word wd to the index
+  auto out_word = [&](int wd) -> uint32_t* {
+    return wd < E ? reinterpret_cast<uint32_t*>(a.obj) + (uint64_t)wd * a.ostride : a.perm;
+  };
+  // the odd word on its own (the last pass, or a remainder of 8 bytes)
+  if (ODD && !fuse_out) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (wbase + (uint64_t)c * kWave + lane < n) L.sv[pos[c]] = odd(c);
+    __syncthreads();
+    uint32_t* out = last ? out_word(W - 1) : a.pay[dstset] + (uint64_t)2 * NP * a.cs;
+    for (uint32_t p = tid; p < tcnt; p += kSortThreads) out[dst(p)] = L.sv[p];
+    if (NP > 0) __syncthreads();
+  }
+  // the pairs, one a round; the next pair's loads are issued before this
+  // round's LDS traffic
+  uint64_t cur[CH];
+  if (NP > 0) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      cur[c] = wbase + (uint64_t)c * kWave + lane < n ? pair(0, c) : 0ull;
+  }
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    uint64_t nxt[CH];
+    if (i + 1 < NP) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+        nxt[c] = wbase + (uint64_t)c * kWave + lane < n ? pair(i + 1, c) : 0ull;
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (wbase + (uint64_t)c * kWave + lane < n) L.sk[pos[c]] = cur[c];
+    __syncthreads();
+    if (last) {
+      uint32_t* o0 = out_word(2 * i);
+      uint32_t* o1 = out_word(2 * i + 1);
+      for (uint32_t p = tid; p < tcnt; p += kSortThreads) {
+        const uint32_t d = dst(p);
+        const uint64_t v = L.sk[p];
+        o0[d] = (uint32_t)v;
+        o1[d] = (uint32_t)(v >> 32);
+      }
+    } else {
+      uint64_t* out = reinterpret_cast<uint64_t*>(a.pay[dstset] + (uint64_t)2 * i * a.cs);
+      for (uint32_t p = tid; p < tcnt; p += kSortThreads) out[dst(p)] = L.sk[p];
+    }
+    if (i + 1 < NP) {
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < CH; ++c) cur[c] = nxt[c];
+    }
+  }
+}
+
+// One pass.  SRC = 0, 1, 2 as for carry_pass_body, fixed by the host (eager
+// builds, whose key width the host knows); SRC = -1: chosen here from the key
+// width (device-planned sorts).
+template <int E, bool IDX, int SRC>
+__global__ __launch_bounds__(kSortThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_sort_carry(
+    CarryArgs a, unsigned pass) {
+  __shared__ SortLds L;
+  unsigned bits = a.bits;
+  if (a.key_or != nullptr) {
+    const unsigned long long ko = *a.key_or;
+    bits = ko ? 64u - (unsigned)__clzll((long long)ko) : 0u;
+  }
+  const unsigned real = (bits + 7) / 8;
+  if (pass >= real) return;  // a device-planned sort: this byte of every key is zero
+  const bool last = pass + 1 == real;
+  if (SRC >= 0)
+    carry_pass_body<E, IDX, (SRC < 0 ? 0 : SRC)>(L, a, pass, bits, last);
+  else if (pass == 0)
+    carry_pass_body<E, IDX, 0>(L, a, pass, bits, last);
+  else if (bits - 8 * pass > 32)
+    carry_pass_body<E, IDX, 1>(L, a, pass, bits, last);
+  else
+    carry_pass_body<E, IDX, 2>(L, a, pass, bits, last);
+}
+
+// A device-planned object-carrying sort whose keys are all equal (no pass
+// reorders): the objects go to the index in input order, the permutation is
+// the identity.  Ends at once when some pass reorders.
+template <int D, bool BOX, bool VEC>
+__global__ __launch_bounds__(256) void k_carry_identity(const float* __restrict__ objs, uint64_t n,
+                                                        const unsigned long long* __restrict__ key_or,
+                                                        float* __restrict__ obj, uint64_t stride,
+                                                        uint32_t* __restrict__ perm_out) {
+  if (*key_or != 0) return;
+  gather_objs_body<D, BOX, VEC, false, 0>(objs, nullptr, nullptr, n, 0, obj, stride, perm_out,
+                                          nullptr, 0, 0);
+}
+
 uint64_t sort_status_words(uint64_t n) { return 2ull * sort_tiles(n > 0 ? n : 1) * 256; }
 
 hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2,
@@ -970,13 +1431,69 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
 
 
 
+hipError_t launch_sort_carry(const CarryArgs& a, unsigned D, bool box, bool idx,
+                             unsigned passes, hipStream_t s) {
+  if (passes > (unsigned)kMaxSortPasses || (D != 2 && D != 3)) return hipErrorInvalidValue;
+  if (a.key_or == nullptr && (a.bits > 64 || (a.bits + 7) / 8 != passes))
+    return hipErrorInvalidValue;
+  if (a.n == 0 || passes == 0) return hipSuccess;
+  const uint32_t nt = sort_tiles(a.n);
+  if (a.words != (uint64_t)nt * 256) return hipErrorInvalidValue;
+  hipError_t e = launch_fill_u32(a.ghist, kMaxSortPasses * 256 + kMaxSortPasses, 0u, s);
+  if (e != hipSuccess) return e;
+  const uint32_t ht = (uint32_t)((a.n + kHistTile - 1) / kHistTile);
+  k_sort_hist_all<<<ht < 2048 ? ht : 2048, kHistThreads, 0, s>>>(a.keys, a.n, passes, a.key_or,
+                                                                  a.ghist, a.status, a.words);
+  const unsigned E = box ? 2 * D : D;
+  for (unsigned p = 0; p < passes; ++p) {
+    // the pass's source kind, when the host knows the key width
+    const int src = a.key_or != nullptr ? -1 : (p == 0 ? 0 : (a.bits - 8 * p > 32 ? 1 : 2));
+#define QCP_SC2(EE, II)                                                                   \
+  switch (src) {                                                                          \
+    case 0: k_sort_carry<EE, II, 0><<<nt, kSortThreads, 0, s>>>(a, p); break;             \
+    case 1: k_sort_carry<EE, II, 1><<<nt, kSortThreads, 0, s>>>(a, p); break;             \
+    case 2: k_sort_carry<EE, II, 2><<<nt, kSortThreads, 0, s>>>(a, p); break;             \
+    default: k_sort_carry<EE, II, -1><<<nt, kSortThreads, 0, s>>>(a, p); break;           \
+  }
+#define QCP_SC(EE) \
+  if (idx) { QCP_SC2(EE, true) } else { QCP_SC2(EE, false) }
+    switch (E) {
+      case 2: QCP_SC(2); break;
+      case 3: QCP_SC(3); break;
+      case 4: QCP_SC(4); break;
+      default: QCP_SC(6); break;
+    }
+#undef QCP_SC
+#undef QCP_SC2
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_carry_identity(const float* objs, uint64_t n, const unsigned long long* key_or,
+                                 unsigned D, bool box, float* obj, uint64_t stride,
+                                 uint32_t* perm_out, hipStream_t s) {
+  if (key_or == nullptr || (D != 2 && D != 3)) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  const unsigned g = grid_for(n, 256, 16384);
+  const bool vec = ((uintptr_t)objs & 15u) == 0;
+#define QCP_CI(DD, BB)                                                                       \
+  if (vec) k_carry_identity<DD, BB, true><<<g, 256, 0, s>>>(objs, n, key_or, obj, stride, perm_out); \
+  else k_carry_identity<DD, BB, false><<<g, 256, 0, s>>>(objs, n, key_or, obj, stride, perm_out)
+  if (D == 2) {
+    if (box) { QCP_CI(2, true); } else { QCP_CI(2, false); }
+  } else {
+    if (box) { QCP_CI(3, true); } else { QCP_CI(3, false); }
+  }
+#undef QCP_CI
+  return hipGetLastError();
+}
+
 hipError_t launch_gather_objs(const float* objs, const uint32_t* perm, const uint64_t* packed,
                               uint64_t n, unsigned pbits, unsigned D, bool box, float* obj,
                               uint64_t stride, uint32_t* perm_out, uint32_t fuse_leaves,
                               float* nbox, uint64_t nstride, uint64_t leaf_off, hipStream_t s) {
   const bool pk = packed != nullptr;
-  if (pk ? (pbits == 0 || pbits > 32 || perm_out == nullptr) : perm == nullptr)
-    return hipErrorInvalidValue;
+  if (pk && (pbits == 0 || pbits > 32 || perm_out == nullptr)) return hipErrorInvalidValue;
   if (fuse_leaves != 0 && fuse_leaves != 32 && fuse_leaves != 64) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
   unsigned g = grid_for(n, 256, 16384);
@@ -1443,6 +1960,25 @@ hipError_t launch_compose(uint64_t* keys, const uint32_t* cid, const uint32_t* r
 hipError_t launch_leaf_boxes(const float* obj, uint64_t ostride, uint64_t n, unsigned D,
                              bool box, uint32_t F, uint64_t nleaves, float* nbox,
                              uint64_t nstride, uint64_t leaf_off, hipStream_t s) {
+  if ((F == 16 || F == 32 || F == 64 || F == 128 || F == 256) && (D == 2 || D == 3)) {
+    // streamed: one wave a run of four leaves
+    const unsigned gs = grid_for((n + 4ull * F - 1) / (4ull * F) * kWave, 256, 16384);
+#define QCP_LB(DD, BB)                                                                        \
+  switch (F) {                                                                                \
+    case 16: k_leaf_boxes_row<DD, BB, 16><<<gs, 256, 0, s>>>(obj, ostride, n, nbox, nstride, leaf_off); break; \
+    case 32: k_leaf_boxes_row<DD, BB, 32><<<gs, 256, 0, s>>>(obj, ostride, n, nbox, nstride, leaf_off); break; \
+    case 64: k_leaf_boxes_row<DD, BB, 64><<<gs, 256, 0, s>>>(obj, ostride, n, nbox, nstride, leaf_off); break; \
+    case 128: k_leaf_boxes_row<DD, BB, 128><<<gs, 256, 0, s>>>(obj, ostride, n, nbox, nstride, leaf_off); break; \
+    default: k_leaf_boxes_row<DD, BB, 256><<<gs, 256, 0, s>>>(obj, ostride, n, nbox, nstride, leaf_off); break; \
+  }
+    if (D == 2) {
+      if (box) { QCP_LB(2, true) } else { QCP_LB(2, false) }
+    } else {
+      if (box) { QCP_LB(3, true) } else { QCP_LB(3, false) }
+    }
+#undef QCP_LB
+    return hipGetLastError();
+  }
   unsigned g = (unsigned)((nleaves * kWave + 255) / 256);
   int mwv = (int)((F + 63) / 64);
   if (D == 2) {

@@ -6,7 +6,8 @@
 //   magic "QCPIDX01" | u32 version | u32 header bytes | qcp_build_cfg |
 //   u64 n | u64 n_nodes | i32 h | u32 pad | u64 level_off[h+1] |
 //   u64 payload bytes | u64 FNV-1a 64 of the payload | payload
-// payload = sorted keys (8n) | perm (4n) | objects SoA (4En) | node boxes SoA
+// payload = sorted keys (8n) | perm (4n; absent when the stored cfg.flags has
+//           QCP_BUILD_NO_PERM: the index kept none) | objects SoA (4En) | node boxes SoA
 //           (8D n_nodes) | [explicit layouts and partitioned MPHR] node first
 //           (4 n_nodes) | node count (4 n_nodes)
 #include <hip/hip_runtime.h>
@@ -52,13 +53,14 @@ qcp_status qcp_save(const qcp_index* ix, const char* path) try {
   const uint32_t E = in.obj_kind == QCP_OBJ_BOXES ? 2 * D : D;
   const bool expl =
       in.layout == QCP_LAYOUT_BVH || in.layout == QCP_LAYOUT_GUTTMAN || in.partitions > 1;
+  const bool has_perm = in.has_perm != 0;
   std::vector<uint64_t> keys(n);
-  std::vector<uint32_t> perm(n);
+  std::vector<uint32_t> perm(has_perm ? n : 0);
   std::vector<float> objs((size_t)E * n), boxes((size_t)2 * D * m);
   std::vector<uint32_t> first(m), count(m);
   std::vector<uint8_t> leaf(m);
-  s = qcp_export(ix, keys.data(), perm.data(), objs.data(), boxes.data(), first.data(),
-                 count.data(), leaf.data());
+  s = qcp_export(ix, keys.data(), has_perm ? perm.data() : nullptr, objs.data(), boxes.data(),
+                 first.data(), count.data(), leaf.data());
   if (s != QCP_OK) return s;
   std::vector<uint8_t> payload;
   auto put = [&](const void* p, size_t bytes) {
@@ -66,7 +68,7 @@ qcp_status qcp_save(const qcp_index* ix, const char* path) try {
     payload.insert(payload.end(), b, b + bytes);
   };
   put(keys.data(), 8 * n);
-  put(perm.data(), 4 * n);
+  if (has_perm) put(perm.data(), 4 * n);
   put(objs.data(), 4 * objs.size());
   put(boxes.data(), 4 * boxes.size());
   if (expl) {
@@ -83,6 +85,7 @@ qcp_status qcp_save(const qcp_index* ix, const char* path) try {
   cfg.key_src = in.key_src;
   cfg.obj_kind = in.obj_kind;
   cfg.partitions = in.partitions;
+  cfg.flags = has_perm ? 0u : (uint32_t)QCP_BUILD_NO_PERM;
   int32_t h = (int32_t)in.height;
   std::vector<uint64_t> level_off(h + 1, 0);
   for (int l = 0; l < h; ++l) level_off[l + 1] = level_off[l] + in.level_count[l];
@@ -138,7 +141,10 @@ qcp_status qcp_load(const char* path, uint32_t device, qcp_index** out) try {
   if (cfg.partitions > 65536) return QCP_EFORMAT;
   const bool expl =
       cfg.layout == QCP_LAYOUT_BVH || cfg.layout == QCP_LAYOUT_GUTTMAN || cfg.partitions > 1;
-  const uint64_t expect = 8 * n + 4 * n + 4ull * E * n + 8ull * D * m + (expl ? 8 * m : 0);
+  if (cfg.flags & ~(uint32_t)QCP_BUILD_NO_PERM) return QCP_EFORMAT;
+  const bool has_perm = !(cfg.flags & QCP_BUILD_NO_PERM);
+  const uint64_t expect =
+      8 * n + (has_perm ? 4 * n : 0) + 4ull * E * n + 8ull * D * m + (expl ? 8 * m : 0);
   if (pbytes != expect) return QCP_EFORMAT;
   std::vector<uint8_t> payload(pbytes);
   if (pbytes && fread(payload.data(), 1, pbytes, fc.f) != pbytes) return QCP_EFORMAT;
@@ -146,8 +152,8 @@ qcp_status qcp_load(const char* path, uint32_t device, qcp_index** out) try {
   const uint8_t* p = payload.data();
   const uint64_t* keys = (const uint64_t*)p;
   p += 8 * n;
-  const uint32_t* perm = (const uint32_t*)p;
-  p += 4 * n;
+  const uint32_t* perm = has_perm ? (const uint32_t*)p : nullptr;
+  if (has_perm) p += 4 * n;
   const float* objs = (const float*)p;
   p += 4ull * E * n;
   const float* boxes = (const float*)p;

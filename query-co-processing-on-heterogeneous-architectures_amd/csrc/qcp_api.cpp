@@ -84,6 +84,8 @@ static qcp_status set_device(int dev) {
 static void index_free_scratch(qcp_index* ix) {
   dfree(ix->d_keys2);
   dfree(ix->d_perm2);
+  dfree(ix->d_carry);
+  ix->carry_words = 0;
   dfree(ix->d_sort_ghist);
   dfree(ix->d_sort_status);
 }
@@ -127,7 +129,157 @@ static qcp_status validate_cfg(const qcp_build_cfg* cfg) {
   if (cfg->order == QCP_ORDER_KMEANS_HILBERT && (cfg->flags & QCP_BUILD_GRAPH)) return QCP_EINVAL;
   if (cfg->partitions > 1 && (cfg->layout != QCP_LAYOUT_MPHR || cfg->partitions > 65536))
     return QCP_EINVAL;  // partitions are MPHR's (src/tree/mphr.cpp:60-108)
+  if (cfg->flags & ~(uint32_t)(QCP_BUILD_GRAPH | QCP_BUILD_NO_PERM | QCP_BUILD_SORT_INDEX |
+                               QCP_BUILD_SORT_OBJECTS))
+    return QCP_EINVAL;
+  if ((cfg->flags & QCP_BUILD_SORT_INDEX) && (cfg->flags & QCP_BUILD_SORT_OBJECTS))
+    return QCP_EINVAL;
+  // the Guttman insertion and the k-means composite keys read the sorted
+  // (key, input index) pairs on the host
+  if ((cfg->flags & QCP_BUILD_SORT_OBJECTS) &&
+      (cfg->layout == QCP_LAYOUT_GUTTMAN || cfg->order == QCP_ORDER_KMEANS_HILBERT))
+    return QCP_EINVAL;
   return QCP_OK;
+}
+
+// Bit width of the input index in a packed sort element (indices < n).
+static unsigned index_bits(uint64_t n) {
+  unsigned pbits = 1;
+  while (pbits < 32 && (1ull << pbits) < n) ++pbits;
+  return pbits;
+}
+
+// First pass of the index form whose output packs the key bits still to be
+// sorted with the input index into one 64-bit word (-1: none).
+static int pack_pass_for(unsigned bits, unsigned pbits) {
+  const unsigned passes = (bits + 7) / 8;
+  for (unsigned j = 0; j + 1 < passes; ++j)
+    if (bits - 8 * (j + 1) + pbits <= 64) return (int)j;
+  return -1;
+}
+
+// Default choice between the two key-sort forms for `bits`-bit keys, by a
+// model of their bytes a key at the rates the build's kernels reach on
+// MI355X (one-sweep passes 3.0 TB/s in the index form and 3.4 TB/s in the
+// object form, whose payload rounds overlap; the gather's raw HBM rate
+// 5.6 TB/s; streamed leaf boxes 4.6 TB/s), which orders the two forms as
+// measured on the 16M and 33.5M-point 3-D, 67M-box and 134M-point 2-D
+// builds.
+// Index form: pass 0 reads the key (the index is the position); a pass
+// writes key + index (12 bytes), or 8 from the pass that packs them on, and
+// the next reads what it wrote; then the gather reads the permutation and
+// each object (a whole 128-byte line once the objects outgrow the 256 MiB
+// Infinity Cache) and writes it, reducing the leaf boxes at fanout 32 / 64.
+// Object form: a pass moves the remainder (8 bytes while more than 32 bits
+// remain, else 4; the odd payload word's 4 bytes then ride in an 8-byte slot
+// or alone) and the W = E (+1) payload words; pass 0 reads the key and the E
+// object words; the last pass writes the W words alone; then the leaf boxes
+// read the objects again.
+static bool carry_cheaper(const qcp_index* ix, unsigned bits, bool keep_perm) {
+  const unsigned passes = (bits + 7) / 8;
+  if (passes == 0) return false;
+  const double E4 = 4.0 * ix->E;
+  const double W4 = E4 + (keep_perm ? 4.0 : 0.0);
+  const int pk = pack_pass_for(bits, index_bits(ix->n));
+  double idx_sort = 0.0, obj_sort = 0.0;
+  for (unsigned p = 0; p < passes; ++p) {
+    const bool packed_in = pk >= 0 && (int)p > pk, packed_out = pk >= 0 && (int)p >= pk;
+    idx_sort += (p == 0 ? 8.0 : (packed_in ? 8.0 : 12.0)) + (packed_out ? 8.0 : 12.0);
+    const bool last = p + 1 == passes;
+    const double in = p == 0 ? 8.0 + E4 : (bits - 8 * p > 32 ? 8.0 : 4.0) + W4;
+    const double out = last ? W4 : (bits - 8 * (p + 1) > 32 ? 8.0 : 4.0) + W4;
+    obj_sort += in + out;
+  }
+  const bool big = (double)ix->n * E4 > 256.0 * 1048576.0;
+  const double gather = 8.0 + (big ? 128.0 : E4) + E4;
+  const bool fused = !ix->expl && (ix->F == 32 || ix->F == 64);
+  const double t_index = idx_sort / 3.0 + gather / 5.6;
+  const double t_object = obj_sort / 3.4 + (fused ? E4 / 4.6 : 0.0);
+  return t_object < t_index;
+}
+
+// The key sort of a build over keys of `bits` bits: objects carried through
+// the passes, or (key, input index) pairs and a gather.
+static bool use_carry_sort(const qcp_index* ix, unsigned bits) {
+  const qcp_build_cfg& c = ix->cfg;
+  if (c.layout == QCP_LAYOUT_GUTTMAN || c.order == QCP_ORDER_KMEANS_HILBERT) return false;
+  if (c.flags & QCP_BUILD_SORT_INDEX) return false;
+  if (c.flags & QCP_BUILD_SORT_OBJECTS) return true;
+  return carry_cheaper(ix, bits, !(c.flags & QCP_BUILD_NO_PERM));
+}
+
+// The object-carrying sort keeps the input indices unless the caller waived them.
+static bool carry_keeps_perm(const qcp_index* ix) { return !(ix->cfg.flags & QCP_BUILD_NO_PERM); }
+
+// Payload row stride of the object-carrying sort's scratch sets.
+static uint64_t carry_stride(uint64_t n) { return ((n + 63) / 64) * 64; }
+
+// Scratch of the object-carrying sort: two sets of E (+1) payload word arrays
+// (the key remainders use d_keys2 and d_keys).
+static qcp_status ensure_carry_scratch(qcp_index* ix) {
+  const uint64_t W = ix->E + (carry_keeps_perm(ix) ? 1 : 0);
+  const uint64_t words = 2 * W * carry_stride(ix->n > 0 ? ix->n : 1);
+  if (ix->carry_words >= words) return QCP_OK;
+  dfree(ix->d_carry);
+  ix->carry_words = 0;
+  QCP_TRY(dalloc(&ix->d_carry, words));
+  ix->carry_words = words;
+  return QCP_OK;
+}
+
+// The sort form of the next build over keys of `bits` bits, and its scratch
+// (allocated before any graph capture of the build).
+static qcp_status prepare_build_form(qcp_index* ix, unsigned bits) {
+  ix->carry = use_carry_sort(ix, bits);
+  ix->perm_valid = !ix->carry || carry_keeps_perm(ix);
+  if (ix->carry && ix->n) QCP_TRY(ensure_carry_scratch(ix));
+  return QCP_OK;
+}
+
+// The sort form of a build about to be captured: the captured sort plans its
+// passes on the device for any key width, and its form is chosen here from
+// the keys of the objects at capture time (one eager key pass); replays keep
+// that form.
+static qcp_status prepare_graph_form(qcp_index* ix, const float* d_objs) {
+  if (ix->n == 0) return prepare_build_form(ix, 0);
+  hipStream_t s = ix->stream;
+  HIP_TRY(launch_fill_u32((uint32_t*)ix->d_key_or, 2, 0u, s));
+  HIP_TRY(launch_fill_u32(ix->d_bad, 1, 0u, s));
+  HIP_TRY(launch_keys(d_objs, ix->n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
+                      ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, nullptr,
+                      ix->d_key_or, ix->d_bad, s));
+  ix->keys_stale = true;  // d_keys now holds the unsorted keys
+  uint64_t key_or = 0;
+  HIP_TRY(hipMemcpyAsync(&key_or, ix->d_key_or, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return prepare_build_form(ix, key_or ? 64u - (unsigned)__builtin_clzll(key_or) : 0u);
+}
+
+// Arguments of the object-carrying sort over the index's buffers: the keys
+// are in d_keys; the key remainders alternate between d_keys2 (set 0) and
+// d_keys (set 1, free once pass 0 has read the keys).
+static CarryArgs carry_args(qcp_index* ix, const float* d_objs, unsigned bits,
+                            const unsigned long long* key_or) {
+  const bool idx = carry_keeps_perm(ix);
+  const uint64_t cs = carry_stride(ix->n);
+  CarryArgs a{};
+  a.keys = ix->d_keys;
+  a.objs = d_objs;
+  a.krem[0] = ix->d_keys2;
+  a.krem[1] = ix->d_keys;
+  a.pay[0] = ix->d_carry;
+  a.pay[1] = ix->d_carry + (ix->E + (idx ? 1 : 0)) * cs;
+  a.cs = cs;
+  a.obj = ix->d_obj;
+  a.ostride = ix->obj_stride;
+  a.perm = idx ? ix->d_perm : nullptr;
+  a.n = ix->n;
+  a.key_or = key_or;
+  a.bits = bits;
+  a.ghist = ix->d_sort_ghist;
+  a.status = ix->d_sort_status;
+  a.words = (uint64_t)sort_tiles(ix->n) * 256;
+  return a;
 }
 
 // Level sizes of a packed tree, root first (Tree::GetLevelNodeCount,
@@ -293,6 +445,8 @@ static void set_info_shape(qcp_index* ix) {
   in.kmeans_iterations = ix->kmeans_its;
   in.kmeans_ms[0] = ix->kmeans_ms[0];
   in.kmeans_ms[1] = ix->kmeans_ms[1];
+  in.sort_form = ix->carry ? 1u : 0u;
+  in.has_perm = ix->perm_valid ? 1u : 0u;
 }
 
 // Guttman layout: insert the objects on the host in their sorted order
@@ -339,13 +493,6 @@ static qcp_status guttman_from_sorted(qcp_index* ix, const float* d_objs, double
 
 // Node boxes of the rank-ordered objects in d_obj: bottom-up over the packed
 // levels, or level by level of an explicit tree.
-// Bit width of the input index in a packed sort element (indices < n).
-static unsigned index_bits(uint64_t n) {
-  unsigned pbits = 1;
-  while (pbits < 32 && (1ull << pbits) < n) ++pbits;
-  return pbits;
-}
-
 // Fanout whose leaf boxes the build's gather writes itself (0: none).
 static uint32_t fused_leaf_fanout(const qcp_index* ix) {
   return (!ix->expl && ix->h >= 1 && (ix->F == 32 || ix->F == 64)) ? ix->F : 0;
@@ -406,6 +553,15 @@ static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned pas
   HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
                       ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, nullptr,
                       ix->d_key_or, ix->d_bad, s));
+  if (ix->carry) {
+    // every pass a key can need; the last one that reorders writes the index
+    const unsigned cpasses = (ix->D * ix->nb + 7) / 8;
+    HIP_TRY(launch_sort_carry(carry_args(ix, d_objs, 0, ix->d_key_or), ix->D, ix->box,
+                              carry_keeps_perm(ix), cpasses, s));
+    HIP_TRY(launch_carry_identity(d_objs, n, ix->d_key_or, ix->D, ix->box, ix->d_obj,
+                                  ix->obj_stride, carry_keeps_perm(ix) ? ix->d_perm : nullptr, s));
+    return enqueue_boxes(ix, false);
+  }
   if (passes & 1u) return QCP_EINVAL;  // odd pass counts are not captured
   const unsigned pbits = index_bits(n);
   HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
@@ -423,6 +579,7 @@ static qcp_status index_run_build_graph(qcp_index* ix, const float* d_objs) {
   hipStream_t s = ix->stream;
   ix->h_nbox.clear();
   if (ix->n == 0) return QCP_OK;
+  if (!ix->build_exec || ix->build_exec_objs != d_objs) QCP_TRY(prepare_graph_form(ix, d_objs));
   if (!ix->build_exec || ix->build_exec_objs != d_objs) {
     if (ix->build_exec) {
       hipGraphExecDestroy(ix->build_exec);
@@ -489,20 +646,29 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
       bits = (unsigned)kb;
     }
     const unsigned passes = (bits + 7) / 8;
+    QCP_TRY(prepare_build_form(ix, bits));
+    if (ix->carry) {
+      // objects carried through the passes: the last pass writes the index
+      uint32_t* perm_out = carry_keeps_perm(ix) ? ix->d_perm : nullptr;
+      if (passes == 0) {  // all keys equal: the input order is the rank order
+        HIP_TRY(launch_gather_objs(d_objs, nullptr, nullptr, n, 0, ix->D, ix->box, ix->d_obj,
+                                   ix->obj_stride, perm_out, 0, nullptr, 0, 0, s));
+      } else {
+        HIP_TRY(launch_sort_carry(carry_args(ix, d_objs, bits, nullptr), ix->D, ix->box,
+                                  perm_out != nullptr, passes, s));
+      }
+      ix->keys_stale = true;  // d_keys holds key remainders
+      HIP_TRY(hipEventRecord(ix->ev[2], s));
+      HIP_TRY(hipEventRecord(ix->ev[3], s));
+      QCP_TRY(enqueue_boxes(ix, false));
+    } else {
     // Packed sort elements from the first pass after which the key bits still
     // to be sorted and the input index fit 64 bits; the gather then recomputes
     // the keys from the objects.  Not for k-means keys (not a function of the
     // object alone) or Guttman builds (their host step reads the sorted keys).
-    int pack_pass = -1;
     const unsigned pbits = index_bits(n);
-    if (!guttman && ix->cfg.order != QCP_ORDER_KMEANS_HILBERT) {
-      for (unsigned j = 0; j + 1 < passes; ++j) {
-        if (bits - 8 * (j + 1) + pbits <= 64) {
-          pack_pass = (int)j;
-          break;
-        }
-      }
-    }
+    const int pack_pass = (!guttman && ix->cfg.order != QCP_ORDER_KMEANS_HILBERT)
+                              ? pack_pass_for(bits, pbits) : -1;
     HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
                              nullptr, ix->d_sort_ghist, ix->d_sort_status, s, pack_pass, pbits));
     if (passes == 0) HIP_TRY(launch_iota_u32(ix->d_perm, n, s));  // all keys equal
@@ -524,6 +690,7 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
     }
     HIP_TRY(hipEventRecord(ix->ev[3], s));
     QCP_TRY(enqueue_boxes(ix, fused_leaf_fanout(ix) != 0));
+    }
   } else {
     if (guttman) QCP_TRY(guttman_adopt(ix, nullptr, nullptr, nullptr, &host_ms));
     HIP_TRY(hipEventRecord(ix->ev[1], s));
@@ -725,6 +892,7 @@ qcp_status qcp_export(const qcp_index* ix, uint64_t* sorted_keys, uint32_t* perm
                       float* node_boxes, uint32_t* node_first, uint32_t* node_count,
                       uint8_t* node_leaf) try {
   if (!ix) return QCP_EINVAL;
+  if (perm && !ix->perm_valid) return QCP_EINVAL;  // the build kept no permutation
   QCP_TRY(set_device(ix->device));
   const uint64_t n = ix->n;
   hipStream_t s = ix->stream;
@@ -1205,6 +1373,7 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
         hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
       }
+      if (b->pipe_objs && ix->n) QCP_TRY(prepare_graph_form(ix, b->pipe_objs));
       hipGraph_t g = nullptr;
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       qcp_status cs = QCP_OK;
@@ -1430,7 +1599,7 @@ __attribute__((visibility("hidden"))) qcp_status qcp_internal_restore(
   std::vector<float> soa;
   if (n) {
     if (hipMemcpyAsync(ix->d_keys, keys, 8 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(ix->d_perm, perm, 4 * n, hipMemcpyHostToDevice, s) != hipSuccess)
+        (perm && hipMemcpyAsync(ix->d_perm, perm, 4 * n, hipMemcpyHostToDevice, s) != hipSuccess))
       return fail(QCP_EHIP);
     soa.resize((size_t)ix->E * n);
     for (uint64_t r = 0; r < n; ++r)
@@ -1453,6 +1622,7 @@ __attribute__((visibility("hidden"))) qcp_status qcp_internal_restore(
     for (uint64_t r = 0; r < n && ix->contain_ok; ++r)
       for (uint32_t d = 0; d < ix->D; ++d)
         if (objs[r * ix->E + d] > objs[r * ix->E + ix->D + d]) ix->contain_ok = false;
+  ix->perm_valid = perm != nullptr || n == 0;
   set_info_shape(ix);
   *out = ix;
   return QCP_OK;

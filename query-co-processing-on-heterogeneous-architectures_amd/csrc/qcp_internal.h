@@ -34,7 +34,12 @@ struct qcp_index {
   uint32_t kmeans_its = 0;       // Lloyd iterations of the last k-means build
   double kmeans_ms[2] = {0, 0};  // its assignment and centre-update device time
   uint64_t* d_keys = nullptr;    // sorted keys [n]
-  uint32_t* d_perm = nullptr;    // [n]
+  uint32_t* d_perm = nullptr;    // [n] input index of each rank (when perm_valid)
+  bool perm_valid = true;
+  bool carry = false;            // the last build carried the objects through the sort
+  // object-carrying sort: payload words of its two scratch sets
+  uint32_t* d_carry = nullptr;
+  uint64_t carry_words = 0;
   // build scratch
   uint64_t* d_keys2 = nullptr;
   uint32_t* d_perm2 = nullptr;

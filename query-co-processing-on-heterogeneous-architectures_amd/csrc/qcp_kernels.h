@@ -58,10 +58,42 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
                             uint32_t* ghist, uint64_t* status, hipStream_t s, int pack_pass = -1,
                             unsigned pbits = 0);
 constexpr uint32_t kSortGhistWords = 8 * 256 + 8;
+// Object-carrying LSD radix sort: every pass moves the key bits still to be
+// sorted and the objects (plus, with idx, their input indices) themselves, and
+// the last pass writes the rank-ordered SoA objects (and the permutation), so
+// the build needs no gather.  Scratch set j holds the key remainders
+// (krem[j], n 8-byte words, read as 4-byte words once at most 32 bits remain)
+// and the payload words (pay[j], [E (+1 with idx)][cs] 4-byte words).
+// key_or null: `bits` is the key width and passes = ceil(bits / 8); otherwise
+// the device word holding the OR of all keys decides how many passes reorder
+// (the others end at once) and which one is last (graph-captured builds,
+// which also launch launch_carry_identity for the all-zero case).
+struct CarryArgs {
+  const uint64_t* keys;    // pass 0: the keys
+  const float* objs;       // pass 0: the objects, row-major
+  uint64_t* krem[2];
+  uint32_t* pay[2];
+  uint64_t cs;             // payload row stride of the scratch sets
+  float* obj;              // last pass: objects [E][ostride]
+  uint64_t ostride;
+  uint32_t* perm;          // last pass (idx): input index of each rank
+  uint64_t n;
+  const unsigned long long* key_or;
+  unsigned bits;
+  uint32_t* ghist;         // kSortGhistWords words
+  uint64_t* status;        // sort_status_words(n) words
+  uint64_t words;          // look-back words of one pass: sort_tiles(n) * 256
+};
+hipError_t launch_sort_carry(const CarryArgs& a, unsigned D, bool box, bool idx,
+                             unsigned passes, hipStream_t s);
+// objects in input order (and perm_out = identity) when *key_or == 0; nothing otherwise
+hipError_t launch_carry_identity(const float* objs, uint64_t n, const unsigned long long* key_or,
+                                 unsigned D, bool box, float* obj, uint64_t stride,
+                                 uint32_t* perm_out, hipStream_t s);
 hipError_t scan_u32_u64(const uint32_t* in, uint64_t n, unsigned long long* out,
                         unsigned long long* block_sums, uint32_t max_blocks, hipStream_t s);
 // The build's gather: obj[e][r] = objs[perm[r]][e], with the permutation
-// either given (perm) or unpacked from packed sort elements (packed, pbits;
+// either given (perm; null = the identity) or unpacked from packed sort elements (packed, pbits;
 // written to perm_out; the sorted keys are then left to launch_keys_sorted).
 // fuse_leaves = 32 or 64 (packed layouts of that fanout): also writes the
 // leaf level's boxes (nbox at leaf_off), as launch_leaf_boxes would.

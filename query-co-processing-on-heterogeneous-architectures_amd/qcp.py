@@ -40,6 +40,18 @@ class BuildCfg(ctypes.Structure):
                  "order", "clusters", "kmeans_iters", "partitions")]
 
 ORDER = {"hilbert": 0, "kmeans": 1}
+# qcp_build_cfg.flags
+BUILD_GRAPH, BUILD_NO_PERM, BUILD_SORT_INDEX, BUILD_SORT_OBJECTS = 1, 2, 4, 8
+SORT = {None: 0, "auto": 0, "index": BUILD_SORT_INDEX, "objects": BUILD_SORT_OBJECTS}
+
+
+def build_flags(graph=False, sort=None, keep_perm=True):
+    """qcp_build_cfg.flags: graph-captured build, the key sort's form (None = the
+    library's choice by size, "index" = (key, input index) + gather, "objects" =
+    objects carried through the passes) and whether the index keeps the input
+    index of each rank."""
+    return ((BUILD_GRAPH if graph else 0) | SORT[sort] |
+            (0 if keep_perm else BUILD_NO_PERM))
 
 
 class IndexInfo(ctypes.Structure):
@@ -50,7 +62,8 @@ class IndexInfo(ctypes.Structure):
                 ("level_count", ctypes.c_uint64 * 32), ("tree_bytes", ctypes.c_uint64),
                 ("build_ms", ctypes.c_double), ("build_phase_ms", ctypes.c_double * 6),
                 ("partitions", ctypes.c_uint32), ("kmeans_iterations", ctypes.c_uint32),
-                ("kmeans_ms", ctypes.c_double * 2)]
+                ("kmeans_ms", ctypes.c_double * 2), ("sort_form", ctypes.c_uint32),
+                ("has_perm", ctypes.c_uint32)]
 
 
 class SearchCfg(ctypes.Structure):
@@ -252,20 +265,21 @@ class Index:
     @classmethod
     def build(cls, objs, dims, fanout=64, layout="mphr", key_mode="ref", key_src="lo",
               obj_kind="points", device=0, graph=False, order="hilbert", clusters=0,
-              kmeans_iters=0, partitions=1):
+              kmeans_iters=0, partitions=1, sort=None, keep_perm=True):
         """Build from host objects (N x D points or N x 2D boxes).  With
         graph=True the device pipeline runs as a captured hipGraph; order="kmeans"
         ranks k-means clusters by their centres' Hilbert keys first; partitions=P
-        splits an MPHR into P subtrees over contiguous rank ranges."""
+        splits an MPHR into P subtrees over contiguous rank ranges; sort and
+        keep_perm as for build_flags."""
         if isinstance(objs, DeviceBuffer):
             return cls.build_device(objs, dims, fanout, layout, key_mode, key_src, obj_kind,
-                                    device, graph, partitions)
+                                    device, graph, partitions, sort, keep_perm)
         objs = np.ascontiguousarray(objs, dtype=np.float32)
         E = dims * (2 if obj_kind == "boxes" else 1)
         n = objs.size // E if objs.size else 0
         cfg = BuildCfg(dims, fanout, LAYOUT[layout], KEY[key_mode], SRC[key_src],
-                       OBJ[obj_kind], device, 1 if graph else 0, ORDER[order], clusters,
-                       kmeans_iters, partitions)
+                       OBJ[obj_kind], device, build_flags(graph, sort, keep_perm),
+                       ORDER[order], clusters, kmeans_iters, partitions)
         h = P()
         check(lib().qcp_build(ctypes.byref(cfg), _p(objs) if n else None, n, ctypes.byref(h)),
               "qcp_build")
@@ -287,12 +301,14 @@ class Index:
 
     @classmethod
     def build_device(cls, buf, dims, fanout=64, layout="mphr", key_mode="ref", key_src="lo",
-                     obj_kind="points", device=0, graph=False, partitions=1):
+                     obj_kind="points", device=0, graph=False, partitions=1, sort=None,
+                     keep_perm=True):
         """Build from objects already resident in HBM (a DeviceBuffer)."""
         E = dims * (2 if obj_kind == "boxes" else 1)
         n = buf.nbytes // (4 * E)
         cfg = BuildCfg(dims, fanout, LAYOUT[layout], KEY[key_mode], SRC[key_src],
-                       OBJ[obj_kind], device, 1 if graph else 0, 0, 0, 0, partitions)
+                       OBJ[obj_kind], device, build_flags(graph, sort, keep_perm), 0, 0, 0,
+                       partitions)
         h = P()
         check(lib().qcp_build_device(ctypes.byref(cfg), buf.ptr, n, ctypes.byref(h)),
               "qcp_build_device")
@@ -337,15 +353,17 @@ class Index:
         n, m, D = inf["n"], inf["n_nodes"], inf["dims"]
         E = D * (2 if inf["obj_kind"] == 1 else 1)
         keys = np.zeros(max(n, 1), np.uint64)
-        perm = np.zeros(max(n, 1), np.uint32)
+        perm = np.zeros(max(n, 1), np.uint32) if inf["has_perm"] else None
         sobjs = np.zeros((max(n, 1), E), np.float32)
         boxes = np.zeros((max(m, 1), 2 * D), np.float32)
         first = np.zeros(max(m, 1), np.uint32)
         count = np.zeros(max(m, 1), np.uint32)
         leaf = np.zeros(max(m, 1), np.uint8)
-        check(lib().qcp_export(self._h, _p(keys), _p(perm), _p(sobjs), _p(boxes), _p(first),
-                               _p(count), _p(leaf)), "qcp_export")
-        return {"keys": keys[:n], "perm": perm[:n], "sobjs": sobjs[:n], "boxes": boxes[:m],
+        check(lib().qcp_export(self._h, _p(keys), _p(perm) if perm is not None else None,
+                               _p(sobjs), _p(boxes), _p(first), _p(count), _p(leaf)),
+              "qcp_export")
+        return {"keys": keys[:n], "perm": perm[:n] if perm is not None else None,
+                "sobjs": sobjs[:n], "boxes": boxes[:m],
                 "first": first[:m], "count": count[:m], "leaf": leaf[:m], "info": inf}
 
     def save(self, path):

@@ -22,18 +22,24 @@ def _oracle_tree(O, objs, D, is_box, F, key_mode, key_src, layout):
 
 
 def check_case(gpu, O, objs, D, is_box, F, key_mode="ref", key_src="lo", layout="mphr",
-               nq=200, seed=0, plain=True, queries=None):
-    tag = "D=%d box=%d F=%d key=%s src=%s layout=%s n=%d" % (D, is_box, F, key_mode, key_src,
-                                                             layout, len(objs))
+               nq=200, seed=0, plain=True, queries=None, sort=None, keep_perm=True):
+    tag = "D=%d box=%d F=%d key=%s src=%s layout=%s n=%d sort=%s perm=%d" % (
+        D, is_box, F, key_mode, key_src, layout, len(objs), sort, keep_perm)
     print("case", tag, flush=True)
     ix = gpu.Index.build(objs, D, fanout=F, layout=layout, key_mode=key_mode, key_src=key_src,
-                         obj_kind="boxes" if is_box else "points")
+                         obj_kind="boxes" if is_box else "points", sort=sort,
+                         keep_perm=keep_perm)
     t = _oracle_tree(O, objs, D, is_box, F, key_mode, key_src, layout)
     ex = ix.export()
     n = len(objs)
     assert ex["info"]["n"] == n
+    if sort is not None and n:
+        assert ex["info"]["sort_form"] == (1 if sort == "objects" else 0), tag
     assert_bits_equal(ex["keys"], t.keys, tag + " sorted keys")
-    assert_bits_equal(ex["perm"], t.perm, tag + " permutation")
+    if ex["perm"] is None:
+        assert not keep_perm and ex["info"]["sort_form"] == 1, tag + " permutation dropped"
+    else:
+        assert_bits_equal(ex["perm"], t.perm, tag + " permutation")
     assert_bits_equal(ex["sobjs"], t.sobjs, tag + " rank-ordered objects")
     assert ex["info"]["n_nodes"] == t.n_nodes, tag
     assert_bits_equal(ex["first"], t.first, tag + " node first")

@@ -160,3 +160,21 @@ def test_read_reference_mphr_rejects_malformed(qcp, tmp_path, bad):
     with pytest.raises(qcp.QcpError) as e:
         qcp.read_reference_mphr(p, 3, 192)
     assert e.value.status == -5
+
+
+def test_build_flag_rules(qcp):
+    """Sort-form and permutation flags are checked before any device work: both
+    sort forms at once, the object form for Guttman layouts or the k-means order
+    (whose host steps read the sorted input indices), and unknown bits are refused."""
+    import ctypes
+
+    L = qcp.lib()
+    h = ctypes.c_void_p()
+    both = qcp.BUILD_SORT_INDEX | qcp.BUILD_SORT_OBJECTS
+    for cfg in (qcp.BuildCfg(2, 64, 0, 0, 0, 0, 0, both),
+                qcp.BuildCfg(2, 64, 3, 0, 0, 0, 0, qcp.BUILD_SORT_OBJECTS),
+                qcp.BuildCfg(2, 64, 0, 0, 0, 0, 0, qcp.BUILD_SORT_OBJECTS, 1),
+                qcp.BuildCfg(2, 64, 0, 0, 0, 0, 0, 16)):
+        assert L.qcp_build(ctypes.byref(cfg), None, 0, ctypes.byref(h)) == -1
+    assert qcp.build_flags(graph=True, sort="objects", keep_perm=False) == 1 | 8 | 2
+    assert qcp.build_flags() == 0
