@@ -32,7 +32,7 @@ namespace qcp {
 // s = flip ^ above and the rotation.  Takes this level's digit t (bit d =
 // coordinate d), returns the D bits it appends to the index, advances (s, rot).
 template <int D>
-QCP_HD uint32_t hilbert_level(uint32_t& s, unsigned& rot, uint32_t t) {
+QCP_HD constexpr uint32_t hilbert_level(uint32_t& s, unsigned& rot, uint32_t t) {
   const uint32_t nd_ones = (1u << D) - 1u;
   const uint32_t x = s ^ t;
   const uint32_t bits = ((x >> rot) | (x << (D - rot))) & nd_ones;
@@ -87,7 +87,7 @@ struct HilbertTable {
 };
 
 template <int D>
-QCP_HD uint16_t hilbert_table_entry(uint32_t e) {
+QCP_HD constexpr uint16_t hilbert_table_entry(uint32_t e) {
   constexpr int L = HilbertTable<D>::L, IN = HilbertTable<D>::IN;
   const uint32_t in = e & ((1u << IN) - 1u);
   const uint32_t st = e >> IN;
@@ -155,7 +155,7 @@ QCP_HD uint64_t hilbert_key_table(const uint32_t* c, unsigned nb, const uint16_t
 // levels from the walk's initial state: entry g_{D-1} << (D-1)r | ... | g_0
 // (r bits each) holds the D*r index bits (low 8) and the state after them.
 template <int D>
-QCP_HD uint16_t hilbert_table_entry_cm(uint32_t e) {
+QCP_HD constexpr uint16_t hilbert_table_entry_cm(uint32_t e) {
   constexpr int L = HilbertTable<D>::L, IN = HilbertTable<D>::IN;
   const uint32_t st = e >> IN;
   uint32_t in = 0;  // level-major digits, top level first
@@ -168,7 +168,7 @@ QCP_HD uint16_t hilbert_table_entry_cm(uint32_t e) {
 }
 
 template <int D>
-QCP_HD uint16_t hilbert_start_entry(int r, uint32_t e) {
+QCP_HD constexpr uint16_t hilbert_start_entry(int r, uint32_t e) {
   uint32_t s = 0;
   unsigned rot = 0;
   uint32_t out = 0;

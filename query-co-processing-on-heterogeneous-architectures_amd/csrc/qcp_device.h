@@ -178,4 +178,27 @@ __device__ __forceinline__ bool overlap_test(const float* ql, const float* qh, c
   return !(t > 0.0f);
 }
 
+// Point v inside the closed box [pl, ph] given canonical bounds (no NaN):
+// med3(v[d], pl[d], ph[d]) returns v[d] itself when pl[d] <= v[d] <= ph[d]
+// and the bound it passed otherwise, so the point is inside iff every
+// difference med3 - v is zero; the differences of distinct floats are never
+// zero (denormals kept), and one D-operand max of their magnitudes decides
+// all dimensions.  A difference is NaN only for an infinite coordinate equal
+// to its bound (inf - inf), which is inside; fmaxf ignores it, and the final
+// !(t > 0) accepts a max that is NaN.  With pl[d] = ql[d] (or -inf for a NaN
+// ql[d]) and ph[d] = qh[d] (or +inf for a NaN qh[d]) and pl <= ph, this is the
+// reference's test for a point (a lower and upper corner both equal to v):
+// rejected iff ql[d] > v[d] or qh[d] < v[d] for some d
+// (src/node/node_soa.cpp:94-113), in D med3, D subtractions, a max and a
+// compare where overlap_test takes 2D subtractions, D maxes and a compare.
+// A query with pl[d] > ph[d] rejects every point; callers handle it apart.
+template <int D>
+__device__ __forceinline__ bool point_test(const float* pl, const float* ph, const float* v) {
+  float t = fabsf(__builtin_amdgcn_fmed3f(v[0], pl[0], ph[0]) - v[0]);
+#pragma unroll
+  for (int d = 1; d < D; ++d)
+    t = fmaxf(t, fabsf(__builtin_amdgcn_fmed3f(v[d], pl[d], ph[d]) - v[d]));
+  return !(t > 0.0f);
+}
+
 }  // namespace qcp

@@ -113,6 +113,10 @@ struct Frame {
 template <int D>
 struct Ctx {
   float ql[D], qh[D];
+  // the point test's canonical bounds: NaN bounds as -inf / +inf (a NaN bound
+  // rejects nothing); pempty: some dimension has pl > ph (no point can hit)
+  float pl[D], ph[D];
+  bool pempty;
   uint32_t q;
   uint32_t cnt;           // hits of this item (wave-uniform part)
   uint32_t cnt_lane;      // per-lane hits of the 4-entries-a-lane leaf path
@@ -130,6 +134,16 @@ struct Ctx {
   uint64_t ie, le;        // entries visited (the canonical traversal unless contain is on)
   uint32_t cr;            // contained subtree runs
 };
+
+// The leaf test of one entry (lo, hi: its lower and upper corners; the same
+// for a point): the box overlap test, or for points the med3 form over the
+// canonical bounds (scan_runs skips the leaf runs of a query that no point
+// can hit, pempty, after counting their entries).
+template <int D, bool BOX>
+__device__ __forceinline__ bool leaf_hit(const Ctx<D>& c, const float* lo, const float* hi) {
+  if (BOX) return overlap_test<D>(c.ql, c.qh, lo, hi);
+  return point_test<D>(c.pl, c.ph, lo);
+}
 
 // Output modes of the persistent form.
 enum { kOutCount = 0, kOutHash = 1, kOutIds = 2 };
@@ -280,8 +294,9 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
       // entries a lane from one 16-byte load per column, U / 4 groups in
       // flight (a quarter of the chunk path's load instructions for the same
       // bytes; the order of the hits does not matter for a count or a sum).
-      // Hits are counted per lane and summed when the item ends.  The run
-      // must start on a 16-byte boundary.
+      // Hits are counted per lane and summed when the item ends (a scalar
+      // popcount of each entry's hit mask loads the scalar unit, which the
+      // traversal also needs).  The run must start on a 16-byte boundary.
       constexpr int G = U / 4;
       if ((cs & 3u) == 0) {
         while (ce - cs >= (uint32_t)(G * 256)) {
@@ -300,7 +315,7 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
               float v[E];
 #pragma unroll
               for (int e = 0; e < E; ++e) v[e] = w[g][e][j];
-              const bool hit = overlap_test<D>(c.ql, c.qh, v, BOX ? v + D : v);
+              const bool hit = leaf_hit<D, BOX>(c, v, BOX ? v + D : v);
               c.cnt_lane += hit ? 1u : 0u;
               if (OUT == kOutHash) {
                 const uint32_t id = cs + (uint32_t)(g * 256) + 4u * lane + (uint32_t)j + 1u;
@@ -331,7 +346,7 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
 #pragma unroll
       for (int u = 0; u < U; ++u)
         chunk_out<D, OUT>(a, c, cs + (uint32_t)(u * kWave),
-                          overlap_test<D>(c.ql, c.qh, w[u], BOX ? w[u] + D : w[u]));
+                          leaf_hit<D, BOX>(c, w[u], BOX ? w[u] + D : w[u]));
       cs += (uint32_t)(U * kWave);
     }
     uint32_t st[U];
@@ -363,6 +378,7 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
           cs = uni((uint32_t)s);
           ce = uni((uint32_t)e);
           c.le += ce - cs;
+          if (!BOX && c.pempty) cs = ce;  // visited, and no point can hit
         }
         break;
       }
@@ -391,7 +407,7 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
       if (len[u]) {
         const float* lo = v[u];
         const float* hi = BOX ? v[u] + D : v[u];
-        chunk_out<D, OUT>(a, c, st[u], (lane < len[u]) & overlap_test<D>(c.ql, c.qh, lo, hi));
+        chunk_out<D, OUT>(a, c, st[u], (lane < len[u]) & leaf_hit<D, BOX>(c, lo, hi));
       }
     }
   }
@@ -400,12 +416,17 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
 template <int D>
 __device__ __forceinline__ void load_query(const SearchArgs& a, uint32_t q, Ctx<D>& c) {
   const float* qp = a.queries + (uint64_t)q * 2 * D;
+  c.pempty = false;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     c.ql[d] = qp[d];
     c.qh[d] = qp[D + d];
+    c.pl[d] = c.ql[d] != c.ql[d] ? -__builtin_huge_valf() : c.ql[d];
+    c.ph[d] = c.qh[d] != c.qh[d] ? __builtin_huge_valf() : c.qh[d];
+    c.pempty |= c.pl[d] > c.ph[d];
   }
 }
+
 
 __device__ __forceinline__ int level_of(const SearchArgs& a, uint32_t node) {
   int l = 0;
