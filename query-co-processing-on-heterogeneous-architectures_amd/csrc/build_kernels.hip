@@ -101,16 +101,35 @@ __device__ __forceinline__ void object_point(const float* v, uint32_t (&c)[D], b
 // 256 x kKeysPerThread block of the grid-stride loop), their walks interleaved.
 constexpr int kKeysPerThread = 4;
 
-template <int D, bool BOX, bool REF, bool CENTER>
+// HIST: also the digit counts of every byte a key of this width can hold,
+// for the radix sort that follows (ghist[p][256], zero at entry; the sort
+// then skips its own counting pass, which read every key again), and zero the
+// sort's first look-back words (status0[0 .. status_words)).  A byte whose
+// digit all of a wave's keys share is added once for the wave (the zero bytes
+// above the keys' width, and clustered keys' high bytes), others one atomic a
+// key, in LDS; each workgroup adds its counts to ghist once at the end.
+template <int D, bool BOX, bool REF, bool CENTER, bool HIST>
 __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, uint64_t n,
                                               uint64_t* __restrict__ keys,
                                               uint32_t* __restrict__ idx,
-                                              unsigned long long* key_or, uint32_t* bad) {
+                                              unsigned long long* key_or, uint32_t* bad,
+                                              uint32_t* __restrict__ ghist,
+                                              uint64_t* __restrict__ status0,
+                                              uint64_t status_words) {
   constexpr int E = BOX ? 2 * D : D;
   constexpr int R = kKeysPerThread;
   constexpr int NB = key_nb<D, REF>();
+  constexpr int NBYTES = (D * NB + 7) / 8;
   __shared__ KeyTables<D, NB> kt;
-  fill_key_tables<D, NB>(kt);
+  __shared__ uint32_t hh[HIST ? NBYTES : 1][256];
+  if (HIST) {
+#pragma unroll
+    for (int p = 0; p < NBYTES; ++p) hh[p][threadIdx.x] = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < status_words;
+         i += (uint64_t)gridDim.x * 256)
+      status0[i] = 0;
+  }
+  fill_key_tables<D, NB>(kt);  // ends with a barrier
   uint64_t acc = 0;
   bool any_bad = false, any_inv = false;
   for (uint64_t base = (uint64_t)blockIdx.x * (256 * R); base < n;
@@ -145,6 +164,37 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
         if (idx != nullptr) idx[i] = (uint32_t)i;  // callers whose sort reads input indices
         acc |= k[r];
       }
+    }
+    if (HIST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const bool valid = base + (uint64_t)r * 256 + threadIdx.x < n;
+        const uint64_t vmask = ballot(valid);
+        if (vmask == 0) continue;
+        const int first = __builtin_ctzll(vmask);
+        const uint32_t lo = (uint32_t)k[r], hi = (uint32_t)(k[r] >> 32);
+        const uint32_t xl = lo ^ (uint32_t)__builtin_amdgcn_readlane((int)lo, first);
+        const uint32_t xh = hi ^ (uint32_t)__builtin_amdgcn_readlane((int)hi, first);
+#pragma unroll
+        for (int p = 0; p < NBYTES; ++p) {
+          const uint32_t word = p < 4 ? lo : hi, x = p < 4 ? xl : xh;
+          const int sh = 8 * (p % 4);
+          const uint32_t dig = (word >> sh) & 0xFFu;
+          if (ballot(valid && ((x >> sh) & 0xFFu) != 0) == 0) {
+            if ((int)lane_id() == first) atomicAdd(&hh[p][dig], (uint32_t)__popcll(vmask));
+          } else if (valid) {
+            atomicAdd(&hh[p][dig], 1u);
+          }
+        }
+      }
+    }
+  }
+  if (HIST) {
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NBYTES; ++p) {
+      const uint32_t v = hh[p][threadIdx.x];
+      if (v) atomicAdd(&ghist[p * 256 + threadIdx.x], v);
     }
   }
   // wave OR, then one atomic per wave, and only when it adds bits: every wave
@@ -721,11 +771,21 @@ static inline unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 655
 
 hipError_t launch_keys(const float* objs, uint64_t n, unsigned D, bool box, bool ref,
                        bool center, unsigned nb, uint64_t* keys, uint32_t* idx,
-                       unsigned long long* key_or, uint32_t* bad, hipStream_t s) {
+                       unsigned long long* key_or, uint32_t* bad, hipStream_t s,
+                       uint32_t* ghist, uint64_t* status0, uint64_t status_words) {
   if (nb != (ref ? (D == 2 ? 31u : 20u) : 16u)) return hipErrorInvalidValue;
-  unsigned g = grid_for((n + kKeysPerThread - 1) / kKeysPerThread, 256, 8192);
-#define QCP_KEYS(DD, BB, RR, CC) \
-  k_keys<DD, BB, RR, CC><<<g, 256, 0, s>>>(objs, n, keys, idx, key_or, bad)
+  if (ghist == nullptr && status_words != 0) return hipErrorInvalidValue;
+  // counting workgroups also add their digit counts to ghist once each: fewer
+  // of them, each walking more keys
+  unsigned g = grid_for((n + kKeysPerThread - 1) / kKeysPerThread, 256,
+                        ghist != nullptr ? 2048 : 8192);
+#define QCP_KEYS(DD, BB, RR, CC)                                                                \
+  if (ghist != nullptr)                                                                         \
+    k_keys<DD, BB, RR, CC, true><<<g, 256, 0, s>>>(objs, n, keys, idx, key_or, bad, ghist,      \
+                                                   status0, status_words);                     \
+  else                                                                                          \
+    k_keys<DD, BB, RR, CC, false><<<g, 256, 0, s>>>(objs, n, keys, idx, key_or, bad, nullptr,   \
+                                                    nullptr, 0)
   if (D == 2) {
     if (box) {
       if (ref) { if (center) QCP_KEYS(2, true, true, true); else QCP_KEYS(2, true, true, false); }
@@ -1423,7 +1483,7 @@ uint64_t sort_status_words(uint64_t n) { return 2ull * sort_tiles(n > 0 ? n : 1)
 hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2,
                             uint64_t n, unsigned passes, const unsigned long long* key_or,
                             uint32_t* ghist, uint64_t* status, hipStream_t s, int pack_pass,
-                            unsigned pbits) {
+                            unsigned pbits, bool counted) {
   if (passes > (unsigned)kMaxSortPasses) return hipErrorInvalidValue;
   if (pack_pass >= 0 && (key_or != nullptr || pack_pass + 1 >= (int)passes || pbits == 0))
     return hipErrorInvalidValue;
@@ -1432,11 +1492,13 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
   const uint32_t nt = sort_tiles(n);
   const uint64_t words = (uint64_t)nt * 256;
   uint32_t* tile_ctr = ghist + kMaxSortPasses * 256;
-  hipError_t e = launch_fill_u32(ghist, kMaxSortPasses * 256 + kMaxSortPasses, 0u, s);
-  if (e != hipSuccess) return e;
-  const uint32_t ht = (uint32_t)((n + kHistTile - 1) / kHistTile);
-  const unsigned g = ht < 2048 ? ht : 2048;
-  k_sort_hist_all<<<g, kHistThreads, 0, s>>>(keys, n, passes, key_or, ghist, status, words);
+  if (!counted) {
+    hipError_t e = launch_fill_u32(ghist, kMaxSortPasses * 256 + kMaxSortPasses, 0u, s);
+    if (e != hipSuccess) return e;
+    const uint32_t ht = (uint32_t)((n + kHistTile - 1) / kHistTile);
+    const unsigned g = ht < 2048 ? ht : 2048;
+    k_sort_hist_all<<<g, kHistThreads, 0, s>>>(keys, n, passes, key_or, ghist, status, words);
+  }
   uint64_t* kin = keys;
   uint32_t* vin = nullptr;  // the first pass takes positions as the input indices
   uint64_t* kout = keys2;
@@ -1472,18 +1534,20 @@ hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uin
 
 
 hipError_t launch_sort_carry(const CarryArgs& a, unsigned D, bool box, bool idx,
-                             unsigned passes, hipStream_t s) {
+                             unsigned passes, hipStream_t s, bool counted) {
   if (passes > (unsigned)kMaxSortPasses || (D != 2 && D != 3)) return hipErrorInvalidValue;
   if (a.key_or == nullptr && (a.bits > 64 || (a.bits + 7) / 8 != passes))
     return hipErrorInvalidValue;
   if (a.n == 0 || passes == 0) return hipSuccess;
   const uint32_t nt = sort_tiles(a.n);
   if (a.words != (uint64_t)nt * 256) return hipErrorInvalidValue;
-  hipError_t e = launch_fill_u32(a.ghist, kMaxSortPasses * 256 + kMaxSortPasses, 0u, s);
-  if (e != hipSuccess) return e;
-  const uint32_t ht = (uint32_t)((a.n + kHistTile - 1) / kHistTile);
-  k_sort_hist_all<<<ht < 2048 ? ht : 2048, kHistThreads, 0, s>>>(a.keys, a.n, passes, a.key_or,
-                                                                  a.ghist, a.status, a.words);
+  if (!counted) {
+    hipError_t e = launch_fill_u32(a.ghist, kMaxSortPasses * 256 + kMaxSortPasses, 0u, s);
+    if (e != hipSuccess) return e;
+    const uint32_t ht = (uint32_t)((a.n + kHistTile - 1) / kHistTile);
+    k_sort_hist_all<<<ht < 2048 ? ht : 2048, kHistThreads, 0, s>>>(a.keys, a.n, passes, a.key_or,
+                                                                    a.ghist, a.status, a.words);
+  }
   const unsigned E = box ? 2 * D : D;
   for (unsigned p = 0; p < passes; ++p) {
     // the pass's source kind, when the host knows the key width

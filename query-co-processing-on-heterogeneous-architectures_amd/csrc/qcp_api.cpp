@@ -550,14 +550,18 @@ static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned pas
   ix->keys_stale = true;
   HIP_TRY(launch_fill_u32((uint32_t*)ix->d_key_or, 2, 0u, s));
   HIP_TRY(launch_fill_u32(ix->d_bad, 1, 0u, s));
+  // the key kernel also counts the sort's digits (graph builds have no
+  // k-means keys)
+  HIP_TRY(launch_fill_u32(ix->d_sort_ghist, kSortGhistWords, 0u, s));
   HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
                       ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, nullptr,
-                      ix->d_key_or, ix->d_bad, s));
+                      ix->d_key_or, ix->d_bad, s, ix->d_sort_ghist, ix->d_sort_status,
+                      (uint64_t)sort_tiles(n) * 256));
   if (ix->carry) {
     // every pass a key can need; the last one that reorders writes the index
     const unsigned cpasses = (ix->D * ix->nb + 7) / 8;
     HIP_TRY(launch_sort_carry(carry_args(ix, d_objs, 0, ix->d_key_or), ix->D, ix->box,
-                              carry_keeps_perm(ix), cpasses, s));
+                              carry_keeps_perm(ix), cpasses, s, true));
     HIP_TRY(launch_carry_identity(d_objs, n, ix->d_key_or, ix->D, ix->box, ix->d_obj,
                                   ix->obj_stride, carry_keeps_perm(ix) ? ix->d_perm : nullptr, s));
     return enqueue_boxes(ix, false);
@@ -565,7 +569,8 @@ static qcp_status enqueue_build(qcp_index* ix, const float* d_objs, unsigned pas
   if (passes & 1u) return QCP_EINVAL;  // odd pass counts are not captured
   const unsigned pbits = index_bits(n);
   HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
-                           ix->d_key_or, ix->d_sort_ghist, ix->d_sort_status, s, -1, pbits));
+                           ix->d_key_or, ix->d_sort_ghist, ix->d_sort_status, s, -1, pbits,
+                           true));
   const uint32_t fl = fused_leaf_fanout(ix);
   HIP_TRY(launch_gather_objs_planned(d_objs, ix->d_perm, ix->d_perm2, ix->d_keys, ix->d_keys2,
                                      n, pbits, passes, ix->d_key_or, ix->D, ix->box, ix->d_obj,
@@ -627,11 +632,17 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
   uint32_t bad = 0;
   if (n > 0) {
     ix->keys_stale = false;
+    // the key kernel also counts the sort's digits, unless k-means replaces
+    // the keys it writes
+    const bool counted = ix->cfg.order != QCP_ORDER_KMEANS_HILBERT;
     HIP_TRY(launch_fill_u32((uint32_t*)ix->d_key_or, 2, 0u, s));
     HIP_TRY(launch_fill_u32(ix->d_bad, 1, 0u, s));
+    if (counted) HIP_TRY(launch_fill_u32(ix->d_sort_ghist, kSortGhistWords, 0u, s));
     HIP_TRY(launch_keys(d_objs, n, ix->D, ix->box, ix->cfg.key_mode == QCP_KEY_REF,
                         ix->cfg.key_src == QCP_SRC_CENTER, ix->nb, ix->d_keys, nullptr,
-                        ix->d_key_or, ix->d_bad, s));
+                        ix->d_key_or, ix->d_bad, s, counted ? ix->d_sort_ghist : nullptr,
+                        counted ? ix->d_sort_status : nullptr,
+                        counted ? (uint64_t)sort_tiles(n) * 256 : 0));
     HIP_TRY(hipMemcpyAsync(&key_or, ix->d_key_or, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(&bad, ix->d_bad, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipEventRecord(ix->ev[1], s));
@@ -655,7 +666,7 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
                                    ix->obj_stride, perm_out, 0, nullptr, 0, 0, s));
       } else {
         HIP_TRY(launch_sort_carry(carry_args(ix, d_objs, bits, nullptr), ix->D, ix->box,
-                                  perm_out != nullptr, passes, s));
+                                  perm_out != nullptr, passes, s, counted));
       }
       ix->keys_stale = true;  // d_keys holds key remainders
       HIP_TRY(hipEventRecord(ix->ev[2], s));
@@ -670,7 +681,8 @@ static qcp_status index_run_build(qcp_index* ix, const float* d_objs) {
     const int pack_pass = (!guttman && ix->cfg.order != QCP_ORDER_KMEANS_HILBERT)
                               ? pack_pass_for(bits, pbits) : -1;
     HIP_TRY(launch_sort_keys(ix->d_keys, ix->d_perm, ix->d_keys2, ix->d_perm2, n, passes,
-                             nullptr, ix->d_sort_ghist, ix->d_sort_status, s, pack_pass, pbits));
+                             nullptr, ix->d_sort_ghist, ix->d_sort_status, s, pack_pass, pbits,
+                             counted));
     if (passes == 0) HIP_TRY(launch_iota_u32(ix->d_perm, n, s));  // all keys equal
     if (pack_pass < 0 && (passes & 1u)) {
       std::swap(ix->d_keys, ix->d_keys2);

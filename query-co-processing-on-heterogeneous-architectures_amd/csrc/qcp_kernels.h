@@ -13,10 +13,16 @@ namespace qcp {
 constexpr uint32_t kBadDomain = 1u;    // NaN, or outside the quantiser's domain
 constexpr uint32_t kBadInverted = 2u;  // a box with lo > hi in some dimension
 // keys[i] of each object and, when idx is non-null, idx[i] = i; the OR of all
-// keys into *key_or and domain / inverted-box flags into *bad.
+// keys into *key_or and domain / inverted-box flags into *bad.  With ghist
+// (kSortGhistWords words, zero at entry) the kernel also counts the digits of
+// every byte a key can hold and zeroes the sort's first look-back words
+// (status0, status_words = sort_tiles(n) * 256), so that the sort of these
+// keys runs with `counted` set.
 hipError_t launch_keys(const float* objs, uint64_t n, unsigned D, bool box, bool ref,
                        bool center, unsigned nb, uint64_t* keys, uint32_t* idx,
-                       unsigned long long* key_or, uint32_t* bad, hipStream_t s);
+                       unsigned long long* key_or, uint32_t* bad, hipStream_t s,
+                       uint32_t* ghist = nullptr, uint64_t* status0 = nullptr,
+                       uint64_t status_words = 0);
 uint32_t sort_tiles(uint64_t n);
 hipError_t launch_assign(const float* objs, uint64_t n, unsigned D, bool box, bool center,
                          const float* centers, uint32_t K, uint32_t* cid, uint32_t* changes,
@@ -53,10 +59,12 @@ hipError_t launch_km_finish(const double* sum, const uint64_t* total, uint32_t K
 // pbits > 0, from which pass the elements are packed; launch_gather_objs_planned
 // finds the result and unpacks it.
 uint64_t sort_status_words(uint64_t n);
+// counted: the digit counts are already in ghist (launch_keys with ghist), and
+// ghist's tile counters and the first look-back words are zero.
 hipError_t launch_sort_keys(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2,
                             uint64_t n, unsigned passes, const unsigned long long* key_or,
                             uint32_t* ghist, uint64_t* status, hipStream_t s, int pack_pass = -1,
-                            unsigned pbits = 0);
+                            unsigned pbits = 0, bool counted = false);
 constexpr uint32_t kSortGhistWords = 8 * 256 + 8;
 // Object-carrying LSD radix sort: every pass moves the key bits still to be
 // sorted and the objects (plus, with idx, their input indices) themselves, and
@@ -85,7 +93,7 @@ struct CarryArgs {
   uint64_t words;          // look-back words of one pass: sort_tiles(n) * 256
 };
 hipError_t launch_sort_carry(const CarryArgs& a, unsigned D, bool box, bool idx,
-                             unsigned passes, hipStream_t s);
+                             unsigned passes, hipStream_t s, bool counted = false);
 // objects in input order (and perm_out = identity) when *key_or == 0; nothing otherwise
 hipError_t launch_carry_identity(const float* objs, uint64_t n, const unsigned long long* key_or,
                                  unsigned D, bool box, float* obj, uint64_t stride,

@@ -62,7 +62,8 @@ def _compare_tree(ix, t, tag):
     ex = ix.export()
     assert ex["info"]["n"] == t.n and ex["info"]["n_nodes"] == t.n_nodes, tag
     assert_bits_equal(ex["keys"], t.keys, tag + " sorted keys")
-    assert_bits_equal(ex["perm"], t.perm, tag + " permutation")
+    if ex["perm"] is not None:  # an object-form build may keep no permutation
+        assert_bits_equal(ex["perm"], t.perm, tag + " permutation")
     assert_bits_equal(ex["sobjs"], t.sobjs, tag + " rank-ordered objects")
     assert_bits_equal(ex["first"], t.first, tag + " node first")
     assert_bits_equal(ex["count"], t.count, tag + " node count")
@@ -192,8 +193,9 @@ def test_c4_served_results(gpu, O):
 
 def test_c5_timed_build_paths(gpu, O):
     """The skewed 2-D config from device-resident points through the timed
-    build paths (build, then rebuild in place; eager and graph-captured),
-    searched with subtree donation on."""
+    build paths (build, then rebuild in place; eager and graph-captured; the
+    library's form and the object-carrying sort with and without the
+    permutation), searched with subtree donation on."""
     t0 = time.time()
     w = gpu.workload("c5", scale=C5_SCALE)
     pts, queries = w["objs"], w["queries"]
@@ -224,5 +226,20 @@ def test_c5_timed_build_paths(gpu, O):
                      (tag, len(idx), n_ids))
             ix.close()
             _log(t0, tag + " equal")
+        # the object-carrying sort (the library's form for these points beyond the
+        # Infinity Cache), with and without the permutation, eager and graph
+        for graph in (False, True):
+            for keep in (True, False):
+                tag = "c5 objects %s perm=%d" % ("graph" if graph else "eager", keep)
+                ix = gpu.Index.build_device(buf, 2, fanout=64, layout="mphr", graph=graph,
+                                            sort="objects", keep_perm=keep)
+                ix.rebuild(buf)
+                assert ix.info()["sort_form"] == 1 and ix.info()["has_perm"] == int(keep), tag
+                _compare_tree(ix, t, tag)
+                with ix.batch(queries) as b:
+                    st = b.run(donate=1)
+                    _check_results(b, counts, hashes, tag + " donation", st)
+                ix.close()
+                _log(t0, tag + " equal")
     finally:
         buf.close()
