@@ -22,7 +22,8 @@ def _oracle_tree(O, objs, D, is_box, F, key_mode, key_src, layout):
 
 
 def check_case(gpu, O, objs, D, is_box, F, key_mode="ref", key_src="lo", layout="mphr",
-               nq=200, seed=0, plain=True, queries=None, sort=None, keep_perm=True):
+               nq=200, seed=0, plain=True, queries=None, sort=None, keep_perm=True,
+               zero_sign=True):
     tag = "D=%d box=%d F=%d key=%s src=%s layout=%s n=%d sort=%s perm=%d" % (
         D, is_box, F, key_mode, key_src, layout, len(objs), sort, keep_perm)
     print("case", tag, flush=True)
@@ -45,7 +46,16 @@ def check_case(gpu, O, objs, D, is_box, F, key_mode="ref", key_src="lo", layout=
     assert_bits_equal(ex["first"], t.first, tag + " node first")
     assert_bits_equal(ex["count"], t.count, tag + " node count")
     assert_bits_equal(ex["leaf"], t.leaf, tag + " node leaf flag")
-    assert_bits_equal(ex["boxes"], t.boxes[: t.n_nodes], tag + " node MBRs")
+    if zero_sign:
+        assert_bits_equal(ex["boxes"], t.boxes[: t.n_nodes], tag + " node MBRs")
+    else:
+        # objects holding both -0.0 and +0.0: the reference's min/max trees keep
+        # whichever zero their pairing order meets first, so a zero bound's sign
+        # is not pinned; every other bound is compared bit for bit
+        got_b, exp_b = ex["boxes"], t.boxes[: t.n_nodes]
+        assert_bits_equal(got_b == 0, exp_b == 0, tag + " node MBR zero bounds")
+        assert_bits_equal(np.where(got_b == 0, 0, got_b), np.where(exp_b == 0, 0, exp_b),
+                          tag + " node MBRs (zero bounds by value)")
     if queries is None:
         queries = rand_queries(nq, D, seed + 17, objs if n else None)
     exp = t.search_dfs(queries, want_ids=True)
@@ -438,3 +448,60 @@ def test_subnormal_coordinates(gpu, O, layout, box):
     assert (np.abs(q[q != 0]) < np.float32(2.0 ** -126)).mean() > 0.5
     check_case(gpu, O, objs, D, box, 64, layout=layout, nq=0, plain=layout != "bvh",
                queries=q)
+
+
+@pytest.mark.parametrize("D,box,layout,sort", [(2, False, "mphr", None), (2, False, "mphr", "objects"),
+                                               (3, False, "mphr", None), (3, False, "bvh", "objects"),
+                                               (2, True, "rtree", None), (3, True, "bvh", "objects")])
+def test_special_values(gpu, O, D, box, layout, sort):
+    """Infinite coordinates (b16 keys clamp them, so such objects build), signed
+    zeros, and query bounds that are NaN, infinite, inverted or degenerate,
+    against the reference's compares: a NaN bound rejects nothing, a query with
+    lo > hi in some dimension hits nothing, and an infinite coordinate equal to
+    its bound is inside.  The point test's med3 form and the box test's
+    subtraction form must reproduce each case, in every search form."""
+    rng = np.random.default_rng(D * 7 + int(box) * 3 + len(layout))
+    n = 20000
+    special = np.array([np.inf, -np.inf, 0.0, -0.0, 1.0, 0.5], np.float32)
+    lo = rng.random((n, D), dtype=np.float32)
+    m = rng.random((n, D)) < 0.05
+    lo[m] = rng.choice(special, size=int(m.sum()))
+    if box:
+        side = (rng.random((n, D), dtype=np.float32) * np.float32(0.01)).astype(np.float32)
+        side[(rng.random((n, D)) < 0.02) & (lo != -np.inf)] = np.inf  # boxes reaching +inf
+        objs = np.ascontiguousarray(np.concatenate([lo, lo + side], 1), np.float32)
+    else:
+        objs = np.ascontiguousarray(lo)
+    inf, nan = np.float32(np.inf), np.float32(np.nan)
+
+    def qry(l, h):
+        return np.array(list(l) + list(h), np.float32)
+
+    qs = [qry([-inf] * D, [inf] * D),              # everything
+          qry([nan] * D, [nan] * D),               # NaN bounds reject nothing
+          qry([nan] + [0.25] * (D - 1), [0.5] + [nan] * (D - 1)),
+          qry([inf] + [-inf] * (D - 1), [inf] * D),  # the +inf plane of dimension 0
+          qry([-inf] * D, [-inf] + [inf] * (D - 1)),  # the -inf plane
+          qry([0.0] * D, [0.0] * D),               # signed zeros on both bounds
+          qry([-0.0] * D, [-0.0] * D),
+          qry([-0.0] + [0.0] * (D - 1), [0.0] + [1.0] * (D - 1)),
+          qry([0.6] + [0.0] * (D - 1), [0.4] + [1.0] * (D - 1)),  # inverted in dimension 0
+          qry([0.6] + [nan] * (D - 1), [0.4] + [nan] * (D - 1)),  # inverted beside NaN bounds
+          qry([1.0] * D, [1.0] * D),               # degenerate, on a special value
+          qry([0.5, 0.5] + [-inf] * (D - 2), [0.5, inf] + [inf] * (D - 2))]
+    for _ in range(60):  # random boxes with one bound replaced by a special value
+        c = rng.random(D, dtype=np.float32)
+        h = np.float32(0.05)
+        ql, qh = c - h, c + h
+        d = rng.integers(0, D)
+        ql[d] = rng.choice(np.array([nan, -inf, -0.0, 0.0, inf], np.float32))
+        d = rng.integers(0, D)
+        qh[d] = rng.choice(np.array([nan, inf, 0.0, -0.0, -inf], np.float32))
+        qs.append(np.concatenate([ql, qh]))
+    qs.extend(rand_queries(60, D, 5, objs))
+    q = np.ascontiguousarray(np.array(qs, np.float32))
+    exp = check_case(gpu, O, objs, D, box, 64, key_mode="b16", layout=layout, queries=q,
+                     plain=layout != "bvh", zero_sign=False, sort=sort)
+    assert exp["counts"][0] == n and exp["counts"][1] == n  # the oracle agrees on the basics
+    if not box:  # a box can span an inverted interval; a point cannot
+        assert exp["counts"][8] == 0 and exp["counts"][9] == 0
