@@ -297,8 +297,72 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
       // Hits are counted per lane and summed when the item ends (a scalar
       // popcount of each entry's hit mask loads the scalar unit, which the
       // traversal also needs).  The run must start on a 16-byte boundary.
+      // Hashes take the first loop below, counts alone the second.
       constexpr int G = U / 4;
-      if ((cs & 3u) == 0) {
+      if (OUT == kOutHash && (cs & 3u) == 0 && ce - cs >= (uint32_t)(G * 256)) {
+        // ID sums from per-group hit moments.  Lane l's entries j = 0..3 of
+        // group k (256 entries from the segment start) have IDs
+        // A_k + 4l + j, A_k = R + 256k.  Per group the lane forms, in one
+        // register, n (hits), J1 = sum of j and J2 = sum of j^2 over its hits
+        // (9, 9 and 14-bit fields), adds that word to cp, and adds k*n, k*J1
+        // and k^2*n to g1, gj1, g2; flush() turns these exact small integers
+        // into the 64-bit sums of the IDs and of their squares (mod 2^64, as
+        // the per-entry sums are), every 64 groups and at the segment's end.
+        // That is 12 VALU per four entries where a select, a 64-bit add, a
+        // 64-bit multiply-add and the ID's add per entry took 16 and more.
+        const uint32_t lane4 = 4u * lane;
+        uint32_t r0 = cs + 1u;
+        uint32_t cp = 0, g1 = 0, gj1 = 0, g2 = 0, gk = 0;
+        auto flush = [&]() {
+          const uint64_t N = cp & 0x1FFu, J1 = (cp >> 9) & 0x1FFu, J2 = cp >> 18;
+          const uint64_t R = r0, L = lane4;
+          const uint64_t an = R * N + 256ull * g1;  // sum over groups of n_k A_k
+          c.cnt_lane += (uint32_t)N;
+          c.s1_lane += an + L * N + J1;
+          c.s2_lane += R * R * N + 512ull * R * g1 + 65536ull * g2 +
+                       2ull * (L * an + R * J1 + 256ull * gj1) + L * L * N + 2ull * L * J1 + J2;
+          cp = g1 = gj1 = g2 = 0;
+        };
+        while (ce - cs >= (uint32_t)(G * 256)) {
+          F32x4 w[G][E];
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const GlobalF32x4* base = reinterpret_cast<const GlobalF32x4*>(
+                uni64(reinterpret_cast<uint64_t>(col[e] + cs)));
+#pragma unroll
+            for (int g = 0; g < G; ++g) w[g][e] = base[g * kWave + (int)lane];
+          }
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            uint32_t pw = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              float v[E];
+#pragma unroll
+              for (int e = 0; e < E; ++e) v[e] = w[g][e][j];
+              constexpr uint32_t kPk[4] = {1u, 1u | (1u << 9) | (1u << 18),
+                                           1u | (2u << 9) | (4u << 18),
+                                           1u | (3u << 9) | (9u << 18)};
+              pw += leaf_hit<D, BOX>(c, v, BOX ? v + D : v) ? kPk[j] : 0u;
+            }
+            const uint32_t k = uni(gk + (uint32_t)g);  // < 66: 24-bit products
+            cp += pw;
+            const uint32_t n = pw & 0x1FFu, j1 = (pw >> 9) & 0x1FFu;
+            g1 += __umul24(k, n);
+            gj1 += __umul24(k, j1);
+            g2 += __umul24(uni(k * k), n);
+          }
+          cs += (uint32_t)(G * 256);
+          gk += (uint32_t)G;
+          if (gk >= 64u) {
+            flush();
+            r0 = cs + 1u;
+            gk = 0;
+          }
+        }
+        flush();
+      }
+      if (OUT == kOutCount && (cs & 3u) == 0) {
         while (ce - cs >= (uint32_t)(G * 256)) {
           F32x4 w[G][E];
 #pragma unroll
@@ -315,14 +379,7 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
               float v[E];
 #pragma unroll
               for (int e = 0; e < E; ++e) v[e] = w[g][e][j];
-              const bool hit = leaf_hit<D, BOX>(c, v, BOX ? v + D : v);
-              c.cnt_lane += hit ? 1u : 0u;
-              if (OUT == kOutHash) {
-                const uint32_t id = cs + (uint32_t)(g * 256) + 4u * lane + (uint32_t)j + 1u;
-                const uint32_t x = hit ? id : 0u;
-                c.s1_lane += x;
-                c.s2_lane += (uint64_t)x * x;
-              }
+              c.cnt_lane += leaf_hit<D, BOX>(c, v, BOX ? v + D : v) ? 1u : 0u;
             }
           }
           cs += (uint32_t)(G * 256);
