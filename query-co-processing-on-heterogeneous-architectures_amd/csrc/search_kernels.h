@@ -436,6 +436,11 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
           ce = uni((uint32_t)e);
           c.le += ce - cs;
           if (!BOX && c.pempty) cs = ce;  // visited, and no point can hit
+          // a run left empty takes the next one here: this loop may end
+          // only on a run with entries to scan, an empty mask, or a child
+          // that is neither a leaf nor inside the query (which the caller
+          // then descends into), never on a leaf run still in the mask
+          continue;
         }
         break;
       }

@@ -505,3 +505,45 @@ def test_special_values(gpu, O, D, box, layout, sort):
     assert exp["counts"][0] == n and exp["counts"][1] == n  # the oracle agrees on the basics
     if not box:  # a box can span an inverted interval; a point cannot
         assert exp["counts"][8] == 0 and exp["counts"][9] == 0
+
+
+def _max_leaf_runs(O, objs, D, F, q):
+    """Most runs of consecutive leaves under one leaf-parent node of the
+    oracle's packed tree whose boxes overlap query q (the reference's test)."""
+    t = O.build_packed(objs, D, False, F, O.KEY_REF)
+    nl = O.level_counts(len(objs), F)[-1]
+    b = t.boxes[t.n_nodes - nl:t.n_nodes]
+    ov = np.ones(nl, bool)
+    for d in range(D):
+        ov &= ~((q[d] > b[:, D + d]) | (q[D + d] < b[:, d]))
+    most = 0
+    for j in range(0, nl, F):
+        o = ov[j:j + F].astype(np.int8)
+        most = max(most, int(o[0] + ((o[1:] == 1) & (o[:-1] == 0)).sum()))
+    return most
+
+
+@pytest.mark.parametrize("F", [64, 128])
+def test_inverted_queries_many_leaf_runs(gpu, O, F):
+    """Queries with lo > hi in one dimension hit no point, but the reference
+    still visits every node whose box spans the inverted interval.  Over
+    uniform 3-D points a thin inverted slab gives leaf-parent nodes with ten or
+    more separate runs of such leaves, more than a wave's chunks in flight:
+    each run must be visited (its entries counted as visited, none scanned)
+    without the traversal ever taking a leaf for an internal node."""
+    n, D = 200000, 3
+    objs = rand_points(n, D, seed=5)
+    qs = []
+    for d in range(D):
+        for b in (0.3, 0.5, 0.7):
+            lo = np.zeros(D, np.float32)
+            hi = np.ones(D, np.float32)
+            lo[d], hi[d] = np.float32(b + 1e-4), np.float32(b)
+            qs.append(np.concatenate([lo, hi]))
+    runs = [_max_leaf_runs(O, objs, D, F, q) for q in qs]
+    assert max(runs) > 8, runs  # more runs in one node than any chunk setting keeps
+    qs.extend(rand_queries(40, D, 6, objs))
+    q = np.ascontiguousarray(np.array(qs, np.float32))
+    exp = check_case(gpu, O, objs, D, False, F, queries=q)
+    assert (exp["counts"][:len(runs)] == 0).all()
+    assert (exp["leaf_entries"][:len(runs)] > 0).all()
