@@ -148,6 +148,10 @@ __device__ __forceinline__ bool leaf_hit(const Ctx<D>& c, const float* lo, const
 // Output modes of the persistent form.
 enum { kOutCount = 0, kOutHash = 1, kOutIds = 2 };
 
+// Shortest leaf run that counts and hashes scan in 256-entry groups (four
+// entries a lane) rather than in 64-entry chunks.
+constexpr uint32_t kGroupMin = 128;
+
 // FP32 in the global address space (a pointer rebuilt from an integer is
 // otherwise generic, which would turn its loads into flat loads)
 typedef __attribute__((address_space(1))) float GlobalF32;
@@ -297,9 +301,29 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
       // Hits are counted per lane and summed when the item ends (a scalar
       // popcount of each entry's hit mask loads the scalar unit, which the
       // traversal also needs).  The run must start on a 16-byte boundary.
-      // Hashes take the first loop below, counts alone the second.
+      // Hashes take the first block below, counts alone the second.  A run
+      // reaches them whole (the refill below leaves aligned runs of at least
+      // kGroupMin entries to them): groups of G, then single groups, the
+      // last one partial.  A lane whose entries lie past the run's end still
+      // loads them (the object arrays hold 1024 slack entries past N) and
+      // does not count them.
       constexpr int G = U / 4;
-      if (OUT == kOutHash && (cs & 3u) == 0 && ce - cs >= (uint32_t)(G * 256)) {
+      constexpr uint32_t kPk[4] = {1u, 1u | (1u << 9) | (1u << 18), 1u | (2u << 9) | (4u << 18),
+                                   1u | (3u << 9) | (9u << 18)};
+      // the leaf test of lane entry j of a group (w: its four entries a column)
+      auto hit_of = [&](const F32x4* w, int j) {
+        float v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = w[e][j];
+        return leaf_hit<D, BOX>(c, v, BOX ? v + D : v);
+      };
+      auto load_group = [&](F32x4 (&w)[E], uint32_t at) {
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          w[e] = reinterpret_cast<const GlobalF32x4*>(
+              uni64(reinterpret_cast<uint64_t>(col[e] + at)))[lane];
+      };
+      if (OUT == kOutHash && (cs & 3u) == 0 && ce - cs >= kGroupMin) {
         // ID sums from per-group hit moments.  Lane l's entries j = 0..3 of
         // group k (256 entries from the segment start) have IDs
         // A_k + 4l + j, A_k = R + 256k.  Per group the lane forms, in one
@@ -323,6 +347,22 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
                        2ull * (L * an + R * J1 + 256ull * gj1) + L * L * N + 2ull * L * J1 + J2;
           cp = g1 = gj1 = g2 = 0;
         };
+        auto add_group = [&](uint32_t pw, uint32_t k) {  // k < 66: 24-bit products
+          cp += pw;
+          const uint32_t n = pw & 0x1FFu, j1 = (pw >> 9) & 0x1FFu;
+          g1 += __umul24(k, n);
+          gj1 += __umul24(k, j1);
+          g2 += __umul24(uni(k * k), n);
+        };
+        auto next_group = [&](uint32_t step, uint32_t groups) {
+          cs += step;
+          gk += groups;
+          if (gk >= 64u) {
+            flush();
+            r0 = cs + 1u;
+            gk = 0;
+          }
+        };
         while (ce - cs >= (uint32_t)(G * 256)) {
           F32x4 w[G][E];
 #pragma unroll
@@ -336,33 +376,25 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
           for (int g = 0; g < G; ++g) {
             uint32_t pw = 0;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              float v[E];
+            for (int j = 0; j < 4; ++j) pw += hit_of(w[g], j) ? kPk[j] : 0u;
+            add_group(pw, uni(gk + (uint32_t)g));
+          }
+          next_group((uint32_t)(G * 256), (uint32_t)G);
+        }
+        while (ce != cs) {
+          const uint32_t rem = ce - cs;
+          F32x4 w[E];
+          load_group(w, cs);
+          uint32_t pw = 0;
 #pragma unroll
-              for (int e = 0; e < E; ++e) v[e] = w[g][e][j];
-              constexpr uint32_t kPk[4] = {1u, 1u | (1u << 9) | (1u << 18),
-                                           1u | (2u << 9) | (4u << 18),
-                                           1u | (3u << 9) | (9u << 18)};
-              pw += leaf_hit<D, BOX>(c, v, BOX ? v + D : v) ? kPk[j] : 0u;
-            }
-            const uint32_t k = uni(gk + (uint32_t)g);  // < 66: 24-bit products
-            cp += pw;
-            const uint32_t n = pw & 0x1FFu, j1 = (pw >> 9) & 0x1FFu;
-            g1 += __umul24(k, n);
-            gj1 += __umul24(k, j1);
-            g2 += __umul24(uni(k * k), n);
-          }
-          cs += (uint32_t)(G * 256);
-          gk += (uint32_t)G;
-          if (gk >= 64u) {
-            flush();
-            r0 = cs + 1u;
-            gk = 0;
-          }
+          for (int j = 0; j < 4; ++j)
+            pw += (lane4 + (uint32_t)j < rem && hit_of(w, j)) ? kPk[j] : 0u;
+          add_group(pw, uni(gk));
+          next_group(rem < 256u ? rem : 256u, 1u);
         }
         flush();
       }
-      if (OUT == kOutCount && (cs & 3u) == 0) {
+      if (OUT == kOutCount && (cs & 3u) == 0 && ce - cs >= kGroupMin) {
         while (ce - cs >= (uint32_t)(G * 256)) {
           F32x4 w[G][E];
 #pragma unroll
@@ -375,14 +407,19 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
 #pragma unroll
           for (int g = 0; g < G; ++g) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              float v[E];
-#pragma unroll
-              for (int e = 0; e < E; ++e) v[e] = w[g][e][j];
-              c.cnt_lane += leaf_hit<D, BOX>(c, v, BOX ? v + D : v) ? 1u : 0u;
-            }
+            for (int j = 0; j < 4; ++j) c.cnt_lane += hit_of(w[g], j) ? 1u : 0u;
           }
           cs += (uint32_t)(G * 256);
+        }
+        const uint32_t lane4 = 4u * lane;
+        while (ce != cs) {
+          const uint32_t rem = ce - cs;
+          F32x4 w[E];
+          load_group(w, cs);
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            c.cnt_lane += (lane4 + (uint32_t)j < rem && hit_of(w, j)) ? 1u : 0u;
+          cs += rem < 256u ? rem : 256u;
         }
       }
     }
@@ -409,6 +446,7 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
     uint32_t st[U];
     uint32_t len[U];
     bool blocked = false;
+    bool defer = false;  // a run for the group path above: no chunks from it
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       while (cs == ce && !blocked) {
@@ -436,6 +474,7 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
           ce = uni((uint32_t)e);
           c.le += ce - cs;
           if (!BOX && c.pempty) cs = ce;  // visited, and no point can hit
+          if (OUT != kOutIds && U >= 4 && (cs & 3u) == 0 && ce - cs >= kGroupMin) defer = true;
           // a run left empty takes the next one here: this loop may end
           // only on a run with entries to scan, an empty mask, or a child
           // that is neither a leaf nor inside the query (which the caller
@@ -444,12 +483,16 @@ __device__ __forceinline__ void scan_runs(const SearchArgs& a, Ctx<D>& c, Mask<M
         }
         break;
       }
-      const uint32_t rem = ce - cs;  // cs <= ce: chunks never pass the run's end
+      // cs <= ce: chunks never pass the run's end
+      const uint32_t rem = defer ? 0u : ce - cs;
       len[u] = rem < (uint32_t)kWave ? rem : (uint32_t)kWave;
       st[u] = cs;
       cs += len[u];
     }
-    if (len[0] == 0) break;
+    if (len[0] == 0) {
+      if (defer) continue;  // the group path takes the new run whole
+      break;
+    }
     float v[U][E];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
