@@ -26,17 +26,18 @@ namespace qcp {
 // first D entries of the branch, src/tree/tree.cpp:600) or the box centre
 // (lo+hi)*0.5f.  The walk's transitions over L levels, indexed by each
 // coordinate's L bits, and over the top nb % L levels from the initial state
-// (hilbert_key.h) are evaluated at compile time into constant memory; each
-// workgroup copies them into LDS (4 KB), and a key then takes one start
-// lookup and nb/L table lookups, with no bit interleaving.  (Computing the
-// 1536 / 2048 entries in every workgroup cost about 47 ns of the device per
-// workgroup, 0.39 ms for the key kernel's 8192 workgroups whatever the
-// number of keys.)
+// (hilbert_key.h) are evaluated at compile time into constant memory, in the
+// form with the parity of the bits above (so the walk itself yields the
+// inverse Gray code); each workgroup copies them into LDS (8 / 6 KB), and a
+// key then takes one start lookup, nb/L table lookups and one constant XOR,
+// with no bit interleaving.  (Computing the 1536 / 2048 entries in every
+// workgroup cost about 47 ns of the device per workgroup, 0.39 ms for the key
+// kernel's 8192 workgroups whatever the number of keys.)
 template <int D>
 struct alignas(16) KeyTabConst {
-  uint16_t tab[HilbertTable<D>::SIZE];
+  uint16_t tab[HilbertTable<D>::PSIZE];
   constexpr KeyTabConst() : tab() {
-    for (uint32_t e = 0; e < HilbertTable<D>::SIZE; ++e) tab[e] = hilbert_table_entry_cm<D>(e);
+    for (uint32_t e = 0; e < HilbertTable<D>::PSIZE; ++e) tab[e] = hilbert_ptable_entry_cm<D>(e);
   }
 };
 // start table of the top TOP levels (a placeholder entry when TOP = 0)
@@ -45,7 +46,7 @@ struct alignas(16) KeyStartConst {
   uint16_t start[1u << (D * (TOP ? TOP : 1))];
   constexpr KeyStartConst() : start() {
     if (TOP > 0)
-      for (uint32_t e = 0; e < (1u << (D * TOP)); ++e) start[e] = hilbert_start_entry<D>(TOP, e);
+      for (uint32_t e = 0; e < (1u << (D * TOP)); ++e) start[e] = hilbert_pstart_entry<D>(TOP, e);
   }
 };
 __constant__ const KeyTabConst<2> kKeyTab2 = KeyTabConst<2>();
@@ -56,7 +57,7 @@ __constant__ const KeyStartConst<2, 3> kKeyStart2r = KeyStartConst<2, 3>();
 template <int D, int NB>
 struct alignas(4) KeyTables {
   static constexpr int TOP = NB % HilbertTable<D>::L;
-  uint16_t tab[HilbertTable<D>::SIZE];
+  uint16_t tab[HilbertTable<D>::PSIZE];
   uint16_t start[1u << (D * (TOP ? TOP : 1))];
 };
 
@@ -66,7 +67,7 @@ __device__ __forceinline__ void fill_key_tables(KeyTables<D, NB>& t) {
   static_assert(TOP == 0 || (D == 2 && TOP == 3), "start tables exist for 2-D 31-bit keys");
   const uint32_t* src = reinterpret_cast<const uint32_t*>(D == 2 ? kKeyTab2.tab : kKeyTab3.tab);
   uint32_t* dst = reinterpret_cast<uint32_t*>(t.tab);
-  for (uint32_t e = threadIdx.x; e < HilbertTable<D>::SIZE / 2; e += blockDim.x) dst[e] = src[e];
+  for (uint32_t e = threadIdx.x; e < HilbertTable<D>::PSIZE / 2; e += blockDim.x) dst[e] = src[e];
   if (TOP > 0)
     for (uint32_t e = threadIdx.x; e < (1u << (D * TOP)); e += blockDim.x)
       t.start[e] = kKeyStart2r.start[e];
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(256) void k_keys(const float* __restrict__ objs, ui
       object_point<D, BOX, REF, CENTER>(v[r], c[r], any_bad);
     }
     uint64_t k[R];
-    hilbert_keys_cm<D, NB, R>(c, kt.tab, kt.start, k);
+    hilbert_keys_pcm<D, NB, R>(c, kt.tab, kt.start, k);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint64_t i = base + (uint64_t)r * 256 + threadIdx.x;
@@ -519,7 +520,7 @@ __global__ __launch_bounds__(256) void k_keys_sorted(const float* __restrict__ o
       object_point<D, BOX, REF, CENTER>(v, c[r], bad);
     }
     uint64_t k[R];
-    hilbert_keys_cm<D, NB, R>(c, kt.tab, kt.start, k);
+    hilbert_keys_pcm<D, NB, R>(c, kt.tab, kt.start, k);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint64_t i = base + (uint64_t)r * 256 + threadIdx.x;

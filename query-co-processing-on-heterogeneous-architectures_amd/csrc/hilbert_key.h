@@ -84,6 +84,7 @@ struct HilbertTable {
   static constexpr int IN = D * L;                  // input bits per step
   static constexpr int STATES = (1 << D) * D;
   static constexpr uint32_t SIZE = (uint32_t)STATES << IN;
+  static constexpr uint32_t PSIZE = 2 * SIZE;       // with the parity bit (below)
 };
 
 template <int D>
@@ -180,6 +181,53 @@ QCP_HD constexpr uint16_t hilbert_start_entry(int r, uint32_t e) {
   return (uint16_t)(out | ((s | (rot << D)) << 8));
 }
 
+// Inverse Gray code of the low w bits of x alone: bit i becomes the XOR of
+// bits i .. w-1.
+QCP_HD constexpr uint32_t prefix_xor_bits(uint32_t x, int w) {
+  for (int s = 1; s < w; s *= 2) x ^= x >> s;
+  return x & ((1u << w) - 1u);
+}
+
+// The walk's tables with the final inverse Gray code folded in.  Bit i of
+// hilbert_finish's prefix XOR is the XOR of the index bits at and above i:
+// within a step's IN bits that is prefix_xor_bits of the step's output, and
+// from the steps above it, the parity P of all their output bits.  So the
+// state gains P (entries at state + P * STATES), an entry's output is
+// prefix_xor_bits(out) with every bit flipped when P is set, and its next
+// state carries P ^ parity(out), which is bit 0 of prefix_xor_bits(out).
+// After the last step only the constant hilbert_finish_const remains.
+template <int D>
+QCP_HD constexpr uint16_t hilbert_ptable_entry_cm(uint32_t e) {
+  constexpr int IN = HilbertTable<D>::IN;
+  constexpr uint32_t ST = (uint32_t)HilbertTable<D>::STATES;
+  const uint32_t in = e & ((1u << IN) - 1u);
+  const uint32_t sp = e >> IN;
+  const uint32_t p = sp >= ST ? 1u : 0u;
+  const uint32_t base = hilbert_table_entry_cm<D>(((sp - p * ST) << IN) | in);
+  const uint32_t px = prefix_xor_bits(base & 0xFFu, IN);
+  const uint32_t out = px ^ (p ? (1u << IN) - 1u : 0u);
+  return (uint16_t)(out | (((base >> 8) + (p ^ (px & 1u)) * ST) << 8));
+}
+
+// The start table in the same form (the parity above the top levels is 0).
+template <int D>
+QCP_HD constexpr uint16_t hilbert_pstart_entry(int r, uint32_t e) {
+  constexpr uint32_t ST = (uint32_t)HilbertTable<D>::STATES;
+  const uint32_t base = hilbert_start_entry<D>(r, e);
+  const uint32_t px = prefix_xor_bits(base & 0xFFu, D * r);
+  return (uint16_t)(px | (((base >> 8) + (px & 1u) * ST) << 8));
+}
+
+// What hilbert_finish adds to the prefix XOR of the walk's index: the prefix
+// XOR of nthbits >> 1 (the prefix XOR is linear).
+template <int D, int NB>
+QCP_HD constexpr uint64_t hilbert_finish_const() {
+  const uint64_t nthbits = ((1ull << (D * NB)) - 1ull) / ((1ull << D) - 1ull);
+  uint64_t x = nthbits >> 1;
+  for (unsigned s = 1; s < (unsigned)D * NB; s *= 2) x ^= x >> s;
+  return x;
+}
+
 // R keys of NB bits a coordinate at once, from the coordinate-major table
 // `tab` and the start table `start` (entries for the top NB % L levels; not
 // read when NB is a multiple of L): the R walks run side by side, so that the
@@ -218,6 +266,45 @@ QCP_HD void hilbert_keys_cm(const uint32_t (&c)[R][D], const uint16_t* tab, cons
   }
   QCP_UNROLL
   for (int r = 0; r < R; ++r) out[r] = hilbert_finish<D>(out[r], NB);
+}
+
+// hilbert_keys_cm from the parity tables (hilbert_ptable_entry_cm,
+// hilbert_pstart_entry): the steps already produce the inverse Gray code, so
+// each key ends with one constant XOR instead of hilbert_finish's six
+// shift-and-XOR rounds.  out[r] equals hilbert_key<D>(c[r], NB).
+template <int D, int NB, int R>
+QCP_HD void hilbert_keys_pcm(const uint32_t (&c)[R][D], const uint16_t* ptab,
+                             const uint16_t* pstart, uint64_t (&out)[R]) {
+  constexpr int L = HilbertTable<D>::L, IN = HilbertTable<D>::IN;
+  constexpr int TOP = NB % L;
+  uint32_t st[R];  // s | rot << D, + STATES when the bits above have odd parity
+  QCP_UNROLL
+  for (int r = 0; r < R; ++r) {
+    out[r] = 0;
+    st[r] = 0;
+    if (TOP > 0) {
+      uint32_t g = 0;
+      QCP_UNROLL
+      for (int d = 0; d < D; ++d) g |= ((c[r][d] >> (NB - TOP)) & ((1u << TOP) - 1u)) << (d * TOP);
+      const uint32_t e = pstart[g];
+      out[r] = e & 0xFFu;
+      st[r] = e >> 8;
+    }
+  }
+  QCP_UNROLL
+  for (int b = NB - TOP - L; b >= 0; b -= L) {
+    QCP_UNROLL
+    for (int r = 0; r < R; ++r) {
+      uint32_t g = st[r] << IN;
+      QCP_UNROLL
+      for (int d = 0; d < D; ++d) g |= ((c[r][d] >> b) & ((1u << L) - 1u)) << (d * L);
+      const uint32_t e = ptab[g];
+      out[r] = (out[r] << IN) | (e & 0xFFu);
+      st[r] = e >> 8;
+    }
+  }
+  QCP_UNROLL
+  for (int r = 0; r < R; ++r) out[r] ^= hilbert_finish_const<D, NB>();
 }
 
 // Quantiser of src/mapper/hilbert_mapper.cpp:22 -- (ll)(1000000*p), one FP32

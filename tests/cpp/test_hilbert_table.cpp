@@ -4,9 +4,11 @@
 // level-major table form (hilbert_key_table) on every point of small grids,
 // then random and edge coordinates at every bit width (D = 2: nb 2..31;
 // D = 3: nb 2..21); the interleave against bit-by-bit placement; and the form
-// the GPU key kernels run (hilbert_keys_cm: coordinate-major table, start
-// table for the top nb % L levels, four keys side by side) at the key widths
-// they use and at widths that leave every remainder of nb % L.
+// the GPU key kernels run (hilbert_keys_pcm: coordinate-major table with the
+// parity of the bits above folded in, start table for the top nb % L levels,
+// four keys side by side), and its form without the parity (hilbert_keys_cm),
+// at the key widths they use and at widths that leave every remainder of
+// nb % L.
 // Prints "ok <cases>" and exits 0, or names the first mismatch and exits 1.
 #include <stdint.h>
 #include <stdio.h>
@@ -97,14 +99,23 @@ static std::vector<uint16_t> make_table_cm() {
   return t;
 }
 
-// hilbert_keys_cm<D, NB, 4> on random, edge and (for small NB) every point.
-template <int D, int NB>
+template <int D>
+static std::vector<uint16_t> make_ptable_cm() {
+  std::vector<uint16_t> t(qcp::HilbertTable<D>::PSIZE);
+  for (uint32_t e = 0; e < t.size(); ++e) t[e] = qcp::hilbert_ptable_entry_cm<D>(e);
+  return t;
+}
+
+// hilbert_keys_cm<D, NB, 4> (P = false) or hilbert_keys_pcm<D, NB, 4> (the
+// parity tables the GPU kernels use, P = true) on random and edge points.
+template <int D, int NB, bool P>
 static bool run_cm() {
   constexpr int L = qcp::HilbertTable<D>::L;
   constexpr int TOP = NB % L;
-  const std::vector<uint16_t> tab = make_table_cm<D>();
+  const std::vector<uint16_t> tab = P ? make_ptable_cm<D>() : make_table_cm<D>();
   std::vector<uint16_t> start(1u << (D * (TOP ? TOP : 1)));
-  for (uint32_t e = 0; e < start.size(); ++e) start[e] = qcp::hilbert_start_entry<D>(TOP, e);
+  for (uint32_t e = 0; e < start.size(); ++e)
+    start[e] = P ? qcp::hilbert_pstart_entry<D>(TOP, e) : qcp::hilbert_start_entry<D>(TOP, e);
   const uint32_t lim = (uint32_t)((1ull << NB) - 1);
   std::mt19937_64 rng(99 + D * 100 + NB);
   std::vector<uint32_t> pts;  // D coordinates per point
@@ -124,14 +135,18 @@ static bool run_cm() {
     for (int r = 0; r < 4; ++r)
       for (int d = 0; d < D; ++d) c[r][d] = pts[((i + r) % n) * D + d];
     uint64_t got[4];
-    qcp::hilbert_keys_cm<D, NB, 4>(c, tab.data(), start.data(), got);
+    if (P)
+      qcp::hilbert_keys_pcm<D, NB, 4>(c, tab.data(), start.data(), got);
+    else
+      qcp::hilbert_keys_cm<D, NB, 4>(c, tab.data(), start.data(), got);
     for (int r = 0; r < 4; ++r) {
       uint64_t c64[3] = {0, 0, 0};
       for (int d = 0; d < D; ++d) c64[d] = c[r][d];
       const uint64_t want = orc_hilbert_c2i(D, NB, c64);
       ++cases;
       if (got[r] != want) {
-        printf("mismatch (coordinate-major) D=%d nb=%d c=(%u,%u,%u): %llx want %llx\n", D, NB,
+        printf("mismatch (coordinate-major%s) D=%d nb=%d c=(%u,%u,%u): %llx want %llx\n",
+               P ? ", parity tables" : "", D, NB,
                c[r][0], c[r][1], D > 2 ? c[r][D - 1] : 0u, (unsigned long long)got[r],
                (unsigned long long)want);
         return false;
@@ -145,9 +160,13 @@ int main() {
   if (!run<2>(31, 6)) return 1;
   if (!run<3>(21, 4)) return 1;
   // the GPU kernels' widths (REF 31 / 20, b16 16) and every remainder of nb % L
-  if (!run_cm<2, 31>() || !run_cm<2, 16>() || !run_cm<2, 29>() || !run_cm<2, 30>() ||
-      !run_cm<2, 5>() || !run_cm<3, 20>() || !run_cm<3, 16>() || !run_cm<3, 21>() ||
-      !run_cm<3, 3>())
+  if (!run_cm<2, 31, false>() || !run_cm<2, 16, false>() || !run_cm<2, 29, false>() ||
+      !run_cm<2, 30, false>() || !run_cm<2, 5, false>() || !run_cm<3, 20, false>() ||
+      !run_cm<3, 16, false>() || !run_cm<3, 21, false>() || !run_cm<3, 3, false>())
+    return 1;
+  if (!run_cm<2, 31, true>() || !run_cm<2, 16, true>() || !run_cm<2, 29, true>() ||
+      !run_cm<2, 30, true>() || !run_cm<2, 5, true>() || !run_cm<3, 20, true>() ||
+      !run_cm<3, 16, true>() || !run_cm<3, 21, true>() || !run_cm<3, 3, true>())
     return 1;
   printf("ok %llu\n", (unsigned long long)cases);
   return 0;
