@@ -939,8 +939,8 @@ def main():
                          "--serve-seconds per rate), c5 or kmeans print their own line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the box has one); --gpus %d is not "
-              "supported" % args.gpus, file=sys.stderr)
+        print("bench.py: measures the one-GPU path only (sharding across GPUs is a design, "
+              "not built); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     qcp = load_qcp()
     if qcp.device_count() < 1:
