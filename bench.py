@@ -797,6 +797,35 @@ def profile_c5(qcp, args):
           flush=True)
 
 
+def profile_c3(qcp, args):
+    """A short, fixed run on the 67M-box 2-D config (objects 1.07 GB, larger
+    than the 256 MB Infinity Cache), for kernel traces and HBM counters: for
+    the R-tree and the BVH, a graph-captured build, then `steps` graph-captured
+    counts+hash searches of all 1,000,000 queries at the launch shape the C3
+    line's autotuner picks (8 waves a workgroup, 8 / 4 leaf chunks, no
+    donation).  Prints one JSON line with each launch's algorithmic bytes and
+    device time."""
+    w = qcp.workload("c3", scale=args.scale)
+    buf = qcp.DeviceBuffer(w["objs"])
+    q = w["queries"]
+    out = {}
+    for layout, chunks in (("rtree", 8), ("bvh", 4)):
+        ix = qcp.Index.build(buf, 2, fanout=128, layout=layout, obj_kind="boxes",
+                             key_src="center", graph=True)
+        launches = []
+        with ix.batch(q) as b:
+            for _ in range(args.steps):
+                st = b.run(graph=True, block_waves=8, chunks=chunks, donate=0)
+                launches.append({"algorithmic_bytes": algorithmic_bytes(st, len(q), 2, 4),
+                                 "search_ms": st["search_ms"],
+                                 "total_hits": int(st["total_hits"])})
+                log("c3 %s profile: %.3f ms search kernel" % (layout, st["search_ms"]))
+        ix.close()
+        out[layout] = launches
+    buf.close()
+    print(json.dumps({"profile": "c3 search", "queries": len(q), "layouts": out}), flush=True)
+
+
 def profile_build(qcp, args):
     """A short, fixed run of the build on the 134M-point 2-D skewed config, for
     kernel traces, in each key-sort form: (key, input index) pairs and a
@@ -923,6 +952,9 @@ def main():
                          "then a 1 GiB device-to-device copy as the calibration kernel")
     ap.add_argument("--profile-c5", action="store_true",
                     help="short search run on the 134M-point 2-D config (for rocprofv3)")
+    ap.add_argument("--profile-c3", action="store_true",
+                    help="short graph search run on the 67M-box config, both layouts "
+                         "(for rocprofv3)")
     ap.add_argument("--build-config", default="c5", choices=["c2", "c3", "c4", "c5"],
                     help="--profile-build: the workload whose build is traced")
     ap.add_argument("--profile-build", action="store_true",
@@ -957,6 +989,9 @@ def main():
         return
     if args.profile_c5:
         profile_c5(qcp, args)
+        return
+    if args.profile_c3:
+        profile_c3(qcp, args)
         return
     if args.profile_build:
         profile_build(qcp, args)
