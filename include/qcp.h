@@ -126,9 +126,9 @@ typedef struct {
   uint32_t mode;           /* QCP_SEARCH_*                                          */
   uint32_t outputs;        /* QCP_OUT_* mask                                        */
   uint32_t plain_batch;    /* plain form: queries per launch (reference default 128) */
-  uint32_t block_waves;    /* persistent form: waves per workgroup (0 = tuned)      */
+  uint32_t block_waves;    /* persistent form: waves per workgroup, 1..8 (0 = 8)    */
   uint32_t grid_blocks;    /* persistent form: workgroups (0 = fill the device)     */
-  uint32_t leaf_chunks;    /* persistent form: 64-entry leaf chunks in flight per wave (0 = tuned) */
+  uint32_t leaf_chunks;    /* persistent form: 64-entry leaf chunks in flight per wave, 2/4/8 (0 = 8) */
   uint32_t donate;         /* persistent form: subtree donation (work stealing) on/off */
   uint32_t order;          /* persistent form: 0 = process queries in Hilbert order of their
                               centres, split into one range per XCD (batches of 8192 queries

@@ -181,8 +181,8 @@ qcp_status qcp::hybrid_run(qcp_batch* b, const qcp_search_cfg& cfg, qcp_search_s
     }
   }
   const int out_mode = (cfg.outputs & QCP_OUT_HASH) ? 1 : 0;
-  const unsigned block = cfg.block_waves ? 64u * cfg.block_waves : 256u;
-  const int u = cfg.leaf_chunks ? (int)cfg.leaf_chunks : 4;
+  const unsigned block = 64u * (cfg.block_waves ? cfg.block_waves : kDefaultBlockWaves);
+  const int u = cfg.leaf_chunks ? (int)cfg.leaf_chunks : kDefaultLeafChunks;
   const unsigned full_grid = persistent_grid(ix, out_mode, block);
   const uint64_t qb = cfg.hybrid_batch ? cfg.hybrid_batch : 16384;
   unsigned nt = cfg.cpu_threads ? cfg.cpu_threads : std::thread::hardware_concurrency();

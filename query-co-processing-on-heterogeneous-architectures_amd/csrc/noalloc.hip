@@ -107,9 +107,9 @@ extern "C" qcp_status qcp_search_noalloc(qcp_index* ix, const float* d_queries, 
 
   // the same launch sequence and configuration rules as qcp_batch_run's
   // persistent form, with the scratch array's parts as its buffers
-  const unsigned block = cfg.block_waves ? 64u * cfg.block_waves : 256u;
+  const unsigned block = 64u * (cfg.block_waves ? cfg.block_waves : kDefaultBlockWaves);
   const int out_mode = hash ? 1 : 0;
-  const int u = cfg.leaf_chunks ? (int)cfg.leaf_chunks : 4;
+  const int u = cfg.leaf_chunks ? (int)cfg.leaf_chunks : kDefaultLeafChunks;
   const bool donate = cfg.donate != 0;
   const unsigned wpb = block / 64u;
   unsigned grid = cfg.grid_blocks;

@@ -1322,9 +1322,9 @@ qcp_status qcp_batch_run(qcp_batch* b, const qcp_search_cfg* cfg_in, qcp_search_
     return QCP_OK;
   }
   if (cfg.block_waves > 8) return QCP_EINVAL;
-  const unsigned block = cfg.block_waves ? 64u * cfg.block_waves : 256u;
+  const unsigned block = 64u * (cfg.block_waves ? cfg.block_waves : kDefaultBlockWaves);
   const int out_mode = (cfg.outputs & QCP_OUT_HASH) ? 1 : 0;
-  const int u = cfg.leaf_chunks ? (int)cfg.leaf_chunks : 4;
+  const int u = cfg.leaf_chunks ? (int)cfg.leaf_chunks : kDefaultLeafChunks;
   const bool donate = cfg.donate != 0;
   // Small batches: a wave per query is enough (no donation) and a spatial
   // order buys nothing, so the grid is sized to the batch and ordering skipped.

@@ -419,9 +419,9 @@ class Batch:
             hybrid_batch=0, plain_waves=0):
         """One search of the resident batch.  donate=1 enables subtree donation
         (work stealing) in the persistent form; chunks = 64-entry leaf chunks in
-        flight per wave (0 = default); order=0 processes the queries in Hilbert
+        flight per wave (0 = 8); order=0 processes the queries in Hilbert
         order of their centres, one range per XCD, order=1 as given;
-        block_waves = waves per workgroup (0 = 4); contain=True reports subtrees
+        block_waves = waves per workgroup (0 = 8); contain=True reports subtrees
         whose box lies inside the query without visiting them (same results);
         plain_waves = 3 runs the plain form with the reference's 192-thread workgroups.
         mode="hybrid": host threads walk the levels above handoff_level (0 = the
