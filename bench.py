@@ -813,9 +813,11 @@ def profile_c3(qcp, args):
         ix = qcp.Index.build(buf, 2, fanout=128, layout=layout, obj_kind="boxes",
                              key_src="center", graph=True)
         launches = []
+        bw = args.block_waves or 8  # --block-waves / --chunks override the tuned shape
+        ch = args.chunks or chunks
         with ix.batch(q) as b:
             for _ in range(args.steps):
-                st = b.run(graph=True, block_waves=8, chunks=chunks, donate=0)
+                st = b.run(graph=True, block_waves=bw, chunks=ch, donate=0)
                 launches.append({"algorithmic_bytes": algorithmic_bytes(st, len(q), 2, 4),
                                  "search_ms": st["search_ms"],
                                  "total_hits": int(st["total_hits"])})
@@ -961,8 +963,10 @@ def main():
                     help="short build run on the 134M-point 2-D config (for rocprofv3)")
     ap.add_argument("--profile-ids", action="store_true",
                     help="short ID-list search run on the 16M-point config (for rocprofv3)")
-    ap.add_argument("--block-waves", type=int, default=0, help="--profile-traffic: waves/WG")
-    ap.add_argument("--chunks", type=int, default=0, help="--profile-traffic: leaf chunks")
+    ap.add_argument("--block-waves", type=int, default=0,
+                    help="--profile-traffic / --profile-c3: waves/WG")
+    ap.add_argument("--chunks", type=int, default=0,
+                    help="--profile-traffic / --profile-c3: leaf chunks")
     ap.add_argument("--donate", type=int, default=0, help="--profile-traffic: donation")
     ap.add_argument("--sweep", action="store_true",
                     help="--profile-traffic: every configuration the autotuner can pick")
